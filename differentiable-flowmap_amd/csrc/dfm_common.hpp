@@ -1,0 +1,265 @@
+// Shared device helpers for the flow-map kernels (gfx950 / CDNA4).
+//
+// Layout: every scalar field is a dense (sx, sy, sz) array, z fastest (2D uses
+// sz = 1).  The MAC velocity component a lives on a (n + e_a) face grid.
+// Flow-map point sets (psi, psi_a, T_a) are SoA: component c of point i is at
+// [c * npoints + i], so a wavefront reads each component fully coalesced.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define DFM_HD __device__ __forceinline__
+
+namespace dfm {
+
+struct Geo {
+  int dim;
+  int n[3];  // cells per axis (n[2] = 1 in 2D)
+  float dx;
+  float inv_dx;
+};
+
+// Shape of a face grid (axis a < dim) or of the cell grid (a == 3).
+DFM_HD void shape_of(const Geo& g, int a, int s[3]) {
+  for (int b = 0; b < 3; ++b) s[b] = g.n[b] + ((b == a && b < g.dim) ? 1 : 0);
+}
+inline void shape_of_host(const Geo& g, int a, int s[3]) {
+  for (int b = 0; b < 3; ++b) s[b] = g.n[b] + ((b == a && b < g.dim) ? 1 : 0);
+}
+inline int64_t count_of_host(const Geo& g, int a) {
+  int s[3];
+  shape_of_host(g, a, s);
+  return (int64_t)s[0] * s[1] * s[2];
+}
+
+DFM_HD int64_t lin(const int s[3], int i, int j, int k) { return ((int64_t)i * s[1] + j) * s[2] + k; }
+
+DFM_HD void unlin(const int s[3], int64_t id, int& i, int& j, int& k) {
+  k = (int)(id % s[2]);
+  int64_t t = id / s[2];
+  j = (int)(t % s[1]);
+  i = (int)(t / s[1]);
+}
+
+// Node position of index (i,j,k) on grid `a` (face axis or 3 = cells):
+// offset 0 along the face normal, 0.5 elsewhere.
+DFM_HD void node_pos(const Geo& g, int a, int i, int j, int k, float p[3]) {
+  int I[3] = {i, j, k};
+  for (int b = 0; b < 3; ++b) p[b] = ((float)I[b] + ((b == a) ? 0.0f : 0.5f)) * g.dx;
+}
+
+// XCD-aware block remap: the hardware deals consecutive workgroups round-robin
+// over the 8 XCDs; give each XCD one contiguous range of logical blocks so the
+// spatial neighbourhood a gather touches stays in that XCD's L2.
+DFM_HD int64_t xcd_block(int64_t bid, int64_t nblocks) {
+  int64_t nb8 = nblocks & ~int64_t(7);
+  if (bid >= nb8) return bid;
+  return (bid & 7) * (nb8 >> 3) + (bid >> 3);
+}
+
+// ----------------------------------------------------------- B-splines ----
+// Piecewise polynomials of the uniform B-splines of order 2 (quadratic,
+// support 1.5), 3 (cubic, support 2) and 4 (quartic, support 2.5), written
+// from their closed forms; derivative sign follows x.
+template <int K> struct Spline;
+
+template <> struct Spline<2> {
+  static constexpr int NT = 4;  // taps floor(s)-1 .. floor(s)+2
+  static constexpr int LO = -1;
+  DFM_HD static float w(float x) {
+    float a = fabsf(x);
+    return a < 0.5f ? 0.75f - a * a : (a < 1.5f ? 0.5f * (1.5f - a) * (1.5f - a) : 0.0f);
+  }
+  DFM_HD static float dw(float x) {
+    float a = fabsf(x);
+    float r = a < 0.5f ? -2.0f * a : (a < 1.5f ? a - 1.5f : 0.0f);
+    return x < 0.0f ? -r : r;
+  }
+};
+
+template <> struct Spline<3> {
+  static constexpr int NT = 6;
+  static constexpr int LO = -2;
+  DFM_HD static float w(float x) {
+    float a = fabsf(x);
+    if (a < 1.0f) return (0.5f * a - 1.0f) * a * a + (2.0f / 3.0f);
+    if (a < 2.0f) {
+      float t = 2.0f - a;
+      return t * t * t * (1.0f / 6.0f);
+    }
+    return 0.0f;
+  }
+  DFM_HD static float dw(float x) {
+    float a = fabsf(x), r = 0.0f;
+    if (a < 1.0f) r = (1.5f * a - 2.0f) * a;
+    else if (a < 2.0f) {
+      float t = 2.0f - a;
+      r = -0.5f * t * t;
+    }
+    return x < 0.0f ? -r : r;
+  }
+};
+
+template <> struct Spline<4> {
+  static constexpr int NT = 8;
+  static constexpr int LO = -3;
+  DFM_HD static float w(float x) {
+    float a = fabsf(x), a2 = a * a;
+    if (a < 0.5f) return (115.0f / 192.0f) + a2 * (-0.625f + 0.25f * a2);
+    if (a < 1.5f) return (55.0f / 96.0f) + a * ((5.0f / 24.0f) + a * (-1.25f + a * ((5.0f / 6.0f) - a * (1.0f / 6.0f))));
+    if (a < 2.5f) {
+      float t = 2.5f - a;
+      float t2 = t * t;
+      return t2 * t2 * (1.0f / 24.0f);
+    }
+    return 0.0f;
+  }
+  DFM_HD static float dw(float x) {
+    float a = fabsf(x), r = 0.0f;
+    if (a < 0.5f) r = a * (-1.25f + a * a);
+    else if (a < 1.5f) r = (5.0f / 24.0f) + a * (-2.5f + a * (2.5f - a * (2.0f / 3.0f)));
+    else if (a < 2.5f) {
+      float t = 2.5f - a;
+      r = -t * t * t * (1.0f / 6.0f);
+    }
+    return x < 0.0f ? -r : r;
+  }
+};
+
+// Per-axis tap set of one sample: clamped coordinate s in [1, S-2-1e-6],
+// first tap index and the NT weights (and derivative weights).  For the
+// quadratic spline the window is pulled back by one node when rounding puts
+// floor(s) at S-2; the dropped tap then carries zero weight, so the value is
+// unchanged and no tap leaves [0, S-1].  Wider splines clamp every tap index
+// to the field, as the reference's sample() does.
+template <int K, bool D> struct Axis {
+  int idx[Spline<K>::NT];
+  float w[Spline<K>::NT];
+  float dw[D ? Spline<K>::NT : 1];
+  DFM_HD void setup(float p_over_dx, float bl, int S) {
+    float u = p_over_dx - bl;
+    float s = fmaxf(1.0f, fminf(u, (float)(S - 2) - 1e-6f));
+    float fl = floorf(s);
+    int f0 = (int)fl;
+    if (K == 2 && f0 > S - 3) {
+      f0 = S - 3;
+      fl = (float)f0;
+    }
+#pragma unroll
+    for (int t = 0; t < Spline<K>::NT; ++t) {
+      float x = s - (fl + (float)(Spline<K>::LO + t));
+      w[t] = Spline<K>::w(x);
+      if (D) dw[t] = Spline<K>::dw(x);
+      int id = f0 + Spline<K>::LO + t;
+      if (K != 2) id = id < 0 ? 0 : (id > S - 1 ? S - 1 : id);
+      idx[t] = id;
+    }
+  }
+};
+
+// Value (and optionally the full gradient) of the B-spline interpolant of
+// field f (shape S, node offset bl) at physical point p.  Separable: the z
+// taps are contracted first, then y, then x.
+template <int K, bool GRAD, int DIM>
+DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[3], const float p[3], float inv_dx,
+                    float g[3]) {
+  constexpr int NT = Spline<K>::NT;
+  Axis<K, GRAD> ay;
+  ay.setup(p[1] * inv_dx, bl[1], S[1]);
+  if (DIM == 2) {
+    Axis<K, GRAD> ax;
+    ax.setup(p[0] * inv_dx, bl[0], S[0]);
+    float v = 0.f, gx = 0.f, gy = 0.f;
+#pragma unroll
+    for (int a = 0; a < NT; ++a) {
+      const float* row = f + (int64_t)ax.idx[a] * S[1];
+      float rv = 0.f, rd = 0.f;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        float val = row[ay.idx[b]];
+        rv = fmaf(val, ay.w[b], rv);
+        if (GRAD) rd = fmaf(val, ay.dw[b], rd);
+      }
+      v = fmaf(rv, ax.w[a], v);
+      if (GRAD) {
+        gx = fmaf(rv, ax.dw[a], gx);
+        gy = fmaf(rd, ax.w[a], gy);
+      }
+    }
+    if (GRAD) {
+      g[0] = gx * inv_dx;
+      g[1] = gy * inv_dx;
+    }
+    return v;
+  } else {
+    // x taps stay rolled (their weights are recomputed per iteration, which
+    // is cheaper than the register pressure of a fully unrolled 3D loop)
+    Axis<K, GRAD> az;
+    az.setup(p[2] * inv_dx, bl[2], S[2]);
+    float ux = p[0] * inv_dx - bl[0];
+    float sx = fmaxf(1.0f, fminf(ux, (float)(S[0] - 2) - 1e-6f));
+    float flx = floorf(sx);
+    int f0x = (int)flx;
+    if (K == 2 && f0x > S[0] - 3) {
+      f0x = S[0] - 3;
+      flx = (float)f0x;
+    }
+    float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll 1
+    for (int a = 0; a < NT; ++a) {
+      float x = sx - (flx + (float)(Spline<K>::LO + a));
+      float wxa = Spline<K>::w(x);
+      int ia = f0x + Spline<K>::LO + a;
+      if (K != 2) ia = ia < 0 ? 0 : (ia > S[0] - 1 ? S[0] - 1 : ia);
+      float yv = 0.f, yd = 0.f, yz = 0.f;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        const float* row = f + ((int64_t)ia * S[1] + ay.idx[b]) * S[2];
+        float zv = 0.f, zd = 0.f;
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+          float val = row[az.idx[c]];
+          zv = fmaf(val, az.w[c], zv);
+          if (GRAD) zd = fmaf(val, az.dw[c], zd);
+        }
+        yv = fmaf(zv, ay.w[b], yv);
+        if (GRAD) {
+          yd = fmaf(zv, ay.dw[b], yd);
+          yz = fmaf(zd, ay.w[b], yz);
+        }
+      }
+      v = fmaf(yv, wxa, v);
+      if (GRAD) {
+        gx = fmaf(yv, Spline<K>::dw(x), gx);
+        gy = fmaf(yd, wxa, gy);
+        gz = fmaf(yz, wxa, gz);
+      }
+    }
+    if (GRAD) {
+      g[0] = gx * inv_dx;
+      g[1] = gy * inv_dx;
+      g[2] = gz * inv_dx;
+    }
+    return v;
+  }
+}
+
+// MAC velocity: component pointers and their face-grid shapes.
+struct Mac {
+  const float* c[3];
+};
+
+// u(p) for the MAC field (quadratic), optionally J[a][b] = d u_a / d x_b.
+template <bool GRAD, int DIM>
+DFM_HD void mac_eval(const Geo& g, const Mac& u, const float p[3], float val[3], float J[3][3]) {
+#pragma unroll 1
+  for (int a = 0; a < DIM; ++a) {
+    int S[3];
+    shape_of(g, a, S);
+    float bl[3] = {0.5f, 0.5f, 0.5f};
+    bl[a] = 0.0f;
+    val[a] = interp<2, GRAD, DIM>(u.c[a], S, bl, p, g.inv_dx, GRAD ? J[a] : nullptr);
+  }
+}
+
+}  // namespace dfm

@@ -1,0 +1,195 @@
+"""Torch-tensor front end of the libdfm.so kernels.
+
+Fields are float32 CUDA (HIP) tensors.  A MAC field is a list of face tensors
+(component a has shape n + e_a); a flow map holds SoA point sets
+(psi: (dim, *n), psi_f[a] / T_f[a]: (dim, *face_a)).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib as L
+
+
+def face_shape(n, a):
+    return tuple(n[b] + (1 if b == a else 0) for b in range(len(n)))
+
+
+class MacField:
+    """One tensor per face axis."""
+
+    def __init__(self, comps):
+        self.c = list(comps)
+        self._s = None
+
+    @staticmethod
+    def empty(n, device, zero=False):
+        f = torch.zeros if zero else torch.empty
+        return MacField([f(face_shape(n, a), dtype=torch.float32, device=device) for a in range(len(n))])
+
+    @staticmethod
+    def from_arrays(arrs, device):
+        return MacField([torch.as_tensor(a, dtype=torch.float32).to(device).contiguous() for a in arrs])
+
+    def clone(self):
+        return MacField([t.clone() for t in self.c])
+
+    def copy_(self, other):
+        for a, b in zip(self.c, other.c):
+            a.copy_(b)
+        return self
+
+    def zero_(self):
+        for t in self.c:
+            t.zero_()
+        return self
+
+    def numpy(self):
+        return [t.detach().cpu().numpy() for t in self.c]
+
+    @property
+    def s(self):
+        if self._s is None:
+            self._s = L.make_mac(self.c)
+        return self._s
+
+
+class FlowMap:
+    """Cell map psi plus per-face maps psi_f[a] and Jacobian columns T_f[a]."""
+
+    def __init__(self, n, device, with_cells=True):
+        dim = len(n)
+        self.n = tuple(n)
+        self.psi = torch.empty((dim,) + tuple(n), dtype=torch.float32, device=device) if with_cells else None
+        self.psi_f = [torch.empty((dim,) + face_shape(n, a), dtype=torch.float32, device=device) for a in range(dim)]
+        self.T_f = [torch.empty((dim,) + face_shape(n, a), dtype=torch.float32, device=device) for a in range(dim)]
+        self._s = None
+
+    @property
+    def s(self):
+        if self._s is None:
+            self._s = L.make_map(self.psi, self.psi_f, self.T_f)
+        return self._s
+
+    def copy_(self, other):
+        if self.psi is not None:
+            self.psi.copy_(other.psi)
+        for a in range(len(self.psi_f)):
+            self.psi_f[a].copy_(other.psi_f[a])
+            self.T_f[a].copy_(other.T_f[a])
+        return self
+
+
+class Ops:
+    """Kernel launcher bound to one grid; every call is asynchronous on the
+    current torch stream."""
+
+    def __init__(self, n, dx):
+        self.n = tuple(n)
+        self.dim = len(n)
+        self.dx = float(dx)
+        self.g = L.make_grid(n, dx)
+        self.lib = L.lib()
+
+    def _st(self):
+        return L.stream_handle()
+
+    def reset_identity(self, fm: FlowMap):
+        L.check(self.lib.dfm_reset_identity(self.g, fm.s, self._st()), "reset_identity")
+
+    def march(self, u: MacField, fm: FlowMap, dt, order):
+        L.check(self.lib.dfm_march(self.g, u.s, fm.s, float(dt), int(order), self._st()), "march")
+
+    def pullback_u(self, src: MacField, fm: FlowMap, out: MacField, alpha=1.0, beta=0.0, add: MacField = None):
+        L.check(self.lib.dfm_pullback_u(self.g, src.s, fm.s, float(alpha), float(beta),
+                                        add.s if add is not None else None, out.s, self._st()), "pullback_u")
+
+    def pullback_s(self, src, psi, out, alpha=1.0, beta=0.0, add=None):
+        L.check(self.lib.dfm_pullback_s(self.g, src.data_ptr(), psi.data_ptr(), float(alpha), float(beta),
+                                        L.ptr(add), out.data_ptr(), self._st()), "pullback_s")
+
+    def bfecc_u(self, init: MacField, fwd: FlowMap, bwd: FlowMap, err: MacField, out: MacField):
+        L.check(self.lib.dfm_bfecc_u(self.g, init.s, fwd.s, bwd.s, err.s, out.s, self._st()), "bfecc_u")
+
+    def bfecc_s(self, init, psi, phi, err, out):
+        L.check(self.lib.dfm_bfecc_s(self.g, init.data_ptr(), psi.data_ptr(), phi.data_ptr(), err.data_ptr(),
+                                     out.data_ptr(), self._st()), "bfecc_s")
+
+    def accumulate(self, f: MacField, fm: FlowMap, init: MacField, h1: MacField, h2: MacField, ind, dt,
+                   f2: MacField = None, dt2=0.0):
+        L.check(self.lib.dfm_accumulate(self.g, f.s, fm.s, init.s, h1.s, h2.s, int(ind), float(dt),
+                                        f2.s if f2 is not None else None, float(dt2), self._st()), "accumulate")
+
+    def sl_advect(self, vel: MacField, dt, order, vec_src=(), vec_dst=(), s_src=(), s_dst=()):
+        nv, ns = len(vec_src), len(s_src)
+        src_arr = (L.Mac * max(1, nv))(*[m.s for m in vec_src])
+        dst_arr = (L.Mac * max(1, nv))(*[m.s for m in vec_dst])
+        L.check(self.lib.dfm_sl_advect(self.g, vel.s, float(dt), int(order), nv, src_arr, dst_arr, ns,
+                                       L.ptr_array(s_src), L.ptr_array(s_dst), self._st()), "sl_advect")
+
+    def adj_conv_source(self, u: MacField, w: MacField, kern, r: MacField):
+        L.check(self.lib.dfm_adj_conv_source(self.g, u.s, w.s, int(kern), r.s, self._st()), "adj_conv_source")
+
+    def adj_scalar_source(self, s_list, lam_list, r: MacField):
+        L.check(self.lib.dfm_adj_scalar_source(self.g, len(s_list), L.ptr_array(s_list), L.ptr_array(lam_list),
+                                               r.s, self._st()), "adj_scalar_source")
+
+    def axpy(self, x: MacField, alpha, y: MacField, out: MacField, beta=0.0, z: MacField = None):
+        L.check(self.lib.dfm_axpy_mac(self.g, x.s, float(alpha), y.s, float(beta),
+                                      z.s if z is not None else None, out.s, self._st()), "axpy")
+
+
+class MGPCG:
+    """Pressure projection on an all-Neumann box (the reference's
+    ``solver.Poisson`` / ``Poisson_adj``, 2D/vortex/mgpcg.py:363-368,
+    3D/fast_mgpcg.py:241-323), backed by the libdfm MGPCG handle."""
+
+    def __init__(self, n, dx, levels=0, pre_post=2, bottom=50, tol=1e-6, relative=True, max_iters=400,
+                 raise_on_noconv=False):
+        self.n = tuple(n)
+        self.g = L.make_grid(n, dx)
+        self.lib = L.lib()
+        h = L.c_void_p()
+        L.check(self.lib.dfm_mgpcg_create(self.g, int(levels), int(pre_post), int(bottom), L.ctypes.byref(h)),
+                "mgpcg_create")
+        self.h = h
+        self.tol, self.relative, self.max_iters = tol, relative, max_iters
+        self.raise_on_noconv = raise_on_noconv
+        self.last_iters = 0
+        self.noconv_count = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.dfm_mgpcg_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def status(self):
+        it = L.c_int(0)
+        res = L.c_double(0)
+        L.check(self.lib.dfm_mgpcg_status(self.h, L.ctypes.byref(it), L.ctypes.byref(res)), "mgpcg_status")
+        return it.value, res.value
+
+    def _done(self, rc, what):
+        if rc == L.DFM_E_NOCONV and not self.raise_on_noconv:
+            self.noconv_count += 1
+            rc = 0
+        L.check(rc, what)
+        self.last_iters = self.status()[0]
+
+    def Poisson(self, u: MacField):
+        rc = self.lib.dfm_mgpcg_project(self.h, u.s, float(self.tol), 1 if self.relative else 0,
+                                        int(self.max_iters), L.stream_handle())
+        self._done(rc, "mgpcg_project")
+
+    # the projection is symmetric and idempotent, so its adjoint is itself
+    Poisson_adj = Poisson
+
+    def solve(self, b, x):
+        rc = self.lib.dfm_mgpcg_solve(self.h, b.data_ptr(), x.data_ptr(), float(self.tol),
+                                      1 if self.relative else 0, int(self.max_iters), L.stream_handle())
+        self._done(rc, "mgpcg_solve")
+        return x

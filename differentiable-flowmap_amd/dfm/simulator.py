@@ -1,0 +1,384 @@
+"""Drop-in LFM_Simulator / LFM_Diff_Simulator backed by libdfm.so.
+
+Mirrors the reference's simulator-class API (the boundary its optimisers call):
+
+* ``LFM_Simulator(res_x, res_y[, res_z], dx, dt, reinit_every, log_dir,
+  log_target_dir=None)`` (2D/vortex/lfm_midpoint_simulator.py:188,
+  3D/lfm_midpoint_simulator.py:169), ``.init(theta=None, forward=True,
+  target=False, init_passive_np=None)`` (2D :370), ``.step_midpoint(
+  write_flow_map, write_passive, control_force_para)`` -> ``(output_frame,
+  final_flag)`` (2D :472-515);
+* ``LFM_Diff_Simulator.forward_step_midpoint`` / ``init_gradient`` /
+  ``backtrack_step_midpoint`` (2D/vortex/lfm_midpoint_diff_simulator.py:118-258);
+* attributes ``u_x, u_y[, u_z]``, ``adj_u_*``, ``passive``, ``adj_passive``,
+  ``loss``, ``step_num[None]``, ``frame_idx[None]`` with ``to_numpy()`` /
+  ``from_numpy()`` like Taichi fields.
+
+Hyper-parameters that the reference reads from its star-imported
+``hyperparameters.py`` (RK_number, total_steps, add_passive_scalar ...) are
+keyword arguments here.  Supported configuration (the one every benchmark
+uses): viscosity 0, no control force, no obstacle mask, use_short_BFECC=False,
+BFECC_clamp=False, backward_u=True, final-time density loss.
+
+The reference's disk cache (RW_Cache, 2D/vortex/lfm_midpoint_simulator.py:46-184)
+is replaced by device-resident storage: the midpoint velocity of every step,
+the passive scalars at every reinitialisation boundary and the forward map
+(phi, F) of every interval stay in HBM.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import scenes
+from .ops import FlowMap, MacField, MGPCG, Ops
+
+
+class _Scalar:
+    """``field[None]`` access like a 0-d Taichi field."""
+
+    def __init__(self, v=0):
+        self.v = v
+
+    def __getitem__(self, k):
+        return self.v
+
+    def __setitem__(self, k, v):
+        self.v = v
+
+
+class FieldView:
+    """Taichi-field-like view of a device tensor (scalar field, or an SoA
+    point set shown as AoS (..., dim) like ti.Vector.field)."""
+
+    def __init__(self, owner, getter, soa=False):
+        self._owner = owner
+        self._get = getter
+        self._soa = soa
+
+    @property
+    def tensor(self):
+        return self._get(self._owner)
+
+    @property
+    def shape(self):
+        t = self.tensor
+        return tuple(t.shape[1:]) if self._soa else tuple(t.shape)
+
+    def to_numpy(self):
+        a = self.tensor.detach().cpu().numpy()
+        return np.moveaxis(a, 0, -1).copy() if self._soa else a
+
+    def to_torch(self, device=None):
+        t = self.tensor
+        return torch.movedim(t, 0, -1) if self._soa else t
+
+    def from_numpy(self, arr):
+        t = self.tensor
+        src = torch.as_tensor(np.asarray(arr), dtype=t.dtype)
+        if self._soa:
+            src = torch.movedim(src, -1, 0)
+        t.copy_(src.to(t.device))
+
+    def fill(self, v):
+        self.tensor.fill_(v)
+
+
+def _parse_ctor(args):
+    """(res_x, res_y, dx, dt, R, log_dir[, target]) or (res_x, res_y, res_z, dx, ...)."""
+    if len(args) >= 3 and isinstance(args[2], (int, np.integer)):
+        n = (int(args[0]), int(args[1]), int(args[2]))
+        rest = args[3:]
+    else:
+        n = (int(args[0]), int(args[1]))
+        rest = args[2:]
+    dx, dt, R = float(rest[0]), float(rest[1]), int(rest[2])
+    log_dir = rest[3] if len(rest) > 3 else None
+    log_target_dir = rest[4] if len(rest) > 4 else None
+    return n, dx, dt, R, log_dir, log_target_dir
+
+
+class LFM_Simulator:
+    """Forward leapfrog flow-map midpoint simulator (LFM_Simulator,
+    2D/vortex/lfm_midpoint_simulator.py:187-661, 3D/lfm_midpoint_simulator.py:167-807)."""
+
+    def __init__(self, *args, order=None, total_steps=None, n_passive=1, adj_kernel=None, mg_levels=0,
+                 mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda"):
+        n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
+        self.n = n
+        self.dim = len(n)
+        self.res_x, self.res_y = n[0], n[1]
+        if self.dim == 3:
+            self.res_z = n[2]
+        self.dx, self.dt, self.reinit_every = dx, dt, R
+        self.log_dir, self.log_target_dir = log_dir, log_target_dir
+        # RK_number: 2 in 2D, 4 in 3D (2D/vortex/hyperparameters.py:11, 3D/hyperparameters.py:281)
+        self.order = order or (2 if self.dim == 2 else 4)
+        # calculate_nabla_u_w kernel: quartic in 2D, cubic in 3D (q5)
+        self.adj_kernel = adj_kernel or (4 if self.dim == 2 else 3)
+        self.total_steps = total_steps
+        self.n_passive = n_passive
+        self.record = record
+        self.device = torch.device(device)
+        if not torch.cuda.is_available():
+            raise RuntimeError("dfm.LFM_Simulator needs a HIP device (torch.cuda.is_available() is False)")
+        self.ops = Ops(n, dx)
+        self.solver = MGPCG(n, dx, levels=mg_levels, pre_post=mg_pre_post, bottom=mg_bottom, tol=tol,
+                            max_iters=max_iters)
+        dev = self.device
+        self.u = MacField.empty(n, dev, zero=True)
+        self._u_new = MacField.empty(n, dev, zero=True)
+        self.init_u = MacField.empty(n, dev, zero=True)
+        self.err_u = MacField.empty(n, dev, zero=True)
+        self.passive = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(n_passive)]
+        self._passive_new = [torch.zeros_like(p) for p in self.passive]
+        self.init_passive = [torch.zeros_like(p) for p in self.passive]
+        self._err_s = torch.zeros(n, dtype=torch.float32, device=dev)
+        self.phi_map = FlowMap(n, dev)
+        self.psi_map = FlowMap(n, dev)
+        self.step_num = _Scalar(-1)
+        self.frame_idx = _Scalar(0)
+        self.mid_u = {}        # step -> MacField (midpoint velocity)
+        self.passive_start = {}  # interval-start step -> [passive]
+        self.flow_maps = {}    # interval index -> FlowMap (phi, F)
+        self._views()
+
+    # Taichi-like attribute views -------------------------------------------
+    def _views(self):
+        names = ["u_x", "u_y", "u_z"][: self.dim]
+        for a, nm in enumerate(names):
+            setattr(self, nm, FieldView(self, lambda s, a=a: s.u.c[a]))
+        if self.n_passive >= 1:
+            self.passive1 = FieldView(self, lambda s: s.passive[0])
+        self.psi_view = FieldView(self, lambda s: s.psi_map.psi, soa=True)
+        self.phi_view = FieldView(self, lambda s: s.phi_map.psi, soa=True)
+
+    # set-up -----------------------------------------------------------------
+    def _initial_velocity(self, theta):
+        if self.dim == 2:
+            uc = scenes.vortex_velocity_2d(np.asarray(theta, dtype=np.float64).reshape(-1, 4), self.n, self.dx)
+        else:
+            uc = scenes.vortex_tubes_3d(np.asarray(theta, dtype=np.float64).reshape(-1, 8), self.n, self.dx)
+        return scenes.split_central_vector(uc)
+
+    def init(self, theta=None, forward=True, target=False, init_passive_np=None, u0=None):
+        """LFM_Simulator.init (2D/vortex/lfm_midpoint_simulator.py:370-439): build
+        u0 (MAC arrays `u0`, or the vortex generator at `theta`), project it,
+        set the passive scalars, reset the forward map and init_u = u."""
+        if not forward:
+            return
+        if u0 is None:
+            if theta is None:
+                raise ValueError("init needs theta (vortex parameters) or u0 (MAC velocity arrays)")
+            u0 = self._initial_velocity(theta)
+        comps = u0.c if isinstance(u0, MacField) else u0
+        for a in range(self.dim):
+            self.u.c[a].copy_(torch.as_tensor(comps[a], dtype=torch.float32).to(self.device))
+        self.solver.Poisson(self.u)
+        if init_passive_np is not None:
+            ps = init_passive_np if isinstance(init_passive_np, (list, tuple)) else [init_passive_np]
+            for k, p in enumerate(ps[: self.n_passive]):
+                self.passive[k].copy_(torch.as_tensor(p, dtype=torch.float32).to(self.device))
+        for k in range(self.n_passive):
+            self.init_passive[k].copy_(self.passive[k])
+        self.step_num[None] = -1
+        self.frame_idx[None] = 0
+        self.ops.reset_identity(self.phi_map)
+        self.init_u.copy_(self.u)
+        self.mid_u.clear()
+        self.passive_start.clear()
+        self.flow_maps.clear()
+
+    def final_condition(self):
+        return self.total_steps is not None and self.step_num[None] == self.total_steps - 1
+
+    # forward step -------------------------------------------------------------
+    def step_midpoint(self, write_flow_map=False, write_passive=False, control_force_para=None):
+        """step_midpoint (2D/vortex/lfm_midpoint_simulator.py:472-515)."""
+        if control_force_para is not None:
+            raise NotImplementedError("control forces are not supported by this build")
+        self.step_num[None] += 1
+        j = self.step_num[None] % self.reinit_every
+        if j == 0 and self.record:
+            self.passive_start[self.step_num[None]] = [p.clone() for p in self.passive]
+        self.step_flow_map(j, self.dt)
+        return False, self.final_condition()
+
+    def _mid_buffer(self, step):
+        if step in self.mid_u:
+            return self.mid_u[step]
+        m = MacField.empty(self.n, self.device)
+        self.mid_u[step] = m
+        return m
+
+    def step_flow_map(self, j, dt):
+        """step_flow_map (2D/vortex/lfm_midpoint_simulator.py:517-661)."""
+        ops, order, step = self.ops, self.order, self.step_num[None]
+        mid = self._mid_buffer(step)
+        # half-step self-advection of u, then projection -> midpoint velocity
+        ops.sl_advect(self.u, 0.5 * dt, order, [self.u], [mid])
+        self.solver.Poisson(mid)
+        # forward map phi/F marched with the midpoint velocity (dt sign flipped)
+        ops.march(mid, self.phi_map, -dt, order)
+        # passive and full-step velocity advected by the midpoint velocity
+        ops.sl_advect(mid, dt, order, [self.u], [self._u_new], self.passive, self._passive_new)
+        self.u, self._u_new = self._u_new, self.u
+        self.passive, self._passive_new = self._passive_new, self.passive
+        self.solver.Poisson(self.u)
+        if j == self.reinit_every - 1:
+            self._reinit(step, j, dt)
+
+    def _reinit(self, step, j, dt):
+        """Reinitialisation: re-march the backward map psi/T from identity
+        through the stored midpoint velocities in reverse (the last step uses
+        the end-of-step velocity, q3; backtrack_psi_grid,
+        2D/vortex/lfm_midpoint_simulator.py:338-353), BFECC pullback of init_u,
+        project, BFECC of the passives, store (phi, F), reset."""
+        ops, order = self.ops, self.order
+        ops.reset_identity(self.psi_map)
+        ops.march(self.u, self.psi_map, dt, order)
+        for st in reversed(range(step - j, step)):
+            ops.march(self.mid_u[st], self.psi_map, dt, order)
+        ops.bfecc_u(self.init_u, self.psi_map, self.phi_map, self.err_u, self.u)
+        self.solver.Poisson(self.u)
+        for k in range(self.n_passive):
+            ops.bfecc_s(self.init_passive[k], self.psi_map.psi, self.phi_map.psi, self._err_s, self.passive[k])
+            self.init_passive[k].copy_(self.passive[k])
+        if self.record:
+            fm = FlowMap(self.n, self.device)
+            fm.copy_(self.phi_map)
+            self.flow_maps[step // self.reinit_every] = fm
+        else:
+            for st in range(step - j, step + 1):
+                self.mid_u.pop(st, None)
+        ops.reset_identity(self.phi_map)
+        self.init_u.copy_(self.u)
+
+    # diagnostics ---------------------------------------------------------------
+    def calculate_w(self):
+        """Vorticity at cell centres (get_central_vector + curl,
+        2D/vortex/taichi_utils.py:152-156,198-205); host-side diagnostic."""
+        u = [t.detach().double() for t in self.u.c]
+        if self.dim == 2:
+            uc = torch.stack([0.5 * (u[0][1:] + u[0][:-1]), 0.5 * (u[1][:, 1:] + u[1][:, :-1])], -1)
+            pad = torch.nn.functional.pad(uc.permute(2, 0, 1)[None], (1, 1, 1, 1), mode="replicate")[0]
+            dvy_dx = (pad[1, 2:, 1:-1] - pad[1, :-2, 1:-1]) / (2 * self.dx)
+            dvx_dy = (pad[0, 1:-1, 2:] - pad[0, 1:-1, :-2]) / (2 * self.dx)
+            return (dvy_dx - dvx_dy).cpu().numpy()
+        raise NotImplementedError("3D vorticity diagnostic is not provided")
+
+
+class LFM_Diff_Simulator(LFM_Simulator):
+    """Forward simulator plus the continuous adjoint sweep on flow maps
+    (LFM_Diff_Simulator, 2D/vortex/lfm_midpoint_diff_simulator.py:17-503,
+    3D/lfm_midpoint_diff_simulator.py:15-594)."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        n, dev = self.n, self.device
+        self.adj_u = MacField.empty(n, dev, zero=True)
+        self._adj_u_new = MacField.empty(n, dev, zero=True)
+        self.init_adj_u = MacField.empty(n, dev, zero=True)
+        self.err_adj_u = MacField.empty(n, dev, zero=True)
+        self.r_u = MacField.empty(n, dev, zero=True)     # ajd_uw_source
+        self.r_s = MacField.empty(n, dev, zero=True)     # sum of adj_passive_f
+        self.ab_h1 = MacField.empty(n, dev, zero=True)   # adj_u_*_AB_tem
+        self.ab_h2 = MacField.empty(n, dev, zero=True)   # adj_u_*_AB_tem2
+        self.adj_passive = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(self.n_passive)]
+        self._adj_passive_new = [torch.zeros_like(p) for p in self.adj_passive]
+        self.init_adj_passive = [torch.zeros_like(p) for p in self.adj_passive]
+        self.target_passive = [torch.zeros_like(p) for p in self.adj_passive]
+        self.loss = 0.0
+        self.final_ind = None
+        names = ["adj_u_x", "adj_u_y", "adj_u_z"][: self.dim]
+        for a, nm in enumerate(names):
+            setattr(self, nm, FieldView(self, lambda s, a=a: s.adj_u.c[a]))
+        if self.n_passive >= 1:
+            self.adj_passive1 = FieldView(self, lambda s: s.adj_passive[0])
+
+    def forward_step_midpoint(self, write_flow_map=False, write_passive=False, control_force_para=None):
+        return self.step_midpoint(write_flow_map, write_passive, control_force_para)
+
+    def set_target(self, target):
+        ts = target if isinstance(target, (list, tuple)) else [target]
+        for k, t in enumerate(ts[: self.n_passive]):
+            self.target_passive[k].copy_(torch.as_tensor(t, dtype=torch.float32).to(self.device))
+
+    def init_gradient(self, theta=None, target=None):
+        """init_gradient (2D/vortex/lfm_midpoint_diff_simulator.py:126-207):
+        terminal adjoint lam_s = 2 (s - s*), lam_u = 0, loss = sum (s - s*)^2,
+        starting from the final forward state held on the device."""
+        if target is not None:
+            self.set_target(target)
+        self.final_ind = self.step_num[None] + 1
+        self.adj_u.zero_()
+        loss = 0.0
+        for k in range(self.n_passive):
+            d = self.passive[k] - self.target_passive[k]
+            self.adj_passive[k].copy_(2.0 * d)
+            loss += float(torch.sum(d.double() ** 2))
+        self.loss = loss
+        self.step_num[None] = self.final_ind
+        self.init_adj_u.copy_(self.adj_u)
+        for k in range(self.n_passive):
+            self.init_adj_passive[k].copy_(self.adj_passive[k])
+        self.init_u.copy_(self.u)
+        for k in range(self.n_passive):
+            self.init_passive[k].copy_(self.passive[k])
+        self.ops.reset_identity(self.psi_map)
+        self.ab_h1.zero_()
+        self.ab_h2.zero_()
+
+    def backtrack_step_midpoint(self, write_gradient=False, write_passive=False, control_force_para=None):
+        """backtrack_step_midpoint (2D/vortex/lfm_midpoint_diff_simulator.py:220-258)."""
+        if control_force_para is not None:
+            raise NotImplementedError("control forces are not supported by this build")
+        self.step_num[None] -= 1
+        j = self.step_num[None] % self.reinit_every
+        self.backtrack_flow_map(j, self.dt)
+        return False, self.step_num[None] == 0
+
+    def backtrack_flow_map(self, j, dt):
+        """backtrack_flow_map (2D/vortex/lfm_midpoint_diff_simulator.py:260-503)
+        with use_short_BFECC=False and backward_u=True."""
+        ops, order, step = self.ops, self.order, self.step_num[None]
+        mid = self.mid_u[step]
+        # adjoint sources at the grid faces, from the current (end-of-step) state
+        ops.adj_conv_source(self.u, self.adj_u, self.adj_kernel, self.r_u)
+        ops.adj_scalar_source(self.passive, self.adj_passive, self.r_s)
+        # one back-trace through mid over -dt serves lam_u, u, s and lam_s
+        ops.sl_advect(mid, -dt, order, [self.adj_u, self.u], [self._adj_u_new, self._u_new],
+                      self.passive + self.adj_passive, self._passive_new + self._adj_passive_new)
+        self.adj_u, self._adj_u_new = self._adj_u_new, self.adj_u
+        self.u, self._u_new = self._u_new, self.u
+        self.passive, self._passive_new = self._passive_new, self.passive
+        self.adj_passive, self._adj_passive_new = self._adj_passive_new, self.adj_passive
+        # lam_u += -dt r_u - dt r_s  (2D/vortex/lfm_midpoint_diff_simulator.py:359-374)
+        ops.axpy(self.adj_u, -dt, self.r_u, self.adj_u, beta=-dt, z=self.r_s)
+        # psi/T march and source accumulation (AB3 at -2dt for r_u, -dt for r_s)
+        ops.march(mid, self.psi_map, dt, order)
+        ops.accumulate(self.r_u, self.psi_map, self.init_adj_u, self.ab_h1, self.ab_h2,
+                       self.reinit_every - 1 - j, -2.0 * dt, f2=self.r_s, dt2=-dt)
+        self.solver.Poisson_adj(self.adj_u)
+        self.solver.Poisson(self.u)
+        if j == 0:
+            fwd = self.flow_maps[step // self.reinit_every]
+            ops.bfecc_u(self.init_adj_u, fwd, self.psi_map, self.err_adj_u, self.adj_u)
+            self.solver.Poisson_adj(self.adj_u)
+            self.init_adj_u.copy_(self.adj_u)
+            ops.bfecc_u(self.init_u, fwd, self.psi_map, self.err_adj_u, self.u)
+            self.solver.Poisson(self.u)
+            self.init_u.copy_(self.u)
+            for k in range(self.n_passive):
+                self.passive[k].copy_(self.passive_start[step][k])
+                self.init_passive[k].copy_(self.passive[k])
+                ops.bfecc_s(self.init_adj_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.adj_passive[k])
+                self.init_adj_passive[k].copy_(self.adj_passive[k])
+            self.ops.reset_identity(self.psi_map)
+
+    def calculate_gw(self):
+        u, self.u = self.u, self.adj_u
+        try:
+            return self.calculate_w()
+        finally:
+            self.u = u

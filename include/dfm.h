@@ -1,0 +1,158 @@
+/*
+ * dfm.h -- C-ABI of libdfm.so, the MI355X (gfx950) HIP implementation of the
+ * leapfrog flow-map fluid step and its adjoint.
+ *
+ * The reference (pearseven/differentiable-flowmap) has no native FFI on this
+ * path: its kernels are Taichi @ti.kernel functions called from Python.  Each
+ * entry point below replaces one of those kernels (or a fixed composition of
+ * them), cited as file:line; the Python drop-in (dfm.simulator) binds them
+ * with ctypes.
+ *
+ * Conventions
+ *  - All field pointers are device pointers (e.g. torch.Tensor.data_ptr()).
+ *  - Scalar fields are dense (sx, sy, sz) arrays, z fastest; 2D uses sz = 1.
+ *    MAC component a lives on the face grid n + e_a; cell fields on n.
+ *  - Flow-map point sets are SoA: component c of point i at [c*npts + i].
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *    asynchronous and never synchronise the host, except where noted.
+ *  - Return value: 0 = OK, < 0 = error (see DFM_E_*); dfm_last_error()
+ *    returns a thread-local description of the last error.
+ */
+#ifndef DFM_H
+#define DFM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFM_OK 0
+#define DFM_E_ARG (-1)        /* bad dimension, order, kernel or size */
+#define DFM_E_NULL (-2)       /* required pointer is NULL */
+#define DFM_E_HIP (-3)        /* HIP runtime error (launch / allocation) */
+#define DFM_E_NOCONV (-4)     /* PCG reached max_iters without converging */
+#define DFM_E_NONFINITE (-5)  /* PCG residual became NaN/Inf */
+
+typedef struct dfm_grid {
+  int32_t dim;  /* 2 or 3 */
+  int32_t n[3]; /* cells per axis; n[2] = 1 in 2D */
+  float dx;     /* cell size */
+} dfm_grid;
+
+/* MAC field: c[a] is the face-a component (c[2] = NULL in 2D). */
+typedef struct dfm_mac {
+  float* c[3];
+} dfm_mac;
+
+/* Flow map: cell-centred map psi (dim x cells, may be NULL), per face axis the
+ * face map psi_f[a] (dim x faces_a) and one Jacobian column T_f[a]
+ * (dim x faces_a; NULL -> march positions only). */
+typedef struct dfm_flowmap {
+  float* psi;
+  float* psi_f[3];
+  float* T_f[3];
+} dfm_flowmap;
+
+const char* dfm_last_error(void);
+int dfm_version(void);
+
+/* reset_to_identity: psi = X, psi_a = X_a, T_a = e_a.
+ * Replaces 2D/vortex/flowmap.py:343-358, 3D/flowmap.py:336-356. */
+int dfm_reset_identity(const dfm_grid* g, const dfm_flowmap* m, void* stream);
+
+/* RK2/RK4 march of every point set of m through velocity u:
+ * psi += -dt u(psi), T += -dt (grad u)(psi) T (quadratic MAC interpolation).
+ * One launch covers RK_grid on all face sets and RK_grid_only_psi on psi.
+ * Replaces RK_grid / RK_grid_only_psi, 2D/vortex/flowmap.py:40-57,125-215;
+ * 3D/flowmap.py:6-25,85-179. */
+int dfm_march(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m, float dt, int order, void* stream);
+
+/* Impulse pullback with epilogue: out_a = alpha * (T_a . src(psi_a)) + beta * add_a
+ * (add may be NULL when beta == 0; out may alias add).
+ * alpha=1, beta=0: advect_u, 2D/vortex/flowmap.py:83-97, 3D/flowmap.py:51-69.
+ * alpha=dt, beta=1, add=out=init: accumulate_init, 2D/vortex/flowmap.py:508-520. */
+int dfm_pullback_u(const dfm_grid* g, const dfm_mac* src, const dfm_flowmap* m, float alpha, float beta,
+                   const dfm_mac* add, const dfm_mac* out, void* stream);
+
+/* Scalar pullback: out = alpha * src(psi) + beta * add (cells, quadratic).
+ * Replaces advect_scalar, 2D/vortex/flowmap.py:99-107, 3D/flowmap.py:71-79. */
+int dfm_pullback_s(const dfm_grid* g, const float* src, const float* psi, float alpha, float beta, const float* add,
+                   float* out, void* stream);
+
+/* BFECC of a MAC field (BFECC_clamp = False): out = T.init(psi);
+ * err = (F.out(phi) - init)/2; out -= T.err(psi), where (T, psi) = fwd and
+ * (F, phi) = bwd.  err is caller-provided scratch.  out must not alias init.
+ * Replaces BFECC, 2D/vortex/flowmap.py:440-463, 3D/flowmap.py:405-483. */
+int dfm_bfecc_u(const dfm_grid* g, const dfm_mac* init, const dfm_flowmap* fwd, const dfm_flowmap* bwd,
+                const dfm_mac* err, const dfm_mac* out, void* stream);
+
+/* BFECC of a cell scalar.  Replaces BFECC_scalar, 2D/vortex/flowmap.py:465-481. */
+int dfm_bfecc_s(const dfm_grid* g, const float* init, const float* psi, const float* phi, float* err, float* out,
+                void* stream);
+
+/* accumulate_init_new (AB3 (23,-16,5)/12 with the reference's ind==1 form):
+ * g = T_a . f(psi_a); init += dt * AB3(g, h1, h2); h2 = h1; h1 = g.
+ * If f2 is non-NULL, additionally init += dt2 * (T_a . f2(psi_a))
+ * (accumulate_init of a second source through the same map).
+ * Replaces 2D/vortex/flowmap.py:508-553, 3D/flowmap.py:485-545. */
+int dfm_accumulate(const dfm_grid* g, const dfm_mac* f, const dfm_flowmap* m, const dfm_mac* init,
+                   const dfm_mac* h1, const dfm_mac* h2, int ind, float dt, const dfm_mac* f2, float dt2,
+                   void* stream);
+
+/* Semi-Lagrangian advection: back-trace every face node (and, if nscal > 0,
+ * every cell centre) through vel over dt with RK order 2/4, then sample
+ * component a of each of the nvec MAC sources (into dst_vec) and each of the
+ * nscal cell sources (into dst_s) there.  nvec <= 3, nscal <= 8.
+ * Destinations must not alias sources.
+ * Replaces advect_u_grid / advect_scalar_grid (RK2_/RK4_advect_*_grid),
+ * 2D/vortex/flowmap.py:59-81,233-329; 3D/flowmap.py:26-50,202-320. */
+int dfm_sl_advect(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec, const dfm_mac* src_vec,
+                  const dfm_mac* dst_vec, int nscal, const float* const* src_s, float* const* dst_s, void* stream);
+
+/* Adjoint convective source r_a(X_a) = ((grad u)^T w)_a with the B-spline of
+ * order kern (4 = quartic as in 2D, 3 = cubic as in 3D).
+ * Replaces calculate_nabla_u_w, 2D/vortex/flowmap.py:361-377, 3D/flowmap.py:359-380. */
+int dfm_adj_conv_source(const dfm_grid* g, const dfm_mac* u, const dfm_mac* w, int kern, const dfm_mac* r,
+                        void* stream);
+
+/* Adjoint scalar source summed over nscal passive pairs (cubic):
+ * r_a(X_a) = sum_m d_a s_m(X_a) * lam_m(X_a).  nscal <= 8.
+ * Replaces calculate_nabla_scalar_adjoint, 2D/vortex/flowmap.py:379-395,
+ * 3D/flowmap.py:382-403. */
+int dfm_adj_scalar_source(const dfm_grid* g, int nscal, const float* const* s, const float* const* lam,
+                          const dfm_mac* r, void* stream);
+
+/* out_a = x_a + alpha * y_a + beta * z_a on every face grid (z may be NULL
+ * when beta == 0).  Replaces add_fields / scale_field on MAC fields,
+ * 2D/vortex/taichi_utils.py:105-113. */
+int dfm_axpy_mac(const dfm_grid* g, const dfm_mac* x, float alpha, const dfm_mac* y, float beta, const dfm_mac* z,
+                 const dfm_mac* out, void* stream);
+
+/* ---------------------------------------------------------------- MGPCG --
+ * Handle owning the multigrid workspaces for one grid.  Levels, smoothing and
+ * bottom sweeps follow MGPCG (2D/vortex/mgpcg.py:17-31): levels <= 0 picks
+ * log2(min n) - 3 + 1.  Requires every n divisible by 2^(levels-1). */
+int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int bottom, void** handle);
+int dfm_mgpcg_destroy(void* handle);
+
+/* Pressure projection in place (MGPCG_2/3.Poisson: apply_bc, divergence,
+ * PCG solve of A p = -div with all-Neumann A, subtract_grad_p, apply_bc;
+ * 2D/vortex/mgpcg.py:358-368, 3D/mgpcg.py:441-451, 3D/fast_mgpcg.py:241-323).
+ * Stops when r.r < tol^2 * b.b (relative != 0) or r.r < tol (relative == 0),
+ * or after max_iters.  Returns DFM_E_NOCONV / DFM_E_NONFINITE after
+ * synchronising on non-convergence; the pressure is kept in the handle. */
+int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int relative, int max_iters, void* stream);
+
+/* Solve A x = b (b recentred to mean zero first).  Same stopping rule. */
+int dfm_mgpcg_solve(void* handle, const float* b, float* x, double tol, int relative, int max_iters, void* stream);
+
+/* Iterations and relative residual sqrt(r.r / b.b) of the last solve. */
+int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
+
+/* Device pointer of the pressure (cells) from the last projection. */
+int dfm_mgpcg_pressure(void* handle, float** p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

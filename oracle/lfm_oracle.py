@@ -1,0 +1,568 @@
+"""FP64 (or FP32-mode) CPU restatement of the leapfrog flow-map step and its
+adjoint -- TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Point kernels (B-spline interpolation, RK march, pullback, semi-Lagrangian
+back-trace, MGPCG) live in ``dfm_oracle.cpp`` (built to ``liboracle.so`` by
+``oracle/Makefile``); this module holds the field-level compositions and the
+step orchestration, each citing the reference lines it restates.
+
+What pins each function (beyond the shared reading of the reference):
+
+=====================  ==========================================================
+function               pinned by (tests/test_oracle_pins.py)
+=====================  ==========================================================
+N_k, dN_k              partition of unity, sum dN = 0, node weights, knot
+                       continuity, dN = finite difference of N
+interp / mac_eval      exact reproduction of constant / linear fields and their
+                       gradients away from the clamp band; clamp behaviour by
+                       brute force; MAC offsets with u = (x, 2y, 3z)
+march                  uniform flow (exact), linear flow vs matrix exponential
+                       (observed order 2 / 4), rotation det(T) ~ 1
+pullback / bfecc       identity maps reproduce the B-spline smoothing in closed
+                       form; linear fields pulled back exactly
+sl_advect              uniform flow == exact shift of a linear field
+project_exact          DCT-II eigen-decomposition of the Neumann Laplacian;
+                       checked against scipy.sparse spsolve and divergence-free
+                       output, P idempotent and symmetric
+project_mgpcg          converges to project_exact
+adj sources            closed forms on polynomial fields
+OracleLFM (whole step) composition of the above; the trajectory itself is
+                       **parity unpinned** against the reference (Taichi cannot
+                       run here) -- only its pieces are pinned.
+=====================  ==========================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    """Compile liboracle.so with gcc (no GPU toolchain involved)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+        for sfx in ("d", "f"):
+            getattr(_LIB, f"oracle_mgpcg_{sfx}").restype = ctypes.c_int
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _sfx(dtype):
+    return "d" if np.dtype(dtype) == np.float64 else "f"
+
+
+def _real(dtype, x):
+    return ctypes.c_double(x) if np.dtype(dtype) == np.float64 else ctypes.c_float(x)
+
+
+def _shape3(shape):
+    s = list(shape) + [1] * (3 - len(shape))
+    return np.ascontiguousarray(s, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- geometry --
+def face_shape(n, a):
+    return tuple(n[b] + (1 if b == a else 0) for b in range(len(n)))
+
+
+def coords(shape, offs, dx, dtype=np.float64):
+    """Positions (idx + offs) * dx of every node, shape `shape + (dim,)`
+    (center/horizontal/vertical_coords_func, 2D/vortex/taichi_utils.py:137-150;
+    x/y/z_coords_func, 3D/taichi_utils.py:95-128)."""
+    axes = [(np.arange(s, dtype=np.float64) + o) * dx for s, o in zip(shape, offs)]
+    g = np.meshgrid(*axes, indexing="ij")
+    return np.ascontiguousarray(np.stack(g, axis=-1).astype(dtype))
+
+
+def cell_coords(n, dx, dtype=np.float64):
+    return coords(n, [0.5] * len(n), dx, dtype)
+
+
+def face_coords(n, a, dx, dtype=np.float64):
+    return coords(face_shape(n, a), [0.0 if b == a else 0.5 for b in range(len(n))], dx, dtype)
+
+
+def mac_bl(dim, a):
+    return [0.0 if b == a else 0.5 for b in range(dim)]
+
+
+# ------------------------------------------------------------ point kernels --
+def interp(kern, f, bl, pts, dx, grad=False):
+    """B-spline interp of f (kern 2/3/4) at pts; returns value[, gradient]."""
+    dt = f.dtype
+    dim = f.ndim
+    f = np.ascontiguousarray(f)
+    pts = np.ascontiguousarray(pts, dtype=dt)
+    npts = pts.size // dim
+    val = np.empty(pts.shape[:-1], dtype=dt)
+    g = np.empty(pts.shape, dtype=dt) if grad else None
+    blv = np.ascontiguousarray(list(bl) + [0.5] * (3 - dim), dtype=dt)
+    getattr(lib(), f"oracle_interp_{_sfx(dt)}")(
+        ctypes.c_int(kern), ctypes.c_int(dim), _p(f), _p(_shape3(f.shape)), _p(blv), _p(pts),
+        ctypes.c_int64(npts), _real(dt, dx), _p(val), _p(g) if grad else None)
+    return (val, g) if grad else val
+
+
+def _ucomps(u):
+    dt = u[0].dtype
+    comps = [np.ascontiguousarray(c, dtype=dt) for c in u]
+    ptrs = [_p(c) for c in comps] + [None] * (3 - len(comps))
+    return comps, ptrs
+
+
+def _cells(u):
+    n = list(u[0].shape)
+    n[0] -= 1
+    return n
+
+
+def mac_eval(u, pts, dx, kern=2, grad=False):
+    """Sample the MAC velocity u (list of face fields) at pts:
+    value (..., d) and optionally J (..., d, d) with J[a][b] = d u_a / d x_b."""
+    dim = len(u)
+    vals, grads = [], []
+    for a in range(dim):
+        r = interp(kern, u[a], mac_bl(dim, a), pts, dx, grad=grad)
+        if grad:
+            vals.append(r[0])
+            grads.append(r[1])
+        else:
+            vals.append(r)
+    v = np.stack(vals, axis=-1)
+    if grad:
+        return v, np.stack(grads, axis=-2)
+    return v
+
+
+def march(u, psi, T, dx, dt, order):
+    """In-place RK march of points psi (..., d) and Jacobian columns T."""
+    dtp = u[0].dtype
+    dim = len(u)
+    comps, ptrs = _ucomps(u)
+    assert psi.flags.c_contiguous and psi.dtype == dtp
+    if T is not None:
+        assert T.flags.c_contiguous and T.dtype == dtp
+    getattr(lib(), f"oracle_march_{_sfx(dtp)}")(
+        ctypes.c_int(dim), ctypes.c_int(order), _p(_shape3(_cells(u))), _real(dtp, dx), _real(dtp, dt),
+        *ptrs, _p(psi), _p(T) if T is not None else None, ctypes.c_int64(psi.size // dim))
+
+
+def pullback(u0, psi, T, dx):
+    """T . u0(psi) per point."""
+    dtp = u0[0].dtype
+    dim = len(u0)
+    comps, ptrs = _ucomps(u0)
+    psi = np.ascontiguousarray(psi, dtype=dtp)
+    T = np.ascontiguousarray(T, dtype=dtp)
+    out = np.empty(psi.shape[:-1], dtype=dtp)
+    getattr(lib(), f"oracle_pullback_{_sfx(dtp)}")(
+        ctypes.c_int(dim), _p(_shape3(_cells(u0))), _real(dtp, dx), *ptrs, _p(psi), _p(T),
+        ctypes.c_int64(psi.size // dim), _p(out))
+    return out
+
+
+def sl_points(vel, src, src_bl, x, dx, dt, order):
+    """Back-trace x through vel over dt (RK order), sample src there."""
+    dtp = vel[0].dtype
+    dim = len(vel)
+    comps, ptrs = _ucomps(vel)
+    src = np.ascontiguousarray(src, dtype=dtp)
+    x = np.ascontiguousarray(x, dtype=dtp)
+    out = np.empty(x.shape[:-1], dtype=dtp)
+    blv = np.ascontiguousarray(list(src_bl) + [0.5] * (3 - dim), dtype=dtp)
+    getattr(lib(), f"oracle_sl_{_sfx(dtp)}")(
+        ctypes.c_int(dim), ctypes.c_int(order), _p(_shape3(_cells(vel))), _real(dtp, dx), _real(dtp, dt),
+        *ptrs, _p(src), _p(_shape3(src.shape)), _p(blv), _p(x), ctypes.c_int64(x.size // dim), _p(out))
+    return out
+
+
+# ---------------------------------------------------------- field kernels ---
+def sl_advect_u(vel, src, dx, dt, order):
+    """advect_u_grid (2D/vortex/flowmap.py:59-69,233-291; 3D/flowmap.py:202-282):
+    component a of src sampled at the back-traced face-a positions."""
+    n = _cells(vel)
+    dim = len(vel)
+    return [sl_points(vel, src[a], mac_bl(dim, a), face_coords(n, a, dx, vel[0].dtype), dx, dt, order)
+            for a in range(dim)]
+
+
+def sl_advect_s(vel, s, dx, dt, order):
+    """advect_scalar_grid (2D/vortex/flowmap.py:71-81,293-329)."""
+    n = _cells(vel)
+    return sl_points(vel, s, [0.5] * len(vel), cell_coords(n, dx, vel[0].dtype), dx, dt, order)
+
+
+class FlowMap:
+    """psi (cells) plus psi_f / T_f per face axis: one Jacobian column per
+    face grid (the reference's psi, psi_x.., T_x.. fields)."""
+
+    def __init__(self, n, dx, dtype):
+        self.n, self.dx, self.dtype = tuple(n), dx, dtype
+        self.dim = len(n)
+        self.reset()
+
+    def reset(self):
+        """reset_to_identity (2D/vortex/flowmap.py:343-358, 3D/flowmap.py:336-356)."""
+        n, dx, dt = self.n, self.dx, self.dtype
+        self.psi = cell_coords(n, dx, dt)
+        self.psi_f = [face_coords(n, a, dx, dt) for a in range(self.dim)]
+        self.T_f = []
+        for a in range(self.dim):
+            T = np.zeros(face_shape(n, a) + (self.dim,), dtype=dt)
+            T[..., a] = 1.0
+            self.T_f.append(T)
+
+    def march(self, u, dt, order):
+        """RK_grid_only_psi on psi + RK_grid on each (psi_a, T_a)
+        (march_phi_grid / march_psi_grid, 2D/vortex/lfm_midpoint_simulator.py:355-358,
+        2D/vortex/lfm_midpoint_diff_simulator.py:215-218)."""
+        march(u, self.psi, None, self.dx, dt, order)
+        for a in range(self.dim):
+            march(u, self.psi_f[a], self.T_f[a], self.dx, dt, order)
+
+    def copy(self):
+        m = FlowMap.__new__(FlowMap)
+        m.n, m.dx, m.dtype, m.dim = self.n, self.dx, self.dtype, self.dim
+        m.psi = self.psi.copy()
+        m.psi_f = [p.copy() for p in self.psi_f]
+        m.T_f = [t.copy() for t in self.T_f]
+        return m
+
+
+def advect_u(u0, fmap, dx):
+    """Impulse pullback u1_a = T_a . u0(psi_a) (2D/vortex/flowmap.py:83-97)."""
+    return [pullback(u0, fmap.psi_f[a], fmap.T_f[a], dx) for a in range(len(u0))]
+
+
+def advect_scalar(s0, psi, dx):
+    """s1 = s0(psi), quadratic (2D/vortex/flowmap.py:99-107)."""
+    return interp(2, s0, [0.5] * s0.ndim, psi, dx)
+
+
+def bfecc(init, fwd, bwd, dx):
+    """BFECC with BFECC_clamp=False (2D/vortex/flowmap.py:440-463):
+    u = T.init(psi); err = (F.u(phi) - init)/2; u -= T.err(psi),
+    where (T, psi) = `fwd` and (F, phi) = `bwd`."""
+    u = advect_u(init, fwd, dx)
+    e = advect_u(u, bwd, dx)
+    err = [(e[a] - init[a]) * 0.5 for a in range(len(init))]
+    t = advect_u(err, fwd, dx)
+    return [u[a] - t[a] for a in range(len(init))]
+
+
+def bfecc_scalar(init, psi, phi, dx):
+    """BFECC_scalar (2D/vortex/flowmap.py:465-481)."""
+    s = advect_scalar(init, psi, dx)
+    e = advect_scalar(s, phi, dx)
+    err = (e - init) * 0.5
+    return s - advect_scalar(err, psi, dx)
+
+
+def accumulate(f, init, fmap, dx, dt):
+    """accumulate_init: init_a += (F_a . f(phi_a)) dt (2D/vortex/flowmap.py:508-520)."""
+    g = advect_u(f, fmap, dx)
+    for a in range(len(init)):
+        init[a] += g[a] * dt
+
+
+def accumulate_ab3(f, init, fmap, H1, H2, ind, dx, dt):
+    """accumulate_init_new, AB3 (23,-16,5)/12 with the ind==1 quirk (q1)
+    that re-uses the current value (2D/vortex/flowmap.py:523-553)."""
+    g = advect_u(f, fmap, dx)
+    for a in range(len(init)):
+        if ind == 0:
+            init[a] += g[a] * dt
+        elif ind == 1:
+            init[a] += (23 * g[a] - 16 * g[a] + 5 * H1[a]) / 12 * dt
+        else:
+            init[a] += (23 * g[a] - 16 * H1[a] + 5 * H2[a]) / 12 * dt
+        H2[a] = H1[a].copy()
+        H1[a] = g[a].copy()
+
+
+def adj_conv_source(u, w, dx, kern):
+    """calculate_nabla_u_w: r_a = ((grad u)^T w)_a at face-a nodes, with the
+    quartic (2D, kern=4) or cubic (3D, kern=3) kernel
+    (2D/vortex/flowmap.py:361-377, 3D/flowmap.py:359-380)."""
+    dim = len(u)
+    n = _cells(u)
+    out = []
+    for a in range(dim):
+        X = face_coords(n, a, dx, u[0].dtype)
+        r = np.zeros(X.shape[:-1], dtype=u[0].dtype)
+        for c in range(dim):
+            _, g = interp(kern, u[c], mac_bl(dim, c), X, dx, grad=True)
+            wv = interp(kern, w[c], mac_bl(dim, c), X, dx)
+            r += g[..., a] * wv
+        out.append(r)
+    return out
+
+
+def adj_scalar_source(s, lam, dx):
+    """calculate_nabla_scalar_adjoint: r_a = d_a s(X_a) * lam(X_a), cubic
+    (2D/vortex/flowmap.py:379-395, 3D/flowmap.py:382-403)."""
+    dim = s.ndim
+    n = list(s.shape)
+    out = []
+    for a in range(dim):
+        X = face_coords(n, a, dx, s.dtype)
+        _, g = interp(3, s, [0.5] * dim, X, dx, grad=True)
+        lv = interp(3, lam, [0.5] * dim, X, dx)
+        out.append(g[..., a] * lv)
+    return out
+
+
+# ---------------------------------------------------------------- Poisson ---
+def apply_bc(u):
+    """Zero the normal velocity on the box faces (apply_bc, 2D/vortex/mgpcg.py:294-326)."""
+    for a, c in enumerate(u):
+        sl0 = [slice(None)] * c.ndim
+        sl1 = [slice(None)] * c.ndim
+        sl0[a] = 0
+        sl1[a] = -1
+        c[tuple(sl0)] = 0
+        c[tuple(sl1)] = 0
+
+
+def divergence(u):
+    """div without 1/dx (2D/vortex/mgpcg.py:328-336, 3D/mgpcg.py:401-411)."""
+    d = None
+    for a, c in enumerate(u):
+        hi = [slice(None)] * c.ndim
+        lo = [slice(None)] * c.ndim
+        hi[a] = slice(1, None)
+        lo[a] = slice(0, -1)
+        t = c[tuple(hi)] - c[tuple(lo)]
+        d = t if d is None else d + t
+    return d
+
+
+def subtract_grad_p(u, p):
+    """u_a -= p[I] - p[I - e_a] on interior faces (2D/vortex/mgpcg.py:338-356);
+    boundary faces are zeroed by the following apply_bc."""
+    for a, c in enumerate(u):
+        hi = [slice(None)] * c.ndim
+        lo = [slice(None)] * c.ndim
+        mid = [slice(None)] * c.ndim
+        hi[a] = slice(1, None)
+        lo[a] = slice(0, -1)
+        mid[a] = slice(1, -1)
+        c[tuple(mid)] -= (p[tuple(hi)] - p[tuple(lo)]).astype(c.dtype)
+
+
+def neumann_solve_dct(b):
+    """Exact mean-zero solution of A p = b, A the all-Neumann (#nbrs)I - adj
+    Laplacian on cells: A is diagonalised by the DCT-II, eigenvalues
+    sum_d (2 - 2 cos(pi k_d / n_d))."""
+    from scipy.fft import dctn, idctn
+
+    b = np.asarray(b, dtype=np.float64)
+    bh = dctn(b, type=2, norm="ortho")
+    lam = np.zeros(b.shape)
+    for a, n in enumerate(b.shape):
+        k = np.arange(n)
+        l1 = 2.0 - 2.0 * np.cos(np.pi * k / n)
+        shp = [1] * b.ndim
+        shp[a] = n
+        lam = lam + l1.reshape(shp)
+    lam.flat[0] = np.inf
+    return idctn(bh / lam, type=2, norm="ortho")
+
+
+def project_exact(u):
+    """Poisson projection (MGPCG_2.Poisson, 2D/vortex/mgpcg.py:363-368) with the
+    pressure solve done exactly; u modified in place."""
+    apply_bc(u)
+    b = -divergence(u).astype(np.float64)
+    b -= b.mean()
+    p = neumann_solve_dct(b)
+    subtract_grad_p(u, p)
+    apply_bc(u)
+
+
+def mgpcg_solve(b, levels, pre=2, bottom=50, tol=1e-12, relative=False, max_iters=400):
+    """Restated MGPCG solve of A x = b; returns (x, iters, rel_residual, status)."""
+    b = np.ascontiguousarray(b)
+    dtp = b.dtype
+    x = np.zeros_like(b)
+    it = ctypes.c_int(0)
+    res = ctypes.c_double(0)
+    st = getattr(lib(), f"oracle_mgpcg_{_sfx(dtp)}")(
+        ctypes.c_int(b.ndim), _p(_shape3(b.shape)), ctypes.c_int(levels), ctypes.c_int(pre),
+        ctypes.c_int(bottom), _p(b), _p(x), ctypes.c_int(max_iters), ctypes.c_double(tol),
+        ctypes.c_int(1 if relative else 0), ctypes.byref(it), ctypes.byref(res))
+    return x, it.value, res.value, st
+
+
+def default_levels(n, base_level=3):
+    """n_mg_levels = log2(min N) - base_level + 1 (2D/vortex/mgpcg.py:28)."""
+    import math
+
+    return int(math.log2(min(n))) - base_level + 1
+
+
+def project_mgpcg(u, levels=None, pre=2, bottom=50, tol=1e-6, relative=True, max_iters=400):
+    """Poisson projection with the restated MGPCG pressure solve."""
+    n = _cells(u)
+    apply_bc(u)
+    b = -divergence(u)
+    x, it, res, st = mgpcg_solve(b, levels or default_levels(n), pre, bottom, tol, relative, max_iters)
+    subtract_grad_p(u, x)
+    apply_bc(u)
+    return it, res
+
+
+# -------------------------------------------------------------- simulator ---
+class OracleLFM:
+    """Forward LFM midpoint step and its adjoint sweep, restating
+    LFM_Simulator.step_flow_map (2D/vortex/lfm_midpoint_simulator.py:472-661;
+    3D/lfm_midpoint_simulator.py:648-807) and
+    LFM_Diff_Simulator.init_gradient / backtrack_flow_map
+    (2D/vortex/lfm_midpoint_diff_simulator.py:126-503;
+    3D/lfm_midpoint_diff_simulator.py:336-594) for the configuration with
+    viscosity 0, no control force, no obstacle mask, use_short_BFECC=False,
+    BFECC_clamp=False, backward_u=True and a final-time density loss (the
+    per-interval intermediate-target terms, q7, are off).
+
+    Storage follows the reference's disk cache semantics, kept in RAM:
+    midpoint velocity per step, passive at the start of every step, forward
+    map (phi, F) per interval.
+    """
+
+    def __init__(self, n, dx, dt, reinit_every, order, adj_kernel=None, project="exact", dtype=np.float64,
+                 mg_levels=None, mg_tol=1e-6):
+        self.n = tuple(n)
+        self.dim = len(n)
+        self.dx, self.dt, self.R, self.order = dx, dt, reinit_every, order
+        self.adj_kernel = adj_kernel or (4 if self.dim == 2 else 3)
+        self.dtype = dtype
+        self.project_mode = project
+        self.mg_levels = mg_levels
+        self.mg_tol = mg_tol
+        self.phi_map = FlowMap(n, dx, dtype)
+        self.psi_map = FlowMap(n, dx, dtype)
+
+    def project(self, u):
+        if self.project_mode == "exact":
+            project_exact(u)
+        else:
+            project_mgpcg(u, levels=self.mg_levels, tol=self.mg_tol)
+
+    # forward ---------------------------------------------------------------
+    def init(self, u0, passives):
+        """LFM_Simulator.init (2D/vortex/lfm_midpoint_simulator.py:370-439) after
+        the initial field is built: project, reset phi/F, init_u = u."""
+        dt = self.dtype
+        self.u = [np.array(c, dtype=dt) for c in u0]
+        self.project(self.u)
+        self.s = [np.array(s, dtype=dt) for s in passives]
+        self.init_s = [s.copy() for s in self.s]
+        self.init_u = [c.copy() for c in self.u]
+        self.phi_map.reset()
+        self.step_num = -1
+        self.mid = {}
+        self.s_start = {}
+        self.maps = {}
+
+    def step(self):
+        """step_midpoint + step_flow_map (2D/vortex/lfm_midpoint_simulator.py:472-661)."""
+        self.step_num += 1
+        j = self.step_num % self.R
+        dx, dt, order = self.dx, self.dt, self.order
+        self.s_start[self.step_num] = [s.copy() for s in self.s]
+        ad_u = [c.copy() for c in self.u]
+        self.u = sl_advect_u(ad_u, ad_u, dx, 0.5 * dt, order)
+        self.project(self.u)
+        self.mid[self.step_num] = [c.copy() for c in self.u]
+        self.phi_map.march(self.u, -dt, order)
+        self.s = [sl_advect_s(self.u, s, dx, dt, order) for s in self.s]
+        self.u = sl_advect_u(self.u, ad_u, dx, dt, order)
+        self.project(self.u)
+        if j == self.R - 1:
+            # backtrack_psi_grid: last step with the end-of-step u (q3)
+            self.psi_map.reset()
+            self.psi_map.march(self.u, dt, order)
+            for st in reversed(range(self.step_num - j, self.step_num)):
+                self.psi_map.march(self.mid[st], dt, order)
+            self.u = bfecc(self.init_u, self.psi_map, self.phi_map, dx)
+            self.project(self.u)
+            self.s = [bfecc_scalar(si, self.psi_map.psi, self.phi_map.psi, dx) for si in self.init_s]
+            self.init_s = [s.copy() for s in self.s]
+            self.maps[self.step_num // self.R] = self.phi_map.copy()
+            self.phi_map.reset()
+            self.init_u = [c.copy() for c in self.u]
+
+    # adjoint ---------------------------------------------------------------
+    def init_gradient(self, targets):
+        """LFM_Diff_Simulator.init_gradient (2D/vortex/lfm_midpoint_diff_simulator.py:126-207):
+        lam_s = 2 (s - s*), lam_u = 0, loss = sum (s - s*)^2."""
+        self.final_ind = self.step_num + 1
+        self.lam_u = [np.zeros_like(c) for c in self.u]
+        self.lam_s = [2.0 * (s - t.astype(self.dtype)) for s, t in zip(self.s, targets)]
+        self.loss = float(sum(np.sum((s.astype(np.float64) - t) ** 2) for s, t in zip(self.s, targets)))
+        self.step_num = self.final_ind
+        self.init_lam_u = [c.copy() for c in self.lam_u]
+        self.init_lam_s = [c.copy() for c in self.lam_s]
+        self.init_u = [c.copy() for c in self.u]
+        self.init_s = [s.copy() for s in self.s]
+        self.psi_map.reset()
+        self.H1 = [np.zeros_like(c) for c in self.u]
+        self.H2 = [np.zeros_like(c) for c in self.u]
+
+    def backtrack_step(self):
+        """backtrack_step_midpoint + backtrack_flow_map
+        (2D/vortex/lfm_midpoint_diff_simulator.py:220-503)."""
+        self.step_num -= 1
+        j = self.step_num % self.R
+        dx, dt, order = self.dx, self.dt, self.order
+        mid = self.mid[self.step_num]
+        r_u = adj_conv_source(self.u, self.lam_u, dx, self.adj_kernel)
+        r_s = [adj_scalar_source(s, l, dx) for s, l in zip(self.s, self.lam_s)]
+        self.lam_u = sl_advect_u(mid, self.lam_u, dx, -dt, order)
+        self.s = [sl_advect_s(mid, s, dx, -dt, order) for s in self.s]
+        self.u = sl_advect_u(mid, self.u, dx, -dt, order)
+        for a in range(self.dim):
+            self.lam_u[a] = self.lam_u[a] + (-dt) * r_u[a]
+        new_lam_s = []
+        for l, rs in zip(self.lam_s, r_s):
+            new_lam_s.append(sl_advect_s(mid, l, dx, -dt, order))
+            for a in range(self.dim):
+                self.lam_u[a] = self.lam_u[a] + (-dt) * rs[a]
+        self.lam_s = new_lam_s
+        self.psi_map.march(mid, dt, order)
+        accumulate_ab3(r_u, self.init_lam_u, self.psi_map, self.H1, self.H2, self.R - 1 - j, dx, -2 * dt)
+        for rs in r_s:
+            accumulate(rs, self.init_lam_u, self.psi_map, dx, -dt)
+        self.project(self.lam_u)
+        self.project(self.u)
+        if j == 0:
+            fwd = self.maps[self.step_num // self.R]
+            self.lam_u = bfecc(self.init_lam_u, fwd, self.psi_map, dx)
+            self.project(self.lam_u)
+            self.init_lam_u = [c.copy() for c in self.lam_u]
+            self.u = bfecc(self.init_u, fwd, self.psi_map, dx)
+            self.project(self.u)
+            self.init_u = [c.copy() for c in self.u]
+            self.s = [s.copy() for s in self.s_start[self.step_num]]
+            self.init_s = [s.copy() for s in self.s]
+            self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx) for l in self.init_lam_s]
+            self.init_lam_s = [l.copy() for l in self.lam_s]
+            self.psi_map.reset()
+        return self.step_num == 0

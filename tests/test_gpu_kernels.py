@@ -1,0 +1,258 @@
+"""GPU kernel parity: each libdfm.so entry point (through the C-ABI) against
+the FP64 oracle on the same float32-rounded inputs.
+
+Tolerance: rel-L2 <= 1e-5 for FP32 kernels (the forward-field bar), on
+non-square grids so an axis mix-up cannot cancel."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import fmap_to_dev, fmap_to_np, mac_to_dev, mac_to_np, rel_l2, rel_l2_fields, round32
+
+from oracle import lfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+GRIDS = {2: ((24, 32), 1 / 32), 3: ((12, 16, 20), 1 / 20)}
+
+
+def rand_mac(rng, n, scale=1.0):
+    return round32([scale * rng.standard_normal(O.face_shape(n, a)) for a in range(len(n))])
+
+
+def smooth_mac(rng, n, dx):
+    """Smooth-ish divergence-free-ish velocity of magnitude ~1."""
+    dim = len(n)
+    out = []
+    for a in range(dim):
+        X = O.face_coords(n, a, dx)
+        v = np.zeros(X.shape[:-1])
+        for _ in range(3):
+            k = rng.uniform(1, 4, dim) * np.pi
+            ph = rng.uniform(0, 2 * np.pi, dim)
+            v += np.prod(np.cos(k * X + ph), axis=-1) * rng.uniform(0.3, 0.8)
+        out.append(v)
+    return round32(out)
+
+
+def random_map(rng, n, dx, disp=2.0):
+    om = O.FlowMap(n, dx, np.float64)
+    om.psi = round32(om.psi + disp * dx * rng.uniform(-1, 1, om.psi.shape))
+    for a in range(len(n)):
+        om.psi_f[a] = round32(om.psi_f[a] + disp * dx * rng.uniform(-1, 1, om.psi_f[a].shape))
+        om.T_f[a] = round32(om.T_f[a] + 0.2 * rng.standard_normal(om.T_f[a].shape))
+    return om
+
+
+def ops_for(n, dx):
+    from dfm.ops import Ops
+
+    return Ops(n, dx)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reset_identity(dim, device):
+    n, dx = GRIDS[dim]
+    from dfm.ops import FlowMap
+
+    fm = FlowMap(n, device)
+    ops_for(n, dx).reset_identity(fm)
+    om = O.FlowMap(n, dx, np.float64)
+    psi, psi_f, T_f = fmap_to_np(fm)
+    assert rel_l2(psi, om.psi) < 1e-7
+    for a in range(dim):
+        assert rel_l2(psi_f[a], om.psi_f[a]) < 1e-7
+        assert np.array_equal(T_f[a], om.T_f[a])
+
+
+@pytest.mark.parametrize("dim,order", [(2, 2), (2, 4), (3, 2), (3, 4)])
+def test_march(dim, order, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(10 + dim * 3 + order)
+    u = smooth_mac(rng, n, dx)
+    om = random_map(rng, n, dx)
+    fm = fmap_to_dev(om, device)
+    dt = 0.5 * dx
+    ops_for(n, dx).march(mac_to_dev(u, device), fm, dt, order)
+    O.march(u, om.psi, None, dx, dt, order)
+    for a in range(dim):
+        O.march(u, om.psi_f[a], om.T_f[a], dx, dt, order)
+    psi, psi_f, T_f = fmap_to_np(fm)
+    assert rel_l2(psi, om.psi) < TOL
+    for a in range(dim):
+        assert rel_l2(psi_f[a], om.psi_f[a]) < TOL
+        assert rel_l2(T_f[a], om.T_f[a]) < TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pullback_u_and_epilogue(dim, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(20 + dim)
+    u0 = rand_mac(rng, n)
+    add = rand_mac(rng, n)
+    om = random_map(rng, n, dx, disp=3.0)
+    fm = fmap_to_dev(om, device)
+    out = mac_to_dev(add, device)
+    ops_for(n, dx).pullback_u(mac_to_dev(u0, device), fm, out, alpha=0.7, beta=-1.3, add=mac_to_dev(add, device))
+    ref = [0.7 * r - 1.3 * c for r, c in zip(O.advect_u(u0, om, dx), add)]
+    assert rel_l2_fields(mac_to_np(out), ref) < TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pullback_s_and_bfecc_s(dim, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(30 + dim)
+    s0 = round32(rng.standard_normal(n))
+    om1 = random_map(rng, n, dx)
+    om2 = random_map(rng, n, dx)
+    fm1, fm2 = fmap_to_dev(om1, device), fmap_to_dev(om2, device)
+    ops = ops_for(n, dx)
+    s_d = torch.tensor(s0, dtype=torch.float32, device=device)
+    out = torch.empty_like(s_d)
+    ops.pullback_s(s_d, fm1.psi, out)
+    assert rel_l2(out.cpu().numpy(), O.advect_scalar(s0, om1.psi, dx)) < TOL
+    err = torch.empty_like(s_d)
+    ops.bfecc_s(s_d, fm1.psi, fm2.psi, err, out)
+    assert rel_l2(out.cpu().numpy(), O.bfecc_scalar(s0, om1.psi, om2.psi, dx)) < TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bfecc_u(dim, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(40 + dim)
+    u0 = rand_mac(rng, n)
+    om1 = random_map(rng, n, dx)
+    om2 = random_map(rng, n, dx)
+    from dfm.ops import MacField
+
+    out = MacField.empty(n, device)
+    err = MacField.empty(n, device)
+    ops_for(n, dx).bfecc_u(mac_to_dev(u0, device), fmap_to_dev(om1, device), fmap_to_dev(om2, device), err, out)
+    assert rel_l2_fields(mac_to_np(out), O.bfecc(u0, om1, om2, dx)) < TOL
+
+
+@pytest.mark.parametrize("dim,order", [(2, 2), (2, 4), (3, 2), (3, 4)])
+def test_sl_advect_multi_source(dim, order, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(50 + dim * 5 + order)
+    vel = smooth_mac(rng, n, dx)
+    srcs = [rand_mac(rng, n), rand_mac(rng, n)]
+    ss = [round32(rng.standard_normal(n)) for _ in range(3)]
+    dt = -0.5 * dx if order == 2 else 0.5 * dx
+    from dfm.ops import MacField
+
+    dsts = [MacField.empty(n, device) for _ in srcs]
+    s_src = [torch.tensor(s, dtype=torch.float32, device=device) for s in ss]
+    s_dst = [torch.empty_like(t) for t in s_src]
+    ops_for(n, dx).sl_advect(mac_to_dev(vel, device), dt, order, [mac_to_dev(s, device) for s in srcs], dsts,
+                             s_src, s_dst)
+    for src, dst in zip(srcs, dsts):
+        assert rel_l2_fields(mac_to_np(dst), O.sl_advect_u(vel, src, dx, dt, order)) < TOL
+    for s, d in zip(ss, s_dst):
+        assert rel_l2(d.cpu().numpy(), O.sl_advect_s(vel, s, dx, dt, order)) < TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_accumulate_ab3_and_second_source(dim, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(60 + dim)
+    om = random_map(rng, n, dx)
+    fm = fmap_to_dev(om, device)
+    ops = ops_for(n, dx)
+    init = rand_mac(rng, n)
+    h1 = [np.zeros_like(c) for c in init]
+    h2 = [np.zeros_like(c) for c in init]
+    d_init, d_h1, d_h2 = mac_to_dev(init, device), mac_to_dev(h1, device), mac_to_dev(h2, device)
+    for ind in range(4):
+        f = rand_mac(rng, n)
+        f2 = rand_mac(rng, n)
+        ops.accumulate(mac_to_dev(f, device), fm, d_init, d_h1, d_h2, ind, -0.02, f2=mac_to_dev(f2, device),
+                       dt2=-0.01)
+        O.accumulate_ab3(f, init, om, h1, h2, ind, dx, -0.02)
+        O.accumulate(f2, init, om, dx, -0.01)
+        assert rel_l2_fields(mac_to_np(d_init), init) < TOL
+        assert rel_l2_fields(mac_to_np(d_h1), h1) < TOL
+        assert rel_l2_fields(mac_to_np(d_h2), h2) < TOL
+
+
+@pytest.mark.parametrize("dim,kern", [(2, 3), (2, 4), (3, 3), (3, 4)])
+def test_adj_conv_source(dim, kern, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(70 + dim * 5 + kern)
+    u = rand_mac(rng, n)
+    w = rand_mac(rng, n)
+    from dfm.ops import MacField
+
+    r = MacField.empty(n, device)
+    ops_for(n, dx).adj_conv_source(mac_to_dev(u, device), mac_to_dev(w, device), kern, r)
+    assert rel_l2_fields(mac_to_np(r), O.adj_conv_source(u, w, dx, kern)) < TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_adj_scalar_source_summed(dim, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(80 + dim)
+    ss = [round32(rng.standard_normal(n)) for _ in range(2)]
+    ls = [round32(rng.standard_normal(n)) for _ in range(2)]
+    from dfm.ops import MacField
+
+    r = MacField.empty(n, device)
+    ops_for(n, dx).adj_scalar_source([torch.tensor(s, device=device, dtype=torch.float32) for s in ss],
+                                     [torch.tensor(v, device=device, dtype=torch.float32) for v in ls], r)
+    parts = [O.adj_scalar_source(s, v, dx) for s, v in zip(ss, ls)]
+    ref = [parts[0][a] + parts[1][a] for a in range(dim)]
+    assert rel_l2_fields(mac_to_np(r), ref) < TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_axpy(dim, device):
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(90 + dim)
+    x, y, z = rand_mac(rng, n), rand_mac(rng, n), rand_mac(rng, n)
+    dx_, dy_, dz_ = mac_to_dev(x, device), mac_to_dev(y, device), mac_to_dev(z, device)
+    ops_for(n, dx).axpy(dx_, 0.25, dy_, dx_, beta=-0.5, z=dz_)
+    assert rel_l2_fields(mac_to_np(dx_), [a + 0.25 * b - 0.5 * c for a, b, c in zip(x, y, z)]) < 1e-6
+
+
+@pytest.mark.parametrize("n,levels", [((64, 48), 0), ((32, 32, 16), 0), ((256, 256), 6), ((64, 64, 64), 4)])
+def test_mgpcg_projection_matches_exact(n, levels, device):
+    from dfm.ops import MGPCG
+
+    dim = len(n)
+    dx = 1.0 / n[1]
+    rng = np.random.default_rng(100 + len(n) + n[0])
+    u = smooth_mac(rng, n, dx)
+    u = [c + 0.05 * rng.standard_normal(c.shape) for c in u]
+    u = round32(u)
+    solver = MGPCG(n, dx, levels=levels, tol=1e-7, max_iters=200)
+    d = mac_to_dev(u, device)
+    solver.Poisson(d)
+    iters, res = solver.status()
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    out = mac_to_np(d)
+    assert iters < 60, iters
+    assert rel_l2_fields(out, ref) < 2e-6
+    div = O.divergence(out)
+    assert np.abs(div).max() < 1e-4 * max(np.abs(c).max() for c in u)
+    # idempotent: a second projection changes nothing beyond FP32 noise
+    solver.Poisson(d)
+    assert rel_l2_fields(mac_to_np(d), out) < 1e-5
+
+
+def test_mgpcg_solve_and_status(device):
+    from dfm.ops import MGPCG
+
+    n = (32, 32, 32)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(n)
+    b -= b.mean()
+    solver = MGPCG(n, 1 / 32, levels=3, tol=1e-6)
+    x = torch.zeros(n, dtype=torch.float32, device=device)
+    solver.solve(torch.tensor(b, dtype=torch.float32, device=device), x)
+    it, res = solver.status()
+    assert res < 1e-6 and 0 < it < 40
+    xe = O.neumann_solve_dct(round32(b))
+    xg = x.cpu().numpy().astype(np.float64)
+    assert rel_l2(xg - xg.mean(), xe) < 1e-5

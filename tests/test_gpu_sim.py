@@ -1,0 +1,110 @@
+"""Whole-trajectory parity of the drop-in simulators against the FP64 oracle
+(forward fields rel-L2 <= 1e-4 after several reinitialisation intervals,
+adjoint fields / gradient rel-L2 <= 1e-3), plus size-independent properties
+at the benchmark grid sizes."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import mac_to_np, rel_l2, rel_l2_fields, round32
+
+from oracle import lfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-4
+ADJ_TOL = 1e-3
+
+
+def run_pair(dim, n, R, steps, seed):
+    from dfm import LFM_Diff_Simulator, scenes
+
+    if dim == 2:
+        sc = scenes.scene_2d(n, seed=seed, vortices=4)
+    else:
+        sc = scenes.scene_3d(n, seed=seed, tubes=6)
+    dx = sc["dx"]
+    dt = 0.5 * dx
+    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=steps, tol=1e-7, max_iters=200)
+    u0 = round32(sc["u0"])
+    s0 = round32(sc["density"])
+    tgt = round32(sc["target"])
+    sim.init(u0=u0, init_passive_np=s0)
+    orc = O.OracleLFM(n, dx, dt, R, sim.order)
+    orc.init(u0, [s0])
+    return sim, orc, tgt
+
+
+@pytest.mark.parametrize("dim,n,R,steps", [(2, (32, 32), 4, 8), (3, (16, 16, 16), 3, 6)])
+def test_forward_and_adjoint_match_oracle(dim, n, R, steps, device):
+    sim, orc, tgt = run_pair(dim, n, R, steps, seed=3)
+    assert rel_l2_fields(mac_to_np(sim.u), orc.u) < FWD_TOL
+    for k in range(steps):
+        _, final = sim.step_midpoint()
+        orc.step()
+        assert final == (k == steps - 1)
+    assert rel_l2_fields(mac_to_np(sim.u), orc.u) < FWD_TOL
+    assert rel_l2(sim.passive[0].cpu().numpy(), orc.s[0]) < FWD_TOL
+    sim.init_gradient(target=tgt)
+    orc.init_gradient([tgt])
+    assert abs(sim.loss - orc.loss) <= 1e-4 * abs(orc.loss)
+    done = False
+    while not done:
+        _, done = sim.backtrack_step_midpoint()
+        orc.backtrack_step()
+    assert sim.step_num[None] == 0
+    assert rel_l2_fields(mac_to_np(sim.adj_u), orc.lam_u) < ADJ_TOL
+    assert rel_l2(sim.adj_passive[0].cpu().numpy(), orc.lam_s[0]) < ADJ_TOL
+    # the reverse-advected primal returns to the initial state (backward_u)
+    assert rel_l2_fields(mac_to_np(sim.u), orc.u) < ADJ_TOL
+
+
+def test_reference_api_surface(device):
+    from dfm import LFM_Diff_Simulator, scenes
+
+    sc = scenes.scene_2d((32, 32), seed=1, vortices=3)
+    sim = LFM_Diff_Simulator(32, 32, sc["dx"], 0.5 * sc["dx"], 2, "unused_log_dir", total_steps=2)
+    sim.init(theta=sc["theta"], init_passive_np=sc["density"])
+    assert sim.u_x.shape == (33, 32) and sim.u_y.shape == (32, 33)
+    assert sim.step_num[None] == -1
+    out, final = sim.forward_step_midpoint(True, True)
+    assert (out, final) == (False, False)
+    out, final = sim.forward_step_midpoint(True, True)
+    assert final
+    w = sim.calculate_w()
+    assert w.shape == (32, 32) and np.isfinite(w).all()
+    sim.init_gradient(target=sc["target"])
+    assert sim.adj_u_x.to_numpy().shape == (33, 32)
+    flags = [sim.backtrack_step_midpoint()[1] for _ in range(2)]
+    assert flags == [False, True]
+    assert np.isfinite(sim.adj_u_x.to_numpy()).all()
+
+
+@pytest.mark.parametrize("n", [(256, 256), (128, 128, 128)])
+def test_full_size_projection_and_uniform_march(n, device):
+    """At the benchmark sizes: projected divergence small, projection
+    idempotent, and a uniform flow marches the flow map exactly."""
+    from dfm import MGPCG, FlowMap, MacField, Ops
+
+    dim = len(n)
+    dx = 1.0 / n[1]
+    rng = np.random.default_rng(11)
+    u = MacField([torch.tensor(rng.standard_normal(O.face_shape(n, a)), dtype=torch.float32, device=device)
+                  for a in range(dim)])
+    solver = MGPCG(n, dx, tol=1e-6)
+    solver.Poisson(u)
+    it, res = solver.status()
+    assert res < 1e-6 and it < 60
+    div = sum(torch.diff(u.c[a], dim=a) for a in range(dim))
+    assert float(div.abs().max()) < 1e-4
+    c = [0.3, -0.2, 0.1][:dim]
+    U = MacField([torch.full(u.c[a].shape, c[a], device=device) for a in range(dim)])
+    fm = FlowMap(n, device)
+    ops = Ops(n, dx)
+    ops.reset_identity(fm)
+    psi0 = fm.psi.clone()
+    ops.march(U, fm, 0.5 * dx, 4)
+    shift = torch.tensor(c, device=device).view(dim, *([1] * dim)) * 0.5 * dx
+    assert float((fm.psi - (psi0 - shift)).abs().max()) < 1e-6
+    for a in range(dim):
+        assert float((fm.T_f[a][a] - 1).abs().max()) < 1e-6
