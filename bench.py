@@ -1,0 +1,254 @@
+"""Benchmark: 3D forward+adjoint leapfrog flow-map simulation at 128^3.
+
+One "step" = one forward step_midpoint + one adjoint backtrack_step_midpoint
+of the 128^3 MAC-grid configuration (RK4, reinitialisation every 10 steps,
+16 seeded vortex tubes, box-union density, 4-Gaussian target, MGPCG with 5
+levels and 2+2 red-black Gauss-Seidel sweeps, tolerance 1e-6 relative).
+The timed region runs K forward steps, the terminal adjoint and K adjoint
+steps on device-resident inputs; value = K / elapsed.
+
+Prints ONE JSON line (rank 0).  Extra objects: "roofline" (dominant kernel,
+timed with HIP events on the launch stream), "mgpcg" (solver HBM GB/s),
+"cpu_baseline" (the CPU oracle on a bounded sample), "grad_check" (GPU vs
+FP64 oracle on a small grid).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "differentiable-flowmap_amd"))
+
+METRIC = "fwd+adjoint sim steps/s at 128³ MAC grid; MGPCG HBM GB/s; grad rel-err vs FP64"
+HBM_PEAK_GBPS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+# Algorithmic work per unit (flop / byte models; see DESIGN.md "Measurement").
+# 3D quadratic B-spline interpolation of one scalar: value + gradient ~480
+# flop, value only ~220 flop (64 taps, separable weights).
+FLOP_INTERP_GRAD_3D = 480.0
+FLOP_INTERP_VAL_3D = 220.0
+# MGPCG compulsory FP32 traffic: bytes per fine cell per PCG iteration (3D) and
+# per solve (divergence, init, gradient subtraction).
+MG_BYTES_CELL_ITER_3D = 156.0
+MG_BYTES_CELL_SOLVE_3D = 64.0
+
+
+def march_flop(n, order=4):
+    """Flop of one dfm_march launch: face points carry (psi, T), cells psi only."""
+    import numpy as np
+
+    cells = float(np.prod(n))
+    faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
+    stages = 4 if order == 4 else 2
+    per_face = stages * (3 * FLOP_INTERP_GRAD_3D + 2 * 9 + 6 * 2) + 12
+    per_cell = stages * (3 * FLOP_INTERP_VAL_3D + 6) + 6
+    return faces * per_face + cells * per_cell
+
+
+def sl_flop(n, nsrc_vec, nsrc_scal, order=4):
+    import numpy as np
+
+    cells = float(np.prod(n))
+    faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
+    stages = 4 if order == 4 else 2
+    back = stages * 3 * FLOP_INTERP_VAL_3D
+    f = faces * (back + nsrc_vec * FLOP_INTERP_VAL_3D) if nsrc_vec else 0.0
+    c = cells * (back + nsrc_scal * FLOP_INTERP_VAL_3D) if nsrc_scal else 0.0
+    return f + c
+
+
+def run_gpu(args):
+    import numpy as np
+    import torch
+
+    from dfm import LFM_Diff_Simulator, scenes, timing
+
+    n = (args.size,) * 3
+    sc = scenes.scene_3d(n, seed=0, tubes=16)
+    dx = sc["dx"]
+    dt = 0.5 * dx
+    R = 10
+    K, W = args.steps, args.warmup
+    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=5, mg_pre_post=2,
+                             mg_bottom=50, tol=1e-6, max_iters=200)
+    u0, s0, tgt = sc["u0"], sc["density"], sc["target"]
+
+    def forward_adjoint(steps):
+        sim.total_steps = steps
+        sim.init(u0=u0, init_passive_np=s0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step_midpoint()
+        sim.init_gradient(target=tgt)
+        for _ in range(steps):
+            sim.backtrack_step_midpoint()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    if W > 0:
+        forward_adjoint(W)
+    timing.reset()
+    timing.enable(True)
+    elapsed = forward_adjoint(K)
+    timing.enable(False)
+    summ = timing.summary()
+    cnt = timing.counts()
+    lam = torch.cat([c.flatten() for c in sim.adj_u.c])
+    assert bool(torch.isfinite(lam).all()), "non-finite adjoint"
+
+    # dominant flow-map kernel by total device time
+    models = {
+        "march": (march_flop(n), "dfm_march (k_march<3,4>)"),
+        "sl_advect": (None, "dfm_sl_advect (k_sl_advect<3,4>)"),
+        "adj_conv": (None, "dfm_adj_conv_source (k_adj_conv<3,3>)"),
+    }
+    dom = max((k for k in models if k in summ), key=lambda k: summ[k][1])
+    calls, tot_ms = summ[dom]
+    avg_s = tot_ms / calls / 1e3
+    if dom == "march":
+        flop = models["march"][0]
+    elif dom == "sl_advect":
+        flop = sl_flop(n, 2, 2)
+    else:
+        faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
+        flop = faces * 3 * 216 * 4
+    achieved = flop / avg_s / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": models[dom][1], "avg_launch_ms": round(avg_s * 1e3, 4), "calls": calls,
+                "flop_per_launch": flop,
+                "note": "FP32 vector roofline (gather/VALU-bound B-spline stencils; no MFMA on this path)"}
+    mg_calls, mg_ms = summ.get("mgpcg", (0, 0.0))
+    mg_iters = cnt.get("mgpcg_iters", 0)
+    cells = float(np.prod(n))
+    mg_bytes = cells * (MG_BYTES_CELL_ITER_3D * mg_iters + MG_BYTES_CELL_SOLVE_3D * mg_calls)
+    mg_gbps = mg_bytes / (mg_ms / 1e3) / 1e9 if mg_ms > 0 else 0.0
+    mgpcg = {"achieved_GBps": round(mg_gbps, 1), "peak_GBps": HBM_PEAK_GBPS, "frac": round(mg_gbps / HBM_PEAK_GBPS, 4),
+             "solves": mg_calls, "pcg_iters_total": mg_iters,
+             "iters_per_solve": round(mg_iters / max(mg_calls, 1), 2), "ms_per_solve": round(mg_ms / max(mg_calls, 1), 3),
+             "bytes_model": "156 B/cell/PCG-iter + 64 B/cell/solve (FP32 compulsory)"}
+    breakdown = {k: {"calls": v[0], "ms": round(v[1], 2)} for k, v in summ.items()}
+    return elapsed, roofline, mgpcg, breakdown
+
+
+def run_grad_check():
+    """GPU forward+adjoint vs the FP64 oracle on a 32^3 grid (2 intervals)."""
+    import numpy as np
+    import torch
+
+    from dfm import LFM_Diff_Simulator, scenes
+    from oracle import lfm_oracle as O
+
+    n, R, steps = (32, 32, 32), 5, 10
+    sc = scenes.scene_3d(n, seed=7, tubes=8)
+    dx, dt = sc["dx"], 0.5 * sc["dx"]
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)  # noqa: E731
+    u0 = [f32(c) for c in sc["u0"]]
+    s0, tgt = f32(sc["density"]), f32(sc["target"])
+    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=steps, tol=1e-7, max_iters=200)
+    sim.init(u0=u0, init_passive_np=s0)
+    orc = O.OracleLFM(n, dx, dt, R, 4, adj_kernel=3)
+    orc.init(u0, [s0])
+    for _ in range(steps):
+        sim.step_midpoint()
+        orc.step()
+
+    def rel(dev_list, ref_list):
+        num = sum(float(np.sum((d.cpu().numpy() - r) ** 2)) for d, r in zip(dev_list, ref_list))
+        return float(np.sqrt(num / sum(float(np.sum(r ** 2)) for r in ref_list)))
+
+    e_u = rel(sim.u.c, orc.u)
+    e_s = rel(sim.passive, orc.s)
+    sim.init_gradient(target=tgt)
+    orc.init_gradient([tgt])
+    for _ in range(steps):
+        sim.backtrack_step_midpoint()
+        orc.backtrack_step()
+    torch.cuda.synchronize()
+    return {"grid": list(n), "steps": steps, "reinit_every": R, "rel_l2_u_final": e_u, "rel_l2_density_final": e_s,
+            "rel_l2_grad_u0": rel(sim.adj_u.c, orc.lam_u), "rel_l2_grad_s0": rel(sim.adj_passive, orc.lam_s),
+            "oracle": "FP64 CPU restatement, exact (DCT) projection"}
+
+
+def run_cpu_baseline(size):
+    """The oracle in FP32 mode with the restated MGPCG, OpenMP on the host."""
+    import numpy as np
+
+    from dfm import scenes
+    from oracle import lfm_oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    n = (size,) * 3
+    sc = scenes.scene_3d(n, seed=0, tubes=16)
+    dx = sc["dx"]
+    orc = O.OracleLFM(n, dx, 0.5 * dx, 10, 4, adj_kernel=3, project="mgpcg", dtype=np.float32, mg_levels=5,
+                      mg_tol=1e-6)
+    orc.init(sc["u0"], [sc["density"]])
+    orc.step()  # untimed: step 0
+    t0 = time.perf_counter()
+    orc.step()  # step 1 (no reinitialisation)
+    t1 = time.perf_counter()
+    orc.init_gradient([sc["target"].astype(np.float32)])
+    t2 = time.perf_counter()
+    orc.backtrack_step()  # adjoint of step 1 (no reinitialisation)
+    t3 = time.perf_counter()
+    el = (t1 - t0) + (t3 - t2)
+    return {"value": round(1.0 / el, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"1 forward + 1 adjoint step at {size}^3 (a step without reinitialisation; the "
+                      f"1-in-10 reinitialisation cost is not in the sample), FP32, C++/OpenMP oracle with the "
+                      f"restated MGPCG; {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grad-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design note, not a benchmarked path.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps % 10 or args.warmup % 10:
+        sys.exit("--steps and --warmup must be multiples of the reinitialisation interval (10)")
+    elapsed, roofline, mgpcg, breakdown = run_gpu(args)
+    K = args.steps
+    out = {
+        "metric": METRIC,
+        "value": round(K / elapsed, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / K * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded vortex tubes / box-union density / Gaussian target)",
+        "config": {"workload": "3D forward+adjoint leapfrog flow-map, %d^3 MAC grid, RK4, reinit every 10, "
+                               "MGPCG 5 levels 2+2 RBGS tol 1e-6" % args.size,
+                   "grid": [args.size] * 3, "reinit_every": 10, "steps_timed": K},
+        "roofline": roofline,
+        "mgpcg": mgpcg,
+        "breakdown_ms": breakdown,
+    }
+    if not args.no_grad_check:
+        out["grad_check"] = run_grad_check()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = run_cpu_baseline(args.size)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

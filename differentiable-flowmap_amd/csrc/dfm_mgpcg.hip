@@ -110,11 +110,11 @@ DFM_HD void block_sum2(double& a, double& b) {
 }
 
 // Finalize: sum partials and run the scalar step of the PCG recurrence.
-enum FinOp { FIN_MEAN = 0, FIN_BNORM, FIN_ZR_INIT, FIN_ALPHA, FIN_RR, FIN_BETA };
+enum FinOp { FIN_MEAN = 0, FIN_BNORM, FIN_ZR_INIT, FIN_ALPHA, FIN_RR, FIN_BETA, FIN_TRUE };
 
 __global__ void __launch_bounds__(kB) k_finalize(int op, const double* __restrict__ part, int nparts, double* sc,
                                                  int* flags, int64_t ncells, double tol, int relative) {
-  if (op != FIN_MEAN && op != FIN_BNORM && is_done(flags)) return;
+  if (op != FIN_MEAN && op != FIN_BNORM && op != FIN_TRUE && is_done(flags)) return;
   double a = 0, b = 0;
   for (int t = threadIdx.x; t < nparts; t += kB) {
     a += part[t];
@@ -157,6 +157,21 @@ __global__ void __launch_bounds__(kB) k_finalize(int op, const double* __restric
     case FIN_BETA: {  // a = z.r (new)
       sc[S_BETA] = a / (sc[S_ZR] + kEps);
       sc[S_ZR] = a;
+      break;
+    }
+    case FIN_TRUE: {  // a = |b - A x|^2 in FP64, b = sum(b - A x): refinement check
+      sc[S_RR] = a;
+      sc[S_MEAN] = b / (double)ncells;
+      if (!isfinite(a)) {
+        flags[F_STATUS] = 2;
+        flags[F_DONE] = 1;
+      } else if (a < sc[S_THRESH]) {
+        flags[F_STATUS] = 1;
+        flags[F_DONE] = 1;
+      } else {
+        flags[F_STATUS] = 0;
+        flags[F_DONE] = 0;
+      }
       break;
     }
   }
@@ -222,15 +237,16 @@ __global__ void __launch_bounds__(kB) k_load_b(int64_t N, const float* __restric
   store_partials(part, acc, 0.0);
 }
 
-// r -= mean; partial r.r; x = 0
-__global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restrict__ r, float* __restrict__ x,
-                                                      const double* sc, double* part) {
+// r -= mean (kept as b for the refinement residual); partial r.r; x = 0
+__global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restrict__ r, float* __restrict__ bkeep,
+                                                      double* __restrict__ x, const double* sc, double* part) {
   float m = (float)sc[S_MEAN];
   double acc = 0;
   for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
     float v = r[id] - m;
     r[id] = v;
-    x[id] = 0.f;
+    bkeep[id] = v;
+    x[id] = 0.0;
     acc += (double)v * (double)v;
   }
   store_partials(part, acc, 0.0);
@@ -239,7 +255,7 @@ __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restri
 // subtract_grad_p + apply_bc (2D/vortex/mgpcg.py:338-356, 3D/mgpcg.py:413-439):
 // interior faces u_a -= p[I] - p[I - e_a]; boundary faces = 0.
 template <int DIM>
-__global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const float* __restrict__ p) {
+__global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const double* __restrict__ p) {
   int64_t nf[3];
   Dims fd[3];
   for (int a = 0; a < 3; ++a) {
@@ -261,9 +277,43 @@ __global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const float* __
     } else {
       int lo[3] = {i, j, k};
       lo[a] -= 1;
-      c[id] -= p[cidx(d, i, j, k)] - p[cidx(d, lo[0], lo[1], lo[2])];
+      // the pressure difference is formed in FP64: p grows like n |u| in
+      // these units, so an FP32 p would cost ~n ulps of velocity
+      c[id] = (float)((double)c[id] - (p[cidx(d, i, j, k)] - p[cidx(d, lo[0], lo[1], lo[2])]));
     }
   }
+}
+
+// Iterative-refinement residual r = b - A x evaluated in FP64 (x is FP64;
+// b is the recentred FP32 right-hand side); partial |r|^2 and sum r.
+template <int DIM>
+__global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restrict__ x, const float* __restrict__ b,
+                                                   float* __restrict__ r, double* part) {
+  double rr = 0, rs = 0;
+  int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
+  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+    int i, j, k;
+    cell_of(d, id, i, j, k);
+    double s = 0.0, nn = 0.0;
+    if (i > 0) { s += x[cidx(d, i - 1, j, k)]; nn += 1.0; }
+    if (i < d.n[0] - 1) { s += x[cidx(d, i + 1, j, k)]; nn += 1.0; }
+    if (j > 0) { s += x[cidx(d, i, j - 1, k)]; nn += 1.0; }
+    if (j < d.n[1] - 1) { s += x[cidx(d, i, j + 1, k)]; nn += 1.0; }
+    if (DIM == 3) {
+      if (k > 0) { s += x[cidx(d, i, j, k - 1)]; nn += 1.0; }
+      if (k < d.n[2] - 1) { s += x[cidx(d, i, j, k + 1)]; nn += 1.0; }
+    }
+    double v = (double)b[id] - (nn * x[id] - s);
+    r[id] = (float)v;
+    rr += v * v;
+    rs += v;
+  }
+  store_partials(part, rr, rs);
+}
+
+// FP64 solution -> FP32 output
+__global__ void __launch_bounds__(kB) k_to_float(int64_t N, const double* __restrict__ x, float* __restrict__ y) {
+  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) y[id] = (float)x[id];
 }
 
 // ------------------------------------------------------------ PCG steps ----
@@ -287,15 +337,16 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
   store_partials(part, acc, 0.0);
 }
 
-// x += alpha p ; r -= alpha Ap ; partial r.r and sum r
-__global__ void __launch_bounds__(kB) k_update_xr(int64_t N, float* __restrict__ x, float* __restrict__ r,
+// x += alpha p (x in FP64) ; r -= alpha Ap ; partial r.r and sum r
+__global__ void __launch_bounds__(kB) k_update_xr(int64_t N, double* __restrict__ x, float* __restrict__ r,
                                                   const float* __restrict__ p, const float* __restrict__ Ap,
                                                   const double* sc, double* part, const int* flags) {
   if (is_done(flags)) return;
-  float alpha = (float)sc[S_ALPHA];
+  const double alpha_d = sc[S_ALPHA];
+  float alpha = (float)alpha_d;
   double rr = 0, rs = 0;
   for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
-    x[id] = fmaf(alpha, p[id], x[id]);
+    x[id] = fma(alpha_d, (double)p[id], x[id]);
     float v = fmaf(-alpha, Ap[id], r[id]);
     r[id] = v;
     rr += (double)v * (double)v;
@@ -474,7 +525,10 @@ struct Solver {
   Geo G;
   int L, pre, bottom;
   Lvl lv[kMaxLevels];
-  float *x, *p, *Ap;
+  double* x;  // PCG iterate (pressure), FP64
+  float* b;   // recentred right-hand side, kept for the FP64 refinement residual
+  int max_refine;
+  float *p, *Ap;
   double* part;
   double* sc;
   int* flags;
@@ -559,14 +613,19 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   int64_t N = L0.size;
   unsigned gr = gridr(N);
   fin<DIM>(s, st, FIN_MEAN, gr);
-  hipLaunchKernelGGL(k_recenter_norm, dim3(gr), dim3(kB), 0, st, N, L0.r, s.x, s.sc, s.part);
+  hipLaunchKernelGGL(k_recenter_norm, dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, s.sc, s.part);
   fin<DIM>(s, st, FIN_BNORM, gr, tol, relative);
   vcycle<DIM>(s, st, nullptr);
   hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, L0.z, L0.r, (const double*)nullptr, s.part, s.flags, 1, s.p);
   fin<DIM>(s, st, FIN_ZR_INIT, gr);
   Dims d0 = dims_of(L0);
   int it = 0;
+  int refinements = 0;
   bool done = false;
+  auto read_flags = [&]() {
+    return hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipStreamSynchronize(st) == hipSuccess;
+  };
   while (it < max_iters) {
     hipLaunchKernelGGL((k_Ap_dot<DIM>), dim3(gr), dim3(kB), 0, st, d0, s.p, s.Ap, s.part, s.flags);
     fin<DIM>(s, st, FIN_ALPHA, gr);
@@ -575,9 +634,22 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
     ++it;
     bool poll = (it % s.check_every) == 0 || it == max_iters;
     if (poll) {
-      if (hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
-        return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
+      if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
+      if (s.h_flags[F_DONE] && s.h_flags[F_STATUS] == 1 && refinements < s.max_refine) {
+        // Converged on the recursive FP32 residual: measure the true residual
+        // b - A x in FP64; if it misses the tolerance, restart PCG from it.
+        hipLaunchKernelGGL((k_true_resid<DIM>), dim3(gr), dim3(kB), 0, st, d0, s.x, s.b, L0.r, s.part);
+        fin<DIM>(s, st, FIN_TRUE, gr);
+        if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
+        if (!s.h_flags[F_DONE]) {
+          ++refinements;
+          vcycle<DIM>(s, st, s.sc + S_MEAN);
+          hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, L0.z, L0.r, (const double*)(s.sc + S_MEAN), s.part,
+                             s.flags, 1, s.p);
+          fin<DIM>(s, st, FIN_ZR_INIT, gr);
+          continue;
+        }
+      }
       if (s.h_flags[F_DONE]) { done = true; break; }
     }
     if (it == max_iters) break;
@@ -607,6 +679,7 @@ void destroy(Solver* s) {
     if (s->lv[l].z) (void)hipFree(s->lv[l].z);
   }
   if (s->x) (void)hipFree(s->x);
+  if (s->b) (void)hipFree(s->b);
   if (s->p) (void)hipFree(s->p);
   if (s->Ap) (void)hipFree(s->Ap);
   if (s->part) (void)hipFree(s->part);
@@ -648,6 +721,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->pre = pre_post;
   s->bottom = bottom;
   s->check_every = 1;
+  s->max_refine = 3;
   bool ok = true;
   for (int l = 0; l < levels && ok; ++l) {
     for (int a = 0; a < 3; ++a) s->lv[l].n[a] = a < g->dim ? (g->n[a] >> l) : 1;
@@ -656,7 +730,8 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
          hipMalloc(&s->lv[l].z, sizeof(float) * s->lv[l].size) == hipSuccess;
   }
   int64_t N = s->lv[0].size;
-  ok = ok && hipMalloc(&s->x, sizeof(float) * N) == hipSuccess && hipMalloc(&s->p, sizeof(float) * N) == hipSuccess &&
+  ok = ok && hipMalloc(&s->x, sizeof(double) * N) == hipSuccess && hipMalloc(&s->b, sizeof(float) * N) == hipSuccess &&
+       hipMalloc(&s->p, sizeof(float) * N) == hipSuccess &&
        hipMalloc(&s->Ap, sizeof(float) * N) == hipSuccess &&
        hipMalloc(&s->part, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
        hipMalloc(&s->sc, sizeof(double) * S_NSLOT) == hipSuccess &&
@@ -666,7 +741,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   if (ok) ok = hipMemset(s->part, 0, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
                hipMemset(s->sc, 0, sizeof(double) * S_NSLOT) == hipSuccess &&
                hipMemset(s->flags, 0, sizeof(int) * F_NFLAG) == hipSuccess &&
-               hipMemset(s->x, 0, sizeof(float) * N) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+               hipMemset(s->x, 0, sizeof(double) * N) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     destroy(s);
     return set_error(DFM_E_HIP, "dfm_mgpcg_create: device allocation failed");
@@ -727,8 +802,9 @@ extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double to
   hipLaunchKernelGGL(k_load_b, dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, s->part);
   int rc = s->G.dim == 2 ? pcg<2>(*s, st, tol, relative, max_iters) : pcg<3>(*s, st, tol, relative, max_iters);
   if (rc != DFM_OK && rc != DFM_E_NOCONV) return rc;
-  if (hipMemcpyAsync(x, s->x, sizeof(float) * L0.size, hipMemcpyDeviceToDevice, st) != hipSuccess)
-    return set_error(DFM_E_HIP, "mgpcg: copy of the solution failed");
+  hipLaunchKernelGGL(k_to_float, dim3(gr), dim3(kB), 0, st, L0.size, s->x, x);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: solution copy", hipGetErrorString(e));
   return rc;
 }
 
@@ -740,7 +816,7 @@ extern "C" int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual) 
   return DFM_OK;
 }
 
-extern "C" int dfm_mgpcg_pressure(void* handle, float** p) {
+extern "C" int dfm_mgpcg_pressure(void* handle, double** p) {
   Solver* s = static_cast<Solver*>(handle);
   if (!s || !p) return set_error(DFM_E_NULL, "dfm_mgpcg_pressure: NULL argument");
   *p = s->x;

@@ -9,6 +9,7 @@ from __future__ import annotations
 import torch
 
 from . import _lib as L
+from . import timing
 
 
 def face_shape(n, a):
@@ -98,7 +99,8 @@ class Ops:
         L.check(self.lib.dfm_reset_identity(self.g, fm.s, self._st()), "reset_identity")
 
     def march(self, u: MacField, fm: FlowMap, dt, order):
-        L.check(self.lib.dfm_march(self.g, u.s, fm.s, float(dt), int(order), self._st()), "march")
+        with timing.region("march"):
+            L.check(self.lib.dfm_march(self.g, u.s, fm.s, float(dt), int(order), self._st()), "march")
 
     def pullback_u(self, src: MacField, fm: FlowMap, out: MacField, alpha=1.0, beta=0.0, add: MacField = None):
         L.check(self.lib.dfm_pullback_u(self.g, src.s, fm.s, float(alpha), float(beta),
@@ -124,11 +126,13 @@ class Ops:
         nv, ns = len(vec_src), len(s_src)
         src_arr = (L.Mac * max(1, nv))(*[m.s for m in vec_src])
         dst_arr = (L.Mac * max(1, nv))(*[m.s for m in vec_dst])
-        L.check(self.lib.dfm_sl_advect(self.g, vel.s, float(dt), int(order), nv, src_arr, dst_arr, ns,
-                                       L.ptr_array(s_src), L.ptr_array(s_dst), self._st()), "sl_advect")
+        with timing.region("sl_advect"):
+            L.check(self.lib.dfm_sl_advect(self.g, vel.s, float(dt), int(order), nv, src_arr, dst_arr, ns,
+                                           L.ptr_array(s_src), L.ptr_array(s_dst), self._st()), "sl_advect")
 
     def adj_conv_source(self, u: MacField, w: MacField, kern, r: MacField):
-        L.check(self.lib.dfm_adj_conv_source(self.g, u.s, w.s, int(kern), r.s, self._st()), "adj_conv_source")
+        with timing.region("adj_conv"):
+            L.check(self.lib.dfm_adj_conv_source(self.g, u.s, w.s, int(kern), r.s, self._st()), "adj_conv_source")
 
     def adj_scalar_source(self, s_list, lam_list, r: MacField):
         L.check(self.lib.dfm_adj_scalar_source(self.g, len(s_list), L.ptr_array(s_list), L.ptr_array(lam_list),
@@ -181,9 +185,11 @@ class MGPCG:
         self.last_iters = self.status()[0]
 
     def Poisson(self, u: MacField):
-        rc = self.lib.dfm_mgpcg_project(self.h, u.s, float(self.tol), 1 if self.relative else 0,
-                                        int(self.max_iters), L.stream_handle())
+        with timing.region("mgpcg"):
+            rc = self.lib.dfm_mgpcg_project(self.h, u.s, float(self.tol), 1 if self.relative else 0,
+                                            int(self.max_iters), L.stream_handle())
         self._done(rc, "mgpcg_project")
+        timing.add_count("mgpcg_iters", self.last_iters)
 
     # the projection is symmetric and idempotent, so its adjoint is itself
     Poisson_adj = Poisson

@@ -139,8 +139,11 @@ int dfm_mgpcg_destroy(void* handle);
  * PCG solve of A p = -div with all-Neumann A, subtract_grad_p, apply_bc;
  * 2D/vortex/mgpcg.py:358-368, 3D/mgpcg.py:441-451, 3D/fast_mgpcg.py:241-323).
  * Stops when r.r < tol^2 * b.b (relative != 0) or r.r < tol (relative == 0),
- * or after max_iters.  Returns DFM_E_NOCONV / DFM_E_NONFINITE after
- * synchronising on non-convergence; the pressure is kept in the handle. */
+ * or after max_iters; returns DFM_E_NOCONV / DFM_E_NONFINITE in those cases
+ * (the gradient is subtracted either way).  The FP32 V-cycle and residual run
+ * with FP64 dot products, FP64 alpha/beta and an FP64 pressure iterate.
+ * Synchronises the stream once per PCG iteration to poll the device
+ * convergence flag; the pressure is kept in the handle. */
 int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int relative, int max_iters, void* stream);
 
 /* Solve A x = b (b recentred to mean zero first).  Same stopping rule. */
@@ -149,8 +152,8 @@ int dfm_mgpcg_solve(void* handle, const float* b, float* x, double tol, int rela
 /* Iterations and relative residual sqrt(r.r / b.b) of the last solve. */
 int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
 
-/* Device pointer of the pressure (cells) from the last projection. */
-int dfm_mgpcg_pressure(void* handle, float** p);
+/* Device pointer of the pressure (cells, FP64) from the last projection. */
+int dfm_mgpcg_pressure(void* handle, double** p);
 
 #ifdef __cplusplus
 }

@@ -215,8 +215,12 @@ def test_axpy(dim, device):
     assert rel_l2_fields(mac_to_np(dx_), [a + 0.25 * b - 0.5 * c for a, b, c in zip(x, y, z)]) < 1e-6
 
 
-@pytest.mark.parametrize("n,levels", [((64, 48), 0), ((32, 32, 16), 0), ((256, 256), 6), ((64, 64, 64), 4)])
-def test_mgpcg_projection_matches_exact(n, levels, device):
+# FP32 V-cycle/residual with FP64 dots, FP64 pressure and FP64 iterative
+# refinement of the true residual: the projected field sits within ~1e-6 of
+# the exact projection at every size (without refinement 256^2 stalled at 1e-5).
+@pytest.mark.parametrize("n,levels,bound", [((64, 48), 0, 2e-6), ((32, 32, 16), 0, 2e-6), ((256, 256), 6, 2e-6),
+                                            ((64, 64, 64), 4, 2e-6)])
+def test_mgpcg_projection_matches_exact(n, levels, bound, device):
     from dfm.ops import MGPCG
 
     dim = len(n)
@@ -232,13 +236,15 @@ def test_mgpcg_projection_matches_exact(n, levels, device):
     ref = [c.copy() for c in u]
     O.project_exact(ref)
     out = mac_to_np(d)
+    err = rel_l2_fields(out, ref)
+    print(f"n={n} iters={iters} res={res:.2e} rel-L2 vs exact={err:.2e}")
     assert iters < 60, iters
-    assert rel_l2_fields(out, ref) < 2e-6
+    assert err < bound
     div = O.divergence(out)
     assert np.abs(div).max() < 1e-4 * max(np.abs(c).max() for c in u)
     # idempotent: a second projection changes nothing beyond FP32 noise
     solver.Poisson(d)
-    assert rel_l2_fields(mac_to_np(d), out) < 1e-5
+    assert rel_l2_fields(mac_to_np(d), out) < 5e-6
 
 
 def test_mgpcg_solve_and_status(device):
