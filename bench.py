@@ -76,7 +76,9 @@ def run_gpu(args):
     R = 10
     K, W = args.steps, args.warmup
     sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=5, mg_pre_post=2,
-                             mg_bottom=50, tol=1e-6, max_iters=200)
+                             mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
+                             mg_bf16_coarse=args.mg_bf16_coarse, mg_bottom_exact=args.mg_bottom_exact,
+                             mg_check_every=args.mg_check_every)
     u0, s0, tgt = sc["u0"], sc["density"], sc["target"]
 
     def forward_adjoint(steps):
@@ -214,6 +216,11 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad-check", action="store_true")
+    ap.add_argument("--mg-no-tail", action="store_true", help="one launch per sweep on every MG level")
+    ap.add_argument("--mg-bf16-coarse", action="store_true", help="BF16 storage of MG levels >= 2")
+    ap.add_argument("--mg-rbgs-bottom", dest="mg_bottom_exact", action="store_false",
+                    help="the reference's 50 RBGS sweeps on the coarsest level instead of the exact DCT solve")
+    ap.add_argument("--mg-check-every", type=int, default=1, help="PCG iterations per convergence poll")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

@@ -34,11 +34,14 @@ inline int64_t count_of_host(const Geo& g, int a) {
 
 DFM_HD int64_t lin(const int s[3], int i, int j, int k) { return ((int64_t)i * s[1] + j) * s[2] + k; }
 
+// Inverse of lin(); grids stay below 2^32 points, so 32-bit division is used.
 DFM_HD void unlin(const int s[3], int64_t id, int& i, int& j, int& k) {
-  k = (int)(id % s[2]);
-  int64_t t = id / s[2];
-  j = (int)(t % s[1]);
-  i = (int)(t / s[1]);
+  uint32_t u = (uint32_t)id, s2 = (uint32_t)s[2], s1 = (uint32_t)s[1];
+  uint32_t t = u / s2;
+  k = (int)(u - t * s2);
+  uint32_t q = t / s1;
+  j = (int)(t - q * s1);
+  i = (int)q;
 }
 
 // Node position of index (i,j,k) on grid `a` (face axis or 3 = cells):
@@ -61,11 +64,17 @@ DFM_HD int64_t xcd_block(int64_t bid, int64_t nblocks) {
 // Piecewise polynomials of the uniform B-splines of order 2 (quadratic,
 // support 1.5), 3 (cubic, support 2) and 4 (quartic, support 2.5), written
 // from their closed forms; derivative sign follows x.
+//
+// Only the taps inside the support are visited: NT = K + 1 consecutive nodes
+// starting at base(s).  The reference loops over 4 / 6 / 8 taps per axis
+// (2D/vortex/taichi_utils.py:433,470,506); every tap it visits beyond these
+// has |x| >= support and weight exactly 0, so the sums are the same while a
+// 3D quadratic sample gathers 27 values instead of 64 (cubic 64 instead of 216).
 template <int K> struct Spline;
 
 template <> struct Spline<2> {
-  static constexpr int NT = 4;  // taps floor(s)-1 .. floor(s)+2
-  static constexpr int LO = -1;
+  static constexpr int NT = 3;  // nodes round(s)-1 .. round(s)+1
+  DFM_HD static int base(float s) { return (int)floorf(s + 0.5f) - 1; }
   DFM_HD static float w(float x) {
     float a = fabsf(x);
     return a < 0.5f ? 0.75f - a * a : (a < 1.5f ? 0.5f * (1.5f - a) * (1.5f - a) : 0.0f);
@@ -78,8 +87,8 @@ template <> struct Spline<2> {
 };
 
 template <> struct Spline<3> {
-  static constexpr int NT = 6;
-  static constexpr int LO = -2;
+  static constexpr int NT = 4;  // nodes floor(s)-1 .. floor(s)+2
+  DFM_HD static int base(float s) { return (int)floorf(s) - 1; }
   DFM_HD static float w(float x) {
     float a = fabsf(x);
     if (a < 1.0f) return (0.5f * a - 1.0f) * a * a + (2.0f / 3.0f);
@@ -101,8 +110,8 @@ template <> struct Spline<3> {
 };
 
 template <> struct Spline<4> {
-  static constexpr int NT = 8;
-  static constexpr int LO = -3;
+  static constexpr int NT = 5;  // nodes round(s)-2 .. round(s)+2
+  DFM_HD static int base(float s) { return (int)floorf(s + 0.5f) - 2; }
   DFM_HD static float w(float x) {
     float a = fabsf(x), a2 = a * a;
     if (a < 0.5f) return (115.0f / 192.0f) + a2 * (-0.625f + 0.25f * a2);
@@ -126,31 +135,35 @@ template <> struct Spline<4> {
   }
 };
 
-// Per-axis tap set of one sample: clamped coordinate s in [1, S-2-1e-6],
-// first tap index and the NT weights (and derivative weights).  For the
-// quadratic spline the window is pulled back by one node when rounding puts
-// floor(s) at S-2; the dropped tap then carries zero weight, so the value is
-// unchanged and no tap leaves [0, S-1].  Wider splines clamp every tap index
-// to the field, as the reference's sample() does.
+// Clamped sample coordinate s in [1, S-2-1e-6] (the reference's clamp,
+// 2D/vortex/taichi_utils.py:420-421) and the first node of the support.  For
+// the quadratic spline s in [1, S-2] puts the 3 nodes inside [0, S-1]; the
+// base is clamped anyway so that no input (NaN included) can leave the field.
+// Wider splines clamp every node index to the field, as sample() does.
+template <int K> DFM_HD float clamp_coord(float p_over_dx, float bl, int S) {
+  return fmaxf(1.0f, fminf(p_over_dx - bl, (float)(S - 2) - 1e-6f));
+}
+template <int K> DFM_HD int support_base(float s, int S) {
+  int b = Spline<K>::base(s);
+  if (K == 2) b = b < 0 ? 0 : (b > S - 3 ? S - 3 : b);
+  return b;
+}
+
+// Per-axis tap set of one sample: node indices and the NT weights (and
+// derivative weights).
 template <int K, bool D> struct Axis {
   int idx[Spline<K>::NT];
   float w[Spline<K>::NT];
   float dw[D ? Spline<K>::NT : 1];
   DFM_HD void setup(float p_over_dx, float bl, int S) {
-    float u = p_over_dx - bl;
-    float s = fmaxf(1.0f, fminf(u, (float)(S - 2) - 1e-6f));
-    float fl = floorf(s);
-    int f0 = (int)fl;
-    if (K == 2 && f0 > S - 3) {
-      f0 = S - 3;
-      fl = (float)f0;
-    }
+    float s = clamp_coord<K>(p_over_dx, bl, S);
+    int b = support_base<K>(s, S);
 #pragma unroll
     for (int t = 0; t < Spline<K>::NT; ++t) {
-      float x = s - (fl + (float)(Spline<K>::LO + t));
+      float x = s - (float)(b + t);
       w[t] = Spline<K>::w(x);
       if (D) dw[t] = Spline<K>::dw(x);
-      int id = f0 + Spline<K>::LO + t;
+      int id = b + t;
       if (K != 2) id = id < 0 ? 0 : (id > S - 1 ? S - 1 : id);
       idx[t] = id;
     }
@@ -196,20 +209,14 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     // is cheaper than the register pressure of a fully unrolled 3D loop)
     Axis<K, GRAD> az;
     az.setup(p[2] * inv_dx, bl[2], S[2]);
-    float ux = p[0] * inv_dx - bl[0];
-    float sx = fmaxf(1.0f, fminf(ux, (float)(S[0] - 2) - 1e-6f));
-    float flx = floorf(sx);
-    int f0x = (int)flx;
-    if (K == 2 && f0x > S[0] - 3) {
-      f0x = S[0] - 3;
-      flx = (float)f0x;
-    }
+    const float sx = clamp_coord<K>(p[0] * inv_dx, bl[0], S[0]);
+    const int bx = support_base<K>(sx, S[0]);
     float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll 1
     for (int a = 0; a < NT; ++a) {
-      float x = sx - (flx + (float)(Spline<K>::LO + a));
+      float x = sx - (float)(bx + a);
       float wxa = Spline<K>::w(x);
-      int ia = f0x + Spline<K>::LO + a;
+      int ia = bx + a;
       if (K != 2) ia = ia < 0 ? 0 : (ia > S[0] - 1 ? S[0] - 1 : ia);
       float yv = 0.f, yd = 0.f, yz = 0.f;
 #pragma unroll

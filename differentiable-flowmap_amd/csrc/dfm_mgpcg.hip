@@ -79,11 +79,28 @@ DFM_HD float nbr_count(const Dims& d, int i, int j, int k) {
   return nn;
 }
 
+// Cell index -> (i, j, k).  Grids here stay below 2^32 cells, so 32-bit
+// unsigned division is used (64-bit division costs ~5x more on the VALU).
 DFM_HD void cell_of(const Dims& d, int64_t id, int& i, int& j, int& k) {
-  k = (int)(id % d.n[2]);
-  int64_t t = id / d.n[2];
-  j = (int)(t % d.n[1]);
-  i = (int)(t / d.n[1]);
+  uint32_t u = (uint32_t)id;
+  uint32_t n2 = (uint32_t)d.n[2], n1 = (uint32_t)d.n[1];
+  uint32_t t = u / n2;
+  k = (int)(u - t * n2);
+  uint32_t q = t / n1;
+  j = (int)(t - q * n1);
+  i = (int)q;
+}
+
+// 1/nn for the neighbour counts that occur (1..6): one multiply instead of an
+// IEEE divide per update.
+DFM_HD float inv_count(float nn) {
+  const int c = (int)nn;
+  return c == 6 ? (1.0f / 6.0f)
+       : c == 5 ? 0.2f
+       : c == 4 ? 0.25f
+       : c == 3 ? (1.0f / 3.0f)
+       : c == 2 ? 0.5f
+                : 1.0f;
 }
 
 DFM_HD bool is_done(const int* flags) { return flags[F_DONE] != 0; }
@@ -397,19 +414,40 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
   float nn = nbr_count<DIM>(d, i, j, k);
   float rv = r[id] - m;
   if (((i + j + k) & 1) == 0) {
-    z[id] = rv / nn;
+    z[id] = rv * inv_count(nn);
     return;
   }
   float s = 0.f;
-  if (i > 0) s += (r[cidx(d, i - 1, j, k)] - m) / nbr_count<DIM>(d, i - 1, j, k);
-  if (i < d.n[0] - 1) s += (r[cidx(d, i + 1, j, k)] - m) / nbr_count<DIM>(d, i + 1, j, k);
-  if (j > 0) s += (r[cidx(d, i, j - 1, k)] - m) / nbr_count<DIM>(d, i, j - 1, k);
-  if (j < d.n[1] - 1) s += (r[cidx(d, i, j + 1, k)] - m) / nbr_count<DIM>(d, i, j + 1, k);
+  if (i > 0) s += (r[cidx(d, i - 1, j, k)] - m) * inv_count(nbr_count<DIM>(d, i - 1, j, k));
+  if (i < d.n[0] - 1) s += (r[cidx(d, i + 1, j, k)] - m) * inv_count(nbr_count<DIM>(d, i + 1, j, k));
+  if (j > 0) s += (r[cidx(d, i, j - 1, k)] - m) * inv_count(nbr_count<DIM>(d, i, j - 1, k));
+  if (j < d.n[1] - 1) s += (r[cidx(d, i, j + 1, k)] - m) * inv_count(nbr_count<DIM>(d, i, j + 1, k));
   if (DIM == 3) {
-    if (k > 0) s += (r[cidx(d, i, j, k - 1)] - m) / nbr_count<DIM>(d, i, j, k - 1);
-    if (k < d.n[2] - 1) s += (r[cidx(d, i, j, k + 1)] - m) / nbr_count<DIM>(d, i, j, k + 1);
+    if (k > 0) s += (r[cidx(d, i, j, k - 1)] - m) * inv_count(nbr_count<DIM>(d, i, j, k - 1));
+    if (k < d.n[2] - 1) s += (r[cidx(d, i, j, k + 1)] - m) * inv_count(nbr_count<DIM>(d, i, j, k + 1));
   }
-  z[id] = (rv + s) / nn;
+  z[id] = (rv + s) * inv_count(nn);
+}
+
+// Colour-`phase` cell of pair index t (pairs run along the innermost axis);
+// false when the pair's cell of that colour falls outside an odd-sized row.
+template <int DIM>
+DFM_HD bool colour_cell(const Dims& d, uint32_t t, int phase, int& i, int& j, int& k) {
+  const uint32_t inner = DIM == 3 ? d.n[2] : d.n[1];
+  const uint32_t half = (inner + 1) >> 1;
+  uint32_t o = t / half;
+  int hm = (int)(t - o * half);
+  if (DIM == 3) {
+    uint32_t q = o / (uint32_t)d.n[1];
+    i = (int)q;
+    j = (int)(o - q * (uint32_t)d.n[1]);
+    k = 2 * hm + ((phase - i - j) & 1);
+    return k < d.n[2];
+  }
+  i = (int)o;
+  k = 0;
+  j = 2 * hm + ((phase - i) & 1);
+  return j < d.n[1];
 }
 
 // One half-sweep of colour `phase`; threads cover only that colour (the
@@ -426,20 +464,8 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, const float* __restrict__
   const int64_t outer = DIM == 3 ? (int64_t)d.n[0] * d.n[1] : d.n[0];
   int64_t t = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (t >= outer * half) return;
-  int hm = (int)(t % half);
-  int64_t o = t / half;
   int i, j, k;
-  if (DIM == 3) {
-    i = (int)(o / d.n[1]);
-    j = (int)(o % d.n[1]);
-    k = 2 * hm + ((phase - i - j) & 1);
-    if (k >= d.n[2]) return;
-  } else {
-    i = (int)o;
-    k = 0;
-    j = 2 * hm + ((phase - i) & 1);
-    if (j >= d.n[1]) return;
-  }
+  if (!colour_cell<DIM>(d, (uint32_t)t, phase, i, j, k)) return;
   int64_t id = cidx(d, i, j, k);
   float nn, s;
   if (zc) {
@@ -459,7 +485,7 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, const float* __restrict__
   } else {
     s = nbr_sum<DIM>(z, d, i, j, k, nn);
   }
-  z[id] = (r[id] - m + s) / nn;
+  z[id] = (r[id] - m + s) * inv_count(nn);
 }
 
 // restrict (2D/vortex/mgpcg.py:152-158): coarse r = sum over the 2^d children
@@ -490,34 +516,211 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
   rc[cid] = acc * (1.0f / (float)(DIM - 1));
 }
 
-// Coarsest level in one workgroup: `iters` RBGS sweeps from z = 0 in LDS.
-template <int DIM>
-__global__ void __launch_bounds__(1024) k_bottom(Dims d, const float* __restrict__ r, float* __restrict__ z, int iters,
-                                                 const double* shift, const int* flags) {
-  if (is_done(flags)) return;
-  __shared__ float sr[kLdsCells];
-  __shared__ float sz[kLdsCells];
-  float m = shift ? (float)*shift : 0.f;
-  int N = d.n[0] * d.n[1] * d.n[2];
-  for (int id = threadIdx.x; id < N; id += blockDim.x) {
-    sr[id] = r[id] - m;
-    sz[id] = 0.f;
+// --------------------------------------------------- coarse tail in LDS ----
+// Levels Lt..L-1 of the V-cycle run inside one workgroup with every level's
+// r and z resident in LDS (FP32, or BF16 storage with FP32 arithmetic in the
+// mixed-precision mode): pre-smoothing from zero, restriction, the `bottom`
+// sweeps on the coarsest level, prolongation and post-smoothing -- the launch
+// sequence the reference issues per level collapsed into one kernel.
+DFM_HD float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+DFM_HD uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (uint16_t)(u >> 16);
+}
+template <typename T> struct Lds;
+template <> struct Lds<float> {
+  DFM_HD static float ld(const float* p, int i) { return p[i]; }
+  DFM_HD static void st(float* p, int i, float v) { p[i] = v; }
+};
+template <> struct Lds<uint16_t> {
+  DFM_HD static float ld(const uint16_t* p, int i) { return bf16_to_f32(p[i]); }
+  DFM_HD static void st(uint16_t* p, int i, float v) { p[i] = f32_to_bf16(v); }
+};
+
+constexpr int kTailMax = 8;
+struct TailArgs {
+  int nl;             // levels in the tail
+  int n[kTailMax][3]; // their dims
+  int off[kTailMax];  // element offset of level l's r in LDS (z follows r)
+  int pre, bottom;
+  int bottom_exact;   // 1: solve the coarsest level exactly (DCT-II) instead of RBGS
+  float inv_dm1;      // restriction scale 1/(dim-1)
+};
+
+// Orthonormal DCT-II along one axis of an LDS array (or its inverse, DCT-III):
+// out[.., k, ..] = sum_m in[.., m, ..] * c_k cos(pi k (2m+1) / 2N) (forward).
+// The all-Neumann Laplacian factorises over axes and each factor is
+// diagonalised by this basis with eigenvalues 2 - 2 cos(pi k / N).
+template <typename T>
+DFM_HD void lds_dct_axis(const T* in, T* out, const int* n, int axis, bool inverse) {
+  const int N = n[axis];
+  const int stride = axis == 0 ? n[1] * n[2] : (axis == 1 ? n[2] : 1);
+  const int total = n[0] * n[1] * n[2];
+  const float pi_2n = 3.14159265358979f / (2.0f * (float)N);
+  const float c0 = sqrtf(1.0f / (float)N), c1 = sqrtf(2.0f / (float)N);
+  for (int id = threadIdx.x; id < total; id += blockDim.x) {
+    int coord = (id / stride) % N;
+    int base = id - coord * stride;
+    float acc = 0.f;
+    for (int m = 0; m < N; ++m) {
+      // forward: k = coord (output frequency), m = sample; inverse: swapped
+      int kf = inverse ? m : coord, xs = inverse ? coord : m;
+      float c = (kf == 0 ? c0 : c1) * __cosf(pi_2n * (float)(kf * (2 * xs + 1)));
+      acc = fmaf(Lds<T>::ld(in, base + m * stride), c, acc);
+    }
+    Lds<T>::st(out, id, acc);
   }
   __syncthreads();
-  for (int it = 0; it < iters; ++it) {
-    for (int phase = 0; phase < 2; ++phase) {
-      for (int id = threadIdx.x; id < N; id += blockDim.x) {
-        int i, j, k;
-        cell_of(d, id, i, j, k);
-        if (((i + j + k) & 1) != phase) continue;
-        float nn;
-        float s = nbr_sum<DIM>(sz, d, i, j, k, nn);
-        sz[id] = (sr[id] + s) / nn;
-      }
-      __syncthreads();
+}
+
+// z = A^+ r on one LDS level (mean-zero solution of the Neumann problem):
+// forward DCT on every axis, divide by the eigenvalue sum (zero mode -> 0),
+// inverse DCT.  r is used as scratch; the result lands in z.
+template <int DIM, typename T>
+DFM_HD void lds_exact_solve(T* r, T* z, const int* n) {
+  T* a = r;
+  T* b = z;
+  for (int ax = 0; ax < DIM; ++ax) {
+    lds_dct_axis<T>(a, b, n, ax, false);
+    T* t = a; a = b; b = t;
+  }
+  const int total = n[0] * n[1] * n[2];
+  for (int id = threadIdx.x; id < total; id += blockDim.x) {
+    int k = id % n[2], t = id / n[2];
+    int j = t % n[1], i = t / n[1];
+    float lam = 2.0f - 2.0f * __cosf(3.14159265358979f * (float)i / (float)n[0]) +
+                2.0f - 2.0f * __cosf(3.14159265358979f * (float)j / (float)n[1]);
+    if (DIM == 3) lam += 2.0f - 2.0f * __cosf(3.14159265358979f * (float)k / (float)n[2]);
+    float v = Lds<T>::ld(a, id);
+    Lds<T>::st(a, id, (i == 0 && j == 0 && k == 0) ? 0.f : v / lam);
+  }
+  __syncthreads();
+  for (int ax = 0; ax < DIM; ++ax) {
+    lds_dct_axis<T>(a, b, n, ax, true);
+    T* t = a; a = b; b = t;
+  }
+  if (a != z) {
+    for (int id = threadIdx.x; id < total; id += blockDim.x) Lds<T>::st(z, id, Lds<T>::ld(a, id));
+    __syncthreads();
+  }
+}
+
+// One half-sweep of colour `colour` on an LDS-resident level: every thread
+// owns one cell of that colour per pass (pairs along the innermost axis).
+template <int DIM, typename T>
+DFM_HD void lds_half_sweep(T* r, T* z, const int* n, int colour) {
+  const Dims d{{n[0], n[1], n[2]}};
+  const uint32_t inner = DIM == 3 ? n[2] : n[1];
+  const uint32_t pairs = (uint32_t)(DIM == 3 ? n[0] * n[1] : n[0]) * ((inner + 1) >> 1);
+  const int sx = n[1] * n[2], sy = n[2];
+  for (uint32_t t = threadIdx.x; t < pairs; t += blockDim.x) {
+    int i, j, k;
+    if (!colour_cell<DIM>(d, t, colour, i, j, k)) continue;
+    int id = (i * n[1] + j) * n[2] + k;
+    float s = 0.f, nn = 0.f;
+    if (i > 0) { s += Lds<T>::ld(z, id - sx); nn += 1.f; }
+    if (i < n[0] - 1) { s += Lds<T>::ld(z, id + sx); nn += 1.f; }
+    if (j > 0) { s += Lds<T>::ld(z, id - sy); nn += 1.f; }
+    if (j < n[1] - 1) { s += Lds<T>::ld(z, id + sy); nn += 1.f; }
+    if (DIM == 3) {
+      if (k > 0) { s += Lds<T>::ld(z, id - 1); nn += 1.f; }
+      if (k < n[2] - 1) { s += Lds<T>::ld(z, id + 1); nn += 1.f; }
+    }
+    Lds<T>::st(z, id, (Lds<T>::ld(r, id) + s) * inv_count(nn));
+  }
+  __syncthreads();
+}
+
+template <int DIM, typename T>
+__global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restrict__ rin, float* __restrict__ zout,
+                                               const double* shift, const int* flags) {
+  if (is_done(flags)) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* smem = reinterpret_cast<T*>(smem_raw);
+  const float m = shift ? (float)*shift : 0.f;
+  auto size_of = [&](int l) { return A.n[l][0] * A.n[l][1] * A.n[l][2]; };
+  {
+    T* r0 = smem + A.off[0];
+    T* z0 = r0 + size_of(0);
+    for (int id = threadIdx.x; id < size_of(0); id += blockDim.x) {
+      Lds<T>::st(r0, id, rin[id] - m);
+      Lds<T>::st(z0, id, 0.f);
     }
   }
-  for (int id = threadIdx.x; id < N; id += blockDim.x) z[id] = sz[id];
+  __syncthreads();
+  // down: smooth from zero, restrict
+  for (int l = 0; l < A.nl - 1; ++l) {
+    T* r = smem + A.off[l];
+    T* z = r + size_of(l);
+    const int* n = A.n[l];
+    for (int s = 0; s < A.pre; ++s) {
+      lds_half_sweep<DIM, T>(r, z, n, 0);
+      lds_half_sweep<DIM, T>(r, z, n, 1);
+    }
+    T* rc = smem + A.off[l + 1];
+    T* zc = rc + size_of(l + 1);
+    const int* nc = A.n[l + 1];
+    for (int cid = threadIdx.x; cid < size_of(l + 1); cid += blockDim.x) {
+      int ck = cid % nc[2], ct = cid / nc[2];
+      int cj = ct % nc[1], ci = ct / nc[1];
+      float acc = 0.f;
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          for (int c = 0; c < (DIM == 3 ? 2 : 1); ++c) {
+            int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
+            int id = (i * n[1] + j) * n[2] + k;
+            float s = 0.f, nn = 0.f;
+            if (i > 0) { s += Lds<T>::ld(z, id - n[1] * n[2]); nn += 1.f; }
+            if (i < n[0] - 1) { s += Lds<T>::ld(z, id + n[1] * n[2]); nn += 1.f; }
+            if (j > 0) { s += Lds<T>::ld(z, id - n[2]); nn += 1.f; }
+            if (j < n[1] - 1) { s += Lds<T>::ld(z, id + n[2]); nn += 1.f; }
+            if (DIM == 3) {
+              if (k > 0) { s += Lds<T>::ld(z, id - 1); nn += 1.f; }
+              if (k < n[2] - 1) { s += Lds<T>::ld(z, id + 1); nn += 1.f; }
+            }
+            acc += Lds<T>::ld(r, id) - (nn * Lds<T>::ld(z, id) - s);
+          }
+      Lds<T>::st(rc, cid, acc * A.inv_dm1);
+      Lds<T>::st(zc, cid, 0.f);
+    }
+    __syncthreads();
+  }
+  // bottom
+  {
+    int l = A.nl - 1;
+    T* r = smem + A.off[l];
+    T* z = r + size_of(l);
+    if (A.bottom_exact) {
+      lds_exact_solve<DIM, T>(r, z, A.n[l]);
+    } else {
+      for (int s = 0; s < A.bottom; ++s) {
+        lds_half_sweep<DIM, T>(r, z, A.n[l], 0);
+        lds_half_sweep<DIM, T>(r, z, A.n[l], 1);
+      }
+    }
+  }
+  // up: prolongate, post-smooth (colour 1 first)
+  for (int l = A.nl - 2; l >= 0; --l) {
+    T* r = smem + A.off[l];
+    T* z = r + size_of(l);
+    T* zc = smem + A.off[l + 1] + size_of(l + 1);
+    const int* n = A.n[l];
+    const int* nc = A.n[l + 1];
+    for (int id = threadIdx.x; id < size_of(l); id += blockDim.x) {
+      int k = id % n[2], t = id / n[2];
+      int j = t % n[1], i = t / n[1];
+      int cid = ((i >> 1) * nc[1] + (j >> 1)) * nc[2] + (DIM == 3 ? (k >> 1) : 0);
+      Lds<T>::st(z, id, Lds<T>::ld(z, id) + Lds<T>::ld(zc, cid));
+    }
+    __syncthreads();
+    for (int s = 0; s < A.pre; ++s) {
+      lds_half_sweep<DIM, T>(r, z, n, 1);
+      lds_half_sweep<DIM, T>(r, z, n, 0);
+    }
+  }
+  T* z0 = smem + A.off[0] + size_of(0);
+  for (int id = threadIdx.x; id < size_of(0); id += blockDim.x) zout[id] = Lds<T>::ld(z0, id);
 }
 
 // ------------------------------------------------------------- handle -----
@@ -537,7 +740,80 @@ struct Solver {
   int last_iters;
   double last_res;
   int check_every;
+  // coarse tail: levels tail_lt..L-1 run in one workgroup with LDS storage
+  int bf16_coarse;   // option: store levels >= 2 in BF16 inside the tail
+  int bottom_exact;  // option: exact (DCT) coarsest-level solve in the tail
+  int tail_bf16;     // the tail in use is BF16
+  int tail_lt;       // first tail level (L = no tail)
+  size_t tail_bytes; // dynamic LDS of the tail kernel
+  TailArgs tail;
+  float* z0_final;   // level-0 buffer holding M^-1 r after the last V-cycle
+  // captured PCG iterations
+  int use_graph;
+  hipStream_t cap;
+  hipGraphExec_t gexec;
+  int gexec_iters;
 };
+
+// Largest LDS a workgroup may request on this device.
+size_t max_lds_bytes() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) return 65536;
+  return (size_t)v;
+}
+
+// Pick the first tail level: the largest suffix of levels whose r and z fit
+// in LDS (BF16 storage allowed only for levels >= 2).
+void configure_tail(Solver& s, size_t cap) {
+  int best = s.L;
+  size_t best_bytes = 0;
+  for (int l = s.L - 1; l >= 0 && s.L - l <= kTailMax; --l) {
+    const size_t es = (s.bf16_coarse && l >= 2) ? 2 : 4;
+    size_t tot = 0;
+    for (int k = l; k < s.L; ++k) tot += 2 * (size_t)s.lv[k].size * es;
+    if (tot > cap) break;
+    best = l;
+    best_bytes = tot;
+  }
+  s.tail_lt = best;
+  s.tail_bytes = best_bytes;
+  s.tail_bf16 = (s.bf16_coarse && best >= 2) ? 1 : 0;
+  if (best >= s.L) return;
+  s.tail.nl = s.L - best;
+  int off = 0;
+  for (int k = 0; k < s.tail.nl; ++k) {
+    for (int a = 0; a < 3; ++a) s.tail.n[k][a] = s.lv[best + k].n[a];
+    s.tail.off[k] = off;
+    off += 2 * (int)s.lv[best + k].size;
+  }
+  s.tail.pre = s.pre;
+  s.tail.bottom = s.bottom;
+  s.tail.bottom_exact = s.bottom_exact;
+  s.tail.inv_dm1 = 1.0f / (float)(s.G.dim - 1);
+}
+
+// Dynamic-LDS opt-in for the tail instantiation this handle launches.
+bool enable_tail_lds(const Solver& s) {
+  if (s.tail_lt >= s.L) return true;
+  const void* fn = s.G.dim == 2 ? (s.tail_bf16 ? (const void*)k_tail<2, uint16_t> : (const void*)k_tail<2, float>)
+                                : (s.tail_bf16 ? (const void*)k_tail<3, uint16_t> : (const void*)k_tail<3, float>);
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.tail_bytes) == hipSuccess;
+}
+
+// Choose the tail for the current options; falls back to the 64 KB static
+// limit (then to no tail) if the runtime refuses the larger allocation.
+void setup_tail(Solver& s) {
+  configure_tail(s, max_lds_bytes());
+  if (!enable_tail_lds(s)) {
+    (void)hipGetLastError();
+    configure_tail(s, 65536);
+    if (!enable_tail_lds(s)) {
+      (void)hipGetLastError();
+      s.tail_lt = s.L;
+    }
+  }
+}
 
 inline Dims dims_of(const Lvl& l) { return Dims{{l.n[0], l.n[1], l.n[2]}}; }
 
@@ -553,56 +829,122 @@ void fin(Solver& s, hipStream_t st, int op, unsigned nparts, double tol = 0, int
                      rel);
 }
 
+// `sweeps` RBGS sweeps (colour c1 first, then the other) in place on lv[l].z,
+// starting from z = 0 when from_zero (first sweep fused into one kernel).
+// zc_prolong fuses the prolongation into the first half-sweep.  Returns lv[l].z.
 template <int DIM>
-void smooth_levels_global(Solver& s, hipStream_t st, int l, int sweeps, const double* shift) {
+float* sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero,
+                     const double* shift, const float* zc_prolong, Dims dc) {
   Lvl& L = s.lv[l];
   Dims d = dims_of(L);
   Dims none{{1, 1, 1}};
-  hipLaunchKernelGGL((k_first_sweep<DIM>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, L.z, shift, s.flags);
+  int k0 = 0;
+  if (from_zero) {
+    hipLaunchKernelGGL((k_first_sweep<DIM>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, L.z, shift, s.flags);
+    k0 = 1;
+  }
   int inner = DIM == 3 ? d.n[2] : d.n[1];
   int64_t outer = DIM == 3 ? (int64_t)d.n[0] * d.n[1] : d.n[0];
   unsigned gs = grid1(outer * ((inner + 1) / 2));
-  for (int it = 1; it < sweeps; ++it) {
-    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 0, shift, nullptr, none, s.flags);
-    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 1, shift, nullptr, none, s.flags);
+  for (int it = k0; it < sweeps; ++it) {
+    const float* zc = (it == k0 && zc_prolong) ? zc_prolong : nullptr;
+    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, c1, shift, zc, zc ? dc : none,
+                       s.flags);
+    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift, nullptr, none,
+                       s.flags);
   }
+  return L.z;
 }
 
-// apply_preconditioner (2D/vortex/mgpcg.py:182-200): z0 = M^-1 r0
+// apply_preconditioner (2D/vortex/mgpcg.py:182-200): z0 = M^-1 r0.  Levels
+// below tail_lt use global kernels; levels tail_lt.. run in one LDS kernel.
 template <int DIM>
 void vcycle(Solver& s, hipStream_t st, const double* shift0) {
   Dims none{{1, 1, 1}};
-  for (int l = 0; l < s.L - 1; ++l) {
+  float* cur[kMaxLevels] = {nullptr};
+  const bool tail = s.tail_lt < s.L;
+  const int Lg = tail ? s.tail_lt : s.L - 1;  // levels smoothed with global kernels before the bottom
+  for (int l = 0; l < Lg; ++l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    smooth_levels_global<DIM>(s, st, l, s.pre, sh);
+    cur[l] = sweeps_global<DIM>(s, st, l, s.pre, 0, true, sh, nullptr, none);
     Lvl& F = s.lv[l];
     Lvl& C = s.lv[l + 1];
-    hipLaunchKernelGGL((k_restrict<DIM>), dim3(grid1(C.size)), dim3(kB), 0, st, dims_of(F), F.r, F.z, dims_of(C), C.r,
-                       sh, s.flags);
+    hipLaunchKernelGGL((k_restrict<DIM>), dim3(grid1(C.size)), dim3(kB), 0, st, dims_of(F), F.r, cur[l], dims_of(C),
+                       C.r, sh, s.flags);
   }
-  Lvl& B = s.lv[s.L - 1];
-  const double* shb = s.L == 1 ? shift0 : nullptr;
-  if (B.size <= kLdsCells) {
-    hipLaunchKernelGGL((k_bottom<DIM>), dim3(1), dim3(1024), 0, st, dims_of(B), B.r, B.z, s.bottom, shb, s.flags);
+  if (tail) {
+    Lvl& T = s.lv[s.tail_lt];
+    const double* sh = s.tail_lt == 0 ? shift0 : nullptr;
+    if (s.tail_bf16)
+      hipLaunchKernelGGL((k_tail<DIM, uint16_t>), dim3(1), dim3(1024), s.tail_bytes, st, s.tail, T.r, T.z, sh, s.flags);
+    else
+      hipLaunchKernelGGL((k_tail<DIM, float>), dim3(1), dim3(1024), s.tail_bytes, st, s.tail, T.r, T.z, sh, s.flags);
+    cur[s.tail_lt] = T.z;
   } else {
-    smooth_levels_global<DIM>(s, st, s.L - 1, s.bottom, shb);
+    const double* sh = s.L == 1 ? shift0 : nullptr;
+    cur[s.L - 1] = sweeps_global<DIM>(s, st, s.L - 1, s.bottom, 0, true, sh, nullptr, none);
   }
-  for (int l = s.L - 2; l >= 0; --l) {
+  for (int l = Lg - 1; l >= 0; --l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    Lvl& F = s.lv[l];
-    Lvl& C = s.lv[l + 1];
-    Dims d = dims_of(F);
-    int inner = DIM == 3 ? d.n[2] : d.n[1];
-    int64_t outer = DIM == 3 ? (int64_t)d.n[0] * d.n[1] : d.n[0];
-    unsigned gs = grid1(outer * ((inner + 1) / 2));
-    // prolongate + smooth(l, 1), then smooth(l, 0); repeat
-    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, F.r, F.z, 1, sh, C.z, dims_of(C), s.flags);
-    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, F.r, F.z, 0, sh, nullptr, none, s.flags);
-    for (int it = 1; it < s.pre; ++it) {
-      hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, F.r, F.z, 1, sh, nullptr, none, s.flags);
-      hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, F.r, F.z, 0, sh, nullptr, none, s.flags);
-    }
+    cur[l] = sweeps_global<DIM>(s, st, l, s.pre, 1, false, sh, cur[l + 1], dims_of(s.lv[l + 1]));
   }
+  s.z0_final = cur[0];
+}
+
+// One PCG iteration once z, p and z.r are set (solve(), 2D/vortex/mgpcg.py:235-278):
+// Ap and p.Ap, alpha, x/r update and r.r (convergence flag), recentring
+// V-cycle, z.r and beta, p update.  Every kernel is a no-op once `done` is set,
+// so iterations may be queued past convergence.
+template <int DIM>
+void pcg_iteration(Solver& s, hipStream_t st) {
+  Lvl& L0 = s.lv[0];
+  int64_t N = L0.size;
+  unsigned gr = gridr(N);
+  Dims d0 = dims_of(L0);
+  hipLaunchKernelGGL((k_Ap_dot<DIM>), dim3(gr), dim3(kB), 0, st, d0, s.p, s.Ap, s.part, s.flags);
+  fin<DIM>(s, st, FIN_ALPHA, gr);
+  hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, s.p, s.Ap, s.sc, s.part, s.flags);
+  fin<DIM>(s, st, FIN_RR, gr);
+  vcycle<DIM>(s, st, s.sc + S_MEAN);
+  hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, s.z0_final, L0.r, (const double*)(s.sc + S_MEAN), s.part,
+                     s.flags, 0, (float*)nullptr);
+  fin<DIM>(s, st, FIN_BETA, gr);
+  hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, s.z0_final, s.p, s.sc, s.flags);
+}
+
+void drop_graph(Solver& s) {
+  if (s.gexec) (void)hipGraphExecDestroy(s.gexec);
+  s.gexec = nullptr;
+  s.gexec_iters = 0;
+}
+
+// Capture check_every PCG iterations into one executable graph (on a private
+// capture stream, default priority); replayed on the caller's stream.  The
+// graph holds kernel launches only -- every pointer it uses is owned by the
+// handle and stable -- so one capture serves every later solve.
+template <int DIM>
+bool build_graph(Solver& s) {
+  drop_graph(s);
+  if (!s.cap && hipStreamCreateWithFlags(&s.cap, hipStreamNonBlocking) != hipSuccess) {
+    s.cap = nullptr;
+    return false;
+  }
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(s.cap, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  for (int q = 0; q < s.check_every; ++q) pcg_iteration<DIM>(s, s.cap);
+  if (hipStreamEndCapture(s.cap, &g) != hipSuccess || !g) {
+    (void)hipGetLastError();
+    return false;
+  }
+  bool ok = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0) == hipSuccess;
+  (void)hipGraphDestroy(g);
+  if (!ok) {
+    s.gexec = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  s.gexec_iters = s.check_every;
+  return true;
 }
 
 // PCG from r0 (already in lv[0].r, mean sum in partials): solve(),
@@ -616,7 +958,7 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   hipLaunchKernelGGL(k_recenter_norm, dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, s.sc, s.part);
   fin<DIM>(s, st, FIN_BNORM, gr, tol, relative);
   vcycle<DIM>(s, st, nullptr);
-  hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, L0.z, L0.r, (const double*)nullptr, s.part, s.flags, 1, s.p);
+  hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, s.z0_final, L0.r, (const double*)nullptr, s.part, s.flags, 1, s.p);
   fin<DIM>(s, st, FIN_ZR_INIT, gr);
   Dims d0 = dims_of(L0);
   int it = 0;
@@ -626,14 +968,18 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
     return hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, st) == hipSuccess &&
            hipStreamSynchronize(st) == hipSuccess;
   };
+  bool use_graph = s.use_graph;
+  if (use_graph && (!s.gexec || s.gexec_iters != s.check_every)) use_graph = build_graph<DIM>(s);
   while (it < max_iters) {
-    hipLaunchKernelGGL((k_Ap_dot<DIM>), dim3(gr), dim3(kB), 0, st, d0, s.p, s.Ap, s.part, s.flags);
-    fin<DIM>(s, st, FIN_ALPHA, gr);
-    hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, s.p, s.Ap, s.sc, s.part, s.flags);
-    fin<DIM>(s, st, FIN_RR, gr);
-    ++it;
-    bool poll = (it % s.check_every) == 0 || it == max_iters;
-    if (poll) {
+    int k = s.check_every;
+    if (it + k > max_iters) k = max_iters - it;
+    if (use_graph && k == s.gexec_iters) {
+      if (hipGraphLaunch(s.gexec, st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: graph launch failed");
+    } else {
+      for (int q = 0; q < k; ++q) pcg_iteration<DIM>(s, st);
+    }
+    it += k;
+    {
       if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
       if (s.h_flags[F_DONE] && s.h_flags[F_STATUS] == 1 && refinements < s.max_refine) {
         // Converged on the recursive FP32 residual: measure the true residual
@@ -644,7 +990,7 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
         if (!s.h_flags[F_DONE]) {
           ++refinements;
           vcycle<DIM>(s, st, s.sc + S_MEAN);
-          hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, L0.z, L0.r, (const double*)(s.sc + S_MEAN), s.part,
+          hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, s.z0_final, L0.r, (const double*)(s.sc + S_MEAN), s.part,
                              s.flags, 1, s.p);
           fin<DIM>(s, st, FIN_ZR_INIT, gr);
           continue;
@@ -652,12 +998,6 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
       }
       if (s.h_flags[F_DONE]) { done = true; break; }
     }
-    if (it == max_iters) break;
-    vcycle<DIM>(s, st, s.sc + S_MEAN);
-    hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, L0.z, L0.r, (const double*)(s.sc + S_MEAN), s.part,
-                       s.flags, 0, (float*)nullptr);
-    fin<DIM>(s, st, FIN_BETA, gr);
-    hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, L0.z, s.p, s.sc, s.flags);
   }
   bool rb = hipMemcpyAsync(s.h_sc, s.sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st) == hipSuccess;
   if (!done) rb = rb && hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, st) == hipSuccess;
@@ -687,6 +1027,8 @@ void destroy(Solver* s) {
   if (s->flags) (void)hipFree(s->flags);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
   if (s->h_sc) (void)hipHostFree(s->h_sc);
+  drop_graph(*s);
+  if (s->cap) (void)hipStreamDestroy(s->cap);
   delete s;
 }
 
@@ -722,6 +1064,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->bottom = bottom;
   s->check_every = 1;
   s->max_refine = 3;
+  s->use_graph = 1;
   bool ok = true;
   for (int l = 0; l < levels && ok; ++l) {
     for (int a = 0; a < 3; ++a) s->lv[l].n[a] = a < g->dim ? (g->n[a] >> l) : 1;
@@ -746,6 +1089,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
     destroy(s);
     return set_error(DFM_E_HIP, "dfm_mgpcg_create: device allocation failed");
   }
+  setup_tail(*s);
   *handle = s;
   return DFM_OK;
 }
@@ -753,6 +1097,57 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
 extern "C" int dfm_mgpcg_destroy(void* handle) {
   destroy(static_cast<Solver*>(handle));
   return DFM_OK;
+}
+
+extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
+  Solver* s = static_cast<Solver*>(handle);
+  if (!s) return set_error(DFM_E_NULL, "mgpcg handle is NULL");
+  switch (option) {
+    case DFM_MG_BF16_COARSE:
+      s->bf16_coarse = value ? 1 : 0;
+      setup_tail(*s);
+      drop_graph(*s);
+      return DFM_OK;
+    case DFM_MG_TAIL:
+      if (value) setup_tail(*s);
+      else s->tail_lt = s->L;
+      drop_graph(*s);
+      return DFM_OK;
+    case DFM_MG_GRAPH:
+      s->use_graph = value ? 1 : 0;
+      drop_graph(*s);
+      return DFM_OK;
+    case DFM_MG_BOTTOM_EXACT:
+      s->bottom_exact = value ? 1 : 0;
+      setup_tail(*s);
+      drop_graph(*s);
+      return DFM_OK;
+    case DFM_MG_MAX_REFINE:
+      if (value < 0) return set_error(DFM_E_ARG, "max_refine must be >= 0");
+      s->max_refine = value;
+      return DFM_OK;
+    case DFM_MG_CHECK_EVERY:
+      if (value < 1) return set_error(DFM_E_ARG, "check_every must be >= 1");
+      s->check_every = value;
+      drop_graph(*s);
+      return DFM_OK;
+    default:
+      return set_error(DFM_E_ARG, "unknown MGPCG option");
+  }
+}
+
+extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
+  Solver* s = static_cast<Solver*>(handle);
+  if (!s || !value) return set_error(DFM_E_NULL, "dfm_mgpcg_get_option: NULL argument");
+  switch (option) {
+    case DFM_MG_BF16_COARSE: *value = s->tail_bf16; return DFM_OK;
+    case DFM_MG_TAIL: *value = s->tail_lt; return DFM_OK;
+    case DFM_MG_MAX_REFINE: *value = s->max_refine; return DFM_OK;
+    case DFM_MG_CHECK_EVERY: *value = s->check_every; return DFM_OK;
+    case DFM_MG_GRAPH: *value = s->gexec ? 2 : s->use_graph; return DFM_OK;
+    case DFM_MG_BOTTOM_EXACT: *value = (s->tail_lt < s->L) ? s->bottom_exact : 0; return DFM_OK;
+    default: return set_error(DFM_E_ARG, "unknown MGPCG option");
+  }
 }
 
 extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int relative, int max_iters,

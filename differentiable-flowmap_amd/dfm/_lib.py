@@ -57,7 +57,16 @@ SIGNATURES = {
     "dfm_mgpcg_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_int, c_void_p]),
     "dfm_mgpcg_status": (c_int, [c_void_p, P(c_int), P(c_double)]),
     "dfm_mgpcg_pressure": (c_int, [c_void_p, P(c_void_p)]),
+    "dfm_mgpcg_set_option": (c_int, [c_void_p, c_int, c_int]),
+    "dfm_mgpcg_get_option": (c_int, [c_void_p, c_int, P(c_int)]),
 }
+
+MG_BF16_COARSE = 1
+MG_TAIL = 2
+MG_MAX_REFINE = 3
+MG_CHECK_EVERY = 4
+MG_GRAPH = 5
+MG_BOTTOM_EXACT = 6
 
 DFM_E_NOCONV = -4
 

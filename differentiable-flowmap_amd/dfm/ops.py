@@ -149,7 +149,8 @@ class MGPCG:
     3D/fast_mgpcg.py:241-323), backed by the libdfm MGPCG handle."""
 
     def __init__(self, n, dx, levels=0, pre_post=2, bottom=50, tol=1e-6, relative=True, max_iters=400,
-                 raise_on_noconv=False):
+                 raise_on_noconv=False, bf16_coarse=False, tail=True, graph=True, check_every=1,
+                 bottom_exact=False):
         self.n = tuple(n)
         self.g = L.make_grid(n, dx)
         self.lib = L.lib()
@@ -157,6 +158,16 @@ class MGPCG:
         L.check(self.lib.dfm_mgpcg_create(self.g, int(levels), int(pre_post), int(bottom), L.ctypes.byref(h)),
                 "mgpcg_create")
         self.h = h
+        if not tail:
+            self.set_option(L.MG_TAIL, 0)
+        if bf16_coarse:
+            self.set_option(L.MG_BF16_COARSE, 1)
+        if not graph:
+            self.set_option(L.MG_GRAPH, 0)
+        if check_every != 1:
+            self.set_option(L.MG_CHECK_EVERY, check_every)
+        if bottom_exact:
+            self.set_option(L.MG_BOTTOM_EXACT, 1)
         self.tol, self.relative, self.max_iters = tol, relative, max_iters
         self.raise_on_noconv = raise_on_noconv
         self.last_iters = 0
@@ -170,6 +181,14 @@ class MGPCG:
             except Exception:
                 pass
             self.h = None
+
+    def set_option(self, option, value):
+        L.check(self.lib.dfm_mgpcg_set_option(self.h, int(option), int(value)), "mgpcg_set_option")
+
+    def get_option(self, option):
+        v = L.c_int(0)
+        L.check(self.lib.dfm_mgpcg_get_option(self.h, int(option), L.ctypes.byref(v)), "mgpcg_get_option")
+        return v.value
 
     def status(self):
         it = L.c_int(0)

@@ -103,7 +103,8 @@ class LFM_Simulator:
     2D/vortex/lfm_midpoint_simulator.py:187-661, 3D/lfm_midpoint_simulator.py:167-807)."""
 
     def __init__(self, *args, order=None, total_steps=None, n_passive=1, adj_kernel=None, mg_levels=0,
-                 mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda"):
+                 mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda",
+                 mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -124,7 +125,8 @@ class LFM_Simulator:
             raise RuntimeError("dfm.LFM_Simulator needs a HIP device (torch.cuda.is_available() is False)")
         self.ops = Ops(n, dx)
         self.solver = MGPCG(n, dx, levels=mg_levels, pre_post=mg_pre_post, bottom=mg_bottom, tol=tol,
-                            max_iters=max_iters)
+                            max_iters=max_iters, bf16_coarse=mg_bf16_coarse, tail=mg_tail,
+                            bottom_exact=mg_bottom_exact, check_every=mg_check_every)
         dev = self.device
         self.u = MacField.empty(n, dev, zero=True)
         self._u_new = MacField.empty(n, dev, zero=True)

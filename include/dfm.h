@@ -155,6 +155,28 @@ int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
 /* Device pointer of the pressure (cells, FP64) from the last projection. */
 int dfm_mgpcg_pressure(void* handle, double** p);
 
+/* Solver options (no counterpart in the reference; tuning of this build).
+ *  DFM_MG_BF16_COARSE: 1 = store levels >= 2 in BF16 (FP32 arithmetic) so
+ *      the coarse V-cycle (up to 32^3) runs in one workgroup's LDS; 0 = FP32.
+ *  DFM_MG_TAIL: 1 = run the coarse levels that fit in LDS in one kernel
+ *      (default); 0 = one launch per sweep on every level.  get: first tail level.
+ *  DFM_MG_MAX_REFINE: FP64 iterative-refinement restarts (default 3).
+ *  DFM_MG_CHECK_EVERY: PCG iterations between convergence polls (default 1).
+ *  DFM_MG_GRAPH: 1 = replay the check_every PCG iterations between polls as
+ *      one captured hipGraph (default); 0 = launch each kernel.  get: 2 when a
+ *      graph is instantiated.
+ *  DFM_MG_BOTTOM_EXACT: 1 = solve the coarsest level exactly (orthonormal
+ *      DCT-II in LDS, mean-zero) instead of `bottom` RBGS sweeps; applies when
+ *      the coarse tail runs.  Changes only the preconditioner. */
+#define DFM_MG_BF16_COARSE 1
+#define DFM_MG_TAIL 2
+#define DFM_MG_MAX_REFINE 3
+#define DFM_MG_CHECK_EVERY 4
+#define DFM_MG_GRAPH 5
+#define DFM_MG_BOTTOM_EXACT 6
+int dfm_mgpcg_set_option(void* handle, int option, int value);
+int dfm_mgpcg_get_option(void* handle, int option, int* value);
+
 #ifdef __cplusplus
 }
 #endif

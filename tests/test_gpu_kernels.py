@@ -247,6 +247,59 @@ def test_mgpcg_projection_matches_exact(n, levels, bound, device):
     assert rel_l2_fields(mac_to_np(d), out) < 5e-6
 
 
+@pytest.mark.parametrize("n,levels", [((128, 128, 128), 5), ((64, 64, 64), 4), ((256, 256), 6), ((16, 16, 16), 2)])
+def test_mgpcg_tail_and_bf16_modes(n, levels, device):
+    """The one-workgroup LDS tail (FP32) reproduces the per-level launch
+    sequence; BF16 storage of levels >= 2 still converges to the exact
+    projection (it only changes the preconditioner)."""
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    dx = 1.0 / n[1]
+    rng = np.random.default_rng(5 + n[0])
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(len(n))])
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    out = {}
+    for name, kw in [("notail", dict(tail=False)), ("tail", dict()), ("bf16", dict(bf16_coarse=True)),
+                     ("exact", dict(bottom_exact=True)), ("bf16exact", dict(bf16_coarse=True, bottom_exact=True))]:
+        s = MGPCG(n, dx, levels=levels, tol=1e-7, max_iters=100, **kw)
+        d = mac_to_dev(u, device)
+        s.Poisson(d)
+        it, res = s.status()
+        out[name] = (mac_to_np(d), it, s.get_option(L.MG_TAIL), s.get_option(L.MG_BF16_COARSE))
+        print(f"n={n} {name}: iters={it} res={res:.2e} tail_level={out[name][2]} bf16={out[name][3]} "
+              f"err={rel_l2_fields(out[name][0], ref):.2e}")
+        assert rel_l2_fields(out[name][0], ref) < 2e-6
+    assert out["notail"][2] == levels
+    assert abs(out["tail"][1] - out["notail"][1]) <= 1
+    assert rel_l2_fields(out["tail"][0], out["notail"][0]) < 1e-6
+
+
+@pytest.mark.parametrize("check_every", [1, 2])
+def test_mgpcg_graph_replays_match_eager(check_every, device):
+    """The captured PCG-iteration graph, replayed across several solves with
+    eager work and host syncs in between, gives the eager launches' results
+    on every solve."""
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    n = (64, 64, 32)
+    dx = 1 / 64
+    g = MGPCG(n, dx, tol=1e-6, check_every=check_every)
+    e = MGPCG(n, dx, tol=1e-6, graph=False, check_every=check_every)
+    rng = np.random.default_rng(31)
+    for rep in range(4):
+        u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(3)])
+        dg, de = mac_to_dev(u, device), mac_to_dev(u, device)
+        g.Poisson(dg)
+        torch.cuda.synchronize()
+        e.Poisson(de)
+        assert g.get_option(L.MG_GRAPH) == 2
+        assert g.status()[0] == e.status()[0]
+        assert rel_l2_fields(mac_to_np(dg), mac_to_np(de)) < 1e-6, rep
+
+
 def test_mgpcg_solve_and_status(device):
     from dfm.ops import MGPCG
 
