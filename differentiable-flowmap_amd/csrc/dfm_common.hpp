@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "dfm_fastdiv.hpp"
+
 #define DFM_HD __device__ __forceinline__
 
 namespace dfm {

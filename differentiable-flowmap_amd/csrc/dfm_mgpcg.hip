@@ -10,17 +10,20 @@
 //
 // MI355X mapping:
 //  * PCG scalars (alpha, beta, r.r, z.r, mean) live on the device in FP64 and
-//    are produced by deterministic two-stage reductions (per-block partials +
-//    one finalize block), so no per-iteration host round trip is needed to
-//    compute them; a device "done" flag turns every later kernel of the solve
-//    into a no-op, so the host only polls the flag.
+//    are produced by deterministic grid reductions inside the kernels that
+//    compute the summands: every block stores its partial sums, and the last
+//    block to arrive adds them in block order and runs the scalar step of the
+//    recurrence.  No separate finalize launch and no per-iteration host round
+//    trip; a device "done" flag turns every later kernel of the solve into a
+//    no-op, so the host only polls the flag.
 //  * Fusions: divergence + apply_bc + partial sums; the first pre-smoothing
 //    sweep from z = 0 is one kernel (black cells read neighbour r/nn directly);
 //    prolongation is fused into the first post-smoothing half-sweep (the
 //    colour whose value is overwritten next is never materialised); the
 //    recentring of r is applied on the fly by the level-0 V-cycle kernels and
-//    written back by the z.r reduction; the coarsest level is solved by one
-//    workgroup in LDS.
+//    written back, together with the z.r reduction and beta, by the last
+//    level-0 half-sweep; the coarse levels run in one LDS-resident workgroup.
+//  * Cell indices are decomposed with invariant-divisor multiplications.
 #include "dfm_common.hpp"
 #include "dfm_internal.hpp"
 #include "../../include/dfm.h"
@@ -34,13 +37,16 @@ using namespace dfm;
 namespace {
 
 constexpr int kB = 256;          // threads per block
-constexpr int kRedBlocks = 1024; // blocks of the reduction kernels
+constexpr int kRedBlocks = 4096; // most blocks of a reducing kernel (grid-stride beyond)
+constexpr unsigned kCtrGroups = 64;  // arrival-counter groups of a grid reduction
+constexpr unsigned kCtrStride = 32;  // counters 128 B apart
+constexpr unsigned kCtrWords = (kCtrGroups + 1) * kCtrStride;
 constexpr int kMaxLevels = 12;
 constexpr int kLdsCells = 8192;  // bottom solve in one workgroup up to this size
 constexpr double kEps = 1e-12;   // denominator guard (2D/vortex/mgpcg.py:204)
 
 enum Slot { S_ZR = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THRESH, S_MEAN, S_PAP, S_NSLOT };
-enum Flag { F_DONE = 0, F_ITERS, F_STATUS, F_NFLAG };
+enum Flag { F_DONE = 0, F_ITERS, F_STATUS, F_NTRUE, F_NFLAG };  // F_NTRUE: failed true-residual checks
 
 struct Lvl {
   int n[3];
@@ -51,9 +57,27 @@ struct Lvl {
 
 struct Dims {
   int n[3];
+  FastDiv f2;  // / n[2]
+  FastDiv f1;  // / n[1]
+  FastDiv fh;  // / cell pairs along the innermost axis ((inner + 1) / 2)
 };
 
-DFM_HD int64_t cidx(const Dims& d, int i, int j, int k) { return ((int64_t)i * d.n[1] + j) * d.n[2] + k; }
+inline Dims make_dims(int n0, int n1, int n2, int dim) {
+  Dims d;
+  d.n[0] = n0;
+  d.n[1] = n1;
+  d.n[2] = n2;
+  d.f2 = make_fastdiv((uint32_t)n2);
+  d.f1 = make_fastdiv((uint32_t)n1);
+  const int inner = dim == 3 ? n2 : n1;
+  d.fh = make_fastdiv((uint32_t)((inner + 1) >> 1));
+  return d;
+}
+
+// 32-bit: every grid (faces included) stays below 2^31 points (checked at creation)
+DFM_HD uint32_t cidx(const Dims& d, int i, int j, int k) {
+  return ((uint32_t)i * (uint32_t)d.n[1] + (uint32_t)j) * (uint32_t)d.n[2] + (uint32_t)k;
+}
 
 // neighbour sum and count (num_fluid_neighbors / neighbor_sum, 2D/vortex/mgpcg.py:92-115)
 template <int DIM>
@@ -79,15 +103,13 @@ DFM_HD float nbr_count(const Dims& d, int i, int j, int k) {
   return nn;
 }
 
-// Cell index -> (i, j, k).  Grids here stay below 2^32 cells, so 32-bit
-// unsigned division is used (64-bit division costs ~5x more on the VALU).
-DFM_HD void cell_of(const Dims& d, int64_t id, int& i, int& j, int& k) {
+// Cell index -> (i, j, k); grids stay below 2^31 cells (checked at creation).
+DFM_HD void cell_of(const Dims& d, uint32_t id, int& i, int& j, int& k) {
   uint32_t u = (uint32_t)id;
-  uint32_t n2 = (uint32_t)d.n[2], n1 = (uint32_t)d.n[1];
-  uint32_t t = u / n2;
-  k = (int)(u - t * n2);
-  uint32_t q = t / n1;
-  j = (int)(t - q * n1);
+  uint32_t t = fdiv(u, d.f2);
+  k = (int)(u - t * (uint32_t)d.n[2]);
+  uint32_t q = fdiv(t, d.f1);
+  j = (int)(t - q * (uint32_t)d.n[1]);
   i = (int)q;
 }
 
@@ -126,29 +148,35 @@ DFM_HD void block_sum2(double& a, double& b) {
   }
 }
 
-// Finalize: sum partials and run the scalar step of the PCG recurrence.
+// Scalar steps of the PCG recurrence, run on the grid totals (a, b).
 enum FinOp { FIN_MEAN = 0, FIN_BNORM, FIN_ZR_INIT, FIN_ALPHA, FIN_RR, FIN_BETA, FIN_TRUE };
 
-__global__ void __launch_bounds__(kB) k_finalize(int op, const double* __restrict__ part, int nparts, double* sc,
-                                                 int* flags, int64_t ncells, double tol, int relative) {
-  if (op != FIN_MEAN && op != FIN_BNORM && op != FIN_TRUE && is_done(flags)) return;
-  double a = 0, b = 0;
-  for (int t = threadIdx.x; t < nparts; t += kB) {
-    a += part[t];
-    b += part[kRedBlocks + t];
-  }
-  block_sum2(a, b);
-  if (threadIdx.x != 0) return;
+// Grid-reduction context: partial sums (2 x kRedBlocks), the arrival counters
+// (zero between reductions), the FP64 scalars and the solve flags.
+struct Red {
+  double* part;
+  unsigned* ctr;  // kCtrWords arrival counters, zero between reductions
+  double* sc;
+  int* flags;
+  double inv_n;  // 1 / fine cells
+  double tol;    // FIN_BNORM only
+  int relative;  // FIN_BNORM only
+};
+
+DFM_HD void fin_op(const Red& R, int op, double a, double b) {
+  double* sc = R.sc;
+  int* flags = R.flags;
   switch (op) {
     case FIN_MEAN:  // a = sum r
-      sc[S_MEAN] = a / (double)ncells;
+      sc[S_MEAN] = a * R.inv_n;
       break;
     case FIN_BNORM:  // a = b.b (after recentring)
       sc[S_BB] = a;
       sc[S_RR] = a;
-      sc[S_THRESH] = relative ? tol * tol * a : tol;
+      sc[S_THRESH] = R.relative ? R.tol * R.tol * a : R.tol;
       flags[F_ITERS] = 0;
       flags[F_STATUS] = 0;
+      flags[F_NTRUE] = 0;
       flags[F_DONE] = (a == 0.0) ? 1 : 0;
       break;
     case FIN_ZR_INIT:  // a = z.r
@@ -160,7 +188,7 @@ __global__ void __launch_bounds__(kB) k_finalize(int op, const double* __restric
       break;
     case FIN_RR: {  // a = r.r, b = sum r
       sc[S_RR] = a;
-      sc[S_MEAN] = b / (double)ncells;
+      sc[S_MEAN] = b * R.inv_n;
       flags[F_ITERS] += 1;
       if (!isfinite(a)) {
         flags[F_STATUS] = 2;
@@ -178,7 +206,7 @@ __global__ void __launch_bounds__(kB) k_finalize(int op, const double* __restric
     }
     case FIN_TRUE: {  // a = |b - A x|^2 in FP64, b = sum(b - A x): refinement check
       sc[S_RR] = a;
-      sc[S_MEAN] = b / (double)ncells;
+      sc[S_MEAN] = b * R.inv_n;
       if (!isfinite(a)) {
         flags[F_STATUS] = 2;
         flags[F_DONE] = 1;
@@ -188,18 +216,79 @@ __global__ void __launch_bounds__(kB) k_finalize(int op, const double* __restric
       } else {
         flags[F_STATUS] = 0;
         flags[F_DONE] = 0;
+        flags[F_NTRUE] += 1;
       }
       break;
     }
   }
 }
 
-DFM_HD void store_partials(double* part, double a, double b) {
+// Deterministic grid reduction of (a, b) followed by fin_op: every block
+// stores its block sums and counts itself in; the last block to arrive adds
+// the partials in block order and runs the scalar step, then returns the
+// counter to zero for the next reduction on the stream.  Called by every
+// thread of every block (no thread may have returned).  Grids of reducing
+// kernels hold at most kRedBlocks blocks.
+//
+// Publication without a device-scope fence: a device-scope release fence
+// writes back the whole (per-XCD, mutually non-coherent) L2 and costs
+// ~100 us over a few thousand blocks.  Only the two partials need publishing,
+// so they are stored as device-scope atomics (coherent stores that bypass the
+// non-coherent L2 state), the store is waited for before the counter
+// increment, and the last block reads them back as device-scope atomic loads.
+// Everything else a reducing kernel writes is consumed by later kernels.
+// Arrivals are counted in two levels (kCtrGroups group counters on separate
+// cache lines, then one top counter) because increments of one address
+// serialise (~10 ns each): a single counter would add ~40 us to a
+// 4096-block reduction.
+DFM_HD void wait_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+DFM_HD void grid_reduce(const Red& R, int op, double a, double b) {
+  __shared__ int s_last;
   block_sum2(a, b);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = a;
-    part[kRedBlocks + blockIdx.x] = b;
+    __hip_atomic_store(&R.part[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&R.part[kRedBlocks + blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    wait_stores();  // both stores complete before the increment is issued
+    const unsigned G = gridDim.x, g = blockIdx.x % kCtrGroups;
+    const unsigned ngroups = G < kCtrGroups ? G : kCtrGroups;
+    const unsigned members = G / kCtrGroups + (g < G % kCtrGroups ? 1u : 0u);
+    unsigned* gc = R.ctr + g * kCtrStride;
+    unsigned* top = R.ctr + kCtrGroups * kCtrStride;
+    int last = 0;
+    if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+      __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      wait_stores();
+      last = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1;
+      if (last) __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_last = last;
   }
+  __syncthreads();
+  if (!s_last) return;
+  // all of a thread's partials are requested before any is summed (a rolled
+  // loop would wait one memory latency per element)
+  constexpr int U = kRedBlocks / kB;
+  double va[U], vb[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const unsigned t = threadIdx.x + u * kB;
+    va[u] = t < gridDim.x ? __hip_atomic_load(&R.part[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+    vb[u] = t < gridDim.x ? __hip_atomic_load(&R.part[kRedBlocks + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : 0.0;
+  }
+  double ta = 0, tb = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    ta += va[u];
+    tb += vb[u];
+  }
+  block_sum2(ta, tb);
+  if (threadIdx.x == 0) fin_op(R, op, ta, tb);
 }
 
 // ------------------------------------------------------ projection ends ----
@@ -210,10 +299,10 @@ struct Faces {
 // apply_bc + divergence (no 1/dx) + b = -div + partial sum of b
 // (apply_bc/divergence, 2D/vortex/mgpcg.py:294-336; 3D/mgpcg.py:380-411).
 template <int DIM>
-__global__ void __launch_bounds__(kB) k_div(Dims d, Faces u, float* __restrict__ b, double* part) {
+__global__ void __launch_bounds__(kB) k_div(Dims d, Faces u, float* __restrict__ b, Red R) {
   double acc = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     int i, j, k;
     cell_of(d, id, i, j, k);
     // face grids: x (n0+1, n1, n2), y (n0, n1+1, n2), z (n0, n1, n2+1)
@@ -239,47 +328,47 @@ __global__ void __launch_bounds__(kB) k_div(Dims d, Faces u, float* __restrict__
     b[id] = bv;
     acc += (double)bv;
   }
-  store_partials(part, acc, 0.0);
+  grid_reduce(R, FIN_MEAN, acc, 0.0);
 }
 
 // copy b into r (for dfm_mgpcg_solve) with partial sum
 __global__ void __launch_bounds__(kB) k_load_b(int64_t N, const float* __restrict__ b, float* __restrict__ r,
-                                               double* part) {
+                                               Red R) {
   double acc = 0;
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     float v = b[id];
     r[id] = v;
     acc += (double)v;
   }
-  store_partials(part, acc, 0.0);
+  grid_reduce(R, FIN_MEAN, acc, 0.0);
 }
 
 // r -= mean (kept as b for the refinement residual); partial r.r; x = 0
 __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restrict__ r, float* __restrict__ bkeep,
-                                                      double* __restrict__ x, const double* sc, double* part) {
-  float m = (float)sc[S_MEAN];
+                                                      double* __restrict__ x, Red R) {
+  float m = (float)R.sc[S_MEAN];
   double acc = 0;
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     float v = r[id] - m;
     r[id] = v;
     bkeep[id] = v;
     x[id] = 0.0;
     acc += (double)v * (double)v;
   }
-  store_partials(part, acc, 0.0);
+  grid_reduce(R, FIN_BNORM, acc, 0.0);
 }
 
 // subtract_grad_p + apply_bc (2D/vortex/mgpcg.py:338-356, 3D/mgpcg.py:413-439):
 // interior faces u_a -= p[I] - p[I - e_a]; boundary faces = 0.
+struct FaceDims {
+  Dims f[3];  // face grid of axis a (cells + e_a)
+};
+
 template <int DIM>
-__global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const double* __restrict__ p) {
+__global__ void __launch_bounds__(kB) k_subgrad(Dims d, FaceDims fdims, Faces u, const double* __restrict__ p) {
   int64_t nf[3];
-  Dims fd[3];
-  for (int a = 0; a < 3; ++a) {
-    fd[a] = d;
-    if (a < DIM) fd[a].n[a] += 1;
-    nf[a] = (a < DIM) ? (int64_t)fd[a].n[0] * fd[a].n[1] * fd[a].n[2] : 0;
-  }
+  const Dims* fd = fdims.f;
+  for (int a = 0; a < 3; ++a) nf[a] = (a < DIM) ? (int64_t)fd[a].n[0] * fd[a].n[1] * fd[a].n[2] : 0;
   int64_t total = nf[0] + nf[1] + nf[2];
   for (int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x; g < total; g += (int64_t)gridDim.x * kB) {
     int a = 0;
@@ -305,10 +394,13 @@ __global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const double* _
 // b is the recentred FP32 right-hand side); partial |r|^2 and sum r.
 template <int DIM>
 __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restrict__ x, const float* __restrict__ b,
-                                                   float* __restrict__ r, double* part) {
+                                                   float* __restrict__ r, Red R, int if_converged) {
+  // only after convergence on the recursive residual (uniform over the grid:
+  // the flags change only in the last block, after every block has read them)
+  if (if_converged && !(R.flags[F_DONE] && R.flags[F_STATUS] == 1)) return;
   double rr = 0, rs = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     int i, j, k;
     cell_of(d, id, i, j, k);
     double s = 0.0, nn = 0.0;
@@ -325,23 +417,23 @@ __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restr
     rr += v * v;
     rs += v;
   }
-  store_partials(part, rr, rs);
+  grid_reduce(R, FIN_TRUE, rr, rs);
 }
 
 // FP64 solution -> FP32 output
 __global__ void __launch_bounds__(kB) k_to_float(int64_t N, const double* __restrict__ x, float* __restrict__ y) {
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) y[id] = (float)x[id];
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) y[id] = (float)x[id];
 }
 
 // ------------------------------------------------------------ PCG steps ----
 // Ap = A p ; partial p.Ap
 template <int DIM>
 __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__ p, float* __restrict__ Ap,
-                                               double* part, const int* flags) {
-  if (is_done(flags)) return;
+                                               Red R) {
+  if (is_done(R.flags)) return;
   double acc = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     int i, j, k;
     cell_of(d, id, i, j, k);
     float nn;
@@ -351,42 +443,43 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
     Ap[id] = v;
     acc += (double)pv * (double)v;
   }
-  store_partials(part, acc, 0.0);
+  grid_reduce(R, FIN_ALPHA, acc, 0.0);
 }
 
 // x += alpha p (x in FP64) ; r -= alpha Ap ; partial r.r and sum r
 __global__ void __launch_bounds__(kB) k_update_xr(int64_t N, double* __restrict__ x, float* __restrict__ r,
                                                   const float* __restrict__ p, const float* __restrict__ Ap,
-                                                  const double* sc, double* part, const int* flags) {
-  if (is_done(flags)) return;
-  const double alpha_d = sc[S_ALPHA];
+                                                  Red R) {
+  if (is_done(R.flags)) return;
+  const double alpha_d = R.sc[S_ALPHA];
   float alpha = (float)alpha_d;
   double rr = 0, rs = 0;
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     x[id] = fma(alpha_d, (double)p[id], x[id]);
     float v = fmaf(-alpha, Ap[id], r[id]);
     r[id] = v;
     rr += (double)v * (double)v;
     rs += (double)v;
   }
-  store_partials(part, rr, rs);
+  grid_reduce(R, FIN_RR, rr, rs);
 }
 
-// r -= shift (written back) ; partial z.r   (shift = 0 for the first pass)
+// r -= shift (written back) ; z.r and its scalar step `op` (FIN_ZR_INIT or
+// FIN_BETA); p = z when init_p.  Used when level 0 is not smoothed by the
+// global kernels (otherwise the last level-0 half-sweep does this).
 __global__ void __launch_bounds__(kB) k_zr(int64_t N, const float* __restrict__ z, float* __restrict__ r,
-                                           const double* shift, double* part, const int* flags, int init_p,
-                                           float* __restrict__ p) {
-  if (is_done(flags)) return;
+                                           const double* shift, Red R, int op, int init_p, float* __restrict__ p) {
+  if (is_done(R.flags)) return;
   float m = shift ? (float)*shift : 0.f;
   double acc = 0;
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB) {
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     float rv = r[id] - m;
     if (shift) r[id] = rv;
     float zv = z[id];
     if (init_p) p[id] = zv;
     acc += (double)zv * (double)rv;
   }
-  store_partials(part, acc, 0.0);
+  grid_reduce(R, op, acc, 0.0);
 }
 
 // p = z + beta p
@@ -394,7 +487,7 @@ __global__ void __launch_bounds__(kB) k_update_p(int64_t N, const float* __restr
                                                  const double* sc, const int* flags) {
   if (is_done(flags)) return;
   float beta = (float)sc[S_BETA];
-  for (int64_t id = (int64_t)blockIdx.x * kB + threadIdx.x; id < N; id += (int64_t)gridDim.x * kB)
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB)
     p[id] = fmaf(beta, p[id], z[id]);
 }
 
@@ -407,7 +500,7 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
   if (is_done(flags)) return;
   float m = shift ? (float)*shift : 0.f;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
-  int64_t id = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  uint32_t id = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (id >= N) return;
   int i, j, k;
   cell_of(d, id, i, j, k);
@@ -429,44 +522,51 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
   z[id] = (rv + s) * inv_count(nn);
 }
 
-// Colour-`phase` cell of pair index t (pairs run along the innermost axis);
-// false when the pair's cell of that colour falls outside an odd-sized row.
+// Pair t of cells along the innermost axis (cells 2h and 2h + 1 of a row):
+// (i, j) of the row, the innermost index `own` of its colour-`phase` cell and
+// `other` of the other cell.  Either may equal the row length (outside an
+// odd-sized row).
 template <int DIM>
-DFM_HD bool colour_cell(const Dims& d, uint32_t t, int phase, int& i, int& j, int& k) {
-  const uint32_t inner = DIM == 3 ? d.n[2] : d.n[1];
-  const uint32_t half = (inner + 1) >> 1;
-  uint32_t o = t / half;
-  int hm = (int)(t - o * half);
+DFM_HD void colour_pair(const Dims& d, uint32_t t, int phase, int& i, int& j, int& own, int& other) {
+  const uint32_t half = d.fh.d;
+  uint32_t o = fdiv(t, d.fh);
+  int base = 2 * (int)(t - o * half);
+  int off;
   if (DIM == 3) {
-    uint32_t q = o / (uint32_t)d.n[1];
+    uint32_t q = fdiv(o, d.f1);
     i = (int)q;
     j = (int)(o - q * (uint32_t)d.n[1]);
-    k = 2 * hm + ((phase - i - j) & 1);
+    off = (phase - i - j) & 1;
+  } else {
+    i = (int)o;
+    j = 0;
+    off = (phase - i) & 1;
+  }
+  own = base + off;
+  other = base + 1 - off;
+}
+
+// Colour-`phase` cell of pair index t; false when it falls outside an
+// odd-sized row.
+template <int DIM>
+DFM_HD bool colour_cell(const Dims& d, uint32_t t, int phase, int& i, int& j, int& k) {
+  int own, other;
+  colour_pair<DIM>(d, t, phase, i, j, own, other);
+  if (DIM == 3) {
+    k = own;
     return k < d.n[2];
   }
-  i = (int)o;
+  j = own;
   k = 0;
-  j = 2 * hm + ((phase - i) & 1);
   return j < d.n[1];
 }
 
-// One half-sweep of colour `phase`; threads cover only that colour (the
-// innermost axis is walked in steps of two).  If zc is given, the
-// neighbours' values are read as z + zc[coarse parent] (fused prolongation).
+// Gauss-Seidel update of cell (i, j, k): (r - m + sum of neighbour z) / nn.
+// With zc, the neighbours' values are read as z + zc[coarse parent] (fused
+// prolongation; the cell's own value is overwritten, so it never needs it).
 template <int DIM>
-__global__ void __launch_bounds__(kB) k_smooth(Dims d, const float* __restrict__ r, float* __restrict__ z, int phase,
-                                               const double* shift, const float* __restrict__ zc, Dims dc,
-                                               const int* flags) {
-  if (is_done(flags)) return;
-  float m = shift ? (float)*shift : 0.f;
-  const int inner = DIM == 3 ? d.n[2] : d.n[1];
-  const int half = (inner + 1) >> 1;
-  const int64_t outer = DIM == 3 ? (int64_t)d.n[0] * d.n[1] : d.n[0];
-  int64_t t = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-  if (t >= outer * half) return;
-  int i, j, k;
-  if (!colour_cell<DIM>(d, (uint32_t)t, phase, i, j, k)) return;
-  int64_t id = cidx(d, i, j, k);
+DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* __restrict__ z,
+                      const float* __restrict__ zc, const Dims& dc, float m, int i, int j, int k, uint32_t id) {
   float nn, s;
   if (zc) {
     auto zf = [&](int a, int b, int c) {
@@ -485,7 +585,48 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, const float* __restrict__
   } else {
     s = nbr_sum<DIM>(z, d, i, j, k, nn);
   }
-  z[id] = (r[id] - m + s) * inv_count(nn);
+  return (r[id] - m + s) * inv_count(nn);
+}
+
+// One half-sweep of colour `phase`; threads cover only that colour (the
+// innermost axis is walked in steps of two).  ZR: this is the last level-0
+// half-sweep of the V-cycle, so it also writes the recentred r back for both
+// cells of each pair, forms z.r over them (the other colour's z is final) and
+// runs the scalar step `op` on the total (p = z as well when p is given).
+template <int DIM, bool ZR>
+__global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, float* __restrict__ z, int phase,
+                                               const double* shift, const float* __restrict__ zc, Dims dc, Red R,
+                                               int op, float* __restrict__ p) {
+  if (is_done(R.flags)) return;
+  const float m = shift ? (float)*shift : 0.f;
+  const int inner = DIM == 3 ? d.n[2] : d.n[1];
+  const uint32_t pairs = (uint32_t)(DIM == 3 ? d.n[0] * d.n[1] : d.n[0]) * d.fh.d;
+  double acc = 0;
+  for (uint32_t t = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x; t < pairs; t += gridDim.x * kB) {
+    int i, j, own, other;
+    colour_pair<DIM>(d, t, phase, i, j, own, other);
+    if (own < inner) {
+      const int a = DIM == 3 ? j : own, k = DIM == 3 ? own : 0;
+      const uint32_t id = cidx(d, i, a, k);
+      const float zn = gs_value<DIM>(d, r, z, zc, dc, m, i, a, k, id);
+      z[id] = zn;
+      if (ZR) {
+        const float rv = r[id] - m;
+        if (shift) r[id] = rv;
+        if (p) p[id] = zn;
+        acc += (double)zn * (double)rv;
+      }
+    }
+    if (ZR && other < inner) {
+      const uint32_t id = DIM == 3 ? cidx(d, i, j, other) : cidx(d, i, other, 0);
+      const float rv = r[id] - m;
+      if (shift) r[id] = rv;
+      const float zo = z[id];
+      if (p) p[id] = zo;
+      acc += (double)zo * (double)rv;
+    }
+  }
+  if (ZR) grid_reduce(R, op, acc, 0.0);
 }
 
 // restrict (2D/vortex/mgpcg.py:152-158): coarse r = sum over the 2^d children
@@ -498,7 +639,7 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
   if (is_done(flags)) return;
   float m = shift ? (float)*shift : 0.f;
   int64_t Nc = (int64_t)dc.n[0] * dc.n[1] * dc.n[2];
-  int64_t cid = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  uint32_t cid = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (cid >= Nc) return;
   int ci, cj, ck;
   cell_of(dc, cid, ci, cj, ck);
@@ -510,7 +651,7 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
         int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
         float nn;
         float s = nbr_sum<DIM>(z, d, i, j, k, nn);
-        int64_t id = cidx(d, i, j, k);
+        uint32_t id = cidx(d, i, j, k);
         acc += (r[id] - m) - (nn * z[id] - s);
       }
   rc[cid] = acc * (1.0f / (float)(DIM - 1));
@@ -546,6 +687,7 @@ struct TailArgs {
   int pre, bottom;
   int bottom_exact;   // 1: solve the coarsest level exactly (DCT-II) instead of RBGS
   float inv_dm1;      // restriction scale 1/(dim-1)
+  Dims d[kTailMax];   // index helpers of each level
 };
 
 // Orthonormal DCT-II along one axis of an LDS array (or its inverse, DCT-III):
@@ -609,10 +751,9 @@ DFM_HD void lds_exact_solve(T* r, T* z, const int* n) {
 // One half-sweep of colour `colour` on an LDS-resident level: every thread
 // owns one cell of that colour per pass (pairs along the innermost axis).
 template <int DIM, typename T>
-DFM_HD void lds_half_sweep(T* r, T* z, const int* n, int colour) {
-  const Dims d{{n[0], n[1], n[2]}};
-  const uint32_t inner = DIM == 3 ? n[2] : n[1];
-  const uint32_t pairs = (uint32_t)(DIM == 3 ? n[0] * n[1] : n[0]) * ((inner + 1) >> 1);
+DFM_HD void lds_half_sweep(T* r, T* z, const Dims& d, int colour) {
+  const int* n = d.n;
+  const uint32_t pairs = (uint32_t)(DIM == 3 ? n[0] * n[1] : n[0]) * d.fh.d;
   const int sx = n[1] * n[2], sy = n[2];
   for (uint32_t t = threadIdx.x; t < pairs; t += blockDim.x) {
     int i, j, k;
@@ -655,8 +796,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
     T* z = r + size_of(l);
     const int* n = A.n[l];
     for (int s = 0; s < A.pre; ++s) {
-      lds_half_sweep<DIM, T>(r, z, n, 0);
-      lds_half_sweep<DIM, T>(r, z, n, 1);
+      lds_half_sweep<DIM, T>(r, z, A.d[l], 0);
+      lds_half_sweep<DIM, T>(r, z, A.d[l], 1);
     }
     T* rc = smem + A.off[l + 1];
     T* zc = rc + size_of(l + 1);
@@ -695,8 +836,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
       lds_exact_solve<DIM, T>(r, z, A.n[l]);
     } else {
       for (int s = 0; s < A.bottom; ++s) {
-        lds_half_sweep<DIM, T>(r, z, A.n[l], 0);
-        lds_half_sweep<DIM, T>(r, z, A.n[l], 1);
+        lds_half_sweep<DIM, T>(r, z, A.d[l], 0);
+        lds_half_sweep<DIM, T>(r, z, A.d[l], 1);
       }
     }
   }
@@ -715,8 +856,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
     }
     __syncthreads();
     for (int s = 0; s < A.pre; ++s) {
-      lds_half_sweep<DIM, T>(r, z, n, 1);
-      lds_half_sweep<DIM, T>(r, z, n, 0);
+      lds_half_sweep<DIM, T>(r, z, A.d[l], 1);
+      lds_half_sweep<DIM, T>(r, z, A.d[l], 0);
     }
   }
   T* z0 = smem + A.off[0] + size_of(0);
@@ -728,11 +869,13 @@ struct Solver {
   Geo G;
   int L, pre, bottom;
   Lvl lv[kMaxLevels];
+  Dims ld[kMaxLevels];  // index helpers per level
   double* x;  // PCG iterate (pressure), FP64
   float* b;   // recentred right-hand side, kept for the FP64 refinement residual
   int max_refine;
   float *p, *Ap;
-  double* part;
+  double* part;   // 2 x kRedBlocks block partials
+  unsigned* ctr;  // grid-reduction arrival counters (kCtrWords)
   double* sc;
   int* flags;
   int* h_flags;
@@ -750,6 +893,7 @@ struct Solver {
   float* z0_final;   // level-0 buffer holding M^-1 r after the last V-cycle
   // captured PCG iterations
   int use_graph;
+  int predict;       // iterations the previous solve took to first convergence
   hipStream_t cap;
   hipGraphExec_t gexec;
   int gexec_iters;
@@ -784,6 +928,7 @@ void configure_tail(Solver& s, size_t cap) {
   int off = 0;
   for (int k = 0; k < s.tail.nl; ++k) {
     for (int a = 0; a < 3; ++a) s.tail.n[k][a] = s.lv[best + k].n[a];
+    s.tail.d[k] = make_dims(s.tail.n[k][0], s.tail.n[k][1], s.tail.n[k][2], s.G.dim);
     s.tail.off[k] = off;
     off += 2 * (int)s.lv[best + k].size;
   }
@@ -815,61 +960,78 @@ void setup_tail(Solver& s) {
   }
 }
 
-inline Dims dims_of(const Lvl& l) { return Dims{{l.n[0], l.n[1], l.n[2]}}; }
-
 inline unsigned grid1(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 inline unsigned gridr(int64_t n) {
   int64_t b = (n + kB - 1) / kB;
   return (unsigned)(b < kRedBlocks ? b : kRedBlocks);
 }
 
-template <int DIM>
-void fin(Solver& s, hipStream_t st, int op, unsigned nparts, double tol = 0, int rel = 0) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kB), 0, st, op, s.part, (int)nparts, s.sc, s.flags, s.lv[0].size, tol,
-                     rel);
+inline Red red_of(const Solver& s, double tol = 0.0, int relative = 0) {
+  return Red{s.part, s.ctr, s.sc, s.flags, 1.0 / (double)s.lv[0].size, tol, relative};
 }
+
+inline uint32_t pairs_of(const Dims& d, int dim) {
+  return (uint32_t)(dim == 3 ? d.n[0] * d.n[1] : d.n[0]) * d.fh.d;
+}
+
+// z.r of the V-cycle result and its scalar step, fused into the last level-0
+// half-sweep when level 0 is smoothed by the global kernels.
+struct ZrFuse {
+  int op;    // FIN_ZR_INIT or FIN_BETA
+  float* p;  // also p = z (start and restart of PCG), else nullptr
+};
 
 // `sweeps` RBGS sweeps (colour c1 first, then the other) in place on lv[l].z,
 // starting from z = 0 when from_zero (first sweep fused into one kernel).
-// zc_prolong fuses the prolongation into the first half-sweep.  Returns lv[l].z.
+// zc_prolong fuses the prolongation into the first half-sweep; zr (if given)
+// is fused into the last half-sweep.  Returns whether zr was done.
 template <int DIM>
-float* sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero,
-                     const double* shift, const float* zc_prolong, Dims dc) {
+bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero, const double* shift,
+                   const float* zc_prolong, const ZrFuse* zr) {
   Lvl& L = s.lv[l];
-  Dims d = dims_of(L);
-  Dims none{{1, 1, 1}};
+  const Dims& d = s.ld[l];
+  const Dims& dc = s.ld[l + 1 < s.L ? l + 1 : l];
+  const Red R = red_of(s);
   int k0 = 0;
   if (from_zero) {
     hipLaunchKernelGGL((k_first_sweep<DIM>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, L.z, shift, s.flags);
     k0 = 1;
   }
-  int inner = DIM == 3 ? d.n[2] : d.n[1];
-  int64_t outer = DIM == 3 ? (int64_t)d.n[0] * d.n[1] : d.n[0];
-  unsigned gs = grid1(outer * ((inner + 1) / 2));
+  const uint32_t pairs = pairs_of(d, DIM);
+  const unsigned gs = grid1(pairs);
+  bool zr_done = false;
   for (int it = k0; it < sweeps; ++it) {
     const float* zc = (it == k0 && zc_prolong) ? zc_prolong : nullptr;
-    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, c1, shift, zc, zc ? dc : none,
-                       s.flags);
-    hipLaunchKernelGGL((k_smooth<DIM>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift, nullptr, none,
-                       s.flags);
+    hipLaunchKernelGGL((k_smooth<DIM, false>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, c1, shift, zc, dc, R, 0,
+                       (float*)nullptr);
+    if (zr && it == sweeps - 1) {
+      hipLaunchKernelGGL((k_smooth<DIM, true>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift,
+                         (const float*)nullptr, dc, R, zr->op, zr->p);
+      zr_done = true;
+    } else {
+      hipLaunchKernelGGL((k_smooth<DIM, false>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift,
+                         (const float*)nullptr, dc, R, 0, (float*)nullptr);
+    }
   }
-  return L.z;
+  return zr_done;
 }
 
-// apply_preconditioner (2D/vortex/mgpcg.py:182-200): z0 = M^-1 r0.  Levels
-// below tail_lt use global kernels; levels tail_lt.. run in one LDS kernel.
+// apply_preconditioner (2D/vortex/mgpcg.py:182-200): z0 = M^-1 r0, followed by
+// z.r and its scalar step `zr`.  Levels below tail_lt use global kernels;
+// levels tail_lt.. run in one LDS kernel.
 template <int DIM>
-void vcycle(Solver& s, hipStream_t st, const double* shift0) {
-  Dims none{{1, 1, 1}};
+void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   float* cur[kMaxLevels] = {nullptr};
   const bool tail = s.tail_lt < s.L;
   const int Lg = tail ? s.tail_lt : s.L - 1;  // levels smoothed with global kernels before the bottom
+  bool zr_done = false;
   for (int l = 0; l < Lg; ++l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    cur[l] = sweeps_global<DIM>(s, st, l, s.pre, 0, true, sh, nullptr, none);
+    sweeps_global<DIM>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr);
+    cur[l] = s.lv[l].z;
     Lvl& F = s.lv[l];
     Lvl& C = s.lv[l + 1];
-    hipLaunchKernelGGL((k_restrict<DIM>), dim3(grid1(C.size)), dim3(kB), 0, st, dims_of(F), F.r, cur[l], dims_of(C),
+    hipLaunchKernelGGL((k_restrict<DIM>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r, cur[l], s.ld[l + 1],
                        C.r, sh, s.flags);
   }
   if (tail) {
@@ -882,34 +1044,38 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0) {
     cur[s.tail_lt] = T.z;
   } else {
     const double* sh = s.L == 1 ? shift0 : nullptr;
-    cur[s.L - 1] = sweeps_global<DIM>(s, st, s.L - 1, s.bottom, 0, true, sh, nullptr, none);
+    zr_done = sweeps_global<DIM>(s, st, s.L - 1, s.bottom, 0, true, sh, nullptr, s.L == 1 ? &zr : nullptr);
+    cur[s.L - 1] = s.lv[s.L - 1].z;
   }
   for (int l = Lg - 1; l >= 0; --l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    cur[l] = sweeps_global<DIM>(s, st, l, s.pre, 1, false, sh, cur[l + 1], dims_of(s.lv[l + 1]));
+    const bool d0 = sweeps_global<DIM>(s, st, l, s.pre, 1, false, sh, cur[l + 1], l == 0 ? &zr : nullptr);
+    if (l == 0) zr_done = d0;
+    cur[l] = s.lv[l].z;
   }
   s.z0_final = cur[0];
+  if (!zr_done) {
+    Lvl& L0 = s.lv[0];
+    hipLaunchKernelGGL(k_zr, dim3(gridr(L0.size)), dim3(kB), 0, st, L0.size, (const float*)s.z0_final, L0.r, shift0,
+                       red_of(s), zr.op, zr.p ? 1 : 0, zr.p);
+  }
 }
 
 // One PCG iteration once z, p and z.r are set (solve(), 2D/vortex/mgpcg.py:235-278):
-// Ap and p.Ap, alpha, x/r update and r.r (convergence flag), recentring
-// V-cycle, z.r and beta, p update.  Every kernel is a no-op once `done` is set,
-// so iterations may be queued past convergence.
+// Ap and p.Ap -> alpha; x/r update and r.r -> convergence flag and mean;
+// recentring V-cycle with z.r -> beta; p update.  Every kernel is a no-op once
+// `done` is set, so iterations may be queued past convergence.
 template <int DIM>
 void pcg_iteration(Solver& s, hipStream_t st) {
   Lvl& L0 = s.lv[0];
   int64_t N = L0.size;
   unsigned gr = gridr(N);
-  Dims d0 = dims_of(L0);
-  hipLaunchKernelGGL((k_Ap_dot<DIM>), dim3(gr), dim3(kB), 0, st, d0, s.p, s.Ap, s.part, s.flags);
-  fin<DIM>(s, st, FIN_ALPHA, gr);
-  hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, s.p, s.Ap, s.sc, s.part, s.flags);
-  fin<DIM>(s, st, FIN_RR, gr);
-  vcycle<DIM>(s, st, s.sc + S_MEAN);
-  hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, s.z0_final, L0.r, (const double*)(s.sc + S_MEAN), s.part,
-                     s.flags, 0, (float*)nullptr);
-  fin<DIM>(s, st, FIN_BETA, gr);
-  hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, s.z0_final, s.p, s.sc, s.flags);
+  const Red R = red_of(s);
+  hipLaunchKernelGGL((k_Ap_dot<DIM>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
+  hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, (const float*)s.p, (const float*)s.Ap, R);
+  vcycle<DIM>(s, st, s.sc + S_MEAN, ZrFuse{FIN_BETA, nullptr});
+  hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, (const float*)s.z0_final, s.p, (const double*)s.sc,
+                     (const int*)s.flags);
 }
 
 void drop_graph(Solver& s) {
@@ -947,20 +1113,15 @@ bool build_graph(Solver& s) {
   return true;
 }
 
-// PCG from r0 (already in lv[0].r, mean sum in partials): solve(),
-// 2D/vortex/mgpcg.py:202-282.
+// PCG from r0 (already in lv[0].r, its mean in sc[S_MEAN]): solve(),
+// 2D/vortex/mgpcg.py:202-282, with FP64 iterative refinement.
 template <int DIM>
 int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   Lvl& L0 = s.lv[0];
   int64_t N = L0.size;
   unsigned gr = gridr(N);
-  fin<DIM>(s, st, FIN_MEAN, gr);
-  hipLaunchKernelGGL(k_recenter_norm, dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, s.sc, s.part);
-  fin<DIM>(s, st, FIN_BNORM, gr, tol, relative);
-  vcycle<DIM>(s, st, nullptr);
-  hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, s.z0_final, L0.r, (const double*)nullptr, s.part, s.flags, 1, s.p);
-  fin<DIM>(s, st, FIN_ZR_INIT, gr);
-  Dims d0 = dims_of(L0);
+  hipLaunchKernelGGL(k_recenter_norm, dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, red_of(s, tol, relative));
+  vcycle<DIM>(s, st, nullptr, ZrFuse{FIN_ZR_INIT, s.p});
   int it = 0;
   int refinements = 0;
   bool done = false;
@@ -970,33 +1131,38 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   };
   bool use_graph = s.use_graph;
   if (use_graph && (!s.gexec || s.gexec_iters != s.check_every)) use_graph = build_graph<DIM>(s);
+  // Queue as many iterations as the previous solve needed before polling the
+  // first time: consecutive projections converge in nearly the same number of
+  // iterations, and iterations queued past convergence are no-ops.  The FP64
+  // true-residual check is queued behind them and runs only if they converged,
+  // so the common case costs one host synchronisation per solve.
+  int batch = s.predict > 0 ? s.predict : s.check_every;
+  bool first_poll = true;
   while (it < max_iters) {
-    int k = s.check_every;
+    int k = first_poll ? batch : s.check_every;
     if (it + k > max_iters) k = max_iters - it;
-    if (use_graph && k == s.gexec_iters) {
+    int left = k;
+    while (use_graph && left >= s.gexec_iters) {
       if (hipGraphLaunch(s.gexec, st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: graph launch failed");
-    } else {
-      for (int q = 0; q < k; ++q) pcg_iteration<DIM>(s, st);
+      left -= s.gexec_iters;
     }
+    for (int q = 0; q < left; ++q) pcg_iteration<DIM>(s, st);
     it += k;
-    {
-      if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
-      if (s.h_flags[F_DONE] && s.h_flags[F_STATUS] == 1 && refinements < s.max_refine) {
-        // Converged on the recursive FP32 residual: measure the true residual
-        // b - A x in FP64; if it misses the tolerance, restart PCG from it.
-        hipLaunchKernelGGL((k_true_resid<DIM>), dim3(gr), dim3(kB), 0, st, d0, s.x, s.b, L0.r, s.part);
-        fin<DIM>(s, st, FIN_TRUE, gr);
-        if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
-        if (!s.h_flags[F_DONE]) {
-          ++refinements;
-          vcycle<DIM>(s, st, s.sc + S_MEAN);
-          hipLaunchKernelGGL(k_zr, dim3(gr), dim3(kB), 0, st, N, s.z0_final, L0.r, (const double*)(s.sc + S_MEAN), s.part,
-                             s.flags, 1, s.p);
-          fin<DIM>(s, st, FIN_ZR_INIT, gr);
-          continue;
-        }
-      }
-      if (s.h_flags[F_DONE]) { done = true; break; }
+    first_poll = false;
+    // Converged on the recursive FP32 residual: measure the true residual
+    // b - A x in FP64; if it misses the tolerance, PCG restarts from it.
+    if (refinements < s.max_refine)
+      hipLaunchKernelGGL((k_true_resid<DIM>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const double*)s.x,
+                         (const float*)s.b, L0.r, red_of(s), 1);
+    if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
+    if (s.h_flags[F_DONE]) {
+      if (s.h_flags[F_STATUS] == 1 && refinements == 0) s.predict = s.h_flags[F_ITERS];
+      done = true;
+      break;
+    }
+    if (s.h_flags[F_NTRUE] > refinements) {
+      ++refinements;
+      vcycle<DIM>(s, st, s.sc + S_MEAN, ZrFuse{FIN_ZR_INIT, s.p});
     }
   }
   bool rb = hipMemcpyAsync(s.h_sc, s.sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st) == hipSuccess;
@@ -1023,6 +1189,7 @@ void destroy(Solver* s) {
   if (s->p) (void)hipFree(s->p);
   if (s->Ap) (void)hipFree(s->Ap);
   if (s->part) (void)hipFree(s->part);
+  if (s->ctr) (void)hipFree(s->ctr);
   if (s->sc) (void)hipFree(s->sc);
   if (s->flags) (void)hipFree(s->flags);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
@@ -1042,6 +1209,8 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   int nmin = g->n[0];
   for (int a = 1; a < g->dim; ++a) nmin = g->n[a] < nmin ? g->n[a] : nmin;
   if (nmin < 2) return set_error(DFM_E_ARG, "grid too small");
+  if ((int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1) >= (int64_t(1) << 31))
+    return set_error(DFM_E_ARG, "grid too large (faces must stay below 2^31 per axis)");
   if (levels <= 0) {
     int lg = 0;
     while ((1 << (lg + 1)) <= nmin) ++lg;
@@ -1069,6 +1238,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   for (int l = 0; l < levels && ok; ++l) {
     for (int a = 0; a < 3; ++a) s->lv[l].n[a] = a < g->dim ? (g->n[a] >> l) : 1;
     s->lv[l].size = (int64_t)s->lv[l].n[0] * s->lv[l].n[1] * s->lv[l].n[2];
+    s->ld[l] = make_dims(s->lv[l].n[0], s->lv[l].n[1], s->lv[l].n[2], g->dim);
     ok = hipMalloc(&s->lv[l].r, sizeof(float) * s->lv[l].size) == hipSuccess &&
          hipMalloc(&s->lv[l].z, sizeof(float) * s->lv[l].size) == hipSuccess;
   }
@@ -1077,11 +1247,13 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
        hipMalloc(&s->p, sizeof(float) * N) == hipSuccess &&
        hipMalloc(&s->Ap, sizeof(float) * N) == hipSuccess &&
        hipMalloc(&s->part, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
+       hipMalloc(&s->ctr, sizeof(unsigned) * kCtrWords) == hipSuccess &&
        hipMalloc(&s->sc, sizeof(double) * S_NSLOT) == hipSuccess &&
        hipMalloc(&s->flags, sizeof(int) * F_NFLAG) == hipSuccess &&
        hipHostMalloc(&s->h_flags, sizeof(int) * F_NFLAG, hipHostMallocDefault) == hipSuccess &&
        hipHostMalloc(&s->h_sc, sizeof(double) * S_NSLOT, hipHostMallocDefault) == hipSuccess;
   if (ok) ok = hipMemset(s->part, 0, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
+               hipMemset(s->ctr, 0, sizeof(unsigned) * kCtrWords) == hipSuccess &&
                hipMemset(s->sc, 0, sizeof(double) * S_NSLOT) == hipSuccess &&
                hipMemset(s->flags, 0, sizeof(int) * F_NFLAG) == hipSuccess &&
                hipMemset(s->x, 0, sizeof(double) * N) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
@@ -1160,15 +1332,15 @@ extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int
   if (max_iters < 1) return set_error(DFM_E_ARG, "max_iters must be >= 1");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   Lvl& L0 = s->lv[0];
-  Dims d0 = dims_of(L0);
+  const Dims& d0 = s->ld[0];
   Faces F{{u->c[0], u->c[1], u->c[2]}};
   unsigned gr = gridr(L0.size);
   int rc;
   if (s->G.dim == 2) {
-    hipLaunchKernelGGL((k_div<2>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, s->part);
+    hipLaunchKernelGGL((k_div<2>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, red_of(*s));
     rc = pcg<2>(*s, st, tol, relative, max_iters);
   } else {
-    hipLaunchKernelGGL((k_div<3>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, s->part);
+    hipLaunchKernelGGL((k_div<3>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, red_of(*s));
     rc = pcg<3>(*s, st, tol, relative, max_iters);
   }
   if (rc != DFM_OK && rc != DFM_E_NOCONV) return rc;
@@ -1178,8 +1350,14 @@ extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int
   for (int a = 0; a < s->G.dim; ++a) nf += (L0.size / L0.n[a]) * (L0.n[a] + 1);
   unsigned gf = (unsigned)((nf + kB - 1) / kB);
   if (gf > 65535u * 8u) gf = 65535u * 8u;
-  if (s->G.dim == 2) hipLaunchKernelGGL((k_subgrad<2>), dim3(gf), dim3(kB), 0, st, d0, F, s->x);
-  else hipLaunchKernelGGL((k_subgrad<3>), dim3(gf), dim3(kB), 0, st, d0, F, s->x);
+  FaceDims fd;
+  for (int a = 0; a < 3; ++a) {
+    int n[3] = {L0.n[0], L0.n[1], L0.n[2]};
+    if (a < s->G.dim) n[a] += 1;
+    fd.f[a] = make_dims(n[0], n[1], n[2], s->G.dim);
+  }
+  if (s->G.dim == 2) hipLaunchKernelGGL((k_subgrad<2>), dim3(gf), dim3(kB), 0, st, d0, fd, F, (const double*)s->x);
+  else hipLaunchKernelGGL((k_subgrad<3>), dim3(gf), dim3(kB), 0, st, d0, fd, F, (const double*)s->x);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg subtract_grad_p", hipGetErrorString(e));
   return rc;
@@ -1194,7 +1372,7 @@ extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double to
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   Lvl& L0 = s->lv[0];
   unsigned gr = gridr(L0.size);
-  hipLaunchKernelGGL(k_load_b, dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, s->part);
+  hipLaunchKernelGGL(k_load_b, dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, red_of(*s));
   int rc = s->G.dim == 2 ? pcg<2>(*s, st, tol, relative, max_iters) : pcg<3>(*s, st, tol, relative, max_iters);
   if (rc != DFM_OK && rc != DFM_E_NOCONV) return rc;
   hipLaunchKernelGGL(k_to_float, dim3(gr), dim3(kB), 0, st, L0.size, s->x, x);

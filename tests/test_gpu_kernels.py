@@ -300,6 +300,34 @@ def test_mgpcg_graph_replays_match_eager(check_every, device):
         assert rel_l2_fields(mac_to_np(dg), mac_to_np(de)) < 1e-6, rep
 
 
+@pytest.mark.parametrize("n,levels", [((40, 36, 20), 3), ((72, 56), 3), ((128, 128, 128), 5)])
+def test_mgpcg_repeat_solves_bitwise_identical(n, levels, device):
+    """The grid reductions (per-block partials, summed in block order by the
+    last block to arrive) are deterministic: the same input solved repeatedly
+    by one handle (graph replays included) gives bitwise-identical fields and
+    iteration counts.  Sizes with odd coarse rows exercise the index
+    decomposition's odd-pair edge cases."""
+    from dfm.ops import MGPCG
+
+    dx = 1.0 / n[1]
+    rng = np.random.default_rng(17 + n[0])
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(len(n))])
+    s = MGPCG(n, dx, levels=levels, tol=1e-7, max_iters=100)
+    outs, its = [], []
+    for rep in range(5):
+        d = mac_to_dev(u, device)
+        s.Poisson(d)
+        outs.append(mac_to_np(d))
+        its.append(s.status()[0])
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+    assert len(set(its)) == 1, its
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    assert rel_l2_fields(outs[0], ref) < 2e-6
+
+
 def test_mgpcg_solve_and_status(device):
     from dfm.ops import MGPCG
 
