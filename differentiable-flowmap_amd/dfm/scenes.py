@@ -5,7 +5,8 @@ Velocity profiles restate the reference's Gaussian ("Lamb-Oseen-like")
 vortex, ``angular_vel_func`` (2D/vortex/init_conditions.py:141-145)
 with the per-vortex form of ``sixteen_vortex_vel_func_with_pos_coef``
 (2D/vortex/init_conditions.py:368-383): u(x) = sum_k w_k * s/r^2 *
-(1 - exp(-r^2/a_k^2)) * (-p_y, p_x) * 2.5 with s = -0.01, r = |x - c_k| + 1e-6.
+(1 - exp(-r^2/a_k^2)) * (-p_y, p_x) * 2.5 with s = -0.01, r = |x - c_k| + 1e-6
+and a_k = 0.05 rr_k.
 Cell-centred velocities are split to the MAC faces with
 ``split_central_vector`` (2D/vortex/taichi_utils.py:158-167,
 3D/taichi_utils.py:170-183) and then projected by the simulator.
@@ -47,21 +48,34 @@ def split_central_vector(uc):
 
 
 # ------------------------------------------------------------------- 2D -----
+# theta follows the reference's control vector layout (4K values, blocks of K):
+# [w_0..w_K-1, cx_0.., cy_0.., rr_0..] with vortex radius a = 0.05 rr
+# (sixteen_vortex_vel_func_with_pos_coef, 2D/vortex/init_conditions.py:368-383;
+# 2D/vortex/optimize_vortex.py:16-50 builds it for K = 16).
+RADIUS_SCALE = 0.05
+
+
 def vortex_params_2d(seed, count=8):
-    """theta (count x 4) = [strength w, centre x, centre y, radius a]:
-    centres U[0.2,0.8]^2, radii U[0.02,0.06], strengths +-U[0.5,1.5]."""
+    """Flat theta (4 count): strengths +-U[0.5,1.5], centres U[0.2,0.8]^2,
+    radii a = 0.05 rr in U[0.02,0.06]."""
     rng = np.random.default_rng(seed)
     w = rng.uniform(0.5, 1.5, count) * rng.choice([-1.0, 1.0], count)
     c = rng.uniform(0.2, 0.8, (count, 2))
     a = rng.uniform(0.02, 0.06, count)
-    return np.stack([w, c[:, 0], c[:, 1], a], axis=1)
+    return np.concatenate([w, c[:, 0], c[:, 1], a / RADIUS_SCALE])
+
+
+def unpack_theta_2d(theta):
+    """(K, 4) rows (w, cx, cy, a) from the flat reference-layout theta."""
+    t = np.asarray(theta, dtype=np.float64).reshape(4, -1)
+    return np.stack([t[0], t[1], t[2], RADIUS_SCALE * t[3]], axis=1)
 
 
 def vortex_velocity_2d(theta, n, dx):
     """Cell-centred velocity of the vortex sum, shape (nx, ny, 2)."""
     X = cell_centers(n, dx)
     u = np.zeros(tuple(n) + (2,))
-    for w, cx, cy, a in np.asarray(theta, dtype=np.float64):
+    for w, cx, cy, a in unpack_theta_2d(theta):
         p = X - np.array([cx, cy])
         r = np.sqrt(p[..., 0] ** 2 + p[..., 1] ** 2)
         f = angular_vel(r, a) * w * SWIRL_GAIN

@@ -21,9 +21,15 @@ uses): viscosity 0, no control force, no obstacle mask, use_short_BFECC=False,
 BFECC_clamp=False, backward_u=True, final-time density loss.
 
 The reference's disk cache (RW_Cache, 2D/vortex/lfm_midpoint_simulator.py:46-184)
-is replaced by device-resident storage: the midpoint velocity of every step,
-the passive scalars at every reinitialisation boundary and the forward map
-(phi, F) of every interval stay in HBM.
+is replaced by device-resident storage.  ``record=True`` (default) keeps the
+midpoint velocity of every step, the passive scalars at every
+reinitialisation boundary and the forward map (phi, F) of every interval in
+HBM, as the reference keeps them on disk.  ``checkpoint=True`` keeps only the
+velocity and passive scalars at each interval start; the adjoint sweep
+re-runs an interval's forward steps from its checkpoint when it enters that
+interval, so adjoint storage is O(reinit interval) instead of O(steps).  The
+kernels are deterministic, so the recomputed midpoint velocities and maps
+are bitwise those of the first forward pass.
 """
 from __future__ import annotations
 
@@ -104,7 +110,7 @@ class LFM_Simulator:
 
     def __init__(self, *args, order=None, total_steps=None, n_passive=1, adj_kernel=None, mg_levels=0,
                  mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda",
-                 mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1):
+                 mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -119,7 +125,9 @@ class LFM_Simulator:
         self.adj_kernel = adj_kernel or (4 if self.dim == 2 else 3)
         self.total_steps = total_steps
         self.n_passive = n_passive
-        self.record = record
+        self.record = record or checkpoint
+        self.checkpoint = checkpoint
+        self._replaying = False
         self.device = torch.device(device)
         if not torch.cuda.is_available():
             raise RuntimeError("dfm.LFM_Simulator needs a HIP device (torch.cuda.is_available() is False)")
@@ -143,6 +151,7 @@ class LFM_Simulator:
         self.mid_u = {}        # step -> MacField (midpoint velocity)
         self.passive_start = {}  # interval-start step -> [passive]
         self.flow_maps = {}    # interval index -> FlowMap (phi, F)
+        self.u_start = {}      # interval index -> MacField (checkpoint mode)
         self._views()
 
     # Taichi-like attribute views -------------------------------------------
@@ -158,9 +167,13 @@ class LFM_Simulator:
     # set-up -----------------------------------------------------------------
     def _initial_velocity(self, theta):
         if self.dim == 2:
-            uc = scenes.vortex_velocity_2d(np.asarray(theta, dtype=np.float64).reshape(-1, 4), self.n, self.dx)
-        else:
-            uc = scenes.vortex_tubes_3d(np.asarray(theta, dtype=np.float64).reshape(-1, 8), self.n, self.dx)
+            # sixteen_vortex_vel_func_with_pos_coef + split_central_vector on the device
+            from .autograd import vortex_velocity_2d
+
+            th = torch.as_tensor(np.asarray(theta, dtype=np.float32).reshape(-1)).to(self.device)
+            with torch.no_grad():
+                return list(vortex_velocity_2d(th, self.n, self.dx))
+        uc = scenes.vortex_tubes_3d(np.asarray(theta, dtype=np.float64).reshape(-1, 8), self.n, self.dx)
         return scenes.split_central_vector(uc)
 
     def init(self, theta=None, forward=True, target=False, init_passive_np=None, u0=None):
@@ -190,6 +203,7 @@ class LFM_Simulator:
         self.mid_u.clear()
         self.passive_start.clear()
         self.flow_maps.clear()
+        self.u_start.clear()
 
     def final_condition(self):
         return self.total_steps is not None and self.step_num[None] == self.total_steps - 1
@@ -201,8 +215,10 @@ class LFM_Simulator:
             raise NotImplementedError("control forces are not supported by this build")
         self.step_num[None] += 1
         j = self.step_num[None] % self.reinit_every
-        if j == 0 and self.record:
+        if j == 0 and self.record and not self._replaying:
             self.passive_start[self.step_num[None]] = [p.clone() for p in self.passive]
+            if self.checkpoint:
+                self.u_start[self.step_num[None] // self.reinit_every] = self.u.clone()
         self.step_flow_map(j, self.dt)
         return False, self.final_condition()
 
@@ -228,7 +244,16 @@ class LFM_Simulator:
         self.passive, self._passive_new = self._passive_new, self.passive
         self.solver.Poisson(self.u)
         if j == self.reinit_every - 1:
-            self._reinit(step, j, dt)
+            if self._replaying:
+                # recomputation for the adjoint only needs the interval's (phi, F)
+                self._store_phi(step)
+            else:
+                self._reinit(step, j, dt)
+
+    def _store_phi(self, step):
+        fm = FlowMap(self.n, self.device)
+        fm.copy_(self.phi_map)
+        self.flow_maps[step // self.reinit_every] = fm
 
     def _reinit(self, step, j, dt):
         """Reinitialisation: re-march the backward map psi/T from identity
@@ -246,10 +271,8 @@ class LFM_Simulator:
         for k in range(self.n_passive):
             ops.bfecc_s(self.init_passive[k], self.psi_map.psi, self.phi_map.psi, self._err_s, self.passive[k])
             self.init_passive[k].copy_(self.passive[k])
-        if self.record:
-            fm = FlowMap(self.n, self.device)
-            fm.copy_(self.phi_map)
-            self.flow_maps[step // self.reinit_every] = fm
+        if self.record and not self.checkpoint:
+            self._store_phi(step)
         else:
             for st in range(step - j, step + 1):
                 self.mid_u.pop(st, None)
@@ -306,20 +329,39 @@ class LFM_Diff_Simulator(LFM_Simulator):
         for k, t in enumerate(ts[: self.n_passive]):
             self.target_passive[k].copy_(torch.as_tensor(t, dtype=torch.float32).to(self.device))
 
-    def init_gradient(self, theta=None, target=None):
+    def init_gradient(self, theta=None, target=None, grad_passive=None, grad_u=None):
         """init_gradient (2D/vortex/lfm_midpoint_diff_simulator.py:126-207):
         terminal adjoint lam_s = 2 (s - s*), lam_u = 0, loss = sum (s - s*)^2,
-        starting from the final forward state held on the device."""
+        starting from the final forward state held on the device.
+
+        For a general final-time loss the caller passes dL/ds_T
+        (``grad_passive``, one tensor per passive) and/or dL/du_T (``grad_u``,
+        MAC tensors); lam_u starts at P grad_u, the adjoint of the final
+        projection, and ``loss`` is not computed."""
         if target is not None:
             self.set_target(target)
         self.final_ind = self.step_num[None] + 1
-        self.adj_u.zero_()
-        loss = 0.0
-        for k in range(self.n_passive):
-            d = self.passive[k] - self.target_passive[k]
-            self.adj_passive[k].copy_(2.0 * d)
-            loss += float(torch.sum(d.double() ** 2))
-        self.loss = loss
+        if self.final_ind % self.reinit_every:
+            raise ValueError("the adjoint sweep needs a whole number of reinitialisation intervals "
+                             f"({self.final_ind} steps, reinit_every={self.reinit_every}): the forward map of "
+                             "an interval is stored at its reinitialisation")
+        if grad_u is None:
+            self.adj_u.zero_()
+        else:
+            for a in range(self.dim):
+                self.adj_u.c[a].copy_(grad_u[a])
+            self.solver.Poisson_adj(self.adj_u)
+        if grad_passive is None:
+            loss = 0.0
+            for k in range(self.n_passive):
+                d = self.passive[k] - self.target_passive[k]
+                self.adj_passive[k].copy_(2.0 * d)
+                loss += float(torch.sum(d.double() ** 2))
+            self.loss = loss
+        else:
+            for k in range(self.n_passive):
+                self.adj_passive[k].copy_(grad_passive[k])
+            self.loss = None
         self.step_num[None] = self.final_ind
         self.init_adj_u.copy_(self.adj_u)
         for k in range(self.n_passive):
@@ -344,6 +386,10 @@ class LFM_Diff_Simulator(LFM_Simulator):
         """backtrack_flow_map (2D/vortex/lfm_midpoint_diff_simulator.py:260-503)
         with use_short_BFECC=False and backward_u=True."""
         ops, order, step = self.ops, self.order, self.step_num[None]
+        if step not in self.mid_u:
+            if not self.checkpoint:
+                raise KeyError(f"no stored midpoint velocity for step {step} (forward run with record=False)")
+            self._replay_interval(step // self.reinit_every)
         mid = self.mid_u[step]
         # adjoint sources at the grid faces, from the current (end-of-step) state
         ops.adj_conv_source(self.u, self.adj_u, self.adj_kernel, self.r_u)
@@ -377,6 +423,48 @@ class LFM_Diff_Simulator(LFM_Simulator):
                 ops.bfecc_s(self.init_adj_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.adj_passive[k])
                 self.init_adj_passive[k].copy_(self.adj_passive[k])
             self.ops.reset_identity(self.psi_map)
+            if self.checkpoint:
+                m = step // self.reinit_every
+                self.flow_maps.pop(m, None)
+                for st in range(step, step + self.reinit_every):
+                    self.mid_u.pop(st, None)
+
+    # checkpoint recomputation ------------------------------------------------
+    _REPLAY_FIELDS = ("u", "_u_new", "passive", "_passive_new", "phi_map")
+
+    def _replay_interval(self, m):
+        """Re-run the forward steps of interval m from its checkpoint (velocity
+        and passives at the interval start) on scratch state, leaving the
+        interval's midpoint velocities and forward map (phi, F) in storage."""
+        if m not in self.u_start:
+            raise KeyError(f"no checkpoint for interval {m}")
+        if not hasattr(self, "_rp"):
+            n, dev = self.n, self.device
+            self._rp = {"u": MacField.empty(n, dev), "_u_new": MacField.empty(n, dev),
+                        "passive": [torch.empty_like(p) for p in self.passive],
+                        "_passive_new": [torch.empty_like(p) for p in self.passive],
+                        "phi_map": FlowMap(n, dev)}
+        R = self.reinit_every
+        start = m * R
+        saved = {k: getattr(self, k) for k in self._REPLAY_FIELDS}
+        saved_step = self.step_num[None]
+        for k, v in self._rp.items():
+            setattr(self, k, v)
+        try:
+            self.u.copy_(self.u_start[m])
+            for k in range(self.n_passive):
+                self.passive[k].copy_(self.passive_start[start][k])
+            self.ops.reset_identity(self.phi_map)
+            self.step_num[None] = start - 1
+            self._replaying = True
+            for _ in range(R):
+                self.step_midpoint()
+        finally:
+            self._replaying = False
+            self._rp = {k: getattr(self, k) for k in self._REPLAY_FIELDS}
+            for k, v in saved.items():
+                setattr(self, k, v)
+            self.step_num[None] = saved_step
 
     def calculate_gw(self):
         u, self.u = self.u, self.adj_u

@@ -177,6 +177,23 @@ int dfm_mgpcg_pressure(void* handle, double** p);
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);
 
+/* ---------------------------------------------- 2D vortex control problem */
+/* Initial velocity of K vortices on the MAC faces: the cell-centred vortex
+ * sum of sixteen_vortex_vel_func_with_pos_coef (2D/vortex/init_conditions.py:368-383,
+ * profile angular_vel_func :141-145) split to faces by split_central_vector
+ * (2D/vortex/taichi_utils.py:158-167); the caller projects it (the
+ * reference's init does, 2D/vortex/lfm_midpoint_simulator.py:420-422).
+ * theta: device FP32 [w (K), cx (K), cy (K), rr (K)], radius 0.05 rr. */
+int dfm_vortex_velocity_2d(const dfm_grid* g, const float* theta, int K, const dfm_mac* u, void* stream);
+
+/* dtheta (device FP32, 4K, theta's layout) = J^T lam, J = d u_faces / d theta
+ * of dfm_vortex_velocity_2d (analytic, FP64 sums, deterministic).  Replaces
+ * the per-component finite differences of calculate_dtheta / _ind
+ * (2D/vortex/optimize_vortex.py:193-226), which form the same inner product
+ * <lam, d u / d theta_i> with eps = 1e-5. */
+int dfm_vortex_velocity_2d_vjp(const dfm_grid* g, const float* theta, int K, const dfm_mac* lam, float* dtheta,
+                               void* stream);
+
 #ifdef __cplusplus
 }
 #endif
