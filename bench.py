@@ -75,7 +75,7 @@ def run_gpu(args):
     dt = 0.5 * dx
     R = 10
     K, W = args.steps, args.warmup
-    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=5, mg_pre_post=2,
+    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=args.mg_levels, mg_pre_post=2,
                              mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
                              mg_bf16_coarse=args.mg_bf16_coarse, mg_bottom_exact=args.mg_bottom_exact,
                              mg_check_every=args.mg_check_every)
@@ -221,6 +221,8 @@ def main():
     ap.add_argument("--mg-rbgs-bottom", dest="mg_bottom_exact", action="store_false",
                     help="the reference's 50 RBGS sweeps on the coarsest level instead of the exact DCT solve")
     ap.add_argument("--mg-check-every", type=int, default=1, help="PCG iterations per convergence poll")
+    ap.add_argument("--mg-levels", type=int, default=0,
+                    help="MG levels (0: log2(n) - 2, the reference's count: 5 at 128^3, 6 at 256^3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -244,7 +246,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded vortex tubes / box-union density / Gaussian target)",
         "config": {"workload": "3D forward+adjoint leapfrog flow-map, %d^3 MAC grid, RK4, reinit every 10, "
-                               "MGPCG 5 levels 2+2 RBGS tol 1e-6" % args.size,
+                               "MGPCG %s levels 2+2 RBGS tol 1e-6%s" % (
+                                   args.size, args.mg_levels or "auto", ", BF16 coarse levels" if args.mg_bf16_coarse else ""),
                    "grid": [args.size] * 3, "reinit_every": 10, "steps_timed": K},
         "roofline": roofline,
         "mgpcg": mgpcg,
