@@ -63,6 +63,18 @@ def sl_flop(n, nsrc_vec, nsrc_scal, order=4):
     return f + c
 
 
+def measured_traffic(dom, n):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when one
+    exists for this kernel and grid; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_3d128.json")
+    if dom != "march" or tuple(n) != (128, 128, 128) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("traffic_bytes")
+
+
 def run_gpu(args):
     import numpy as np
     import torch
@@ -123,7 +135,7 @@ def run_gpu(args):
         flop = faces * 3 * 216 * 4
     achieved = flop / avg_s / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, n),
                 "kernel": models[dom][1], "avg_launch_ms": round(avg_s * 1e3, 4), "calls": calls,
                 "flop_per_launch": flop,
                 "note": "FP32 vector roofline (gather/VALU-bound B-spline stencils; no MFMA on this path)"}
@@ -208,6 +220,21 @@ def run_cpu_baseline(size):
                       f"restated MGPCG; {el:.1f} s"}
 
 
+def run_copy_calibration():
+    """Plain device-to-device copy of a known size (256 MiB, above the 256 MB
+    MALL), the reference point that scales the FETCH_SIZE / WRITE_SIZE
+    counters of a profiled kernel to HBM bytes."""
+    import torch
+
+    nbytes = 256 << 20
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,7 +250,12 @@ def main():
     ap.add_argument("--mg-check-every", type=int, default=1, help="PCG iterations per convergence poll")
     ap.add_argument("--mg-levels", type=int, default=0,
                     help="MG levels (0: log2(n) - 2, the reference's count: 5 at 128^3, 6 at 256^3)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 256 MiB copy used to calibrate the HBM traffic counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        run_copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design note, not a benchmarked path.", file=sys.stderr)
