@@ -45,7 +45,7 @@ constexpr int kMaxLevels = 12;
 constexpr int kLdsCells = 8192;  // bottom solve in one workgroup up to this size
 constexpr double kEps = 1e-12;   // denominator guard (2D/vortex/mgpcg.py:204)
 
-enum Slot { S_ZR = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THRESH, S_MEAN, S_PAP, S_NSLOT };
+enum Slot { S_ZR = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THRESH, S_MEAN, S_PAP, S_NFLUID, S_NSLOT };
 enum Flag { F_DONE = 0, F_ITERS, F_STATUS, F_NTRUE, F_NFLAG };  // F_NTRUE: failed true-residual checks
 
 struct Lvl {
@@ -60,6 +60,7 @@ struct Dims {
   FastDiv f2;  // / n[2]
   FastDiv f1;  // / n[1]
   FastDiv fh;  // / cell pairs along the innermost axis ((inner + 1) / 2)
+  const uint8_t* bm;  // obstacle flags (1 = solid) of this level, or nullptr
 };
 
 inline Dims make_dims(int n0, int n1, int n2, int dim) {
@@ -71,6 +72,7 @@ inline Dims make_dims(int n0, int n1, int n2, int dim) {
   d.f1 = make_fastdiv((uint32_t)n1);
   const int inner = dim == 3 ? n2 : n1;
   d.fh = make_fastdiv((uint32_t)((inner + 1) >> 1));
+  d.bm = nullptr;
   return d;
 }
 
@@ -79,27 +81,50 @@ DFM_HD uint32_t cidx(const Dims& d, int i, int j, int k) {
   return ((uint32_t)i * (uint32_t)d.n[1] + (uint32_t)j) * (uint32_t)d.n[2] + (uint32_t)k;
 }
 
-// neighbour sum and count (num_fluid_neighbors / neighbor_sum, 2D/vortex/mgpcg.py:92-115)
-template <int DIM>
+template <bool MASK> DFM_HD bool is_fluid(const Dims& d, uint32_t id) { return !MASK || d.bm[id] == 0; }
+
+// Visit the in-grid neighbours of cell (i, j, k) = id, restricted to fluid
+// cells when MASK (num_fluid_neighbors / neighbor_sum, 2D/vortex/mgpcg.py:92-115;
+// the obstacle form num_fluid_neighbors2 / neighbor_sum2,
+// 2D/vortex/mgpcg_solid.py:140-170), in the order -x, +x, -y, +y, -z, +z.
+template <int DIM, bool MASK, typename F>
+DFM_HD void for_nbrs(const Dims& d, int i, int j, int k, uint32_t id, F&& fn) {
+  const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
+  auto visit = [&](bool inside, uint32_t q, int a, int b, int c) {
+    if (inside && is_fluid<MASK>(d, q)) fn(q, a, b, c);
+  };
+  visit(i > 0, id - sx, i - 1, j, k);
+  visit(i < d.n[0] - 1, id + sx, i + 1, j, k);
+  visit(j > 0, id - sy, i, j - 1, k);
+  visit(j < d.n[1] - 1, id + sy, i, j + 1, k);
+  if (DIM == 3) {
+    visit(k > 0, id - 1, i, j, k - 1);
+    visit(k < d.n[2] - 1, id + 1, i, j, k + 1);
+  }
+}
+
+// neighbour sum and count
+template <int DIM, bool MASK = false>
 DFM_HD float nbr_sum(const float* __restrict__ f, const Dims& d, int i, int j, int k, float& nn) {
   float s = 0.f;
   nn = 0.f;
-  if (i > 0) { s += f[cidx(d, i - 1, j, k)]; nn += 1.f; }
-  if (i < d.n[0] - 1) { s += f[cidx(d, i + 1, j, k)]; nn += 1.f; }
-  if (j > 0) { s += f[cidx(d, i, j - 1, k)]; nn += 1.f; }
-  if (j < d.n[1] - 1) { s += f[cidx(d, i, j + 1, k)]; nn += 1.f; }
-  if (DIM == 3) {
-    if (k > 0) { s += f[cidx(d, i, j, k - 1)]; nn += 1.f; }
-    if (k < d.n[2] - 1) { s += f[cidx(d, i, j, k + 1)]; nn += 1.f; }
-  }
+  for_nbrs<DIM, MASK>(d, i, j, k, cidx(d, i, j, k), [&](uint32_t q, int, int, int) {
+    s += f[q];
+    nn += 1.f;
+  });
   return s;
 }
 
-template <int DIM>
+template <int DIM, bool MASK = false>
 DFM_HD float nbr_count(const Dims& d, int i, int j, int k) {
+  if (!MASK) {
+    float nn = 0.f;
+    nn += (i > 0) + (i < d.n[0] - 1) + (j > 0) + (j < d.n[1] - 1);
+    if (DIM == 3) nn += (k > 0) + (k < d.n[2] - 1);
+    return nn;
+  }
   float nn = 0.f;
-  nn += (i > 0) + (i < d.n[0] - 1) + (j > 0) + (j < d.n[1] - 1);
-  if (DIM == 3) nn += (k > 0) + (k < d.n[2] - 1);
+  for_nbrs<DIM, MASK>(d, i, j, k, cidx(d, i, j, k), [&](uint32_t, int, int, int) { nn += 1.f; });
   return nn;
 }
 
@@ -114,7 +139,8 @@ DFM_HD void cell_of(const Dims& d, uint32_t id, int& i, int& j, int& k) {
 }
 
 // 1/nn for the neighbour counts that occur (1..6): one multiply instead of an
-// IEEE divide per update.
+// IEEE divide per update.  A fluid cell walled in by obstacles (nn = 0) has
+// no equation; it gets 0 (the reference would divide by zero).
 DFM_HD float inv_count(float nn) {
   const int c = (int)nn;
   return c == 6 ? (1.0f / 6.0f)
@@ -122,7 +148,8 @@ DFM_HD float inv_count(float nn) {
        : c == 4 ? 0.25f
        : c == 3 ? (1.0f / 3.0f)
        : c == 2 ? 0.5f
-                : 1.0f;
+       : c == 1 ? 1.0f
+                : 0.0f;
 }
 
 DFM_HD bool is_done(const int* flags) { return flags[F_DONE] != 0; }
@@ -149,7 +176,7 @@ DFM_HD void block_sum2(double& a, double& b) {
 }
 
 // Scalar steps of the PCG recurrence, run on the grid totals (a, b).
-enum FinOp { FIN_MEAN = 0, FIN_BNORM, FIN_ZR_INIT, FIN_ALPHA, FIN_RR, FIN_BETA, FIN_TRUE };
+enum FinOp { FIN_MEAN = 0, FIN_BNORM, FIN_ZR_INIT, FIN_ALPHA, FIN_RR, FIN_BETA, FIN_TRUE, FIN_COUNT };
 
 // Grid-reduction context: partial sums (2 x kRedBlocks), the arrival counters
 // (zero between reductions), the FP64 scalars and the solve flags.
@@ -204,6 +231,9 @@ DFM_HD void fin_op(const Red& R, int op, double a, double b) {
       sc[S_ZR] = a;
       break;
     }
+    case FIN_COUNT:  // a = fluid cells of an obstacle mask
+      sc[S_NFLUID] = a;
+      break;
     case FIN_TRUE: {  // a = |b - A x|^2 in FP64, b = sum(b - A x): refinement check
       sc[S_RR] = a;
       sc[S_MEAN] = b * R.inv_n;
@@ -298,58 +328,75 @@ struct Faces {
 
 // apply_bc + divergence (no 1/dx) + b = -div + partial sum of b
 // (apply_bc/divergence, 2D/vortex/mgpcg.py:294-336; 3D/mgpcg.py:380-411).
-template <int DIM>
+// MASK: the obstacle form (MGPCG_2_solid.apply_bc/divergence,
+// 2D/vortex/mgpcg_solid.py:404-431): every face of a solid cell is zeroed and
+// a solid cell's divergence is 0.  A fluid cell reads a face next to a solid
+// cell as 0, so the zeroing writes of other threads never feed a result.
+template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_div(Dims d, Faces u, float* __restrict__ b, Red R) {
   double acc = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
+  const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
   for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     int i, j, k;
     cell_of(d, id, i, j, k);
     // face grids: x (n0+1, n1, n2), y (n0, n1+1, n2), z (n0, n1, n2+1)
     Dims fx{{d.n[0] + 1, d.n[1], d.n[2]}}, fy{{d.n[0], d.n[1] + 1, d.n[2]}}, fz{{d.n[0], d.n[1], d.n[2] + 1}};
+    const bool solid = !is_fluid<MASK>(d, id);
+    // open faces: not a wall, and (MASK) neither adjacent cell solid
+    const bool oxl = i > 0 && !solid && is_fluid<MASK>(d, id - sx);
+    const bool oxr = i < d.n[0] - 1 && !solid && is_fluid<MASK>(d, id + sx);
+    const bool oyl = j > 0 && !solid && is_fluid<MASK>(d, id - sy);
+    const bool oyr = j < d.n[1] - 1 && !solid && is_fluid<MASK>(d, id + sy);
     float div = 0.f;
-    float xl = (i == 0) ? 0.f : u.c[0][cidx(fx, i, j, k)];
-    float xr = (i == d.n[0] - 1) ? 0.f : u.c[0][cidx(fx, i + 1, j, k)];
-    float yl = (j == 0) ? 0.f : u.c[1][cidx(fy, i, j, k)];
-    float yr = (j == d.n[1] - 1) ? 0.f : u.c[1][cidx(fy, i, j + 1, k)];
+    float xl = oxl ? u.c[0][cidx(fx, i, j, k)] : 0.f;
+    float xr = oxr ? u.c[0][cidx(fx, i + 1, j, k)] : 0.f;
+    float yl = oyl ? u.c[1][cidx(fy, i, j, k)] : 0.f;
+    float yr = oyr ? u.c[1][cidx(fy, i, j + 1, k)] : 0.f;
     div = (xr - xl) + (yr - yl);
-    if (i == 0) u.c[0][cidx(fx, 0, j, k)] = 0.f;
-    if (i == d.n[0] - 1) u.c[0][cidx(fx, d.n[0], j, k)] = 0.f;
-    if (j == 0) u.c[1][cidx(fy, i, 0, k)] = 0.f;
-    if (j == d.n[1] - 1) u.c[1][cidx(fy, i, d.n[1], k)] = 0.f;
+    if (i == 0 || solid) u.c[0][cidx(fx, i, j, k)] = 0.f;
+    if (i == d.n[0] - 1 || solid) u.c[0][cidx(fx, i + 1, j, k)] = 0.f;
+    if (j == 0 || solid) u.c[1][cidx(fy, i, j, k)] = 0.f;
+    if (j == d.n[1] - 1 || solid) u.c[1][cidx(fy, i, j + 1, k)] = 0.f;
     if (DIM == 3) {
-      float zl = (k == 0) ? 0.f : u.c[2][cidx(fz, i, j, k)];
-      float zr = (k == d.n[2] - 1) ? 0.f : u.c[2][cidx(fz, i, j, k + 1)];
+      const bool ozl = k > 0 && !solid && is_fluid<MASK>(d, id - 1);
+      const bool ozr = k < d.n[2] - 1 && !solid && is_fluid<MASK>(d, id + 1);
+      float zl = ozl ? u.c[2][cidx(fz, i, j, k)] : 0.f;
+      float zr = ozr ? u.c[2][cidx(fz, i, j, k + 1)] : 0.f;
       div += (zr - zl);
-      if (k == 0) u.c[2][cidx(fz, i, j, 0)] = 0.f;
-      if (k == d.n[2] - 1) u.c[2][cidx(fz, i, j, d.n[2])] = 0.f;
+      if (k == 0 || solid) u.c[2][cidx(fz, i, j, k)] = 0.f;
+      if (k == d.n[2] - 1 || solid) u.c[2][cidx(fz, i, j, k + 1)] = 0.f;
     }
-    float bv = -div;
+    float bv = solid ? 0.f : -div;
     b[id] = bv;
     acc += (double)bv;
   }
   grid_reduce(R, FIN_MEAN, acc, 0.0);
 }
 
-// copy b into r (for dfm_mgpcg_solve) with partial sum
+// copy b into r (for dfm_mgpcg_solve) with partial sum; solid cells get 0
+template <bool MASK>
 __global__ void __launch_bounds__(kB) k_load_b(int64_t N, const float* __restrict__ b, float* __restrict__ r,
-                                               Red R) {
+                                               const uint8_t* __restrict__ bm, Red R) {
   double acc = 0;
   for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
-    float v = b[id];
+    float v = (!MASK || bm[id] == 0) ? b[id] : 0.f;
     r[id] = v;
     acc += (double)v;
   }
   grid_reduce(R, FIN_MEAN, acc, 0.0);
 }
 
-// r -= mean (kept as b for the refinement residual); partial r.r; x = 0
+// r -= mean (kept as b for the refinement residual); partial r.r; x = 0.
+// Solid cells keep r = 0 (the mean is over fluid cells).
+template <bool MASK>
 __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restrict__ r, float* __restrict__ bkeep,
-                                                      double* __restrict__ x, Red R) {
+                                                      double* __restrict__ x, const uint8_t* __restrict__ bm,
+                                                      Red R) {
   float m = (float)R.sc[S_MEAN];
   double acc = 0;
   for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
-    float v = r[id] - m;
+    float v = (!MASK || bm[id] == 0) ? r[id] - m : 0.f;
     r[id] = v;
     bkeep[id] = v;
     x[id] = 0.0;
@@ -358,13 +405,14 @@ __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restri
   grid_reduce(R, FIN_BNORM, acc, 0.0);
 }
 
-// subtract_grad_p + apply_bc (2D/vortex/mgpcg.py:338-356, 3D/mgpcg.py:413-439):
-// interior faces u_a -= p[I] - p[I - e_a]; boundary faces = 0.
+// subtract_grad_p + apply_bc (2D/vortex/mgpcg.py:338-356, 3D/mgpcg.py:413-439;
+// obstacle form 2D/vortex/mgpcg_solid.py:433-461): interior faces
+// u_a -= p[I] - p[I - e_a]; wall faces and (MASK) faces of solid cells = 0.
 struct FaceDims {
   Dims f[3];  // face grid of axis a (cells + e_a)
 };
 
-template <int DIM>
+template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_subgrad(Dims d, FaceDims fdims, Faces u, const double* __restrict__ p) {
   int64_t nf[3];
   const Dims* fd = fdims.f;
@@ -375,24 +423,32 @@ __global__ void __launch_bounds__(kB) k_subgrad(Dims d, FaceDims fdims, Faces u,
     int64_t id = g;
     if (id >= nf[0]) { id -= nf[0]; a = 1; if (id >= nf[1]) { id -= nf[1]; a = 2; } }
     int i, j, k;
-    cell_of(fd[a], id, i, j, k);
+    cell_of(fd[a], (uint32_t)id, i, j, k);
     int I[3] = {i, j, k};
     float* c = u.c[a];
-    if (I[a] == 0 || I[a] == d.n[a]) {
+    int lo[3] = {i, j, k};
+    lo[a] -= 1;
+    bool open = !(I[a] == 0 || I[a] == d.n[a]);
+    uint32_t chi = 0, clo = 0;
+    if (open) {
+      chi = cidx(d, i, j, k);
+      clo = cidx(d, lo[0], lo[1], lo[2]);
+      open = is_fluid<MASK>(d, chi) && is_fluid<MASK>(d, clo);
+    }
+    if (!open) {
       c[id] = 0.f;
     } else {
-      int lo[3] = {i, j, k};
-      lo[a] -= 1;
       // the pressure difference is formed in FP64: p grows like n |u| in
       // these units, so an FP32 p would cost ~n ulps of velocity
-      c[id] = (float)((double)c[id] - (p[cidx(d, i, j, k)] - p[cidx(d, lo[0], lo[1], lo[2])]));
+      c[id] = (float)((double)c[id] - (p[chi] - p[clo]));
     }
   }
 }
 
 // Iterative-refinement residual r = b - A x evaluated in FP64 (x is FP64;
 // b is the recentred FP32 right-hand side); partial |r|^2 and sum r.
-template <int DIM>
+// Solid cells: r = 0 (b and the operator row are 0 there).
+template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restrict__ x, const float* __restrict__ b,
                                                    float* __restrict__ r, Red R, int if_converged) {
   // only after convergence on the recursive residual (uniform over the grid:
@@ -403,16 +459,15 @@ __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restr
   for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     int i, j, k;
     cell_of(d, id, i, j, k);
-    double s = 0.0, nn = 0.0;
-    if (i > 0) { s += x[cidx(d, i - 1, j, k)]; nn += 1.0; }
-    if (i < d.n[0] - 1) { s += x[cidx(d, i + 1, j, k)]; nn += 1.0; }
-    if (j > 0) { s += x[cidx(d, i, j - 1, k)]; nn += 1.0; }
-    if (j < d.n[1] - 1) { s += x[cidx(d, i, j + 1, k)]; nn += 1.0; }
-    if (DIM == 3) {
-      if (k > 0) { s += x[cidx(d, i, j, k - 1)]; nn += 1.0; }
-      if (k < d.n[2] - 1) { s += x[cidx(d, i, j, k + 1)]; nn += 1.0; }
+    double v = 0.0;
+    if (is_fluid<MASK>(d, id)) {
+      double s = 0.0, nn = 0.0;
+      for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int, int, int) {
+        s += x[q];
+        nn += 1.0;
+      });
+      v = (double)b[id] - (nn * x[id] - s);
     }
-    double v = (double)b[id] - (nn * x[id] - s);
     r[id] = (float)v;
     rr += v * v;
     rs += v;
@@ -426,8 +481,9 @@ __global__ void __launch_bounds__(kB) k_to_float(int64_t N, const double* __rest
 }
 
 // ------------------------------------------------------------ PCG steps ----
-// Ap = A p ; partial p.Ap
-template <int DIM>
+// Ap = A p ; partial p.Ap.  MASK: solid rows are 0 (compute_Ap of the
+// obstacle solver updates fluid cells only, 2D/vortex/mgpcg_solid.py:172-178).
+template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__ p, float* __restrict__ Ap,
                                                Red R) {
   if (is_done(R.flags)) return;
@@ -436,17 +492,21 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
   for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
     int i, j, k;
     cell_of(d, id, i, j, k);
-    float nn;
-    float s = nbr_sum<DIM>(p, d, i, j, k, nn);
-    float pv = p[id];
-    float v = nn * pv - s;
+    float v = 0.f, pv = 0.f;
+    if (is_fluid<MASK>(d, id)) {
+      float nn;
+      float s = nbr_sum<DIM, MASK>(p, d, i, j, k, nn);
+      pv = p[id];
+      v = nn * pv - s;
+    }
     Ap[id] = v;
     acc += (double)pv * (double)v;
   }
   grid_reduce(R, FIN_ALPHA, acc, 0.0);
 }
 
-// x += alpha p (x in FP64) ; r -= alpha Ap ; partial r.r and sum r
+// x += alpha p (x in FP64) ; r -= alpha Ap ; partial r.r and sum r.
+// (Solid cells hold p = Ap = r = 0 throughout, so they need no test here.)
 __global__ void __launch_bounds__(kB) k_update_xr(int64_t N, double* __restrict__ x, float* __restrict__ r,
                                                   const float* __restrict__ p, const float* __restrict__ Ap,
                                                   Red R) {
@@ -466,15 +526,19 @@ __global__ void __launch_bounds__(kB) k_update_xr(int64_t N, double* __restrict_
 
 // r -= shift (written back) ; z.r and its scalar step `op` (FIN_ZR_INIT or
 // FIN_BETA); p = z when init_p.  Used when level 0 is not smoothed by the
-// global kernels (otherwise the last level-0 half-sweep does this).
+// global kernels (otherwise the last level-0 half-sweep does this).  Solid
+// cells keep r = 0.
+template <bool MASK>
 __global__ void __launch_bounds__(kB) k_zr(int64_t N, const float* __restrict__ z, float* __restrict__ r,
-                                           const double* shift, Red R, int op, int init_p, float* __restrict__ p) {
+                                           const double* shift, const uint8_t* __restrict__ bm, Red R, int op,
+                                           int init_p, float* __restrict__ p) {
   if (is_done(R.flags)) return;
   float m = shift ? (float)*shift : 0.f;
   double acc = 0;
   for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
-    float rv = r[id] - m;
-    if (shift) r[id] = rv;
+    const bool fl = !MASK || bm[id] == 0;
+    float rv = fl ? r[id] - m : 0.f;
+    if (shift && fl) r[id] = rv;
     float zv = z[id];
     if (init_p) p[id] = zv;
     acc += (double)zv * (double)rv;
@@ -494,7 +558,9 @@ __global__ void __launch_bounds__(kB) k_update_p(int64_t N, const float* __restr
 // -------------------------------------------------------------- V-cycle ----
 // First RBGS sweep from z = 0: colour 0 cells z = r/nn; colour 1 cells use the
 // colour-0 neighbours' r/nn.  `shift` (level 0) recentres r on the fly.
-template <int DIM>
+// MASK: solid cells get z = 0 and are not neighbours (smooth of the obstacle
+// solver, 2D/vortex/mgpcg_solid.py:229-236).
+template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restrict__ r, float* __restrict__ z,
                                                     const double* shift, const int* flags) {
   if (is_done(flags)) return;
@@ -502,23 +568,22 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
   uint32_t id = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
   if (id >= N) return;
+  if (!is_fluid<MASK>(d, id)) {
+    z[id] = 0.f;
+    return;
+  }
   int i, j, k;
   cell_of(d, id, i, j, k);
-  float nn = nbr_count<DIM>(d, i, j, k);
+  float nn = nbr_count<DIM, MASK>(d, i, j, k);
   float rv = r[id] - m;
   if (((i + j + k) & 1) == 0) {
     z[id] = rv * inv_count(nn);
     return;
   }
   float s = 0.f;
-  if (i > 0) s += (r[cidx(d, i - 1, j, k)] - m) * inv_count(nbr_count<DIM>(d, i - 1, j, k));
-  if (i < d.n[0] - 1) s += (r[cidx(d, i + 1, j, k)] - m) * inv_count(nbr_count<DIM>(d, i + 1, j, k));
-  if (j > 0) s += (r[cidx(d, i, j - 1, k)] - m) * inv_count(nbr_count<DIM>(d, i, j - 1, k));
-  if (j < d.n[1] - 1) s += (r[cidx(d, i, j + 1, k)] - m) * inv_count(nbr_count<DIM>(d, i, j + 1, k));
-  if (DIM == 3) {
-    if (k > 0) s += (r[cidx(d, i, j, k - 1)] - m) * inv_count(nbr_count<DIM>(d, i, j, k - 1));
-    if (k < d.n[2] - 1) s += (r[cidx(d, i, j, k + 1)] - m) * inv_count(nbr_count<DIM>(d, i, j, k + 1));
-  }
+  for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
+    s += (r[q] - m) * inv_count(nbr_count<DIM, MASK>(d, a, b, c));
+  });
   z[id] = (rv + s) * inv_count(nn);
 }
 
@@ -564,26 +629,19 @@ DFM_HD bool colour_cell(const Dims& d, uint32_t t, int phase, int& i, int& j, in
 // Gauss-Seidel update of cell (i, j, k): (r - m + sum of neighbour z) / nn.
 // With zc, the neighbours' values are read as z + zc[coarse parent] (fused
 // prolongation; the cell's own value is overwritten, so it never needs it).
-template <int DIM>
+template <int DIM, bool MASK>
 DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* __restrict__ z,
                       const float* __restrict__ zc, const Dims& dc, float m, int i, int j, int k, uint32_t id) {
   float nn, s;
   if (zc) {
-    auto zf = [&](int a, int b, int c) {
-      return z[cidx(d, a, b, c)] + zc[cidx(dc, a >> 1, b >> 1, DIM == 3 ? (c >> 1) : 0)];
-    };
     s = 0.f;
     nn = 0.f;
-    if (i > 0) { s += zf(i - 1, j, k); nn += 1.f; }
-    if (i < d.n[0] - 1) { s += zf(i + 1, j, k); nn += 1.f; }
-    if (j > 0) { s += zf(i, j - 1, k); nn += 1.f; }
-    if (j < d.n[1] - 1) { s += zf(i, j + 1, k); nn += 1.f; }
-    if (DIM == 3) {
-      if (k > 0) { s += zf(i, j, k - 1); nn += 1.f; }
-      if (k < d.n[2] - 1) { s += zf(i, j, k + 1); nn += 1.f; }
-    }
+    for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
+      s += z[q] + zc[cidx(dc, a >> 1, b >> 1, DIM == 3 ? (c >> 1) : 0)];
+      nn += 1.f;
+    });
   } else {
-    s = nbr_sum<DIM>(z, d, i, j, k, nn);
+    s = nbr_sum<DIM, MASK>(z, d, i, j, k, nn);
   }
   return (r[id] - m + s) * inv_count(nn);
 }
@@ -593,7 +651,8 @@ DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* _
 // half-sweep of the V-cycle, so it also writes the recentred r back for both
 // cells of each pair, forms z.r over them (the other colour's z is final) and
 // runs the scalar step `op` on the total (p = z as well when p is given).
-template <int DIM, bool ZR>
+// MASK: solid cells are skipped (z and r stay 0 there).
+template <int DIM, bool ZR, bool MASK>
 __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, float* __restrict__ z, int phase,
                                                const double* shift, const float* __restrict__ zc, Dims dc, Red R,
                                                int op, float* __restrict__ p) {
@@ -608,22 +667,28 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
     if (own < inner) {
       const int a = DIM == 3 ? j : own, k = DIM == 3 ? own : 0;
       const uint32_t id = cidx(d, i, a, k);
-      const float zn = gs_value<DIM>(d, r, z, zc, dc, m, i, a, k, id);
-      z[id] = zn;
-      if (ZR) {
-        const float rv = r[id] - m;
-        if (shift) r[id] = rv;
-        if (p) p[id] = zn;
-        acc += (double)zn * (double)rv;
+      if (is_fluid<MASK>(d, id)) {
+        const float zn = gs_value<DIM, MASK>(d, r, z, zc, dc, m, i, a, k, id);
+        z[id] = zn;
+        if (ZR) {
+          const float rv = r[id] - m;
+          if (shift) r[id] = rv;
+          if (p) p[id] = zn;
+          acc += (double)zn * (double)rv;
+        }
+      } else if (ZR && p) {
+        p[id] = 0.f;
       }
     }
     if (ZR && other < inner) {
       const uint32_t id = DIM == 3 ? cidx(d, i, j, other) : cidx(d, i, other, 0);
-      const float rv = r[id] - m;
-      if (shift) r[id] = rv;
       const float zo = z[id];
+      if (is_fluid<MASK>(d, id)) {
+        const float rv = r[id] - m;
+        if (shift) r[id] = rv;
+        acc += (double)zo * (double)rv;
+      }
       if (p) p[id] = zo;
-      acc += (double)zo * (double)rv;
     }
   }
   if (ZR) grid_reduce(R, op, acc, 0.0);
@@ -631,8 +696,9 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
 
 // restrict (2D/vortex/mgpcg.py:152-158): coarse r = sum over the 2^d children
 // of the fine residual r - A z, times 1/(dim-1).  One thread per coarse cell,
-// so the sum order is fixed.
-template <int DIM>
+// so the sum order is fixed.  MASK: fluid children only
+// (2D/vortex/mgpcg_solid.py:216-224).
+template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict__ r, const float* __restrict__ z,
                                                  Dims dc, float* __restrict__ rc, const double* shift,
                                                  const int* flags) {
@@ -649,12 +715,43 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
     for (int b = 0; b < 2; ++b)
       for (int c = 0; c < kz; ++c) {
         int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
-        float nn;
-        float s = nbr_sum<DIM>(z, d, i, j, k, nn);
         uint32_t id = cidx(d, i, j, k);
+        if (!is_fluid<MASK>(d, id)) continue;
+        float nn;
+        float s = nbr_sum<DIM, MASK>(z, d, i, j, k, nn);
         acc += (r[id] - m) - (nn * z[id] - s);
       }
   rc[cid] = acc * (1.0f / (float)(DIM - 1));
+}
+
+// Obstacle flags of the MG levels: level 0 = mask > 0 (the solver's
+// bm[0] <= 0 fluid test); a coarse cell is solid iff all its 2^d children
+// are (downsample_bm, 2D/vortex/mgpcg_solid.py:98-112).
+__global__ void __launch_bounds__(kB) k_mask_level0(int64_t N, const float* __restrict__ mask,
+                                                    uint8_t* __restrict__ bm, Red R) {
+  double fluid = 0;
+  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
+    const uint8_t sld = mask[id] > 0.f ? 1 : 0;
+    bm[id] = sld;
+    fluid += sld ? 0.0 : 1.0;
+  }
+  grid_reduce(R, FIN_COUNT, fluid, 0.0);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kB) k_mask_coarsen(Dims d, const uint8_t* __restrict__ bm, Dims dc,
+                                                     uint8_t* __restrict__ bmc) {
+  int64_t Nc = (int64_t)dc.n[0] * dc.n[1] * dc.n[2];
+  uint32_t cid = blockIdx.x * kB + threadIdx.x;
+  if (cid >= Nc) return;
+  int ci, cj, ck;
+  cell_of(dc, cid, ci, cj, ck);
+  uint8_t all = 1;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < (DIM == 3 ? 2 : 1); ++c)
+        all &= bm[cidx(d, 2 * ci + a, 2 * cj + b, DIM == 3 ? 2 * ck + c : 0)];
+  bmc[cid] = all;
 }
 
 // --------------------------------------------------- coarse tail in LDS ----
@@ -897,6 +994,10 @@ struct Solver {
   hipStream_t cap;
   hipGraphExec_t gexec;
   int gexec_iters;
+  // obstacle mask (MGPCG_2_solid): solid flags per level, fluid cell count
+  int masked;
+  uint8_t* bm[kMaxLevels];
+  double nfluid;
 };
 
 // Largest LDS a workgroup may request on this device.
@@ -949,6 +1050,10 @@ bool enable_tail_lds(const Solver& s) {
 // Choose the tail for the current options; falls back to the 64 KB static
 // limit (then to no tail) if the runtime refuses the larger allocation.
 void setup_tail(Solver& s) {
+  if (s.masked) {  // the LDS tail and its exact bottom solve assume no obstacles
+    s.tail_lt = s.L;
+    return;
+  }
   configure_tail(s, max_lds_bytes());
   if (!enable_tail_lds(s)) {
     (void)hipGetLastError();
@@ -967,7 +1072,8 @@ inline unsigned gridr(int64_t n) {
 }
 
 inline Red red_of(const Solver& s, double tol = 0.0, int relative = 0) {
-  return Red{s.part, s.ctr, s.sc, s.flags, 1.0 / (double)s.lv[0].size, tol, relative};
+  const double n = s.masked ? (s.nfluid > 0 ? s.nfluid : 1.0) : (double)s.lv[0].size;
+  return Red{s.part, s.ctr, s.sc, s.flags, 1.0 / n, tol, relative};
 }
 
 inline uint32_t pairs_of(const Dims& d, int dim) {
@@ -985,7 +1091,7 @@ struct ZrFuse {
 // starting from z = 0 when from_zero (first sweep fused into one kernel).
 // zc_prolong fuses the prolongation into the first half-sweep; zr (if given)
 // is fused into the last half-sweep.  Returns whether zr was done.
-template <int DIM>
+template <int DIM, bool MASK>
 bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero, const double* shift,
                    const float* zc_prolong, const ZrFuse* zr) {
   Lvl& L = s.lv[l];
@@ -994,7 +1100,8 @@ bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool fr
   const Red R = red_of(s);
   int k0 = 0;
   if (from_zero) {
-    hipLaunchKernelGGL((k_first_sweep<DIM>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, L.z, shift, s.flags);
+    hipLaunchKernelGGL((k_first_sweep<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, L.z, shift,
+                       s.flags);
     k0 = 1;
   }
   const uint32_t pairs = pairs_of(d, DIM);
@@ -1002,14 +1109,14 @@ bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool fr
   bool zr_done = false;
   for (int it = k0; it < sweeps; ++it) {
     const float* zc = (it == k0 && zc_prolong) ? zc_prolong : nullptr;
-    hipLaunchKernelGGL((k_smooth<DIM, false>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, c1, shift, zc, dc, R, 0,
-                       (float*)nullptr);
+    hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, c1, shift, zc, dc, R,
+                       0, (float*)nullptr);
     if (zr && it == sweeps - 1) {
-      hipLaunchKernelGGL((k_smooth<DIM, true>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift,
-                         (const float*)nullptr, dc, R, zr->op, zr->p);
+      hipLaunchKernelGGL((k_smooth<DIM, true, MASK>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, L.z, 1 - c1,
+                         shift, (const float*)nullptr, dc, R, zr->op, zr->p);
       zr_done = true;
     } else {
-      hipLaunchKernelGGL((k_smooth<DIM, false>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift,
+      hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift,
                          (const float*)nullptr, dc, R, 0, (float*)nullptr);
     }
   }
@@ -1018,8 +1125,10 @@ bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool fr
 
 // apply_preconditioner (2D/vortex/mgpcg.py:182-200): z0 = M^-1 r0, followed by
 // z.r and its scalar step `zr`.  Levels below tail_lt use global kernels;
-// levels tail_lt.. run in one LDS kernel.
-template <int DIM>
+// levels tail_lt.. run in one LDS kernel.  With an obstacle mask the bottom
+// level gets bottom/2 sweeps in the order (0, 1) then bottom/2 in (1, 0)
+// (2D/vortex/mgpcg_solid.py:263-268) and no LDS tail.
+template <int DIM, bool MASK>
 void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   float* cur[kMaxLevels] = {nullptr};
   const bool tail = s.tail_lt < s.L;
@@ -1027,12 +1136,12 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   bool zr_done = false;
   for (int l = 0; l < Lg; ++l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    sweeps_global<DIM>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr);
+    sweeps_global<DIM, MASK>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr);
     cur[l] = s.lv[l].z;
     Lvl& F = s.lv[l];
     Lvl& C = s.lv[l + 1];
-    hipLaunchKernelGGL((k_restrict<DIM>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r, cur[l], s.ld[l + 1],
-                       C.r, sh, s.flags);
+    hipLaunchKernelGGL((k_restrict<DIM, MASK>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r, cur[l],
+                       s.ld[l + 1], C.r, sh, s.flags);
   }
   if (tail) {
     Lvl& T = s.lv[s.tail_lt];
@@ -1044,20 +1153,27 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
     cur[s.tail_lt] = T.z;
   } else {
     const double* sh = s.L == 1 ? shift0 : nullptr;
-    zr_done = sweeps_global<DIM>(s, st, s.L - 1, s.bottom, 0, true, sh, nullptr, s.L == 1 ? &zr : nullptr);
+    const ZrFuse* zb = s.L == 1 ? &zr : nullptr;
+    if (MASK && s.bottom >= 2) {
+      const int first = (s.bottom + 1) / 2;
+      sweeps_global<DIM, MASK>(s, st, s.L - 1, first, 0, true, sh, nullptr, nullptr);
+      zr_done = sweeps_global<DIM, MASK>(s, st, s.L - 1, s.bottom - first, 1, false, sh, nullptr, zb);
+    } else {
+      zr_done = sweeps_global<DIM, MASK>(s, st, s.L - 1, s.bottom, 0, true, sh, nullptr, zb);
+    }
     cur[s.L - 1] = s.lv[s.L - 1].z;
   }
   for (int l = Lg - 1; l >= 0; --l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    const bool d0 = sweeps_global<DIM>(s, st, l, s.pre, 1, false, sh, cur[l + 1], l == 0 ? &zr : nullptr);
+    const bool d0 = sweeps_global<DIM, MASK>(s, st, l, s.pre, 1, false, sh, cur[l + 1], l == 0 ? &zr : nullptr);
     if (l == 0) zr_done = d0;
     cur[l] = s.lv[l].z;
   }
   s.z0_final = cur[0];
   if (!zr_done) {
     Lvl& L0 = s.lv[0];
-    hipLaunchKernelGGL(k_zr, dim3(gridr(L0.size)), dim3(kB), 0, st, L0.size, (const float*)s.z0_final, L0.r, shift0,
-                       red_of(s), zr.op, zr.p ? 1 : 0, zr.p);
+    hipLaunchKernelGGL((k_zr<MASK>), dim3(gridr(L0.size)), dim3(kB), 0, st, L0.size, (const float*)s.z0_final, L0.r,
+                       shift0, (const uint8_t*)s.bm[0], red_of(s), zr.op, zr.p ? 1 : 0, zr.p);
   }
 }
 
@@ -1065,15 +1181,15 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
 // Ap and p.Ap -> alpha; x/r update and r.r -> convergence flag and mean;
 // recentring V-cycle with z.r -> beta; p update.  Every kernel is a no-op once
 // `done` is set, so iterations may be queued past convergence.
-template <int DIM>
+template <int DIM, bool MASK>
 void pcg_iteration(Solver& s, hipStream_t st) {
   Lvl& L0 = s.lv[0];
   int64_t N = L0.size;
   unsigned gr = gridr(N);
   const Red R = red_of(s);
-  hipLaunchKernelGGL((k_Ap_dot<DIM>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
+  hipLaunchKernelGGL((k_Ap_dot<DIM, MASK>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
   hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, (const float*)s.p, (const float*)s.Ap, R);
-  vcycle<DIM>(s, st, s.sc + S_MEAN, ZrFuse{FIN_BETA, nullptr});
+  vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_BETA, nullptr});
   hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, (const float*)s.z0_final, s.p, (const double*)s.sc,
                      (const int*)s.flags);
 }
@@ -1088,7 +1204,7 @@ void drop_graph(Solver& s) {
 // capture stream, default priority); replayed on the caller's stream.  The
 // graph holds kernel launches only -- every pointer it uses is owned by the
 // handle and stable -- so one capture serves every later solve.
-template <int DIM>
+template <int DIM, bool MASK>
 bool build_graph(Solver& s) {
   drop_graph(s);
   if (!s.cap && hipStreamCreateWithFlags(&s.cap, hipStreamNonBlocking) != hipSuccess) {
@@ -1097,7 +1213,7 @@ bool build_graph(Solver& s) {
   }
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(s.cap, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
-  for (int q = 0; q < s.check_every; ++q) pcg_iteration<DIM>(s, s.cap);
+  for (int q = 0; q < s.check_every; ++q) pcg_iteration<DIM, MASK>(s, s.cap);
   if (hipStreamEndCapture(s.cap, &g) != hipSuccess || !g) {
     (void)hipGetLastError();
     return false;
@@ -1115,13 +1231,14 @@ bool build_graph(Solver& s) {
 
 // PCG from r0 (already in lv[0].r, its mean in sc[S_MEAN]): solve(),
 // 2D/vortex/mgpcg.py:202-282, with FP64 iterative refinement.
-template <int DIM>
+template <int DIM, bool MASK>
 int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   Lvl& L0 = s.lv[0];
   int64_t N = L0.size;
   unsigned gr = gridr(N);
-  hipLaunchKernelGGL(k_recenter_norm, dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, red_of(s, tol, relative));
-  vcycle<DIM>(s, st, nullptr, ZrFuse{FIN_ZR_INIT, s.p});
+  hipLaunchKernelGGL((k_recenter_norm<MASK>), dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, (const uint8_t*)s.bm[0],
+                     red_of(s, tol, relative));
+  vcycle<DIM, MASK>(s, st, nullptr, ZrFuse{FIN_ZR_INIT, s.p});
   int it = 0;
   int refinements = 0;
   bool done = false;
@@ -1130,7 +1247,7 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
            hipStreamSynchronize(st) == hipSuccess;
   };
   bool use_graph = s.use_graph;
-  if (use_graph && (!s.gexec || s.gexec_iters != s.check_every)) use_graph = build_graph<DIM>(s);
+  if (use_graph && (!s.gexec || s.gexec_iters != s.check_every)) use_graph = build_graph<DIM, MASK>(s);
   // Queue as many iterations as the previous solve needed before polling the
   // first time: consecutive projections converge in nearly the same number of
   // iterations, and iterations queued past convergence are no-ops.  The FP64
@@ -1146,13 +1263,13 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
       if (hipGraphLaunch(s.gexec, st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: graph launch failed");
       left -= s.gexec_iters;
     }
-    for (int q = 0; q < left; ++q) pcg_iteration<DIM>(s, st);
+    for (int q = 0; q < left; ++q) pcg_iteration<DIM, MASK>(s, st);
     it += k;
     first_poll = false;
     // Converged on the recursive FP32 residual: measure the true residual
     // b - A x in FP64; if it misses the tolerance, PCG restarts from it.
     if (refinements < s.max_refine)
-      hipLaunchKernelGGL((k_true_resid<DIM>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const double*)s.x,
+      hipLaunchKernelGGL((k_true_resid<DIM, MASK>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const double*)s.x,
                          (const float*)s.b, L0.r, red_of(s), 1);
     if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
     if (s.h_flags[F_DONE]) {
@@ -1162,7 +1279,7 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
     }
     if (s.h_flags[F_NTRUE] > refinements) {
       ++refinements;
-      vcycle<DIM>(s, st, s.sc + S_MEAN, ZrFuse{FIN_ZR_INIT, s.p});
+      vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_ZR_INIT, s.p});
     }
   }
   bool rb = hipMemcpyAsync(s.h_sc, s.sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st) == hipSuccess;
@@ -1190,6 +1307,8 @@ void destroy(Solver* s) {
   if (s->Ap) (void)hipFree(s->Ap);
   if (s->part) (void)hipFree(s->part);
   if (s->ctr) (void)hipFree(s->ctr);
+  for (int l = 0; l < kMaxLevels; ++l)
+    if (s->bm[l]) (void)hipFree(s->bm[l]);
   if (s->sc) (void)hipFree(s->sc);
   if (s->flags) (void)hipFree(s->flags);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
@@ -1336,12 +1455,23 @@ extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int
   Faces F{{u->c[0], u->c[1], u->c[2]}};
   unsigned gr = gridr(L0.size);
   int rc;
+  const Red R = red_of(*s);
   if (s->G.dim == 2) {
-    hipLaunchKernelGGL((k_div<2>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, red_of(*s));
-    rc = pcg<2>(*s, st, tol, relative, max_iters);
+    if (s->masked) {
+      hipLaunchKernelGGL((k_div<2, true>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
+      rc = pcg<2, true>(*s, st, tol, relative, max_iters);
+    } else {
+      hipLaunchKernelGGL((k_div<2, false>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
+      rc = pcg<2, false>(*s, st, tol, relative, max_iters);
+    }
   } else {
-    hipLaunchKernelGGL((k_div<3>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, red_of(*s));
-    rc = pcg<3>(*s, st, tol, relative, max_iters);
+    if (s->masked) {
+      hipLaunchKernelGGL((k_div<3, true>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
+      rc = pcg<3, true>(*s, st, tol, relative, max_iters);
+    } else {
+      hipLaunchKernelGGL((k_div<3, false>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
+      rc = pcg<3, false>(*s, st, tol, relative, max_iters);
+    }
   }
   if (rc != DFM_OK && rc != DFM_E_NOCONV) return rc;
   // subtract the pressure gradient even when max_iters was hit (the reference
@@ -1356,8 +1486,14 @@ extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int
     if (a < s->G.dim) n[a] += 1;
     fd.f[a] = make_dims(n[0], n[1], n[2], s->G.dim);
   }
-  if (s->G.dim == 2) hipLaunchKernelGGL((k_subgrad<2>), dim3(gf), dim3(kB), 0, st, d0, fd, F, (const double*)s->x);
-  else hipLaunchKernelGGL((k_subgrad<3>), dim3(gf), dim3(kB), 0, st, d0, fd, F, (const double*)s->x);
+  const double* px = s->x;
+  if (s->G.dim == 2) {
+    if (s->masked) hipLaunchKernelGGL((k_subgrad<2, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+    else hipLaunchKernelGGL((k_subgrad<2, false>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+  } else {
+    if (s->masked) hipLaunchKernelGGL((k_subgrad<3, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+    else hipLaunchKernelGGL((k_subgrad<3, false>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg subtract_grad_p", hipGetErrorString(e));
   return rc;
@@ -1372,13 +1508,62 @@ extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double to
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   Lvl& L0 = s->lv[0];
   unsigned gr = gridr(L0.size);
-  hipLaunchKernelGGL(k_load_b, dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, red_of(*s));
-  int rc = s->G.dim == 2 ? pcg<2>(*s, st, tol, relative, max_iters) : pcg<3>(*s, st, tol, relative, max_iters);
+  int rc;
+  if (s->masked) {
+    hipLaunchKernelGGL((k_load_b<true>), dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, (const uint8_t*)s->bm[0],
+                       red_of(*s));
+    rc = s->G.dim == 2 ? pcg<2, true>(*s, st, tol, relative, max_iters) : pcg<3, true>(*s, st, tol, relative, max_iters);
+  } else {
+    hipLaunchKernelGGL((k_load_b<false>), dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, (const uint8_t*)nullptr,
+                       red_of(*s));
+    rc = s->G.dim == 2 ? pcg<2, false>(*s, st, tol, relative, max_iters)
+                       : pcg<3, false>(*s, st, tol, relative, max_iters);
+  }
   if (rc != DFM_OK && rc != DFM_E_NOCONV) return rc;
   hipLaunchKernelGGL(k_to_float, dim3(gr), dim3(kB), 0, st, L0.size, s->x, x);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: solution copy", hipGetErrorString(e));
   return rc;
+}
+
+extern "C" int dfm_mgpcg_set_mask(void* handle, const float* mask, void* stream) {
+  Solver* s = static_cast<Solver*>(handle);
+  if (!s) return set_error(DFM_E_NULL, "mgpcg handle is NULL");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  drop_graph(*s);
+  if (!mask) {
+    s->masked = 0;
+    for (int l = 0; l < s->L; ++l) s->ld[l].bm = nullptr;
+    setup_tail(*s);
+    return DFM_OK;
+  }
+  for (int l = 0; l < s->L; ++l)
+    if (!s->bm[l] && hipMalloc(&s->bm[l], (size_t)s->lv[l].size) != hipSuccess)
+      return set_error(DFM_E_HIP, "dfm_mgpcg_set_mask: device allocation failed");
+  Lvl& L0 = s->lv[0];
+  Red R = red_of(*s);
+  hipLaunchKernelGGL(k_mask_level0, dim3(gridr(L0.size)), dim3(kB), 0, st, L0.size, mask, s->bm[0], R);
+  for (int l = 0; l + 1 < s->L; ++l) {
+    Dims d = s->ld[l], dc = s->ld[l + 1];
+    if (s->G.dim == 2)
+      hipLaunchKernelGGL((k_mask_coarsen<2>), dim3(grid1(s->lv[l + 1].size)), dim3(kB), 0, st, d,
+                         (const uint8_t*)s->bm[l], dc, s->bm[l + 1]);
+    else
+      hipLaunchKernelGGL((k_mask_coarsen<3>), dim3(grid1(s->lv[l + 1].size)), dim3(kB), 0, st, d,
+                         (const uint8_t*)s->bm[l], dc, s->bm[l + 1]);
+  }
+  double nf = 0.0;
+  if (hipMemcpyAsync(&nf, s->sc + S_NFLUID, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return set_error(DFM_E_HIP, "dfm_mgpcg_set_mask: fluid count readback failed");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(DFM_E_HIP, "dfm_mgpcg_set_mask", hipGetErrorString(e));
+  s->masked = 1;
+  s->nfluid = nf;
+  for (int l = 0; l < s->L; ++l) s->ld[l].bm = s->bm[l];
+  setup_tail(*s);
+  s->predict = 0;
+  return DFM_OK;
 }
 
 extern "C" int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual) {

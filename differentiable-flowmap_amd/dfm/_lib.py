@@ -59,6 +59,9 @@ SIGNATURES = {
     "dfm_mgpcg_pressure": (c_int, [c_void_p, P(c_void_p)]),
     "dfm_mgpcg_set_option": (c_int, [c_void_p, c_int, c_int]),
     "dfm_mgpcg_get_option": (c_int, [c_void_p, c_int, P(c_int)]),
+    "dfm_mgpcg_set_mask": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "dfm_mask_velocity": (c_int, [P(Grid), c_void_p, c_void_p, P(Mac), c_void_p]),
+    "dfm_mask_scalar": (c_int, [P(Grid), c_void_p, c_void_p, c_int, c_void_p]),
     "dfm_vortex_velocity_2d": (c_int, [P(Grid), c_void_p, c_int, P(Mac), c_void_p]),
     "dfm_vortex_velocity_2d_vjp": (c_int, [P(Grid), c_void_p, c_int, P(Mac), c_void_p, c_void_p]),
 }

@@ -138,6 +138,16 @@ class Ops:
         L.check(self.lib.dfm_adj_scalar_source(self.g, len(s_list), L.ptr_array(s_list), L.ptr_array(lam_list),
                                                r.s, self._st()), "adj_scalar_source")
 
+    def mask_velocity(self, mask, u: MacField, bvel=None):
+        """mask_velocity_by_boundary (bvel: (dim, *n) cell tensor) or, with
+        bvel None, mask_adj_velocity_by_boundary."""
+        L.check(self.lib.dfm_mask_velocity(self.g, mask.data_ptr(), L.ptr(bvel), u.s, self._st()), "mask_velocity")
+
+    def mask_scalar(self, mask, s, adjoint=False):
+        """mask_passive_by_boundary (adjoint False) / mask_adj_passive_by_boundary."""
+        L.check(self.lib.dfm_mask_scalar(self.g, mask.data_ptr(), s.data_ptr(), 1 if adjoint else 0, self._st()),
+                "mask_scalar")
+
     def axpy(self, x: MacField, alpha, y: MacField, out: MacField, beta=0.0, z: MacField = None):
         L.check(self.lib.dfm_axpy_mac(self.g, x.s, float(alpha), y.s, float(beta),
                                       z.s if z is not None else None, out.s, self._st()), "axpy")
@@ -189,6 +199,11 @@ class MGPCG:
         v = L.c_int(0)
         L.check(self.lib.dfm_mgpcg_get_option(self.h, int(option), L.ctypes.byref(v)), "mgpcg_get_option")
         return v.value
+
+    def set_mask(self, mask):
+        """Obstacle mask (cells, solid where > 0) for the projection, or None.
+        The solver derives and keeps its own per-level solid flags."""
+        L.check(self.lib.dfm_mgpcg_set_mask(self.h, L.ptr(mask), L.stream_handle()), "mgpcg_set_mask")
 
     def status(self):
         it = L.c_int(0)

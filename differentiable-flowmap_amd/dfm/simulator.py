@@ -17,8 +17,10 @@ Mirrors the reference's simulator-class API (the boundary its optimisers call):
 Hyper-parameters that the reference reads from its star-imported
 ``hyperparameters.py`` (RK_number, total_steps, add_passive_scalar ...) are
 keyword arguments here.  Supported configuration (the one every benchmark
-uses): viscosity 0, no control force, no obstacle mask, use_short_BFECC=False,
-BFECC_clamp=False, backward_u=True, final-time density loss.
+uses): viscosity 0, no control force, use_short_BFECC=False, BFECC_clamp=False,
+backward_u=True, final-time density loss.  Obstacles: ``set_boundary_mask``
+switches the projection to the obstacle solver (MGPCG_2_solid) and applies
+the reference's mask_* kernels at its call sites.
 
 The reference's disk cache (RW_Cache, 2D/vortex/lfm_midpoint_simulator.py:46-184)
 is replaced by device-resident storage.  ``record=True`` (default) keeps the
@@ -152,6 +154,8 @@ class LFM_Simulator:
         self.passive_start = {}  # interval-start step -> [passive]
         self.flow_maps = {}    # interval index -> FlowMap (phi, F)
         self.u_start = {}      # interval index -> MacField (checkpoint mode)
+        self.boundary_mask_t = None  # obstacle mask on cells (None: no obstacles)
+        self.boundary_vel_t = None   # boundary velocity, (dim, *n), or None for 0
         self._views()
 
     # Taichi-like attribute views -------------------------------------------
@@ -163,6 +167,37 @@ class LFM_Simulator:
             self.passive1 = FieldView(self, lambda s: s.passive[0])
         self.psi_view = FieldView(self, lambda s: s.psi_map.psi, soa=True)
         self.phi_view = FieldView(self, lambda s: s.phi_map.psi, soa=True)
+
+    # obstacles --------------------------------------------------------------
+    def set_boundary_mask(self, mask, boundary_vel=None):
+        """Obstacle cells (mask >= 1 solid for the field masks, > 0 for the
+        projection, as in the reference) and their boundary velocity (cell
+        array (..., dim) like the reference's boundary_vel, or None for 0);
+        mask None removes the obstacles."""
+        if mask is None:
+            self.boundary_mask_t = None
+            self.boundary_vel_t = None
+            self.solver.set_mask(None)
+            return
+        m = torch.as_tensor(np.asarray(mask, dtype=np.float32)).to(self.device).contiguous()
+        if tuple(m.shape) != self.n:
+            raise ValueError(f"mask shape {tuple(m.shape)} != grid {self.n}")
+        self.boundary_mask_t = m
+        if boundary_vel is not None:
+            bv = torch.as_tensor(np.asarray(boundary_vel, dtype=np.float32)).to(self.device)
+            self.boundary_vel_t = torch.movedim(bv, -1, 0).contiguous()
+        else:
+            self.boundary_vel_t = None
+        self.solver.set_mask(m)
+
+    def _mask_fwd(self, u):
+        """mask_velocity_by_boundary + mask_passive_by_boundary after a
+        forward projection (2D/vortex/lfm_midpoint_simulator.py:429-433,543-547,612-616,632-636)."""
+        if self.boundary_mask_t is None:
+            return
+        self.ops.mask_velocity(self.boundary_mask_t, u, self.boundary_vel_t)
+        for p in self.passive:
+            self.ops.mask_scalar(self.boundary_mask_t, p)
 
     # set-up -----------------------------------------------------------------
     def _initial_velocity(self, theta):
@@ -196,6 +231,7 @@ class LFM_Simulator:
                 self.passive[k].copy_(torch.as_tensor(p, dtype=torch.float32).to(self.device))
         for k in range(self.n_passive):
             self.init_passive[k].copy_(self.passive[k])
+        self._mask_fwd(self.u)
         self.step_num[None] = -1
         self.frame_idx[None] = 0
         self.ops.reset_identity(self.phi_map)
@@ -236,6 +272,7 @@ class LFM_Simulator:
         # half-step self-advection of u, then projection -> midpoint velocity
         ops.sl_advect(self.u, 0.5 * dt, order, [self.u], [mid])
         self.solver.Poisson(mid)
+        self._mask_fwd(mid)
         # forward map phi/F marched with the midpoint velocity (dt sign flipped)
         ops.march(mid, self.phi_map, -dt, order)
         # passive and full-step velocity advected by the midpoint velocity
@@ -243,6 +280,7 @@ class LFM_Simulator:
         self.u, self._u_new = self._u_new, self.u
         self.passive, self._passive_new = self._passive_new, self.passive
         self.solver.Poisson(self.u)
+        self._mask_fwd(self.u)
         if j == self.reinit_every - 1:
             if self._replaying:
                 # recomputation for the adjoint only needs the interval's (phi, F)
@@ -268,6 +306,7 @@ class LFM_Simulator:
             ops.march(self.mid_u[st], self.psi_map, dt, order)
         ops.bfecc_u(self.init_u, self.psi_map, self.phi_map, self.err_u, self.u)
         self.solver.Poisson(self.u)
+        self._mask_fwd(self.u)
         for k in range(self.n_passive):
             ops.bfecc_s(self.init_passive[k], self.psi_map.psi, self.phi_map.psi, self._err_s, self.passive[k])
             self.init_passive[k].copy_(self.passive[k])
@@ -362,6 +401,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
             for k in range(self.n_passive):
                 self.adj_passive[k].copy_(grad_passive[k])
             self.loss = None
+        self._mask_adj()
         self.step_num[None] = self.final_ind
         self.init_adj_u.copy_(self.adj_u)
         for k in range(self.n_passive):
@@ -409,6 +449,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
                        self.reinit_every - 1 - j, -2.0 * dt, f2=self.r_s, dt2=-dt)
         self.solver.Poisson_adj(self.adj_u)
         self.solver.Poisson(self.u)
+        self._mask_adj()
         if j == 0:
             fwd = self.flow_maps[step // self.reinit_every]
             ops.bfecc_u(self.init_adj_u, fwd, self.psi_map, self.err_adj_u, self.adj_u)
@@ -417,6 +458,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
             ops.bfecc_u(self.init_u, fwd, self.psi_map, self.err_adj_u, self.u)
             self.solver.Poisson(self.u)
             self.init_u.copy_(self.u)
+            self._mask_adj()
             for k in range(self.n_passive):
                 self.passive[k].copy_(self.passive_start[step][k])
                 self.init_passive[k].copy_(self.passive[k])
@@ -465,6 +507,20 @@ class LFM_Diff_Simulator(LFM_Simulator):
             for k, v in saved.items():
                 setattr(self, k, v)
             self.step_num[None] = saved_step
+
+    def _mask_adj(self):
+        """The forward masks on u and the passives plus
+        mask_adj_velocity_by_boundary / mask_adj_passive_by_boundary on the
+        adjoint fields (2D/vortex/lfm_midpoint_diff_simulator.py:179-189,405-413,451-459)."""
+        if self.boundary_mask_t is None:
+            return
+        m = self.boundary_mask_t
+        self.ops.mask_velocity(m, self.u, self.boundary_vel_t)
+        self.ops.mask_velocity(m, self.adj_u, None)
+        for p in self.passive:
+            self.ops.mask_scalar(m, p)
+        for p in self.adj_passive:
+            self.ops.mask_scalar(m, p, adjoint=True)
 
     def calculate_gw(self):
         u, self.u = self.u, self.adj_u

@@ -177,6 +177,29 @@ int dfm_mgpcg_pressure(void* handle, double** p);
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);
 
+/* Obstacle mask for the projection (MGPCG_2_solid, 2D/vortex/mgpcg_solid.py:15-461;
+ * the all-solid coarsening downsample_bm :98-112).  mask: device FP32 cells,
+ * solid where > 0; NULL removes the mask.  While a mask is set, the operator
+ * couples fluid cells only, every face of a solid cell is a wall, solid cells
+ * carry zero pressure, the coarse levels run as global kernels (no LDS tail)
+ * and the bottom level gets bottom/2 RBGS sweeps in the order (0,1) then
+ * bottom/2 in (1,0).  Recentring uses the mean over fluid cells.  Synchronises
+ * the stream once (fluid-cell count). */
+int dfm_mgpcg_set_mask(void* handle, const float* mask, void* stream);
+
+/* mask_velocity_by_boundary (bvel = cell boundary velocity, SoA (dim, cells))
+ * and, with bvel = NULL, mask_adj_velocity_by_boundary
+ * (2D/vortex/init_conditions.py:548-556,571-578; 3D/init_conditions.py:535-545,563-572):
+ * every face of a cell with mask >= 1 is set to that cell's boundary velocity
+ * (0 with bvel = NULL). */
+int dfm_mask_velocity(const dfm_grid* g, const float* mask, const float* bvel, const dfm_mac* u, void* stream);
+
+/* mask_passive_by_boundary (adjoint = 0: a cell with mask >= 1 takes the mean
+ * of its 3^d neighbours with mask <= 0) and mask_adj_passive_by_boundary
+ * (adjoint = 1: such cells are zeroed) (2D/vortex/init_conditions.py:558-569,580-585;
+ * 3D/init_conditions.py:547-561,574-578). */
+int dfm_mask_scalar(const dfm_grid* g, const float* mask, float* s, int adjoint, void* stream);
+
 /* ---------------------------------------------- 2D vortex control problem */
 /* Initial velocity of K vortices on the MAC faces: the cell-centred vortex
  * sum of sixteen_vortex_vel_func_with_pos_coef (2D/vortex/init_conditions.py:368-383,

@@ -25,6 +25,12 @@ project_exact          DCT-II eigen-decomposition of the Neumann Laplacian;
                        checked against scipy.sparse spsolve and divergence-free
                        output, P idempotent and symmetric
 project_mgpcg          converges to project_exact
+project_masked_exact   empty mask == project_exact; divergence-free on fluid
+                       cells, zero on walls and solid-cell faces, idempotent,
+                       sparse operator symmetric with zero row sums
+mask_velocity /        zero mask is a no-op; solid-cell faces take the boundary
+mask_scalar            velocity; a constant scalar stays constant; a solid cell
+                       with no fluid neighbour gets 0; adjoint form zeroes
 adj sources            closed forms on polynomial fields
 OracleLFM (whole step) composition of the above; the trajectory itself is
                        **parity unpinned** against the reference (Taichi cannot
@@ -397,6 +403,133 @@ def project_exact(u):
     apply_bc(u)
 
 
+# ------------------------------------------------------ obstacle masks -------
+def _face_solid(solid, a):
+    """Boolean per face of axis a: true where either adjacent cell is solid."""
+    n = solid.shape
+    out = np.zeros(face_shape(n, a), dtype=bool)
+    hi = [slice(None)] * solid.ndim
+    lo = [slice(None)] * solid.ndim
+    hi[a] = slice(0, n[a])
+    lo[a] = slice(1, n[a] + 1)
+    out[tuple(hi)] |= solid
+    out[tuple(lo)] |= solid
+    return out
+
+
+def apply_bc_masked(u, solid):
+    """apply_bc of the obstacle solver (2D/vortex/mgpcg_solid.py:388-402): box
+    walls and every face of a solid cell are zeroed."""
+    apply_bc(u)
+    for a, c in enumerate(u):
+        c[_face_solid(solid, a)] = 0
+
+
+def masked_laplacian(solid):
+    """Sparse fluid-cell operator of the obstacle solver: row i = (#fluid
+    neighbours) x_i - sum of fluid neighbours (num_fluid_neighbors2 /
+    neighbor_sum2, 2D/vortex/mgpcg_solid.py:140-170); solid rows and columns
+    are left out.  Returns (A, fluid flat indices)."""
+    import scipy.sparse as sp
+
+    n = solid.shape
+    fluid = ~solid
+    idx = -np.ones(n, dtype=np.int64)
+    fl = np.flatnonzero(fluid.ravel())
+    idx.ravel()[fl] = np.arange(fl.size)
+    rows, cols, vals = [], [], []
+    diag = np.zeros(fl.size)
+    for a in range(solid.ndim):
+        lo = [slice(None)] * solid.ndim
+        hi = [slice(None)] * solid.ndim
+        lo[a] = slice(0, n[a] - 1)
+        hi[a] = slice(1, n[a])
+        i0, i1 = idx[tuple(lo)], idx[tuple(hi)]
+        both = (i0 >= 0) & (i1 >= 0)
+        r, c = i0[both], i1[both]
+        rows += [r, c]
+        cols += [c, r]
+        vals += [-np.ones(r.size), -np.ones(r.size)]
+        np.add.at(diag, r, 1.0)
+        np.add.at(diag, c, 1.0)
+    rows.append(np.arange(fl.size))
+    cols.append(np.arange(fl.size))
+    vals.append(diag)
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(fl.size, fl.size))
+    return A, fl
+
+
+def project_masked_exact(u, solid):
+    """MGPCG_2_solid.Poisson (2D/vortex/mgpcg_solid.py:455-461) with the
+    pressure solved exactly: apply_bc, divergence on fluid cells (0 in solid
+    ones), the fluid-cell Poisson problem solved directly (its constant null
+    space fixed by the mean-zero condition; the fluid region must be
+    connected), p = 0 in solid cells, gradient subtraction, apply_bc."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    solid = np.asarray(solid, dtype=bool)
+    apply_bc_masked(u, solid)
+    b = -divergence(u).astype(np.float64)
+    b[solid] = 0.0
+    A, fl = masked_laplacian(solid)
+    bf = b.ravel()[fl]
+    bf = bf - bf.mean()
+    # ground the first fluid cell (x_0 = 0): the remaining block of the
+    # singular Neumann operator is SPD on a connected fluid region, and the
+    # consistent right-hand side makes row 0 hold automatically; then recentre
+    x = np.zeros(fl.size)
+    x[1:] = spla.spsolve(sp.csc_matrix(A[1:, 1:]), bf[1:])
+    x -= x.mean()
+    p = np.zeros(solid.shape)
+    p.ravel()[fl] = x
+    subtract_grad_p(u, p)
+    apply_bc_masked(u, solid)
+    return p
+
+
+def mask_velocity(u, mask, bvel=None):
+    """mask_velocity_by_boundary / mask_adj_velocity_by_boundary
+    (2D/vortex/init_conditions.py:548-556,571-578): every face of a cell with
+    mask >= 1 takes that cell's boundary velocity (0 when bvel is None); a
+    face between two such cells takes the high-side cell's value."""
+    solid = np.asarray(mask) >= 1
+    n = solid.shape
+    for a, c in enumerate(u):
+        val = np.zeros(n) if bvel is None else np.asarray(bvel[a], dtype=np.float64)
+        lo = [slice(None)] * solid.ndim
+        hi = [slice(None)] * solid.ndim
+        lo[a] = slice(1, n[a] + 1)   # faces whose low-side cell is I - e_a
+        hi[a] = slice(0, n[a])       # faces whose high-side cell is I
+        v = c[tuple(lo)]
+        v[solid] = val[solid]
+        c[tuple(lo)] = v
+        v = c[tuple(hi)]
+        v[solid] = val[solid]
+        c[tuple(hi)] = v
+
+
+def mask_scalar(s, mask, adjoint=False):
+    """mask_passive_by_boundary (solid cells take the mean of the fluid cells of
+    their 3^d neighbourhood, accumulated in the reference's loop order) and
+    mask_adj_passive_by_boundary (solid cells zeroed)
+    (2D/vortex/init_conditions.py:558-569,580-585; 3D/init_conditions.py:547-578)."""
+    mask = np.asarray(mask)
+    out = s.copy()
+    for I in zip(*np.nonzero(mask >= 1)):
+        if adjoint:
+            out[I] = 0.0
+            continue
+        acc, w = s.dtype.type(0), 0
+        for off in np.ndindex(*(3,) * s.ndim):
+            J = tuple(i + o - 1 for i, o in zip(I, off))
+            if all(0 <= j < m for j, m in zip(J, s.shape)) and mask[J] <= 0:
+                acc = acc + s[J]
+                w += 1
+        out[I] = acc / w if w > 0 else 0.0
+    s[...] = out
+
+
 def mgpcg_solve(b, levels, pre=2, bottom=50, tol=1e-12, relative=False, max_iters=400):
     """Restated MGPCG solve of A x = b; returns (x, iters, rel_residual, status)."""
     b = np.ascontiguousarray(b)
@@ -459,11 +592,46 @@ class OracleLFM:
         self.phi_map = FlowMap(n, dx, dtype)
         self.psi_map = FlowMap(n, dx, dtype)
 
+    mask = None
+    bvel = None
+
+    def set_mask(self, mask, bvel=None):
+        """Obstacle mask on cells (solid where > 0 for the projection, >= 1 for
+        the field masks), as the 2D reference sets boundary_mask; the
+        projection becomes MGPCG_2_solid's (solved exactly) and the mask_*
+        kernels run where the reference calls them."""
+        self.mask = None if mask is None else np.asarray(mask, dtype=np.float64)
+        self.bvel = bvel
+
     def project(self, u):
-        if self.project_mode == "exact":
+        if self.mask is not None:
+            project_masked_exact(u, self.mask > 0)
+        elif self.project_mode == "exact":
             project_exact(u)
         else:
             project_mgpcg(u, levels=self.mg_levels, tol=self.mg_tol)
+
+    def _mask_fwd(self):
+        """mask_velocity_by_boundary + mask_passive_by_boundary after a forward
+        projection (2D/vortex/lfm_midpoint_simulator.py:429-433,543-547,612-616,632-636)."""
+        if self.mask is None:
+            return
+        mask_velocity(self.u, self.mask, self.bvel)
+        for s in self.s:
+            mask_scalar(s, self.mask)
+
+    def _mask_adj(self):
+        """The forward masks on u and s plus mask_adj_velocity_by_boundary and
+        mask_adj_passive_by_boundary on the adjoint fields
+        (2D/vortex/lfm_midpoint_diff_simulator.py:179-189,405-413,451-459)."""
+        if self.mask is None:
+            return
+        mask_velocity(self.u, self.mask, self.bvel)
+        mask_velocity(self.lam_u, self.mask, None)
+        for s in self.s:
+            mask_scalar(s, self.mask)
+        for l in self.lam_s:
+            mask_scalar(l, self.mask, adjoint=True)
 
     # forward ---------------------------------------------------------------
     def init(self, u0, passives):
@@ -474,6 +642,7 @@ class OracleLFM:
         self.project(self.u)
         self.s = [np.array(s, dtype=dt) for s in passives]
         self.init_s = [s.copy() for s in self.s]
+        self._mask_fwd()
         self.init_u = [c.copy() for c in self.u]
         self.phi_map.reset()
         self.step_num = -1
@@ -490,11 +659,13 @@ class OracleLFM:
         ad_u = [c.copy() for c in self.u]
         self.u = sl_advect_u(ad_u, ad_u, dx, 0.5 * dt, order)
         self.project(self.u)
+        self._mask_fwd()
         self.mid[self.step_num] = [c.copy() for c in self.u]
         self.phi_map.march(self.u, -dt, order)
         self.s = [sl_advect_s(self.u, s, dx, dt, order) for s in self.s]
         self.u = sl_advect_u(self.u, ad_u, dx, dt, order)
         self.project(self.u)
+        self._mask_fwd()
         if j == self.R - 1:
             # backtrack_psi_grid: last step with the end-of-step u (q3)
             self.psi_map.reset()
@@ -503,6 +674,7 @@ class OracleLFM:
                 self.psi_map.march(self.mid[st], dt, order)
             self.u = bfecc(self.init_u, self.psi_map, self.phi_map, dx)
             self.project(self.u)
+            self._mask_fwd()
             self.s = [bfecc_scalar(si, self.psi_map.psi, self.phi_map.psi, dx) for si in self.init_s]
             self.init_s = [s.copy() for s in self.s]
             self.maps[self.step_num // self.R] = self.phi_map.copy()
@@ -517,6 +689,7 @@ class OracleLFM:
         self.lam_u = [np.zeros_like(c) for c in self.u]
         self.lam_s = [2.0 * (s - t.astype(self.dtype)) for s, t in zip(self.s, targets)]
         self.loss = float(sum(np.sum((s.astype(np.float64) - t) ** 2) for s, t in zip(self.s, targets)))
+        self._mask_adj()
         self.step_num = self.final_ind
         self.init_lam_u = [c.copy() for c in self.lam_u]
         self.init_lam_s = [c.copy() for c in self.lam_s]
@@ -552,6 +725,7 @@ class OracleLFM:
             accumulate(rs, self.init_lam_u, self.psi_map, dx, -dt)
         self.project(self.lam_u)
         self.project(self.u)
+        self._mask_adj()
         if j == 0:
             fwd = self.maps[self.step_num // self.R]
             self.lam_u = bfecc(self.init_lam_u, fwd, self.psi_map, dx)
@@ -560,6 +734,7 @@ class OracleLFM:
             self.u = bfecc(self.init_u, fwd, self.psi_map, dx)
             self.project(self.u)
             self.init_u = [c.copy() for c in self.u]
+            self._mask_adj()
             self.s = [s.copy() for s in self.s_start[self.step_num]]
             self.init_s = [s.copy() for s in self.s]
             self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx) for l in self.init_lam_s]

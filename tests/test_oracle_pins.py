@@ -378,3 +378,77 @@ def test_mgpcg_restatement_converges_to_exact(shape, levels):
     assert st == 0 and it < 40
     xe = O.neumann_solve_dct(b)
     np.testing.assert_allclose(x - x.mean(), xe, atol=1e-6 * np.abs(xe).max())
+
+
+# ------------------------------------------------------------ obstacle masks --
+def _disk_mask(n, c, r):
+    X = O.cell_coords(n, 1.0 / n[1])
+    return (np.sqrt(np.sum((X - np.asarray(c)) ** 2, axis=-1)) < r).astype(np.float64)
+
+
+@pytest.mark.parametrize("n", [(24, 20), (12, 10, 8)])
+def test_masked_projection_empty_mask_is_exact_projection(n):
+    rng = np.random.default_rng(3)
+    u = [rng.standard_normal(O.face_shape(n, a)) for a in range(len(n))]
+    a, b = [c.copy() for c in u], [c.copy() for c in u]
+    O.project_exact(a)
+    O.project_masked_exact(b, np.zeros(n, dtype=bool))
+    assert max(np.abs(x - y).max() for x, y in zip(a, b)) < 1e-10
+
+
+@pytest.mark.parametrize("n", [(32, 28), (14, 12, 10)])
+def test_masked_projection_properties(n):
+    """Divergence-free on fluid cells, zero on walls and on every face of a
+    solid cell, idempotent; the fluid operator is symmetric with zero row sums."""
+    rng = np.random.default_rng(4)
+    c = [0.45] * len(n)
+    solid = _disk_mask(n, c, 0.22) > 0
+    u = [rng.standard_normal(O.face_shape(n, a)) for a in range(len(n))]
+    O.project_masked_exact(u, solid)
+    div = O.divergence(u)
+    assert np.abs(div[~solid]).max() < 1e-9
+    for a in range(len(n)):
+        assert np.all(u[a][O._face_solid(solid, a)] == 0)
+    again = [x.copy() for x in u]
+    O.project_masked_exact(again, solid)
+    assert max(np.abs(x - y).max() for x, y in zip(again, u)) < 1e-9
+    A, fl = O.masked_laplacian(solid)
+    assert abs(A - A.T).max() == 0
+    assert np.abs(np.asarray(A.sum(axis=1))).max() == 0
+
+
+def test_mask_field_kernels_closed_forms():
+    n = (10, 9)
+    mask = np.zeros(n)
+    mask[3:5, 4:6] = 1.0          # solid block
+    mask[8, 0] = 0.5              # neither solid nor fluid
+    rng = np.random.default_rng(5)
+    # zero mask: no-op
+    u = [rng.standard_normal(O.face_shape(n, a)) for a in range(2)]
+    ref = [c.copy() for c in u]
+    O.mask_velocity(u, np.zeros(n))
+    assert all(np.array_equal(a, b) for a, b in zip(u, ref))
+    # solid-cell faces take the boundary velocity; other faces untouched
+    bvel = [np.full(n, 2.0), np.full(n, -3.0)]
+    O.mask_velocity(u, mask, bvel)
+    assert np.all(u[0][3:6, 4:6] == 2.0) and np.all(u[1][3:5, 4:7] == -3.0)
+    assert np.array_equal(u[0][:3], ref[0][:3]) and np.array_equal(u[1][:, :4], ref[1][:, :4])
+    # constant scalar stays constant; the half-masked cell is left alone
+    s = np.full(n, 1.25)
+    s[8, 0] = 7.0
+    O.mask_scalar(s, mask)
+    assert np.all(s[3:5, 4:6] == 1.25) and s[8, 0] == 7.0
+    # a solid cell with no fluid neighbour gets 0; adjoint form zeroes
+    m2 = np.ones((3, 3))
+    s2 = np.full((3, 3), 4.0)
+    O.mask_scalar(s2, m2)
+    assert np.all(s2 == 0)
+    s3 = rng.standard_normal(n)
+    O.mask_scalar(s3, mask, adjoint=True)
+    assert np.all(s3[3:5, 4:6] == 0)
+    # brute-force mean of the fluid 3x3 neighbours for one solid cell
+    s4 = rng.standard_normal(n)
+    want = np.mean([s4[i, j] for i in range(2, 6) for j in range(3, 7)
+                    if 2 <= i <= 4 + 1 and abs(i - 3) <= 1 and abs(j - 4) <= 1 and mask[i, j] <= 0])
+    O.mask_scalar(s4, mask)
+    assert abs(s4[3, 4] - want) < 1e-12
