@@ -220,6 +220,42 @@ def run_cpu_baseline(size):
                       f"restated MGPCG; {el:.1f} s"}
 
 
+def run_batch(args):
+    """Config 5: 32 independent 2D forward+adjoint problems, grids from
+    {128,192,256,384,512}^2 by seed, 4-12 vortices each, theta gradient per
+    instance.  Sequential run, worker-count autotune, concurrent run."""
+    import numpy as np
+
+    from dfm.batch import BatchedLFM, make_problems
+
+    probs = make_problems(count=32, seed=100, steps=args.batch_steps, reinit_every=12)
+    b = BatchedLFM(probs, workers=1)
+    tail = b.autotune_tail()
+    b.run(12)  # untimed: builds every handle and graph
+    seq, t_seq = b.run()
+    tune = b.autotune(candidates=(1, 2, 4, 8, 16), probe_steps=12)
+    res, t = b.run()
+    steps_total = b.instance_steps()
+    per = []
+    for p, (l0, g0), (l1, g1) in zip(probs, seq, res):
+        per.append({"n": p["n"][0], "vortices": int(p["theta"].size // 4), "loss": round(float(l1), 6),
+                    "grad_norm": round(float(np.linalg.norm(g1)), 6),
+                    "grad_rel_diff_vs_sequential": float(np.linalg.norm(g1 - g0) / max(np.linalg.norm(g0), 1e-30))})
+    return {
+        "metric": "aggregate fwd+adjoint instance-steps/s, 32 uneven 2D instances (128^2..512^2)",
+        "value": round(steps_total / t, 2), "unit": "steps/s", "n_gpus": 1, "steps": args.batch_steps,
+        "ms_total": round(t * 1e3, 1), "sequential_value": round(steps_total / t_seq, 2),
+        "speedup_vs_sequential": round(t_seq / t, 3), "workers": b.workers,
+        "autotune_s": {str(k): round(v, 4) for k, v in tune.items()},
+        "mg_tail_level_ms": {str(n[0]): {str(k): round(v, 3) for k, v in t.items()} for n, t in tail.items()},
+        "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded vortices / disk / Gaussian blobs)",
+        "config": {"workload": "32 independent 2D forward+adjoint LFM problems, RK2, reinit every 12, "
+                               "MGPCG tol 1e-6, theta gradient", "instances": len(probs)},
+        "per_instance": per,
+    }
+
+
 def run_copy_calibration():
     """Plain device-to-device copy of a known size (256 MiB, above the 256 MB
     MALL), the reference point that scales the FETCH_SIZE / WRITE_SIZE
@@ -250,11 +286,23 @@ def main():
     ap.add_argument("--mg-check-every", type=int, default=1, help="PCG iterations per convergence poll")
     ap.add_argument("--mg-levels", type=int, default=0,
                     help="MG levels (0: log2(n) - 2, the reference's count: 5 at 128^3, 6 at 256^3)")
+    ap.add_argument("--batch", action="store_true",
+                    help="the batched configuration: 32 uneven 2D instances, aggregate steps/s")
+    ap.add_argument("--batch-steps", type=int, default=24)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for --batch (1..32; 0 keeps the platform's setting: 16 measured slower than 4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy used to calibrate the HBM traffic counters")
     args = ap.parse_args()
     if args.copy_calibration:
         run_copy_calibration()
+        return
+    if args.batch:
+        from dfm.batch import configure_hw_queues
+
+        if args.hw_queues > 0:
+            configure_hw_queues(args.hw_queues)
+        print(json.dumps(run_batch(args)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

@@ -985,6 +985,7 @@ struct Solver {
   int bottom_exact;  // option: exact (DCT) coarsest-level solve in the tail
   int tail_bf16;     // the tail in use is BF16
   int tail_lt;       // first tail level (L = no tail)
+  int tail_min;      // option: lowest level the tail may start at (0 = any that fits)
   size_t tail_bytes; // dynamic LDS of the tail kernel
   TailArgs tail;
   float* z0_final;   // level-0 buffer holding M^-1 r after the last V-cycle
@@ -1013,7 +1014,7 @@ size_t max_lds_bytes() {
 void configure_tail(Solver& s, size_t cap) {
   int best = s.L;
   size_t best_bytes = 0;
-  for (int l = s.L - 1; l >= 0 && s.L - l <= kTailMax; --l) {
+  for (int l = s.L - 1; l >= s.tail_min && s.L - l <= kTailMax; --l) {
     const size_t es = (s.bf16_coarse && l >= 2) ? 2 : 4;
     size_t tot = 0;
     for (int k = l; k < s.L; ++k) tot += 2 * (size_t)s.lv[k].size * es;
@@ -1408,6 +1409,12 @@ extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
       s->use_graph = value ? 1 : 0;
       drop_graph(*s);
       return DFM_OK;
+    case DFM_MG_TAIL_LEVEL:
+      if (value < 0) return set_error(DFM_E_ARG, "tail level must be >= 0");
+      s->tail_min = value;
+      setup_tail(*s);
+      drop_graph(*s);
+      return DFM_OK;
     case DFM_MG_BOTTOM_EXACT:
       s->bottom_exact = value ? 1 : 0;
       setup_tail(*s);
@@ -1437,6 +1444,7 @@ extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
     case DFM_MG_CHECK_EVERY: *value = s->check_every; return DFM_OK;
     case DFM_MG_GRAPH: *value = s->gexec ? 2 : s->use_graph; return DFM_OK;
     case DFM_MG_BOTTOM_EXACT: *value = (s->tail_lt < s->L) ? s->bottom_exact : 0; return DFM_OK;
+    case DFM_MG_TAIL_LEVEL: *value = s->tail_lt; return DFM_OK;
     default: return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
 }

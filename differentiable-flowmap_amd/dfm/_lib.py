@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 
 import torch  # noqa: F401  (must be imported before the HIP library is loaded)
 
@@ -72,10 +73,12 @@ MG_MAX_REFINE = 3
 MG_CHECK_EVERY = 4
 MG_GRAPH = 5
 MG_BOTTOM_EXACT = 6
+MG_TAIL_LEVEL = 7
 
 DFM_E_NOCONV = -4
 
 _lib = None
+_lib_lock = threading.Lock()
 
 
 class DfmError(RuntimeError):
@@ -85,9 +88,13 @@ class DfmError(RuntimeError):
 
 
 def lib():
-    """Load libdfm.so (raises if it has not been built)."""
+    """Load libdfm.so (raises if it has not been built); thread-safe."""
     global _lib
-    if _lib is None:
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build the HIP library first "

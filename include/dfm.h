@@ -174,6 +174,11 @@ int dfm_mgpcg_pressure(void* handle, double** p);
 #define DFM_MG_CHECK_EVERY 4
 #define DFM_MG_GRAPH 5
 #define DFM_MG_BOTTOM_EXACT 6
+/*  DFM_MG_TAIL_LEVEL: lowest level the LDS tail may start at (default 0: the
+ *      largest suffix of levels that fits); levels below it use global
+ *      kernels.  get: the first tail level in use (L = no tail).  The batch
+ *      executor's autotuner sets it per grid size. */
+#define DFM_MG_TAIL_LEVEL 7
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);
 

@@ -1,0 +1,91 @@
+"""Batched uneven instances (config-5 form at reduced sizes): running the
+instances concurrently on several streams gives, per instance, bitwise the
+results of running them one by one, each instance's theta gradient matches
+the FP64 oracle, and the worker-count autotuner picks its fastest probe."""
+import numpy as np
+import pytest
+
+from gpu_util import rel_l2, round32
+
+from oracle import lfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def problems():
+    from dfm.batch import make_problems
+
+    return make_problems(count=6, seed=40, steps=8, reinit_every=4, sizes=(32, 48, 64))
+
+
+def test_concurrent_equals_sequential_bitwise(device):
+    from dfm.batch import BatchedLFM
+
+    probs = problems()
+    seq, _ = BatchedLFM(probs, workers=1).run()
+    par, _ = BatchedLFM(probs, workers=3).run()
+    for (l1, g1), (l2, g2) in zip(seq, par):
+        assert l1 == l2
+        assert np.array_equal(g1, g2)
+
+
+def test_instance_gradients_match_oracle(device):
+    from dfm import scenes
+    from dfm.batch import BatchedLFM
+
+    probs = problems()
+    res, _ = BatchedLFM(probs, workers=2).run()
+    for i in sorted(range(len(probs)), key=lambda k: probs[k]["n"][0])[:2]:
+        p = probs[i]
+        n, dx = p["n"], p["dx"]
+        th = p["theta"].astype(np.float64)
+        faces = lambda t: scenes.split_central_vector(scenes.vortex_velocity_2d(t, n, dx))  # noqa: E731
+        orc = O.OracleLFM(n, dx, p["dt"], p["R"], 2)
+        orc.init(round32(faces(th)), [p["density"].astype(np.float64)])
+        for _ in range(p["steps"]):
+            orc.step()
+        orc.init_gradient([p["target"].astype(np.float64)])
+        for _ in range(p["steps"]):
+            orc.backtrack_step()
+        g = np.zeros_like(th)
+        for k in range(th.size):
+            e = np.zeros_like(th)
+            e[k] = 1e-6
+            g[k] = sum(np.sum(l * (a - b)) for l, a, b in zip(orc.lam_u, faces(th + e), faces(th - e))) / 2e-6
+        loss, grad = res[i]
+        print(f"instance {i} n={n}: loss rel {abs(loss - orc.loss) / orc.loss:.2e}, grad rel-L2 {rel_l2(grad, g):.2e}")
+        assert abs(loss - orc.loss) <= 1e-4 * orc.loss
+        assert rel_l2(grad, g) < 1e-3
+
+
+def test_autotune_keeps_fastest(device):
+    from dfm.batch import BatchedLFM
+
+    b = BatchedLFM(problems(), workers=1)
+    times = b.autotune(candidates=(1, 2, 4), probe_steps=4)
+    assert set(times) == {1, 2, 4}
+    assert b.workers == min(times, key=times.get)
+
+
+@pytest.mark.parametrize("n", [(128, 128), (64, 64, 32)])
+def test_tail_level_option_every_level_agrees(n, device):
+    """Every admissible first LDS-tail level (down to none) gives the exact
+    projection; get_option reports the level in use."""
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    from gpu_util import mac_to_dev, mac_to_np, rel_l2_fields
+
+    rng = np.random.default_rng(12)
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(len(n))])
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    s = MGPCG(n, 1.0 / n[1], tol=1e-7, max_iters=200)
+    lo = s.get_option(L.MG_TAIL_LEVEL)
+    levels = int(np.log2(min(n))) - 2
+    for lvl in range(lo, levels + 1):
+        s.set_option(L.MG_TAIL_LEVEL, lvl)
+        assert s.get_option(L.MG_TAIL_LEVEL) == lvl
+        d = mac_to_dev(u, device)
+        s.Poisson(d)
+        assert rel_l2_fields(mac_to_np(d), ref) < 2e-6, lvl
