@@ -494,10 +494,16 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
     cell_of(d, id, i, j, k);
     float v = 0.f, pv = 0.f;
     if (is_fluid<MASK>(d, id)) {
-      float nn;
-      float s = nbr_sum<DIM, MASK>(p, d, i, j, k, nn);
+      // formed in FP64 and rounded once: for smooth p, nn p - sum(p) cancels
+      // all but ~(pi/n)^2 of |p|, and the FP32 form's error then drifts the
+      // recursive residual away from b - A x by up to ~1e-2 |b| at 512^2
+      double s = 0.0, nn = 0.0;
+      for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int, int, int) {
+        s += (double)p[q];
+        nn += 1.0;
+      });
       pv = p[id];
-      v = nn * pv - s;
+      v = (float)(nn * (double)pv - s);
     }
     Ap[id] = v;
     acc += (double)pv * (double)v;
