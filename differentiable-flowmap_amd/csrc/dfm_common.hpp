@@ -72,10 +72,16 @@ DFM_HD int64_t xcd_block(int64_t bid, int64_t nblocks) {
 // (2D/vortex/taichi_utils.py:433,470,506); every tap it visits beyond these
 // has |x| >= support and weight exactly 0, so the sums are the same while a
 // 3D quadratic sample gathers 27 values instead of 64 (cubic 64 instead of 216).
+//
+// Within the support, which polynomial piece a tap uses is fixed by its
+// position, so the NT weights are evaluated branch-free as polynomials of the
+// sample's offset t from the support's reference node REF = base + REF_OFF
+// (tap x = t + REF_OFF - tap index); the pieces are those of w(x) below.
 template <int K> struct Spline;
 
 template <> struct Spline<2> {
-  static constexpr int NT = 3;  // nodes round(s)-1 .. round(s)+1
+  static constexpr int NT = 3;       // nodes round(s)-1 .. round(s)+1
+  static constexpr int REF_OFF = 1;  // t = s - round(s) in [-1/2, 1/2)
   DFM_HD static int base(float s) { return (int)floorf(s + 0.5f) - 1; }
   DFM_HD static float w(float x) {
     float a = fabsf(x);
@@ -86,10 +92,22 @@ template <> struct Spline<2> {
     float r = a < 0.5f ? -2.0f * a : (a < 1.5f ? a - 1.5f : 0.0f);
     return x < 0.0f ? -r : r;
   }
+  DFM_HD static void weights(float t, float* wt) {
+    const float l = 0.5f - t, h = 0.5f + t;
+    wt[0] = 0.5f * l * l;
+    wt[1] = 0.75f - t * t;
+    wt[2] = 0.5f * h * h;
+  }
+  DFM_HD static void dweights(float t, float* d) {
+    d[0] = t - 0.5f;
+    d[1] = -2.0f * t;
+    d[2] = t + 0.5f;
+  }
 };
 
 template <> struct Spline<3> {
-  static constexpr int NT = 4;  // nodes floor(s)-1 .. floor(s)+2
+  static constexpr int NT = 4;       // nodes floor(s)-1 .. floor(s)+2
+  static constexpr int REF_OFF = 1;  // t = s - floor(s) in [0, 1)
   DFM_HD static int base(float s) { return (int)floorf(s) - 1; }
   DFM_HD static float w(float x) {
     float a = fabsf(x);
@@ -109,15 +127,36 @@ template <> struct Spline<3> {
     }
     return x < 0.0f ? -r : r;
   }
+  DFM_HD static void weights(float t, float* wt) {
+    const float u = 1.0f - t;
+    wt[0] = u * u * u * (1.0f / 6.0f);              // x = t + 1
+    wt[1] = (0.5f * t - 1.0f) * t * t + (2.0f / 3.0f);  // x = t
+    wt[2] = (0.5f * u - 1.0f) * u * u + (2.0f / 3.0f);  // x = t - 1
+    wt[3] = t * t * t * (1.0f / 6.0f);              // x = t - 2
+  }
+  DFM_HD static void dweights(float t, float* d) {
+    const float u = 1.0f - t;
+    d[0] = -0.5f * u * u;
+    d[1] = (1.5f * t - 2.0f) * t;
+    d[2] = -(1.5f * u - 2.0f) * u;
+    d[3] = 0.5f * t * t;
+  }
 };
 
 template <> struct Spline<4> {
-  static constexpr int NT = 5;  // nodes round(s)-2 .. round(s)+2
+  static constexpr int NT = 5;       // nodes round(s)-2 .. round(s)+2
+  static constexpr int REF_OFF = 2;  // t = s - round(s) in [-1/2, 1/2)
   DFM_HD static int base(float s) { return (int)floorf(s + 0.5f) - 2; }
+  DFM_HD static float inner(float a) {  // piece 1/2 <= |x| < 3/2
+    return (55.0f / 96.0f) + a * ((5.0f / 24.0f) + a * (-1.25f + a * ((5.0f / 6.0f) - a * (1.0f / 6.0f))));
+  }
+  DFM_HD static float dinner(float a) {
+    return (5.0f / 24.0f) + a * (-2.5f + a * (2.5f - a * (2.0f / 3.0f)));
+  }
   DFM_HD static float w(float x) {
     float a = fabsf(x), a2 = a * a;
     if (a < 0.5f) return (115.0f / 192.0f) + a2 * (-0.625f + 0.25f * a2);
-    if (a < 1.5f) return (55.0f / 96.0f) + a * ((5.0f / 24.0f) + a * (-1.25f + a * ((5.0f / 6.0f) - a * (1.0f / 6.0f))));
+    if (a < 1.5f) return inner(a);
     if (a < 2.5f) {
       float t = 2.5f - a;
       float t2 = t * t;
@@ -128,12 +167,28 @@ template <> struct Spline<4> {
   DFM_HD static float dw(float x) {
     float a = fabsf(x), r = 0.0f;
     if (a < 0.5f) r = a * (-1.25f + a * a);
-    else if (a < 1.5f) r = (5.0f / 24.0f) + a * (-2.5f + a * (2.5f - a * (2.0f / 3.0f)));
+    else if (a < 1.5f) r = dinner(a);
     else if (a < 2.5f) {
       float t = 2.5f - a;
       r = -t * t * t * (1.0f / 6.0f);
     }
     return x < 0.0f ? -r : r;
+  }
+  DFM_HD static void weights(float t, float* wt) {
+    const float l = 0.5f - t, h = 0.5f + t, t2 = t * t;
+    wt[0] = l * l * l * l * (1.0f / 24.0f);                   // x = t + 2
+    wt[1] = inner(1.0f + t);                                  // x = t + 1
+    wt[2] = (115.0f / 192.0f) + t2 * (-0.625f + 0.25f * t2);  // x = t
+    wt[3] = inner(1.0f - t);                                  // x = t - 1
+    wt[4] = h * h * h * h * (1.0f / 24.0f);                   // x = t - 2
+  }
+  DFM_HD static void dweights(float t, float* d) {
+    const float l = 0.5f - t, h = 0.5f + t;
+    d[0] = -l * l * l * (1.0f / 6.0f);
+    d[1] = dinner(1.0f + t);
+    d[2] = t * (-1.25f + t * t);
+    d[3] = -dinner(1.0f - t);
+    d[4] = h * h * h * (1.0f / 6.0f);
   }
 };
 
@@ -150,6 +205,11 @@ template <int K> DFM_HD int support_base(float s, int S) {
   if (K == 2) b = b < 0 ? 0 : (b > S - 3 ? S - 3 : b);
   return b;
 }
+template <int K> DFM_HD int tap_index(int b, int t, int S) {
+  int id = b + t;
+  if (K != 2) id = id < 0 ? 0 : (id > S - 1 ? S - 1 : id);
+  return id;
+}
 
 // Per-axis tap set of one sample: node indices and the NT weights (and
 // derivative weights).
@@ -160,21 +220,35 @@ template <int K, bool D> struct Axis {
   DFM_HD void setup(float p_over_dx, float bl, int S) {
     float s = clamp_coord<K>(p_over_dx, bl, S);
     int b = support_base<K>(s, S);
+    const float t = s - (float)(b + Spline<K>::REF_OFF);
+    Spline<K>::weights(t, w);
+    if (D) Spline<K>::dweights(t, dw);
 #pragma unroll
-    for (int t = 0; t < Spline<K>::NT; ++t) {
-      float x = s - (float)(b + t);
-      w[t] = Spline<K>::w(x);
-      if (D) dw[t] = Spline<K>::dw(x);
-      int id = b + t;
-      if (K != 2) id = id < 0 ? 0 : (id > S - 1 ? S - 1 : id);
-      idx[t] = id;
-    }
+    for (int q = 0; q < Spline<K>::NT; ++q) idx[q] = tap_index<K>(b, q, S);
   }
 };
 
+// Weight (and derivative) of tap a of the support for offset t, with a a
+// runtime (wave-uniform) loop index: quadratic taps from a closed form in
+// the signed tap position, wider splines from the piecewise form at
+// x = t + REF_OFF - a.  Register arrays indexed by a loop counter would be
+// placed in scratch memory.
+template <int K> DFM_HD float tap_w(float t, int a) {
+  if (K == 2) {
+    const float h = 0.5f + (float)(a - 1) * t;  // a = 0: 1/2 - t, a = 2: 1/2 + t
+    return a == 1 ? 0.75f - t * t : 0.5f * h * h;
+  }
+  return Spline<K>::w(t + (float)(Spline<K>::REF_OFF - a));
+}
+template <int K> DFM_HD float tap_dw(float t, int a) {
+  if (K == 2) return a == 1 ? -2.0f * t : t + 0.5f * (float)(a - 1);
+  return Spline<K>::dw(t + (float)(Spline<K>::REF_OFF - a));
+}
+
 // Value (and optionally the full gradient) of the B-spline interpolant of
 // field f (shape S, node offset bl) at physical point p.  Separable: the z
-// taps are contracted first, then y, then x.
+// taps are contracted first, then y, then x.  Row addresses are signed, so
+// the consecutive z taps of a row fold into one address plus immediate offsets.
 template <int K, bool GRAD, int DIM>
 DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[3], const float p[3], float inv_dx,
                     float g[3]) {
@@ -207,19 +281,18 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     }
     return v;
   } else {
-    // x taps stay rolled (their weights are recomputed per iteration, which
-    // is cheaper than the register pressure of a fully unrolled 3D loop)
+    // x taps stay rolled (their weights computed per iteration), which is
+    // cheaper than the register pressure of a fully unrolled 3D loop
     Axis<K, GRAD> az;
     az.setup(p[2] * inv_dx, bl[2], S[2]);
     const float sx = clamp_coord<K>(p[0] * inv_dx, bl[0], S[0]);
     const int bx = support_base<K>(sx, S[0]);
+    const float tx = sx - (float)(bx + Spline<K>::REF_OFF);
     float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll 1
     for (int a = 0; a < NT; ++a) {
-      float x = sx - (float)(bx + a);
-      float wxa = Spline<K>::w(x);
-      int ia = bx + a;
-      if (K != 2) ia = ia < 0 ? 0 : (ia > S[0] - 1 ? S[0] - 1 : ia);
+      const float wxa = tap_w<K>(tx, a);
+      const int ia = tap_index<K>(bx, a, S[0]);
       float yv = 0.f, yd = 0.f, yz = 0.f;
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
@@ -239,7 +312,7 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
       }
       v = fmaf(yv, wxa, v);
       if (GRAD) {
-        gx = fmaf(yv, Spline<K>::dw(x), gx);
+        gx = fmaf(yv, tap_dw<K>(tx, a), gx);
         gy = fmaf(yd, wxa, gy);
         gz = fmaf(yz, wxa, gz);
       }
@@ -259,15 +332,28 @@ struct Mac {
 };
 
 // u(p) for the MAC field (quadratic), optionally J[a][b] = d u_a / d x_b.
+// The component loop stays rolled; its results are stored through
+// wave-uniform branches (a dynamically indexed val[a] / J[a] would live in
+// scratch memory).
 template <bool GRAD, int DIM>
 DFM_HD void mac_eval(const Geo& g, const Mac& u, const float p[3], float val[3], float J[3][3]) {
 #pragma unroll 1
   for (int a = 0; a < DIM; ++a) {
     int S[3];
     shape_of(g, a, S);
-    float bl[3] = {0.5f, 0.5f, 0.5f};
-    bl[a] = 0.0f;
-    val[a] = interp<2, GRAD, DIM>(u.c[a], S, bl, p, g.inv_dx, GRAD ? J[a] : nullptr);
+    const float bl[3] = {a == 0 ? 0.0f : 0.5f, a == 1 ? 0.0f : 0.5f, a == 2 ? 0.0f : 0.5f};
+    float gr[3] = {0.f, 0.f, 0.f};
+    const float v = interp<2, GRAD, DIM>(u.c[a], S, bl, p, g.inv_dx, gr);
+    if (a == 0) {
+      val[0] = v;
+      if (GRAD) { J[0][0] = gr[0]; J[0][1] = gr[1]; J[0][2] = gr[2]; }
+    } else if (a == 1) {
+      val[1] = v;
+      if (GRAD) { J[1][0] = gr[0]; J[1][1] = gr[1]; J[1][2] = gr[2]; }
+    } else {
+      val[2] = v;
+      if (GRAD) { J[2][0] = gr[0]; J[2][1] = gr[1]; J[2][2] = gr[2]; }
+    }
   }
 }
 
