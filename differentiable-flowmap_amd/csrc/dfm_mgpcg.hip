@@ -968,6 +968,15 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
 }
 
 // ------------------------------------------------------------- handle -----
+// A solve between its queued start and its completion.
+struct Pending {
+  int active;
+  int it, refinements, max_iters;
+  bool use_graph;
+  hipStream_t st;
+  Faces F;           // velocity of a projection (for the gradient subtraction)
+};
+
 struct Solver {
   Geo G;
   int L, pre, bottom;
@@ -1001,6 +1010,7 @@ struct Solver {
   hipStream_t cap;
   hipGraphExec_t gexec;
   int gexec_iters;
+  Pending pend;      // the solve between project_begin and project_end
   // obstacle mask (MGPCG_2_solid): solid flags per level, fluid cell count
   int masked;
   uint8_t* bm[kMaxLevels];
@@ -1236,58 +1246,79 @@ bool build_graph(Solver& s) {
   return true;
 }
 
-// PCG from r0 (already in lv[0].r, its mean in sc[S_MEAN]): solve(),
-// 2D/vortex/mgpcg.py:202-282, with FP64 iterative refinement.
+// Queue k PCG iterations (graph replays, then single iterations), the FP64
+// true-residual check that runs only if they converged, and an asynchronous
+// copy of the flag block to pinned host memory.
 template <int DIM, bool MASK>
-int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
+int pcg_queue(Solver& s, int k) {
+  Pending& P = s.pend;
+  int left = k;
+  while (P.use_graph && left >= s.gexec_iters) {
+    if (hipGraphLaunch(s.gexec, P.st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: graph launch failed");
+    left -= s.gexec_iters;
+  }
+  for (int q = 0; q < left; ++q) pcg_iteration<DIM, MASK>(s, P.st);
+  P.it += k;
+  // Converged on the recursive FP32 residual: measure the true residual
+  // b - A x in FP64; if it misses the tolerance, PCG restarts from it.
+  if (P.refinements < s.max_refine)
+    hipLaunchKernelGGL((k_true_resid<DIM, MASK>), dim3(gridr(s.lv[0].size)), dim3(kB), 0, P.st, s.ld[0],
+                       (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
+  if (hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, P.st) != hipSuccess)
+    return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
+  return DFM_OK;
+}
+
+// PCG from r0 (already in lv[0].r, its mean in sc[S_MEAN]): solve(),
+// 2D/vortex/mgpcg.py:202-282, with FP64 iterative refinement.  Split in two
+// so that a caller can queue several independent solves on different streams
+// before waiting on any: pcg_begin queues the set-up and as many iterations
+// as the previous solve needed (consecutive projections converge in nearly
+// the same number, and iterations queued past convergence are no-ops);
+// pcg_end waits for the flag block and continues until convergence.
+template <int DIM, bool MASK>
+int pcg_begin(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   Lvl& L0 = s.lv[0];
   int64_t N = L0.size;
   unsigned gr = gridr(N);
   hipLaunchKernelGGL((k_recenter_norm<MASK>), dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, (const uint8_t*)s.bm[0],
                      red_of(s, tol, relative));
   vcycle<DIM, MASK>(s, st, nullptr, ZrFuse{FIN_ZR_INIT, s.p});
-  int it = 0;
-  int refinements = 0;
+  Pending& P = s.pend;
+  P.st = st;
+  P.it = 0;
+  P.refinements = 0;
+  P.max_iters = max_iters;
+  P.use_graph = s.use_graph;
+  if (P.use_graph && (!s.gexec || s.gexec_iters != s.check_every)) P.use_graph = build_graph<DIM, MASK>(s);
+  int k = s.predict > 0 ? s.predict : s.check_every;
+  if (k > max_iters) k = max_iters;
+  P.active = 1;
+  return pcg_queue<DIM, MASK>(s, k);
+}
+
+template <int DIM, bool MASK>
+int pcg_end(Solver& s) {
+  Pending& P = s.pend;
+  P.active = 0;
+  hipStream_t st = P.st;
   bool done = false;
-  auto read_flags = [&]() {
-    return hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, st) == hipSuccess &&
-           hipStreamSynchronize(st) == hipSuccess;
-  };
-  bool use_graph = s.use_graph;
-  if (use_graph && (!s.gexec || s.gexec_iters != s.check_every)) use_graph = build_graph<DIM, MASK>(s);
-  // Queue as many iterations as the previous solve needed before polling the
-  // first time: consecutive projections converge in nearly the same number of
-  // iterations, and iterations queued past convergence are no-ops.  The FP64
-  // true-residual check is queued behind them and runs only if they converged,
-  // so the common case costs one host synchronisation per solve.
-  int batch = s.predict > 0 ? s.predict : s.check_every;
-  bool first_poll = true;
-  while (it < max_iters) {
-    int k = first_poll ? batch : s.check_every;
-    if (it + k > max_iters) k = max_iters - it;
-    int left = k;
-    while (use_graph && left >= s.gexec_iters) {
-      if (hipGraphLaunch(s.gexec, st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: graph launch failed");
-      left -= s.gexec_iters;
-    }
-    for (int q = 0; q < left; ++q) pcg_iteration<DIM, MASK>(s, st);
-    it += k;
-    first_poll = false;
-    // Converged on the recursive FP32 residual: measure the true residual
-    // b - A x in FP64; if it misses the tolerance, PCG restarts from it.
-    if (refinements < s.max_refine)
-      hipLaunchKernelGGL((k_true_resid<DIM, MASK>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const double*)s.x,
-                         (const float*)s.b, L0.r, red_of(s), 1);
-    if (!read_flags()) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
+  for (;;) {
+    if (hipStreamSynchronize(st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
     if (s.h_flags[F_DONE]) {
-      if (s.h_flags[F_STATUS] == 1 && refinements == 0) s.predict = s.h_flags[F_ITERS];
+      if (s.h_flags[F_STATUS] == 1 && P.refinements == 0) s.predict = s.h_flags[F_ITERS];
       done = true;
       break;
     }
-    if (s.h_flags[F_NTRUE] > refinements) {
-      ++refinements;
+    if (s.h_flags[F_NTRUE] > P.refinements) {
+      ++P.refinements;
       vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_ZR_INIT, s.p});
     }
+    if (P.it >= P.max_iters) break;
+    int k = s.check_every;
+    if (P.it + k > P.max_iters) k = P.max_iters - P.it;
+    int rc = pcg_queue<DIM, MASK>(s, k);
+    if (rc != DFM_OK) return rc;
   }
   bool rb = hipMemcpyAsync(s.h_sc, s.sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st) == hipSuccess;
   if (!done) rb = rb && hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, st) == hipSuccess;
@@ -1300,6 +1331,16 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   if (s.h_flags[F_STATUS] == 2) return set_error(DFM_E_NONFINITE, "mgpcg: residual is not finite");
   if (!s.h_flags[F_DONE]) return set_error(DFM_E_NOCONV, "mgpcg: max_iters reached without convergence");
   return DFM_OK;
+}
+
+template <int DIM, bool MASK>
+int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
+  int rc = pcg_begin<DIM, MASK>(s, st, tol, relative, max_iters);
+  if (rc != DFM_OK) {
+    s.pend.active = 0;
+    return rc;
+  }
+  return pcg_end<DIM, MASK>(s);
 }
 
 void destroy(Solver* s) {
@@ -1400,6 +1441,7 @@ extern "C" int dfm_mgpcg_destroy(void* handle) {
 extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
   Solver* s = static_cast<Solver*>(handle);
   if (!s) return set_error(DFM_E_NULL, "mgpcg handle is NULL");
+  if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   switch (option) {
     case DFM_MG_BF16_COARSE:
       s->bf16_coarse = value ? 1 : 0;
@@ -1455,14 +1497,15 @@ extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
   }
 }
 
-extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int relative, int max_iters,
-                                 void* stream) {
+extern "C" int dfm_mgpcg_project_begin(void* handle, const dfm_mac* u, double tol, int relative, int max_iters,
+                                       void* stream) {
   Solver* s = static_cast<Solver*>(handle);
   if (!s) return set_error(DFM_E_NULL, "mgpcg handle is NULL");
   if (!u) return set_error(DFM_E_NULL, "velocity is NULL");
   for (int a = 0; a < s->G.dim; ++a)
     if (!u->c[a]) return set_error(DFM_E_NULL, "velocity component is NULL");
   if (max_iters < 1) return set_error(DFM_E_ARG, "max_iters must be >= 1");
+  if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   Lvl& L0 = s->lv[0];
   const Dims& d0 = s->ld[0];
@@ -1473,23 +1516,39 @@ extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int
   if (s->G.dim == 2) {
     if (s->masked) {
       hipLaunchKernelGGL((k_div<2, true>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
-      rc = pcg<2, true>(*s, st, tol, relative, max_iters);
+      rc = pcg_begin<2, true>(*s, st, tol, relative, max_iters);
     } else {
       hipLaunchKernelGGL((k_div<2, false>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
-      rc = pcg<2, false>(*s, st, tol, relative, max_iters);
+      rc = pcg_begin<2, false>(*s, st, tol, relative, max_iters);
     }
   } else {
     if (s->masked) {
       hipLaunchKernelGGL((k_div<3, true>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
-      rc = pcg<3, true>(*s, st, tol, relative, max_iters);
+      rc = pcg_begin<3, true>(*s, st, tol, relative, max_iters);
     } else {
       hipLaunchKernelGGL((k_div<3, false>), dim3(gr), dim3(kB), 0, st, d0, F, L0.r, R);
-      rc = pcg<3, false>(*s, st, tol, relative, max_iters);
+      rc = pcg_begin<3, false>(*s, st, tol, relative, max_iters);
     }
   }
+  s->pend.F = F;
+  if (rc != DFM_OK) s->pend.active = 0;
+  return rc;
+}
+
+extern "C" int dfm_mgpcg_project_end(void* handle) {
+  Solver* s = static_cast<Solver*>(handle);
+  if (!s) return set_error(DFM_E_NULL, "mgpcg handle is NULL");
+  if (!s->pend.active) return set_error(DFM_E_ARG, "mgpcg: no begun projection on this handle");
+  const bool m = s->masked;
+  int rc = s->G.dim == 2 ? (m ? pcg_end<2, true>(*s) : pcg_end<2, false>(*s))
+                         : (m ? pcg_end<3, true>(*s) : pcg_end<3, false>(*s));
   if (rc != DFM_OK && rc != DFM_E_NOCONV) return rc;
   // subtract the pressure gradient even when max_iters was hit (the reference
   // prints and continues, 2D/vortex/mgpcg.py:280-282)
+  hipStream_t st = s->pend.st;
+  Faces F = s->pend.F;
+  Lvl& L0 = s->lv[0];
+  const Dims& d0 = s->ld[0];
   int64_t nf = 0;
   for (int a = 0; a < s->G.dim; ++a) nf += (L0.size / L0.n[a]) * (L0.n[a] + 1);
   unsigned gf = (unsigned)((nf + kB - 1) / kB);
@@ -1502,15 +1561,22 @@ extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int
   }
   const double* px = s->x;
   if (s->G.dim == 2) {
-    if (s->masked) hipLaunchKernelGGL((k_subgrad<2, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+    if (m) hipLaunchKernelGGL((k_subgrad<2, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
     else hipLaunchKernelGGL((k_subgrad<2, false>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
   } else {
-    if (s->masked) hipLaunchKernelGGL((k_subgrad<3, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+    if (m) hipLaunchKernelGGL((k_subgrad<3, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
     else hipLaunchKernelGGL((k_subgrad<3, false>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg subtract_grad_p", hipGetErrorString(e));
   return rc;
+}
+
+extern "C" int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int relative, int max_iters,
+                                 void* stream) {
+  int rc = dfm_mgpcg_project_begin(handle, u, tol, relative, max_iters, stream);
+  if (rc != DFM_OK) return rc;
+  return dfm_mgpcg_project_end(handle);
 }
 
 extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double tol, int relative, int max_iters,
@@ -1519,6 +1585,7 @@ extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double to
   if (!s) return set_error(DFM_E_NULL, "mgpcg handle is NULL");
   if (!b || !x) return set_error(DFM_E_NULL, "b/x is NULL");
   if (max_iters < 1) return set_error(DFM_E_ARG, "max_iters must be >= 1");
+  if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   Lvl& L0 = s->lv[0];
   unsigned gr = gridr(L0.size);
@@ -1543,6 +1610,7 @@ extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double to
 extern "C" int dfm_mgpcg_set_mask(void* handle, const float* mask, void* stream) {
   Solver* s = static_cast<Solver*>(handle);
   if (!s) return set_error(DFM_E_NULL, "mgpcg handle is NULL");
+  if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   drop_graph(*s);
   if (!mask) {

@@ -55,6 +55,8 @@ SIGNATURES = {
     "dfm_mgpcg_create": (c_int, [P(Grid), c_int, c_int, c_int, P(c_void_p)]),
     "dfm_mgpcg_destroy": (c_int, [c_void_p]),
     "dfm_mgpcg_project": (c_int, [c_void_p, P(Mac), c_double, c_int, c_int, c_void_p]),
+    "dfm_mgpcg_project_begin": (c_int, [c_void_p, P(Mac), c_double, c_int, c_int, c_void_p]),
+    "dfm_mgpcg_project_end": (c_int, [c_void_p]),
     "dfm_mgpcg_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_int, c_void_p]),
     "dfm_mgpcg_status": (c_int, [c_void_p, P(c_int), P(c_double)]),
     "dfm_mgpcg_pressure": (c_int, [c_void_p, P(c_void_p)]),

@@ -228,6 +228,19 @@ class MGPCG:
     # the projection is symmetric and idempotent, so its adjoint is itself
     Poisson_adj = Poisson
 
+    def Poisson_begin(self, u: MacField):
+        """Queue a projection of u on the current stream without waiting;
+        complete it with Poisson_end (independent projections on other
+        handles and streams may be begun in between, and then overlap)."""
+        rc = self.lib.dfm_mgpcg_project_begin(self.h, u.s, float(self.tol), 1 if self.relative else 0,
+                                              int(self.max_iters), L.stream_handle())
+        L.check(rc, "mgpcg_project_begin")
+
+    def Poisson_end(self):
+        rc = self.lib.dfm_mgpcg_project_end(self.h)
+        self._done(rc, "mgpcg_project_end")
+        timing.add_count("mgpcg_iters", self.last_iters)
+
     def solve(self, b, x):
         rc = self.lib.dfm_mgpcg_solve(self.h, b.data_ptr(), x.data_ptr(), float(self.tol),
                                       1 if self.relative else 0, int(self.max_iters), L.stream_handle())

@@ -38,7 +38,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import scenes
+from . import scenes, timing
 from .ops import FlowMap, MacField, MGPCG, Ops
 
 
@@ -112,7 +112,8 @@ class LFM_Simulator:
 
     def __init__(self, *args, order=None, total_steps=None, n_passive=1, adj_kernel=None, mg_levels=0,
                  mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda",
-                 mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False):
+                 mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
+                 overlap_projections=True):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -134,9 +135,15 @@ class LFM_Simulator:
         if not torch.cuda.is_available():
             raise RuntimeError("dfm.LFM_Simulator needs a HIP device (torch.cuda.is_available() is False)")
         self.ops = Ops(n, dx)
-        self.solver = MGPCG(n, dx, levels=mg_levels, pre_post=mg_pre_post, bottom=mg_bottom, tol=tol,
-                            max_iters=max_iters, bf16_coarse=mg_bf16_coarse, tail=mg_tail,
-                            bottom_exact=mg_bottom_exact, check_every=mg_check_every)
+        self._mg_kw = dict(levels=mg_levels, pre_post=mg_pre_post, bottom=mg_bottom, tol=tol, max_iters=max_iters,
+                           bf16_coarse=mg_bf16_coarse, tail=mg_tail, bottom_exact=mg_bottom_exact,
+                           check_every=mg_check_every)
+        self.solver = MGPCG(n, dx, **self._mg_kw)
+        # independent projections of one adjoint step run overlapped on a
+        # second handle and stream
+        self.overlap_projections = overlap_projections
+        self._solver2 = None
+        self._stream2 = None
         dev = self.device
         self.u = MacField.empty(n, dev, zero=True)
         self._u_new = MacField.empty(n, dev, zero=True)
@@ -178,11 +185,15 @@ class LFM_Simulator:
             self.boundary_mask_t = None
             self.boundary_vel_t = None
             self.solver.set_mask(None)
+            if self._solver2 is not None:
+                self._solver2.set_mask(None)
             return
         m = torch.as_tensor(np.asarray(mask, dtype=np.float32)).to(self.device).contiguous()
         if tuple(m.shape) != self.n:
             raise ValueError(f"mask shape {tuple(m.shape)} != grid {self.n}")
         self.boundary_mask_t = m
+        if self._solver2 is not None:
+            self._solver2.set_mask(m)
         if boundary_vel is not None:
             bv = torch.as_tensor(np.asarray(boundary_vel, dtype=np.float32)).to(self.device)
             self.boundary_vel_t = torch.movedim(bv, -1, 0).contiguous()
@@ -447,8 +458,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
         ops.march(mid, self.psi_map, dt, order)
         ops.accumulate(self.r_u, self.psi_map, self.init_adj_u, self.ab_h1, self.ab_h2,
                        self.reinit_every - 1 - j, -2.0 * dt, f2=self.r_s, dt2=-dt)
-        self.solver.Poisson_adj(self.adj_u)
-        self.solver.Poisson(self.u)
+        self._project_pair(self.adj_u, self.u)
         self._mask_adj()
         if j == 0:
             fwd = self.flow_maps[step // self.reinit_every]
@@ -507,6 +517,34 @@ class LFM_Diff_Simulator(LFM_Simulator):
             for k, v in saved.items():
                 setattr(self, k, v)
             self.step_num[None] = saved_step
+
+    def _project_pair(self, first, second):
+        """Poisson_adj(first) and Poisson(second) (2D/vortex/lfm_midpoint_diff_simulator.py:391,403),
+        which do not depend on each other: queued on two MGPCG handles and
+        streams before either waits, so their latency-bound kernels overlap.
+        The results are those of the two projections in sequence."""
+        if not self.overlap_projections:
+            self.solver.Poisson_adj(first)
+            self.solver.Poisson(second)
+            return
+        if self._solver2 is None:
+            self._solver2 = MGPCG(self.n, self.dx, **self._mg_kw)
+            if self.boundary_mask_t is not None:
+                self._solver2.set_mask(self.boundary_mask_t)
+            self._stream2 = torch.cuda.Stream(device=self.device)
+        main = torch.cuda.current_stream()
+        with timing.region("mgpcg", calls=2):
+            ready = torch.cuda.Event()
+            ready.record(main)
+            self.solver.Poisson_begin(first)
+            with torch.cuda.stream(self._stream2):
+                self._stream2.wait_event(ready)
+                self._solver2.Poisson_begin(second)
+            self.solver.Poisson_end()
+            self._solver2.Poisson_end()
+            done = torch.cuda.Event()
+            done.record(self._stream2)
+            main.wait_event(done)
 
     def _mask_adj(self):
         """The forward masks on u and the passives plus

@@ -28,7 +28,9 @@ def reset():
 
 
 @contextlib.contextmanager
-def region(name):
+def region(name, calls=1):
+    """Time the enclosed launches as `calls` calls of `name` (a region that
+    covers several overlapped operations counts each of them)."""
     if not _enabled:
         yield
         return
@@ -39,7 +41,7 @@ def region(name):
         yield
     finally:
         e.record()
-        _events[name].append((s, e))
+        _events[name].append((s, e, calls))
 
 
 def add_count(name, k=1):
@@ -50,7 +52,7 @@ def add_count(name, k=1):
 def summary():
     """{name: (calls, total_ms)} after synchronising."""
     torch.cuda.synchronize()
-    return {k: (len(v), sum(s.elapsed_time(e) for s, e in v)) for k, v in _events.items()}
+    return {k: (sum(c for _, _, c in v), sum(s.elapsed_time(e) for s, e, _ in v)) for k, v in _events.items()}
 
 
 def counts():

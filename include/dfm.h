@@ -149,6 +149,18 @@ int dfm_mgpcg_project(void* handle, const dfm_mac* u, double tol, int relative, 
 /* Solve A x = b (b recentred to mean zero first).  Same stopping rule. */
 int dfm_mgpcg_solve(void* handle, const float* b, float* x, double tol, int relative, int max_iters, void* stream);
 
+/* The projection in two halves, so that independent projections on
+ * different handles and streams overlap on the device:
+ *  _begin queues the divergence, the PCG set-up and as many iterations as
+ *  this handle's previous solve needed (no host synchronisation);
+ *  _end waits for the convergence flags, queues further iterations if
+ *  needed, then the gradient subtraction, on the stream given to _begin.
+ * dfm_mgpcg_project is _begin followed by _end.  One begun projection per
+ * handle at a time; no other call on the handle in between.  No counterpart
+ * in the reference (its solves are synchronous). */
+int dfm_mgpcg_project_begin(void* handle, const dfm_mac* u, double tol, int relative, int max_iters, void* stream);
+int dfm_mgpcg_project_end(void* handle);
+
 /* Iterations and relative residual sqrt(r.r / b.b) of the last solve. */
 int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
 
