@@ -37,7 +37,10 @@ using namespace dfm;
 namespace {
 
 constexpr int kB = 256;          // threads per block
-constexpr int kRedBlocks = 4096; // most blocks of a reducing kernel (grid-stride beyond)
+constexpr int kRedBlocks = 2048;  // most blocks of a reducing kernel (grid-stride beyond)
+// Coarse levels of at most this many cells run each RBGS sweep as one launch
+// (k_sweep2); their half-sweeps are launch-latency bound.
+constexpr int64_t kSweep2MaxCells = int64_t(1) << 19;
 constexpr unsigned kCtrGroups = 64;  // arrival-counter groups of a grid reduction
 constexpr unsigned kCtrStride = 32;  // counters 128 B apart
 constexpr unsigned kCtrWords = (kCtrGroups + 1) * kCtrStride;
@@ -53,6 +56,7 @@ struct Lvl {
   int64_t size;
   float* r;
   float* z;
+  float* z2;  // second iterate buffer of the fused-sweep levels (else nullptr)
 };
 
 struct Dims {
@@ -700,6 +704,42 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
   if (ZR) grid_reduce(R, op, acc, 0.0);
 }
 
+// One full RBGS sweep (colour c1, then the other) as a single launch, for the
+// levels whose half-sweeps are launch-latency bound.  The second colour's
+// update needs the first colour's new values at its neighbours; each such
+// thread recomputes them from the previous iterate.  Reading zin and writing
+// zout (separate buffers) keeps the launch race-free, and every value is
+// formed with the operations of the two half-sweeps in their order, so the
+// result is bitwise theirs.  zc (first sweep after prolongation only) is
+// added to the neighbour values of the first colour's update, as there.
+template <int DIM, bool MASK>
+__global__ void __launch_bounds__(kB) k_sweep2(Dims d, const float* __restrict__ r, const float* __restrict__ zin,
+                                               float* __restrict__ zout, int c1, const double* shift,
+                                               const float* __restrict__ zc, Dims dc, const int* flags) {
+#pragma clang fp contract(off)
+  if (is_done(flags)) return;
+  const float m = shift ? (float)*shift : 0.f;
+  const int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
+  const uint32_t id = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  if (id >= N) return;
+  if (!is_fluid<MASK>(d, id)) {
+    zout[id] = 0.f;
+    return;
+  }
+  int i, j, k;
+  cell_of(d, id, i, j, k);
+  if (((i + j + k) & 1) == c1) {
+    zout[id] = gs_value<DIM, MASK>(d, r, zin, zc, dc, m, i, j, k, id);
+    return;
+  }
+  float s = 0.f, nn = 0.f;
+  for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
+    s += gs_value<DIM, MASK>(d, r, zin, zc, dc, m, a, b, c, q);
+    nn += 1.f;
+  });
+  zout[id] = (r[id] - m + s) * inv_count(nn);
+}
+
 // restrict (2D/vortex/mgpcg.py:152-158): coarse r = sum over the 2^d children
 // of the fine residual r - A z, times 1/(dim-1).  One thread per coarse cell,
 // so the sum order is fixed.  MASK: fluid children only
@@ -1015,6 +1055,7 @@ struct Solver {
   int masked;
   uint8_t* bm[kMaxLevels];
   double nfluid;
+  int fused_sweeps;  // option: one launch per RBGS sweep on levels with z2
 };
 
 // Largest LDS a workgroup may request on this device.
@@ -1104,36 +1145,47 @@ struct ZrFuse {
   float* p;  // also p = z (start and restart of PCG), else nullptr
 };
 
-// `sweeps` RBGS sweeps (colour c1 first, then the other) in place on lv[l].z,
-// starting from z = 0 when from_zero (first sweep fused into one kernel).
-// zc_prolong fuses the prolongation into the first half-sweep; zr (if given)
-// is fused into the last half-sweep.  Returns whether zr was done.
+// `sweeps` RBGS sweeps (colour c1 first, then the other) on level l, starting
+// from z = 0 when from_zero (first sweep fused into one kernel; z is set to
+// lv[l].z) or else from the iterate in buffer z.  zc_prolong fuses the
+// prolongation into the first sweep; zr (if given) is fused into the last
+// half-sweep.  Levels with a second buffer run each full sweep as one k_sweep2
+// launch, alternating between lv[l].z and lv[l].z2; the others sweep in place.
+// On return z names the buffer holding the result.  Returns whether zr was done.
 template <int DIM, bool MASK>
 bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero, const double* shift,
-                   const float* zc_prolong, const ZrFuse* zr) {
+                   const float* zc_prolong, const ZrFuse* zr, float*& z) {
   Lvl& L = s.lv[l];
   const Dims& d = s.ld[l];
   const Dims& dc = s.ld[l + 1 < s.L ? l + 1 : l];
   const Red R = red_of(s);
   int k0 = 0;
   if (from_zero) {
-    hipLaunchKernelGGL((k_first_sweep<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, L.z, shift,
-                       s.flags);
+    z = L.z;
+    hipLaunchKernelGGL((k_first_sweep<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, z, shift, s.flags);
     k0 = 1;
   }
   const uint32_t pairs = pairs_of(d, DIM);
   const unsigned gs = grid1(pairs);
+  const bool fused = s.fused_sweeps && L.z2 && !zr;
   bool zr_done = false;
   for (int it = k0; it < sweeps; ++it) {
     const float* zc = (it == k0 && zc_prolong) ? zc_prolong : nullptr;
-    hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, c1, shift, zc, dc, R,
+    if (fused) {
+      float* zout = z == L.z ? L.z2 : L.z;
+      hipLaunchKernelGGL((k_sweep2<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, (const float*)L.r,
+                         (const float*)z, zout, c1, shift, zc, dc, (const int*)s.flags);
+      z = zout;
+      continue;
+    }
+    hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, z, c1, shift, zc, dc, R,
                        0, (float*)nullptr);
     if (zr && it == sweeps - 1) {
-      hipLaunchKernelGGL((k_smooth<DIM, true, MASK>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, L.z, 1 - c1,
+      hipLaunchKernelGGL((k_smooth<DIM, true, MASK>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, z, 1 - c1,
                          shift, (const float*)nullptr, dc, R, zr->op, zr->p);
       zr_done = true;
     } else {
-      hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, L.z, 1 - c1, shift,
+      hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, z, 1 - c1, shift,
                          (const float*)nullptr, dc, R, 0, (float*)nullptr);
     }
   }
@@ -1153,8 +1205,7 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   bool zr_done = false;
   for (int l = 0; l < Lg; ++l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    sweeps_global<DIM, MASK>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr);
-    cur[l] = s.lv[l].z;
+    sweeps_global<DIM, MASK>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr, cur[l]);
     Lvl& F = s.lv[l];
     Lvl& C = s.lv[l + 1];
     hipLaunchKernelGGL((k_restrict<DIM, MASK>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r, cur[l],
@@ -1171,20 +1222,20 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   } else {
     const double* sh = s.L == 1 ? shift0 : nullptr;
     const ZrFuse* zb = s.L == 1 ? &zr : nullptr;
+    float*& zb_cur = cur[s.L - 1];
     if (MASK && s.bottom >= 2) {
       const int first = (s.bottom + 1) / 2;
-      sweeps_global<DIM, MASK>(s, st, s.L - 1, first, 0, true, sh, nullptr, nullptr);
-      zr_done = sweeps_global<DIM, MASK>(s, st, s.L - 1, s.bottom - first, 1, false, sh, nullptr, zb);
+      sweeps_global<DIM, MASK>(s, st, s.L - 1, first, 0, true, sh, nullptr, nullptr, zb_cur);
+      zr_done = sweeps_global<DIM, MASK>(s, st, s.L - 1, s.bottom - first, 1, false, sh, nullptr, zb, zb_cur);
     } else {
-      zr_done = sweeps_global<DIM, MASK>(s, st, s.L - 1, s.bottom, 0, true, sh, nullptr, zb);
+      zr_done = sweeps_global<DIM, MASK>(s, st, s.L - 1, s.bottom, 0, true, sh, nullptr, zb, zb_cur);
     }
-    cur[s.L - 1] = s.lv[s.L - 1].z;
   }
   for (int l = Lg - 1; l >= 0; --l) {
     const double* sh = l == 0 ? shift0 : nullptr;
-    const bool d0 = sweeps_global<DIM, MASK>(s, st, l, s.pre, 1, false, sh, cur[l + 1], l == 0 ? &zr : nullptr);
+    const bool d0 = sweeps_global<DIM, MASK>(s, st, l, s.pre, 1, false, sh, cur[l + 1], l == 0 ? &zr : nullptr,
+                                             cur[l]);
     if (l == 0) zr_done = d0;
-    cur[l] = s.lv[l].z;
   }
   s.z0_final = cur[0];
   if (!zr_done) {
@@ -1348,6 +1399,7 @@ void destroy(Solver* s) {
   for (int l = 0; l < s->L; ++l) {
     if (s->lv[l].r) (void)hipFree(s->lv[l].r);
     if (s->lv[l].z) (void)hipFree(s->lv[l].z);
+    if (s->lv[l].z2) (void)hipFree(s->lv[l].z2);
   }
   if (s->x) (void)hipFree(s->x);
   if (s->b) (void)hipFree(s->b);
@@ -1401,6 +1453,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->check_every = 1;
   s->max_refine = 3;
   s->use_graph = 1;
+  s->fused_sweeps = 1;
   bool ok = true;
   for (int l = 0; l < levels && ok; ++l) {
     for (int a = 0; a < 3; ++a) s->lv[l].n[a] = a < g->dim ? (g->n[a] >> l) : 1;
@@ -1408,6 +1461,8 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
     s->ld[l] = make_dims(s->lv[l].n[0], s->lv[l].n[1], s->lv[l].n[2], g->dim);
     ok = hipMalloc(&s->lv[l].r, sizeof(float) * s->lv[l].size) == hipSuccess &&
          hipMalloc(&s->lv[l].z, sizeof(float) * s->lv[l].size) == hipSuccess;
+    if (ok && l > 0 && s->lv[l].size <= kSweep2MaxCells)
+      ok = hipMalloc(&s->lv[l].z2, sizeof(float) * s->lv[l].size) == hipSuccess;
   }
   int64_t N = s->lv[0].size;
   ok = ok && hipMalloc(&s->x, sizeof(double) * N) == hipSuccess && hipMalloc(&s->b, sizeof(float) * N) == hipSuccess &&
@@ -1477,6 +1532,10 @@ extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
       s->check_every = value;
       drop_graph(*s);
       return DFM_OK;
+    case DFM_MG_FUSED_SWEEPS:
+      s->fused_sweeps = value ? 1 : 0;
+      drop_graph(*s);
+      return DFM_OK;
     default:
       return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
@@ -1493,6 +1552,7 @@ extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
     case DFM_MG_GRAPH: *value = s->gexec ? 2 : s->use_graph; return DFM_OK;
     case DFM_MG_BOTTOM_EXACT: *value = (s->tail_lt < s->L) ? s->bottom_exact : 0; return DFM_OK;
     case DFM_MG_TAIL_LEVEL: *value = s->tail_lt; return DFM_OK;
+    case DFM_MG_FUSED_SWEEPS: *value = s->fused_sweeps; return DFM_OK;
     default: return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
 }

@@ -76,6 +76,7 @@ MG_CHECK_EVERY = 4
 MG_GRAPH = 5
 MG_BOTTOM_EXACT = 6
 MG_TAIL_LEVEL = 7
+MG_FUSED_SWEEPS = 8
 
 DFM_E_NOCONV = -4
 

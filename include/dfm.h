@@ -191,6 +191,11 @@ int dfm_mgpcg_pressure(void* handle, double** p);
  *      kernels.  get: the first tail level in use (L = no tail).  The batch
  *      executor's autotuner sets it per grid size. */
 #define DFM_MG_TAIL_LEVEL 7
+/*  DFM_MG_FUSED_SWEEPS: 1 = on the coarse global levels (at most 2^19 cells)
+ *      run each red-black Gauss-Seidel sweep as one launch that recomputes
+ *      the first colour's neighbour values (default; bitwise equal to the two
+ *      half-sweeps); 0 = two launches per sweep. */
+#define DFM_MG_FUSED_SWEEPS 8
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);
 
