@@ -67,6 +67,9 @@ SIGNATURES = {
     "dfm_mask_scalar": (c_int, [P(Grid), c_void_p, c_void_p, c_int, c_void_p]),
     "dfm_vortex_velocity_2d": (c_int, [P(Grid), c_void_p, c_int, P(Mac), c_void_p]),
     "dfm_vortex_velocity_2d_vjp": (c_int, [P(Grid), c_void_p, c_int, P(Mac), c_void_p, c_void_p]),
+    "dfm_gaussian_force": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_int, P(Mac), c_void_p]),
+    "dfm_gaussian_force_adjoint": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_int, P(Mac), c_void_p, c_void_p,
+                                           c_void_p, c_void_p, c_void_p]),
 }
 
 MG_BF16_COARSE = 1

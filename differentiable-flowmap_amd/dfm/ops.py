@@ -148,6 +148,20 @@ class Ops:
         L.check(self.lib.dfm_mask_scalar(self.g, mask.data_ptr(), s.data_ptr(), 1 if adjoint else 0, self._st()),
                 "mask_scalar")
 
+    def gaussian_force(self, center, inv_radius, strength, count, f: MacField):
+        """apply_gausion_force: f = the field of the first `count` Gaussian
+        forces (center (N, dim), inv_radius (N,), strength (N, dim); device FP32)."""
+        L.check(self.lib.dfm_gaussian_force(self.g, L.ptr(center), L.ptr(inv_radius), L.ptr(strength), int(count),
+                                            f.s, self._st()), "gaussian_force")
+
+    def gaussian_force_adjoint(self, center, inv_radius, strength, count, adj_u: MacField, adj_center, adj_strength,
+                               penalty_center, penalty_strength):
+        """calculate_adj_gausion_force: the first `count` rows of the four
+        (N, dim) outputs; rows beyond `count` are left as they are."""
+        L.check(self.lib.dfm_gaussian_force_adjoint(
+            self.g, L.ptr(center), L.ptr(inv_radius), L.ptr(strength), int(count), adj_u.s, L.ptr(adj_center),
+            L.ptr(adj_strength), L.ptr(penalty_center), L.ptr(penalty_strength), self._st()), "gaussian_force_adjoint")
+
     def axpy(self, x: MacField, alpha, y: MacField, out: MacField, beta=0.0, z: MacField = None):
         L.check(self.lib.dfm_axpy_mac(self.g, x.s, float(alpha), y.s, float(beta),
                                       z.s if z is not None else None, out.s, self._st()), "axpy")

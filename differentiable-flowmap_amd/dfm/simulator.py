@@ -17,10 +17,15 @@ Mirrors the reference's simulator-class API (the boundary its optimisers call):
 Hyper-parameters that the reference reads from its star-imported
 ``hyperparameters.py`` (RK_number, total_steps, add_passive_scalar ...) are
 keyword arguments here.  Supported configuration (the one every benchmark
-uses): viscosity 0, no control force, use_short_BFECC=False, BFECC_clamp=False,
-backward_u=True, final-time density loss.  Obstacles: ``set_boundary_mask``
-switches the projection to the obstacle solver (MGPCG_2_solid) and applies
-the reference's mask_* kernels at its call sites.
+uses): viscosity 0, use_short_BFECC=False, BFECC_clamp=False, backward_u=True,
+final-time density loss.  Obstacles: ``set_boundary_mask`` switches the
+projection to the obstacle solver (MGPCG_2_solid) and applies the reference's
+mask_* kernels at its call sites.  Control forces: ``control_num=N`` (the
+reference's add_control_force / control_num) enables ``control_force_para``
+({"c": centres (N, dim), "r": inverse squared radii (N,), "s": strengths
+(N, dim), "n": count}) in ``step_midpoint`` and ``backtrack_step_midpoint``;
+the adjoint step leaves the parameter gradients in ``adj_gaussion_force_*``
+and ``penalty_gaussion_force_*`` (N, dim), as the reference does.
 
 The reference's disk cache (RW_Cache, 2D/vortex/lfm_midpoint_simulator.py:46-184)
 is replaced by device-resident storage.  ``record=True`` (default) keeps the
@@ -113,7 +118,7 @@ class LFM_Simulator:
     def __init__(self, *args, order=None, total_steps=None, n_passive=1, adj_kernel=None, mg_levels=0,
                  mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda",
                  mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
-                 overlap_projections=True):
+                 overlap_projections=True, control_num=0):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -163,6 +168,15 @@ class LFM_Simulator:
         self.u_start = {}      # interval index -> MacField (checkpoint mode)
         self.boundary_mask_t = None  # obstacle mask on cells (None: no obstacles)
         self.boundary_vel_t = None   # boundary velocity, (dim, *n), or None for 0
+        # Gaussian control forces (add_control_force, 2D/vortex/lfm_midpoint_simulator.py:271-278)
+        self.control_num = int(control_num)
+        if self.control_num > 0:
+            N, d = self.control_num, self.dim
+            self.control_f = MacField.empty(n, dev, zero=True)
+            self._fc = torch.zeros((N, d), dtype=torch.float32, device=dev)
+            self._fr = torch.zeros((N,), dtype=torch.float32, device=dev)
+            self._fs = torch.zeros((N, d), dtype=torch.float32, device=dev)
+        self._force_store = {}  # step -> (centres, radii, strengths, count) for checkpoint replays
         self._views()
 
     # Taichi-like attribute views -------------------------------------------
@@ -174,6 +188,39 @@ class LFM_Simulator:
             self.passive1 = FieldView(self, lambda s: s.passive[0])
         self.psi_view = FieldView(self, lambda s: s.psi_map.psi, soa=True)
         self.phi_view = FieldView(self, lambda s: s.phi_map.psi, soa=True)
+        if self.control_num > 0:
+            for a, nm in enumerate(["control_force_x", "control_force_y", "control_force_z"][: self.dim]):
+                setattr(self, nm, FieldView(self, lambda s, a=a: s.control_f.c[a]))
+
+    # control forces -----------------------------------------------------------
+    def _load_forces(self, para):
+        """Copy control_force_para {"c", "r", "s", "n"} to the device
+        parameter buffers; returns the force count."""
+        if self.control_num <= 0:
+            raise ValueError("control_force_para needs a simulator built with control_num > 0 "
+                             "(the reference's add_control_force / control_num)")
+        count = int(para["n"])
+        if not 0 <= count <= self.control_num:
+            raise ValueError(f"force count {count} outside [0, control_num={self.control_num}]")
+        d = self.dim
+        for buf, key, shape in ((self._fc, "c", (-1, d)), (self._fr, "r", (-1,)), (self._fs, "s", (-1, d))):
+            src = torch.as_tensor(para[key], dtype=torch.float32).reshape(shape)
+            if src.shape[0] < count:
+                raise ValueError(f"control_force_para[{key!r}] has {src.shape[0]} rows, fewer than n = {count}")
+            buf[: src.shape[0]].copy_(src[: buf.shape[0]].to(self.device))
+        return count
+
+    def apply_gausion_force(self, *args):
+        """apply_gausion_force(f_x, f_y[, f_z], center, inverse_radius,
+        strength, num) (2D/vortex/lfm_midpoint_simulator.py:302-307): f = the
+        Gaussian force field of the first `num` forces."""
+        fields, (c, r, st, num) = args[: self.dim], args[self.dim:]
+        count = self._load_forces({"c": c, "r": r, "s": st, "n": num})
+        comps = [f.tensor if isinstance(f, FieldView) else f for f in fields]
+        self.ops.gaussian_force(self._fc, self._fr, self._fs, count, MacField(comps))
+
+    def _force_field(self, count):
+        self.ops.gaussian_force(self._fc, self._fr, self._fs, count, self.control_f)
 
     # obstacles --------------------------------------------------------------
     def set_boundary_mask(self, mask, boundary_vel=None):
@@ -251,6 +298,7 @@ class LFM_Simulator:
         self.passive_start.clear()
         self.flow_maps.clear()
         self.u_start.clear()
+        self._force_store.clear()
 
     def final_condition(self):
         return self.total_steps is not None and self.step_num[None] == self.total_steps - 1
@@ -258,15 +306,24 @@ class LFM_Simulator:
     # forward step -------------------------------------------------------------
     def step_midpoint(self, write_flow_map=False, write_passive=False, control_force_para=None):
         """step_midpoint (2D/vortex/lfm_midpoint_simulator.py:472-515)."""
-        if control_force_para is not None:
-            raise NotImplementedError("control forces are not supported by this build")
         self.step_num[None] += 1
-        j = self.step_num[None] % self.reinit_every
+        step = self.step_num[None]
+        j = step % self.reinit_every
         if j == 0 and self.record and not self._replaying:
-            self.passive_start[self.step_num[None]] = [p.clone() for p in self.passive]
+            self.passive_start[step] = [p.clone() for p in self.passive]
             if self.checkpoint:
-                self.u_start[self.step_num[None] // self.reinit_every] = self.u.clone()
-        self.step_flow_map(j, self.dt)
+                self.u_start[step // self.reinit_every] = self.u.clone()
+        forces = None
+        if control_force_para is not None:
+            forces = self._load_forces(control_force_para)
+            if self.checkpoint and not self._replaying:
+                self._force_store[step] = (self._fc.clone(), self._fr.clone(), self._fs.clone(), forces)
+        elif self._replaying and step in self._force_store:
+            c, r, st, forces = self._force_store[step]
+            self._fc.copy_(c)
+            self._fr.copy_(r)
+            self._fs.copy_(st)
+        self.step_flow_map(j, self.dt, forces)
         return False, self.final_condition()
 
     def _mid_buffer(self, step):
@@ -276,12 +333,17 @@ class LFM_Simulator:
         self.mid_u[step] = m
         return m
 
-    def step_flow_map(self, j, dt):
-        """step_flow_map (2D/vortex/lfm_midpoint_simulator.py:517-661)."""
+    def step_flow_map(self, j, dt, forces=None):
+        """step_flow_map (2D/vortex/lfm_midpoint_simulator.py:517-661); `forces`:
+        the count of control forces loaded for this step, or None."""
         ops, order, step = self.ops, self.order, self.step_num[None]
         mid = self._mid_buffer(step)
         # half-step self-advection of u, then projection -> midpoint velocity
         ops.sl_advect(self.u, 0.5 * dt, order, [self.u], [mid])
+        if forces is not None:
+            # half-step control force before the first projection (:527-537)
+            self._force_field(forces)
+            ops.axpy(mid, 0.5 * dt, self.control_f, mid)
         self.solver.Poisson(mid)
         self._mask_fwd(mid)
         # forward map phi/F marched with the midpoint velocity (dt sign flipped)
@@ -290,6 +352,12 @@ class LFM_Simulator:
         ops.sl_advect(mid, dt, order, [self.u], [self._u_new], self.passive, self._passive_new)
         self.u, self._u_new = self._u_new, self.u
         self.passive, self._passive_new = self._passive_new, self.passive
+        if forces is not None:
+            # full-step force and accumulate_init through (F, phi) (:597-602);
+            # init_u only feeds the reinitialisation, which a replay skips
+            ops.axpy(self.u, dt, self.control_f, self.u)
+            if not self._replaying:
+                ops.pullback_u(self.control_f, self.phi_map, self.init_u, alpha=dt, beta=1.0, add=self.init_u)
         self.solver.Poisson(self.u)
         self._mask_fwd(self.u)
         if j == self.reinit_every - 1:
@@ -363,6 +431,15 @@ class LFM_Diff_Simulator(LFM_Simulator):
         self._adj_passive_new = [torch.zeros_like(p) for p in self.adj_passive]
         self.init_adj_passive = [torch.zeros_like(p) for p in self.adj_passive]
         self.target_passive = [torch.zeros_like(p) for p in self.adj_passive]
+        if self.control_num > 0:
+            # calculate_adj_gausion_force outputs (2D/shape/lfm_midpoint_diff_simulator.py:58-64)
+            N, d = self.control_num, self.dim
+            self._force_grads = {k: torch.zeros((N, d), dtype=torch.float32, device=dev)
+                                 for k in ("adj_center", "adj_strength", "pen_center", "pen_strength")}
+            for nm, k in (("adj_gaussion_force_center", "adj_center"), ("adj_gaussion_force_strength", "adj_strength"),
+                          ("penalty_gaussion_force_center", "pen_center"),
+                          ("penalty_gaussion_force_strength", "pen_strength")):
+                setattr(self, nm, FieldView(self, lambda s, k=k: s._force_grads[k]))
         self.loss = 0.0
         self.final_ind = None
         names = ["adj_u_x", "adj_u_y", "adj_u_z"][: self.dim]
@@ -426,22 +503,34 @@ class LFM_Diff_Simulator(LFM_Simulator):
 
     def backtrack_step_midpoint(self, write_gradient=False, write_passive=False, control_force_para=None):
         """backtrack_step_midpoint (2D/vortex/lfm_midpoint_diff_simulator.py:220-258)."""
-        if control_force_para is not None:
-            raise NotImplementedError("control forces are not supported by this build")
+        if control_force_para is not None and self.control_num <= 0:
+            self._load_forces(control_force_para)  # raises: no control forces configured
         self.step_num[None] -= 1
         j = self.step_num[None] % self.reinit_every
-        self.backtrack_flow_map(j, self.dt)
+        self.backtrack_flow_map(j, self.dt, control_force_para)
         return False, self.step_num[None] == 0
 
-    def backtrack_flow_map(self, j, dt):
+    def backtrack_flow_map(self, j, dt, control_force_para=None):
         """backtrack_flow_map (2D/vortex/lfm_midpoint_diff_simulator.py:260-503)
-        with use_short_BFECC=False and backward_u=True."""
+        with use_short_BFECC=False and backward_u=True; with control forces as
+        2D/shape/lfm_midpoint_diff_simulator.py:207-343 adds them."""
         ops, order, step = self.ops, self.order, self.step_num[None]
         if step not in self.mid_u:
             if not self.checkpoint:
                 raise KeyError(f"no stored midpoint velocity for step {step} (forward run with record=False)")
             self._replay_interval(step // self.reinit_every)
         mid = self.mid_u[step]
+        forces = None
+        if control_force_para is not None:
+            # force field for the backward_u re-march, and the parameter
+            # gradients from lam_u as the step begins (:209-213)
+            forces = self._load_forces(control_force_para)
+            self._force_field(forces)
+            for t in self._force_grads.values():
+                t.zero_()
+            g = self._force_grads
+            ops.gaussian_force_adjoint(self._fc, self._fr, self._fs, forces, self.adj_u, g["adj_center"],
+                                       g["adj_strength"], g["pen_center"], g["pen_strength"])
         # adjoint sources at the grid faces, from the current (end-of-step) state
         ops.adj_conv_source(self.u, self.adj_u, self.adj_kernel, self.r_u)
         ops.adj_scalar_source(self.passive, self.adj_passive, self.r_s)
@@ -458,6 +547,10 @@ class LFM_Diff_Simulator(LFM_Simulator):
         ops.march(mid, self.psi_map, dt, order)
         ops.accumulate(self.r_u, self.psi_map, self.init_adj_u, self.ab_h1, self.ab_h2,
                        self.reinit_every - 1 - j, -2.0 * dt, f2=self.r_s, dt2=-dt)
+        if forces is not None:
+            # u takes the step's force back, accumulated through (T, psi) (:296-300)
+            ops.axpy(self.u, -dt, self.control_f, self.u)
+            ops.pullback_u(self.control_f, self.psi_map, self.init_u, alpha=-dt, beta=1.0, add=self.init_u)
         self._project_pair(self.adj_u, self.u)
         self._mask_adj()
         if j == 0:
@@ -470,7 +563,12 @@ class LFM_Diff_Simulator(LFM_Simulator):
             self.init_u.copy_(self.u)
             self._mask_adj()
             for k in range(self.n_passive):
-                self.passive[k].copy_(self.passive_start[step][k])
+                if self.control_num > 0:
+                    # with control forces the passive is rebuilt through the
+                    # interval's maps (2D/shape/lfm_midpoint_diff_simulator.py:339-343)
+                    ops.bfecc_s(self.init_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.passive[k])
+                else:
+                    self.passive[k].copy_(self.passive_start[step][k])
                 self.init_passive[k].copy_(self.passive[k])
                 ops.bfecc_s(self.init_adj_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.adj_passive[k])
                 self.init_adj_passive[k].copy_(self.adj_passive[k])

@@ -239,6 +239,29 @@ int dfm_vortex_velocity_2d(const dfm_grid* g, const float* theta, int K, const d
 int dfm_vortex_velocity_2d_vjp(const dfm_grid* g, const float* theta, int K, const dfm_mac* lam, float* dtheta,
                                void* stream);
 
+/* ------------------------------------------------ Gaussian control forces */
+/* Force field of `count` Gaussian forces on every face (fills f; faces no
+ * force reaches get 0).  Force I: centre[I*dim + b], inverse squared radius
+ * inv_radius[I], strength[I*dim + a] (all device FP32).  At face node p of
+ * axis a, with m = |p - centre_I| in FP32, force I adds
+ * exp(-(m*m) r_I) * strength_I[a] when (m*m) r_I < 5, in increasing I.
+ * Replaces apply_gausion_force / apply_gausion_force_kernel
+ * (2D/vortex/lfm_midpoint_simulator.py:302-326, 3D/lfm_midpoint_simulator.py:316-344). */
+int dfm_gaussian_force(const dfm_grid* g, const float* center, const float* inv_radius, const float* strength,
+                       int count, const dfm_mac* f, void* stream);
+
+/* Parameter adjoint of the force field against the face field adj_u: per
+ * force (device FP32, count x dim each) adj_strength[a] = sum e adj_u_a,
+ * adj_center = sum s[a] e 2 (p - c) r adj_u_a, penalty_strength[a] =
+ * sum 2 (e s[a]) e, penalty_center = sum 2 (e s[a]) s[a] e 2 (p - c) r, over
+ * the faces inside the cut-off (the FP32 test above; e and p - c in FP64,
+ * FP64 sums, deterministic).
+ * Replaces calculate_adj_gausion_force / _kernel
+ * (2D/shape/lfm_midpoint_diff_simulator.py:68-103, 3D/lfm_midpoint_diff_simulator.py:63-110). */
+int dfm_gaussian_force_adjoint(const dfm_grid* g, const float* center, const float* inv_radius,
+                               const float* strength, int count, const dfm_mac* adj_u, float* adj_center,
+                               float* adj_strength, float* penalty_center, float* penalty_strength, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,13 @@ mask_velocity /        zero mask is a no-op; solid-cell faces take the boundary
 mask_scalar            velocity; a constant scalar stays constant; a solid cell
                        with no fluid neighbour gets 0; adjoint form zeroes
 adj sources            closed forms on polynomial fields
+gaussian_force         a force far from the walls integrates to
+                       s (pi/r)(1 - e^-5) / dx^d; one force on a node gives s
+                       there; forces add; the FP32 cut-off admits exactly the
+                       nodes with fp32 (m*m) r < 5 (brute force per node)
+gaussian_force_adjoint adj_strength / adj_center = central differences of
+                       <f, lam> in s / c; penalty_* = central differences of
+                       sum over forces of sum f_I^2 (f_I: force I alone)
 OracleLFM (whole step) composition of the above; the trajectory itself is
                        **parity unpinned** against the reference (Taichi cannot
                        run here) -- only its pieces are pinned.
@@ -303,6 +310,74 @@ def accumulate_ab3(f, init, fmap, H1, H2, ind, dx, dt):
         H1[a] = g[a].copy()
 
 
+# ------------------------------------------------------ control forces --
+def _force_weights(n, a, dx, c, r, fp32_test):
+    """(mask, e, p - c) of one force at the nodes of face grid a.  The test
+    (m*m) r < 5 with m = |p - c| is evaluated in FP32, as the reference's FP32
+    fields evaluate it, when fp32_test (else in FP64); e = exp(-|p - c|^2 r)
+    in FP64."""
+    if fp32_test:
+        # FP32 node positions as FP32 fields hold them: float(idx + off) * float(dx)
+        shp = face_shape(n, a)
+        axes = [(np.arange(m) + (0.0 if b == a else 0.5)).astype(np.float32) * np.float32(dx)
+                for b, m in enumerate(shp)]
+        p = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1)
+    else:
+        p = face_coords(n, a, dx)
+    if fp32_test:
+        d32 = p - np.asarray(c, np.float32)
+        n2 = np.zeros(d32.shape[:-1], np.float32)
+        for b in range(len(n)):
+            n2 = n2 + d32[..., b] * d32[..., b]
+        m = np.sqrt(n2)
+        inside = (m * m) * np.float32(r) < np.float32(5.0)
+    d = p.astype(np.float64) - np.asarray(c, np.float64)
+    q = np.sum(d * d, axis=-1) * float(r)
+    if not fp32_test:
+        inside = q < 5.0
+    return inside, np.exp(-q), d
+
+
+def gaussian_force(n, dx, center, inv_radius, strength, count, fp32_test=True):
+    """apply_gausion_force_kernel (2D/vortex/lfm_midpoint_simulator.py:309-326;
+    3D/lfm_midpoint_simulator.py:316-344): on every face of axis a,
+    f_a = sum over the first `count` forces I with (m*m) r_I < 5 of
+    exp(-m^2 r_I) s_I[a], m = |p - c_I|; zero elsewhere."""
+    dim = len(n)
+    f = [np.zeros(face_shape(n, a)) for a in range(dim)]
+    for I in range(int(count)):
+        for a in range(dim):
+            inside, e, _ = _force_weights(n, a, dx, center[I], inv_radius[I], fp32_test)
+            f[a] += np.where(inside, e * float(strength[I][a]), 0.0)
+    return f
+
+
+def gaussian_force_adjoint(n, dx, center, inv_radius, strength, count, lam, fp32_test=True):
+    """calculate_adj_gausion_force_kernel (2D/shape/lfm_midpoint_diff_simulator.py:75-103;
+    3D/lfm_midpoint_diff_simulator.py:70-110): per force I, over the faces
+    inside its cut-off,
+      adj_strength[a] = sum e lam_a,   adj_center = sum s[a] e 2 (p - c) r lam_a,
+      pen_strength[a] = sum 2 (e s[a]) e,   pen_center = sum 2 (e s[a]) s[a] e 2 (p - c) r.
+    Returns (adj_center, adj_strength, pen_center, pen_strength), (count, dim) each."""
+    dim = len(n)
+    cnt = int(count)
+    ac, ast, pc, ps = (np.zeros((cnt, dim)) for _ in range(4))
+    for I in range(cnt):
+        r = float(inv_radius[I])
+        for a in range(dim):
+            inside, e, d = _force_weights(n, a, dx, center[I], r, fp32_test)
+            e = np.where(inside, e, 0.0)
+            sa = float(strength[I][a])
+            la = np.asarray(lam[a], np.float64)
+            ast[I, a] += np.sum(e * la)
+            ps[I, a] += np.sum(2.0 * (e * sa) * e)
+            for b in range(dim):
+                g = e * 2.0 * d[..., b] * r
+                ac[I, b] += np.sum(sa * g * la)
+                pc[I, b] += np.sum(2.0 * (e * sa) * sa * g)
+    return ac, ast, pc, ps
+
+
 def adj_conv_source(u, w, dx, kern):
     """calculate_nabla_u_w: r_a = ((grad u)^T w)_a at face-a nodes, with the
     quartic (2D, kern=4) or cubic (3D, kern=3) kernel
@@ -570,7 +645,7 @@ class OracleLFM:
     LFM_Diff_Simulator.init_gradient / backtrack_flow_map
     (2D/vortex/lfm_midpoint_diff_simulator.py:126-503;
     3D/lfm_midpoint_diff_simulator.py:336-594) for the configuration with
-    viscosity 0, no control force, no obstacle mask, use_short_BFECC=False,
+    viscosity 0, use_short_BFECC=False,
     BFECC_clamp=False, backward_u=True and a final-time density loss (the
     per-interval intermediate-target terms, q7, are off).
 
@@ -580,7 +655,8 @@ class OracleLFM:
     """
 
     def __init__(self, n, dx, dt, reinit_every, order, adj_kernel=None, project="exact", dtype=np.float64,
-                 mg_levels=None, mg_tol=1e-6):
+                 mg_levels=None, mg_tol=1e-6, control_forces=False):
+        self.control_forces = control_forces  # add_control_force: also switches the j == 0 passive rebuild
         self.n = tuple(n)
         self.dim = len(n)
         self.dx, self.dt, self.R, self.order = dx, dt, reinit_every, order
@@ -650,20 +726,35 @@ class OracleLFM:
         self.s_start = {}
         self.maps = {}
 
-    def step(self):
-        """step_midpoint + step_flow_map (2D/vortex/lfm_midpoint_simulator.py:472-661)."""
+    def _force(self, force):
+        return gaussian_force(self.n, self.dx, force["c"], force["r"], force["s"], force["n"])
+
+    def step(self, force=None):
+        """step_midpoint + step_flow_map (2D/vortex/lfm_midpoint_simulator.py:472-661).
+        force: control_force_para {"c", "r", "s", "n"} of this step (the
+        half-step force before the first projection, :527-537; the full-step
+        force and its accumulate_init after advection, :597-602)."""
         self.step_num += 1
         j = self.step_num % self.R
         dx, dt, order = self.dx, self.dt, self.order
         self.s_start[self.step_num] = [s.copy() for s in self.s]
         ad_u = [c.copy() for c in self.u]
         self.u = sl_advect_u(ad_u, ad_u, dx, 0.5 * dt, order)
+        f = None
+        if force is not None:
+            f = self._force(force)
+            for a in range(self.dim):
+                self.u[a] = self.u[a] + 0.5 * dt * f[a]
         self.project(self.u)
         self._mask_fwd()
         self.mid[self.step_num] = [c.copy() for c in self.u]
         self.phi_map.march(self.u, -dt, order)
         self.s = [sl_advect_s(self.u, s, dx, dt, order) for s in self.s]
         self.u = sl_advect_u(self.u, ad_u, dx, dt, order)
+        if f is not None:
+            for a in range(self.dim):
+                self.u[a] = self.u[a] + dt * f[a]
+            accumulate(f, self.init_u, self.phi_map, dx, dt)
         self.project(self.u)
         self._mask_fwd()
         if j == self.R - 1:
@@ -699,13 +790,21 @@ class OracleLFM:
         self.H1 = [np.zeros_like(c) for c in self.u]
         self.H2 = [np.zeros_like(c) for c in self.u]
 
-    def backtrack_step(self):
+    def backtrack_step(self, force=None):
         """backtrack_step_midpoint + backtrack_flow_map
-        (2D/vortex/lfm_midpoint_diff_simulator.py:220-503)."""
+        (2D/vortex/lfm_midpoint_diff_simulator.py:220-503).  force: the
+        step's control_force_para; its parameter gradients (from lam_u as the
+        step begins) land in self.force_grad, and u takes the force back
+        (2D/shape/lfm_midpoint_diff_simulator.py:209-213,296-300)."""
         self.step_num -= 1
         j = self.step_num % self.R
         dx, dt, order = self.dx, self.dt, self.order
         mid = self.mid[self.step_num]
+        f = None
+        if force is not None:
+            f = self._force(force)
+            self.force_grad = gaussian_force_adjoint(self.n, dx, force["c"], force["r"], force["s"], force["n"],
+                                                     self.lam_u)
         r_u = adj_conv_source(self.u, self.lam_u, dx, self.adj_kernel)
         r_s = [adj_scalar_source(s, l, dx) for s, l in zip(self.s, self.lam_s)]
         self.lam_u = sl_advect_u(mid, self.lam_u, dx, -dt, order)
@@ -724,6 +823,10 @@ class OracleLFM:
         for rs in r_s:
             accumulate(rs, self.init_lam_u, self.psi_map, dx, -dt)
         self.project(self.lam_u)
+        if f is not None:
+            for a in range(self.dim):
+                self.u[a] = self.u[a] - dt * f[a]
+            accumulate(f, self.init_u, self.psi_map, dx, -dt)
         self.project(self.u)
         self._mask_adj()
         if j == 0:
@@ -735,7 +838,11 @@ class OracleLFM:
             self.project(self.u)
             self.init_u = [c.copy() for c in self.u]
             self._mask_adj()
-            self.s = [s.copy() for s in self.s_start[self.step_num]]
+            if self.control_forces:
+                # rebuilt through the interval's maps (2D/shape/lfm_midpoint_diff_simulator.py:339-343)
+                self.s = [bfecc_scalar(si, fwd.psi, self.psi_map.psi, dx) for si in self.init_s]
+            else:
+                self.s = [s.copy() for s in self.s_start[self.step_num]]
             self.init_s = [s.copy() for s in self.s]
             self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx) for l in self.init_lam_s]
             self.init_lam_s = [l.copy() for l in self.lam_s]

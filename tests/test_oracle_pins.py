@@ -452,3 +452,93 @@ def test_mask_field_kernels_closed_forms():
                     if 2 <= i <= 4 + 1 and abs(i - 3) <= 1 and abs(j - 4) <= 1 and mask[i, j] <= 0])
     O.mask_scalar(s4, mask)
     assert abs(s4[3, 4] - want) < 1e-12
+
+
+# ---------------------------------------------------------- control forces --
+def _forces(rng, count, dim, dx, radii=(2, 3, 4, 6)):
+    c = rng.uniform(0.3, 0.7, (count, dim))
+    r = np.array([1.0 / (rng.choice(radii) * dx) ** 2 for _ in range(count)])
+    s = rng.uniform(-1, 1, (count, dim))
+    return c, r, s
+
+
+def test_gaussian_force_integral_node_value_and_additivity():
+    n, dx = (96, 80), 1.0 / 80
+    r = 1.0 / (4 * dx) ** 2
+    c = np.array([[0.6, 0.5]])
+    s = np.array([[0.7, -1.3]])
+    f = O.gaussian_force(n, dx, c, [r], s, 1)
+    # truncated Gaussian: integral over m^2 r < 5 is (pi / r)(1 - e^-5)
+    want = (np.pi / r) * (1 - np.exp(-5.0))
+    for a in range(2):
+        assert abs(np.sum(f[a]) * dx * dx / s[0, a] - want) < 2e-3 * want
+    # a centre on an x-face node gives the strength there (e = 1)
+    node = np.array([[30 * dx, (20 + 0.5) * dx]])
+    fx = O.gaussian_force(n, dx, node, [r], s, 1)[0]
+    assert abs(fx[30, 20] - s[0, 0]) < 1e-6 * abs(s[0, 0])
+    # forces add; count limits the forces used
+    rng = np.random.default_rng(4)
+    C, R, S = _forces(rng, 5, 2, dx)
+    tot = O.gaussian_force(n, dx, C, R, S, 5)
+    parts = [O.gaussian_force(n, dx, C[i:i + 1], R[i:i + 1], S[i:i + 1], 1) for i in range(5)]
+    for a in range(2):
+        np.testing.assert_allclose(tot[a], sum(p[a] for p in parts), atol=1e-12)
+    first3 = O.gaussian_force(n, dx, C, R, S, 3)
+    for a in range(2):
+        np.testing.assert_allclose(first3[a], sum(p[a] for p in parts[:3]), atol=1e-12)
+
+
+def test_gaussian_force_cutoff_is_the_fp32_test():
+    """Brute force over nodes: a node is inside exactly when the FP32
+    expression (m*m) r < 5 holds (which the FP64 test can disagree with)."""
+    n, dx = (40, 40), 1.0 / 40
+    r = np.float32(1.0 / (3 * dx) ** 2)
+    c = np.array([[0.51, 0.47]], np.float32)
+    f = O.gaussian_force(n, dx, c, [r], np.ones((1, 2)), 1)
+    for a in range(2):
+        shp = O.face_shape(n, a)
+        for i in range(shp[0]):
+            for j in range(shp[1]):
+                p = np.array([(i + (0 if a == 0 else 0.5)), (j + (0 if a == 1 else 0.5))], np.float32)
+                p = p * np.float32(dx)
+                d = p - c[0]
+                m = np.sqrt(np.float32(d[0] * d[0]) + np.float32(d[1] * d[1]))
+                assert (f[a][i, j] != 0) == bool(np.float32(m * m) * r < np.float32(5)), (a, i, j)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gaussian_force_adjoint_is_the_derivative(dim):
+    """adj_strength / adj_center are d<f, lam>/d(s, c) and the penalty terms
+    d/d(s, c) of sum over forces of sum f_I^2, checked by central
+    differences (FP64 cut-off test; no node lies near the cut-off)."""
+    n = (24, 20) if dim == 2 else (12, 10, 14)
+    dx = 1.0 / n[1]
+    rng = np.random.default_rng(11 + dim)
+    C, R, S = _forces(rng, 3, dim, dx, radii=(2, 3))
+    lam = [rng.standard_normal(O.face_shape(n, a)) for a in range(dim)]
+    for I in range(3):  # keep nodes away from the discontinuity at the cut-off
+        for a in range(dim):
+            _, e, d = O._force_weights(n, a, dx, C[I], R[I], False)
+            assert np.min(np.abs(np.sum(d * d, -1) * R[I] - 5.0)) > 5e-5  # h moves q by < 2e-5
+    ac, ast, pc, ps = O.gaussian_force_adjoint(n, dx, C, R, S, 3, lam, fp32_test=False)
+
+    def J(C_, S_):
+        f = O.gaussian_force(n, dx, C_, R, S_, 3, fp32_test=False)
+        return sum(np.sum(f[a] * lam[a]) for a in range(dim))
+
+    def E(C_, S_):
+        return sum(sum(np.sum(g ** 2) for g in O.gaussian_force(n, dx, C_[i:i + 1], R[i:i + 1], S_[i:i + 1], 1,
+                                                                 fp32_test=False)) for i in range(3))
+
+    h = 1e-6
+    for I in range(3):
+        for b in range(dim):
+            dS = np.zeros_like(S)
+            dS[I, b] = h
+            dC = np.zeros_like(C)
+            dC[I, b] = h
+            for fn, want_s, want_c in ((J, ast, ac), (E, ps, pc)):
+                gs = (fn(C, S + dS) - fn(C, S - dS)) / (2 * h)
+                gc = (fn(C + dC, S) - fn(C - dC, S)) / (2 * h)
+                assert abs(gs - want_s[I, b]) <= 1e-6 * max(1.0, abs(gs)), (fn.__name__, I, b)
+                assert abs(gc - want_c[I, b]) <= 1e-5 * max(1.0, abs(gc)), (fn.__name__, I, b)
