@@ -67,6 +67,12 @@ SIGNATURES = {
     "dfm_mask_scalar": (c_int, [P(Grid), c_void_p, c_void_p, c_int, c_void_p]),
     "dfm_vortex_velocity_2d": (c_int, [P(Grid), c_void_p, c_int, P(Mac), c_void_p]),
     "dfm_vortex_velocity_2d_vjp": (c_int, [P(Grid), c_void_p, c_int, P(Mac), c_void_p, c_void_p]),
+    "dfm_central_vector": (c_int, [P(Grid), P(Mac), c_void_p, c_void_p]),
+    "dfm_split_central_vector": (c_int, [P(Grid), c_void_p, P(Mac), c_void_p]),
+    "dfm_max_speed": (c_int, [P(Grid), P(Mac), c_void_p, c_void_p]),
+    "dfm_tile8_convert": (c_int, [P(Grid), c_void_p, c_void_p, c_int, c_void_p]),
+    "dfm_poisson_operator_check": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P(c_int),
+                                           c_void_p]),
     "dfm_gaussian_force": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_int, P(Mac), c_void_p]),
     "dfm_gaussian_force_adjoint": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_int, P(Mac), c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p]),

@@ -148,6 +148,18 @@ class Ops:
         L.check(self.lib.dfm_mask_scalar(self.g, mask.data_ptr(), s.data_ptr(), 1 if adjoint else 0, self._st()),
                 "mask_scalar")
 
+    def central_vector(self, u: MacField, central):
+        """get_central_vector: central (dim, *n) FP32 = face averages of u."""
+        L.check(self.lib.dfm_central_vector(self.g, u.s, central.data_ptr(), self._st()), "central_vector")
+
+    def split_central_vector(self, central, u: MacField):
+        """split_central_vector: MAC u from the cell-centred (dim, *n) field."""
+        L.check(self.lib.dfm_split_central_vector(self.g, central.data_ptr(), u.s, self._st()), "split_central_vector")
+
+    def max_speed(self, u: MacField, out):
+        """calc_max_speed into the device scalar `out` (FP32)."""
+        L.check(self.lib.dfm_max_speed(self.g, u.s, out.data_ptr(), self._st()), "max_speed")
+
     def gaussian_force(self, center, inv_radius, strength, count, f: MacField):
         """apply_gausion_force: f = the field of the first `count` Gaussian
         forces (center (N, dim), inv_radius (N,), strength (N, dim); device FP32)."""

@@ -162,6 +162,7 @@ class LFM_Simulator:
         self.psi_map = FlowMap(n, dev)
         self.step_num = _Scalar(-1)
         self.frame_idx = _Scalar(0)
+        self.max_speed = _Scalar(0.0)
         self.mid_u = {}        # step -> MacField (midpoint velocity)
         self.passive_start = {}  # interval-start step -> [passive]
         self.flow_maps = {}    # interval index -> FlowMap (phi, F)
@@ -267,7 +268,10 @@ class LFM_Simulator:
             with torch.no_grad():
                 return list(vortex_velocity_2d(th, self.n, self.dx))
         uc = scenes.vortex_tubes_3d(np.asarray(theta, dtype=np.float64).reshape(-1, 8), self.n, self.dx)
-        return scenes.split_central_vector(uc)
+        c = torch.as_tensor(np.ascontiguousarray(np.moveaxis(uc, -1, 0), dtype=np.float32)).to(self.device)
+        u = MacField.empty(self.n, self.device)
+        self.ops.split_central_vector(c, u)
+        return u
 
     def init(self, theta=None, forward=True, target=False, init_passive_np=None, u0=None):
         """LFM_Simulator.init (2D/vortex/lfm_midpoint_simulator.py:370-439): build
@@ -398,13 +402,31 @@ class LFM_Simulator:
         self.init_u.copy_(self.u)
 
     # diagnostics ---------------------------------------------------------------
+    def calc_max_speed(self, *u):
+        """calc_max_speed (2D/vortex/lfm_midpoint_simulator.py:328-335): max
+        cell-centre speed (at least 1e-3) of the given MAC components, or of u,
+        into ``max_speed[None]``; reads one scalar back.  step_midpoint does not
+        call it (the reference only prints the CFL number from it)."""
+        m = MacField([f.tensor if isinstance(f, FieldView) else f for f in u]) if u else self.u
+        out = torch.empty(1, dtype=torch.float32, device=self.device)
+        self.ops.max_speed(m, out)
+        self.max_speed[None] = float(out.item())
+        return self.max_speed[None]
+
+    def get_central_vector(self, u=None):
+        """get_central_vector: (dim, *n) cell-centred average of u (default the
+        current velocity), on the device."""
+        u = self.u if u is None else u
+        c = torch.empty((self.dim, *self.n), dtype=torch.float32, device=self.device)
+        self.ops.central_vector(u, c)
+        return c
+
     def calculate_w(self):
         """Vorticity at cell centres (get_central_vector + curl,
-        2D/vortex/taichi_utils.py:152-156,198-205); host-side diagnostic."""
-        u = [t.detach().double() for t in self.u.c]
+        2D/vortex/taichi_utils.py:152-156,198-205); the curl is a host-side diagnostic."""
         if self.dim == 2:
-            uc = torch.stack([0.5 * (u[0][1:] + u[0][:-1]), 0.5 * (u[1][:, 1:] + u[1][:, :-1])], -1)
-            pad = torch.nn.functional.pad(uc.permute(2, 0, 1)[None], (1, 1, 1, 1), mode="replicate")[0]
+            uc = self.get_central_vector().double()
+            pad = torch.nn.functional.pad(uc[None], (1, 1, 1, 1), mode="replicate")[0]
             dvy_dx = (pad[1, 2:, 1:-1] - pad[1, :-2, 1:-1]) / (2 * self.dx)
             dvx_dy = (pad[0, 1:-1, 2:] - pad[0, 1:-1, :-2]) / (2 * self.dx)
             return (dvy_dx - dvx_dy).cpu().numpy()

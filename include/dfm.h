@@ -239,6 +239,38 @@ int dfm_vortex_velocity_2d(const dfm_grid* g, const float* theta, int K, const d
 int dfm_vortex_velocity_2d_vjp(const dfm_grid* g, const float* theta, int K, const dfm_mac* lam, float* dtheta,
                                void* stream);
 
+/* ------------------------------------------------ cell-centre utilities */
+/* central (device FP32, SoA: component a of cell c at [a*cells + c]) =
+ * 0.5 (u_a[c + e_a] + u_a[c]).  Replaces get_central_vector
+ * (2D/vortex/taichi_utils.py:152-156, 3D/taichi_utils.py:163-168). */
+int dfm_central_vector(const dfm_grid* g, const dfm_mac* u, float* central, void* stream);
+
+/* u_a(face) = 0.5 (central_a(high cell) + central_a(low cell)), cells clamped
+ * to the grid.  Replaces split_central_vector (2D/vortex/taichi_utils.py:158-167,
+ * 3D/taichi_utils.py:170-183). */
+int dfm_split_central_vector(const dfm_grid* g, const float* central, const dfm_mac* u, void* stream);
+
+/* *max_speed (device FP32) = max(1e-3, max over cells of |central velocity|);
+ * NaN propagates.  Replaces calc_max_speed (2D/vortex/lfm_midpoint_simulator.py:328-335,
+ * 3D/lfm_midpoint_simulator.py:347-355). */
+int dfm_max_speed(const dfm_grid* g, const dfm_mac* u, float* max_speed, void* stream);
+
+/* ----------------------------------------- PoissonSolver-compatible entry */
+/* Convert a 3D cell field between the dense (n0, n1, n2) layout and the
+ * 8^3-tiled flat layout MGPCG_3 passes to its external PoissonSolver
+ * (rearrange_tensor_for_cuda / _from_cuda, 3D/fast_mgpcg.py:121-158):
+ * to_tiles != 0: dense -> tiled, else tiled -> dense.  Every n must be a
+ * multiple of 8; src and dst must not overlap. */
+int dfm_tile8_convert(const dfm_grid* g, const float* src, float* dst, int to_tiles, void* stream);
+
+/* Count (into *mismatches, host int) the cells whose coefficients differ from
+ * the all-Neumann 7-point operator MGPCG_3.init builds for the external
+ * solver (construct_adiag / construct_ax, is_dof = 1; 3D/fast_mgpcg.py:48-74,
+ * 162-190); arrays in the tiled layout when tiled != 0.  Synchronises the
+ * stream. */
+int dfm_poisson_operator_check(const dfm_grid* g, const float* a_diag, const float* a_x, const float* a_y,
+                               const float* a_z, const uint8_t* is_dof, int tiled, int* mismatches, void* stream);
+
 /* ------------------------------------------------ Gaussian control forces */
 /* Force field of `count` Gaussian forces on every face (fills f; faces no
  * force reaches get 0).  Force I: centre[I*dim + b], inverse squared radius

@@ -66,3 +66,39 @@ def test_argument_validation_codes(lib):
     odd = L.make_grid((24, 20), 1 / 24)
     assert lib.dfm_mgpcg_create(odd, 4, 2, 50, ctypes.byref(h)) == -1  # 20 not divisible by 8
     assert lib.dfm_mgpcg_project(None, u, 1e-6, 1, 10, None) == -2
+
+
+def test_force_and_tiled_argument_validation(lib):
+    from dfm import _lib as L
+
+    g2 = L.make_grid((16, 16), 1 / 16)
+    u = L.Mac()
+    assert lib.dfm_gaussian_force(g2, None, None, None, -1, u, None) == -1  # negative count
+    assert lib.dfm_gaussian_force(g2, None, None, None, 3, u, None) == -2  # NULL parameters
+    assert lib.dfm_gaussian_force(g2, None, None, None, 0, u, None) == -2  # NULL force field
+    g3 = L.make_grid((16, 16, 20), 1 / 16)
+    assert lib.dfm_tile8_convert(g3, None, None, 1, None) == -1  # 20 is not a multiple of 8
+    assert lib.dfm_tile8_convert(g2, None, None, 1, None) == -1  # 2D
+    g8 = L.make_grid((16, 16, 24), 1 / 16)
+    assert lib.dfm_tile8_convert(g8, None, None, 1, None) == -2
+    n = ctypes.c_int(0)
+    assert lib.dfm_poisson_operator_check(g8, None, None, None, None, None, 1, ctypes.byref(n), None) == -2
+
+
+def test_poisson_solver_shim_host_checks():
+    import torch
+
+    from poisson_solver_permenant_stream import PoissonSolver
+
+    ps = PoissonSolver([2, 2, 3])
+    assert ps.n == (16, 16, 24)
+    t = torch.zeros(16 * 16 * 24)
+    kw = dict(a_x=t, a_y=t, a_z=t, b=t, a_diag=t, is_dof=t.to(torch.uint8), x_initial=t)
+    with pytest.raises(NotImplementedError):
+        ps.setup(**kw, neg_bc_type_vec=[1, 2, 1])
+    with pytest.raises(TypeError):  # host tensors
+        ps.setup(**kw)
+    with pytest.raises(RuntimeError):
+        ps.build()
+    with pytest.raises(ValueError):
+        PoissonSolver([2, 2])

@@ -381,3 +381,48 @@ def test_mgpcg_solve_and_status(device):
     xe = O.neumann_solve_dct(round32(b))
     xg = x.cpu().numpy().astype(np.float64)
     assert rel_l2(xg - xg.mean(), xe) < 1e-5
+
+
+@pytest.mark.parametrize("n", [(48, 40), (20, 16, 24)])
+def test_central_vector_split_and_max_speed(n, device):
+    """get_central_vector / split_central_vector / calc_max_speed against
+    numpy restatements: the central average, the clamped split (a constant
+    central field splits to the same constant; split of a field's central
+    average differs from it only by the two-cell smoothing), and the maximum
+    speed with its 1e-3 floor."""
+    from dfm.ops import MacField, Ops
+
+    dim = len(n)
+    rng = np.random.default_rng(40 + dim)
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(dim)])
+    ops = Ops(n, 1.0 / n[1])
+    du = mac_to_dev(u, device)
+    c = torch.empty((dim, *n), device=device)
+    ops.central_vector(du, c)
+    cen = []
+    for a in range(dim):
+        hi = [slice(None)] * dim
+        lo = [slice(None)] * dim
+        hi[a], lo[a] = slice(1, None), slice(0, -1)
+        cen.append(0.5 * (u[a][tuple(hi)] + u[a][tuple(lo)]))
+    assert np.array_equal(c.cpu().numpy(), np.stack(cen).astype(np.float32))
+    # split: faces average the two adjacent cells, clamped at the walls
+    out = MacField.empty(n, device)
+    ops.split_central_vector(c, out)
+    for a in range(dim):
+        pad = np.concatenate([np.take(cen[a], [0], axis=a), cen[a], np.take(cen[a], [-1], axis=a)], axis=a)
+        hi = [slice(None)] * dim
+        lo = [slice(None)] * dim
+        hi[a], lo[a] = slice(1, None), slice(0, -1)
+        want = 0.5 * (pad[tuple(hi)] + pad[tuple(lo)])
+        assert np.abs(out.c[a].cpu().numpy() - want).max() <= 1e-6 * np.abs(want).max()
+    const = torch.full((dim, *n), 0.75, device=device)
+    ops.split_central_vector(const, out)
+    assert all(bool(torch.all(t == 0.75)) for t in out.c)
+    # maximum speed
+    m = torch.empty(1, device=device)
+    ops.max_speed(du, m)
+    want = max(1e-3, float(np.sqrt(sum(x.astype(np.float64) ** 2 for x in cen)).max()))
+    assert abs(float(m.item()) - want) <= 1e-6 * want
+    ops.max_speed(MacField([torch.zeros_like(t) for t in du.c]), m)
+    assert float(m.item()) == np.float32(1e-3)
