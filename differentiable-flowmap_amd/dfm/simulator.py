@@ -165,6 +165,7 @@ class LFM_Simulator:
         self.max_speed = _Scalar(0.0)
         self.mid_u = {}        # step -> MacField (midpoint velocity)
         self.passive_start = {}  # interval-start step -> [passive]
+        self.passive_steps = {}  # step -> [passive], kept when step_midpoint(write_passive=True)
         self.flow_maps = {}    # interval index -> FlowMap (phi, F)
         self.u_start = {}      # interval index -> MacField (checkpoint mode)
         self.boundary_mask_t = None  # obstacle mask on cells (None: no obstacles)
@@ -302,6 +303,7 @@ class LFM_Simulator:
         self.passive_start.clear()
         self.flow_maps.clear()
         self.u_start.clear()
+        self.passive_steps.clear()
         self._force_store.clear()
 
     def final_condition(self):
@@ -315,6 +317,9 @@ class LFM_Simulator:
         j = step % self.reinit_every
         if j == 0 and self.record and not self._replaying:
             self.passive_start[step] = [p.clone() for p in self.passive]
+        if write_passive and self.record and not self._replaying:
+            # the reference writes the passive of every step to disk (:487-491)
+            self.passive_steps[step] = self.passive_start.get(step) or [p.clone() for p in self.passive]
             if self.checkpoint:
                 self.u_start[step // self.reinit_every] = self.u.clone()
         forces = None
@@ -399,6 +404,102 @@ class LFM_Simulator:
             for st in range(step - j, step + 1):
                 self.mid_u.pop(st, None)
         ops.reset_identity(self.phi_map)
+        self.init_u.copy_(self.u)
+
+    # the reference's file layout ------------------------------------------------
+    def export_cache(self, log_dir=None):
+        """Write the stored run in the reference's on-disk layout under
+        ``log_dir`` (default: the constructor's log_dir): midpoint velocities
+        (midpoint/u_*), each interval's forward map (flowmap_phi, flowmap_phi_xy
+        or _xyz, flowmap_F), the passives at the stored steps (passive/s1_*
+        or s_*) and the final state (final/u_x_k ..., final_ind_0, s1_k), as
+        RW_Cache / write_disk write them (2D/vortex/lfm_midpoint_simulator.py:46-184,
+        472-515,617-643).  Returns the directory."""
+        import os
+
+        from . import rw_cache as W
+
+        d = log_dir or self.log_dir
+        if d is None:
+            raise ValueError("export_cache needs a directory (log_dir)")
+        lay = W.layout(self.dim, self.n_passive)
+        for key in lay:
+            os.makedirs(os.path.join(d, lay[key][0]), exist_ok=True)
+        path = {k: os.path.join(d, v[0]) for k, v in lay.items()}
+        n, cache = self.n, W.RWCache()
+        aos = lambda t: np.moveaxis(t.detach().cpu().numpy(), 0, -1)  # noqa: E731
+        for st in sorted(self.mid_u):
+            cache.write(W.pack_mac(self.mid_u[st].numpy(), n), "u", path["midpoint"], st)
+        cache.force_write("u", path["midpoint"])
+        xy = lay["phi_xy"][1]
+        for m in sorted(self.flow_maps):
+            fm = self.flow_maps[m]
+            cache.write(aos(fm.psi), "phi", path["phi"], m)
+            cache.write(W.pack_face_vectors([aos(t) for t in fm.psi_f], n), xy, path["phi_xy"], m)
+            cache.write(W.pack_face_vectors([aos(t) for t in fm.T_f], n), "F", path["F"], m)
+        for name, key in (("phi", "phi"), (xy, "phi_xy"), ("F", "F")):
+            cache.force_write(name, path[key])
+        # the series is read back by position (entry j in slot j - first), so
+        # it holds every step: the per-step copies kept under write_passive,
+        # else the interval-start copies, else zeros
+        last = max(list(self.passive_steps) + list(self.passive_start), default=-1)
+        for st in range(last + 1):
+            src = self.passive_steps.get(st, self.passive_start.get(st))
+            for k, nm in enumerate(lay["passive"][1]):
+                arr = src[k].cpu().numpy() if src is not None else np.zeros(n, np.float32)
+                cache.write(W.as_channels(arr, self.dim), nm, path["passive"], st)
+        for nm in lay["passive"][1]:
+            cache.force_write(nm, path["passive"])
+        k = self.step_num[None] + 1
+        for a, c in enumerate(self.u.numpy()):
+            W.write_disk(c, "u_" + "xyz"[a], path["final"], k)
+        W.write_disk(int(k), "final_ind", path["final"], 0)
+        for p, nm in zip(self.passive, lay["passive"][1]):
+            W.write_disk(p.cpu().numpy(), nm, path["final"], k)
+        return d
+
+    def import_cache(self, log_dir=None):
+        """Load a run stored in the reference's on-disk layout (``export_cache``
+        or the reference's own forward pass with write_flow_map and
+        write_passive) into device storage and the final state, so that
+        init_gradient / backtrack_step_midpoint can run the adjoint sweep from
+        it (init_gradient's read_disk of final/, 2D/vortex/lfm_midpoint_diff_simulator.py:126-207,
+        and the per-step reads of backtrack_flow_map)."""
+        import os
+
+        from . import rw_cache as W
+
+        if self.checkpoint:
+            raise ValueError("import_cache fills stored-everything storage; construct with checkpoint=False")
+        d = log_dir or self.log_dir
+        lay = W.layout(self.dim, self.n_passive)
+        path = {k: os.path.join(d, v[0]) for k, v in lay.items()}
+        n, dev, cache, R = self.n, self.device, W.RWCache(), self.reinit_every
+        final_ind = int(W.read_disk("final_ind", path["final"], 0)[0])
+        for a in range(self.dim):
+            self.u.c[a].copy_(torch.as_tensor(W.read_disk("u_" + "xyz"[a], path["final"], final_ind)).to(dev))
+        for p, nm in zip(self.passive, lay["passive"][1]):
+            p.copy_(torch.as_tensor(np.asarray(W.read_disk(nm, path["final"], final_ind)).reshape(n)).to(dev))
+        self.mid_u.clear()
+        self.flow_maps.clear()
+        self.passive_start.clear()
+        soa = lambda arr: torch.as_tensor(np.ascontiguousarray(np.moveaxis(arr, -1, 0))).to(dev)  # noqa: E731
+        for st in range(final_ind):
+            comps = W.unpack_mac(cache.read("u", path["midpoint"], st), n)
+            self.mid_u[st] = MacField([torch.as_tensor(c).to(dev) for c in comps])
+        xy = lay["phi_xy"][1]
+        for m in range(final_ind // R):
+            fm = FlowMap(n, dev)
+            fm.psi.copy_(soa(np.asarray(cache.read("phi", path["phi"], m))[tuple(slice(0, s) for s in n)]))
+            for dst, name, key in ((fm.psi_f, xy, "phi_xy"), (fm.T_f, "F", "F")):
+                for a, v in enumerate(W.unpack_face_vectors(cache.read(name, path[key], m), n)):
+                    dst[a].copy_(soa(v))
+            self.flow_maps[m] = fm
+        for st in range(0, final_ind, R):
+            self.passive_start[st] = [
+                torch.as_tensor(np.ascontiguousarray(np.asarray(cache.read(nm, path["passive"], st))[..., 0])).to(dev)
+                for nm in lay["passive"][1]]
+        self.step_num[None] = final_ind - 1
         self.init_u.copy_(self.u)
 
     # diagnostics ---------------------------------------------------------------

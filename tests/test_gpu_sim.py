@@ -134,3 +134,46 @@ def test_overlapped_adjoint_projections_are_bitwise_sequential(dim, device):
     for a, b in zip(out[0][0] + out[0][1], out[1][0] + out[1][1]):
         assert np.array_equal(a, b)
     assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("dim,n,R,steps", [(2, (32, 32), 4, 8), (3, (16, 16, 16), 3, 6)])
+def test_reference_file_layout_round_trip(dim, n, R, steps, tmp_path, device):
+    """export_cache writes the run in the reference's RW_Cache layout, and an
+    adjoint sweep resumed from those files (import_cache in a fresh simulator)
+    is bitwise the sweep of the simulator that ran the forward pass."""
+    from dfm import LFM_Diff_Simulator, scenes
+
+    sc = scenes.scene_2d(n, seed=3, vortices=4) if dim == 2 else scenes.scene_3d(n, seed=3, tubes=6)
+    dx = sc["dx"]
+    kw = dict(total_steps=steps, tol=1e-7, max_iters=200)
+    a = LFM_Diff_Simulator(*n, dx, 0.5 * dx, R, str(tmp_path), **kw)
+    a.init(u0=round32(sc["u0"]), init_passive_np=round32(sc["density"]))
+    for _ in range(steps):
+        a.step_midpoint()
+    a.export_cache()
+    xy = "phi_xy" if dim == 2 else "phi_xyz"
+    s = "s1" if dim == 2 else "s"
+    big = tuple(m + 1 for m in n)
+    files = {
+        "midpoint/u_0_%d.npy" % (steps - 1): (100,) + big + (dim,),
+        "flowmap_%s/%s_0_%d.npy" % (xy, xy, steps // R - 1): (100,) + big + (dim * dim,),
+        "flowmap_F/F_0_%d.npy" % (steps // R - 1): (100,) + big + (dim * dim,),
+        "flowmap_phi/phi_0_%d.npy" % (steps // R - 1): (100,) + n + (dim,),
+        "passive/%s_0_%d.npy" % (s, steps - R): (100,) + n + (1,),
+        "final/u_x_%d.npy" % steps: (n[0] + 1,) + n[1:],
+        "final/%s_%d.npy" % (s, steps): n,
+    }
+    for f, shape in files.items():
+        assert np.load(tmp_path / f).shape == shape, f
+    assert np.load(tmp_path / "final/final_ind_0.npy").tolist() == [steps]
+    b = LFM_Diff_Simulator(*n, dx, 0.5 * dx, R, str(tmp_path), **kw)
+    b.import_cache()
+    assert b.step_num[None] == steps - 1
+    tgt = round32(sc["target"])
+    for sim in (a, b):
+        sim.init_gradient(target=tgt)
+        while not sim.backtrack_step_midpoint()[1]:
+            pass
+    assert a.loss == b.loss
+    for x, y in zip(a.adj_u.c + [a.adj_passive[0]], b.adj_u.c + [b.adj_passive[0]]):
+        assert torch.equal(x, y)
