@@ -45,6 +45,8 @@ SIGNATURES = {
     "dfm_pullback_s": (c_int, [P(Grid), c_void_p, c_void_p, c_float, c_float, c_void_p, c_void_p, c_void_p]),
     "dfm_bfecc_u": (c_int, [P(Grid), P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), P(Mac), c_void_p]),
     "dfm_bfecc_s": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dfm_bfecc_u_clamp": (c_int, [P(Grid), P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), P(Mac), P(Mac), c_void_p]),
+    "dfm_bfecc_s_clamp": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dfm_accumulate": (c_int, [P(Grid), P(Mac), P(FlowMapS), P(Mac), P(Mac), P(Mac), c_int, c_float, P(Mac),
                                c_float, c_void_p]),
     "dfm_sl_advect": (c_int, [P(Grid), P(Mac), c_float, c_int, c_int, P(Mac), P(Mac), c_int, P(c_void_p),

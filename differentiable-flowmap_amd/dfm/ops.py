@@ -117,6 +117,14 @@ class Ops:
         L.check(self.lib.dfm_bfecc_s(self.g, init.data_ptr(), psi.data_ptr(), phi.data_ptr(), err.data_ptr(),
                                      out.data_ptr(), self._st()), "bfecc_s")
 
+    def bfecc_u_clamp(self, init: MacField, fwd: FlowMap, bwd: FlowMap, err: MacField, tmp: MacField, out: MacField):
+        L.check(self.lib.dfm_bfecc_u_clamp(self.g, init.s, fwd.s, bwd.s, err.s, tmp.s, out.s, self._st()),
+                "bfecc_u_clamp")
+
+    def bfecc_s_clamp(self, init, psi, phi, err, tmp, out):
+        L.check(self.lib.dfm_bfecc_s_clamp(self.g, init.data_ptr(), psi.data_ptr(), phi.data_ptr(), err.data_ptr(),
+                                           tmp.data_ptr(), out.data_ptr(), self._st()), "bfecc_s_clamp")
+
     def accumulate(self, f: MacField, fm: FlowMap, init: MacField, h1: MacField, h2: MacField, ind, dt,
                    f2: MacField = None, dt2=0.0):
         L.check(self.lib.dfm_accumulate(self.g, f.s, fm.s, init.s, h1.s, h2.s, int(ind), float(dt),

@@ -17,8 +17,8 @@ Mirrors the reference's simulator-class API (the boundary its optimisers call):
 Hyper-parameters that the reference reads from its star-imported
 ``hyperparameters.py`` (RK_number, total_steps, add_passive_scalar ...) are
 keyword arguments here.  Supported configuration (the one every benchmark
-uses): viscosity 0, use_short_BFECC=False, BFECC_clamp=False, backward_u=True,
-final-time density loss.  Obstacles: ``set_boundary_mask`` switches the
+uses): viscosity 0, backward_u=True, final-time density loss; the keywords
+``BFECC_clamp`` and ``use_short_BFECC`` select the reference's BFECC variants.  Obstacles: ``set_boundary_mask`` switches the
 projection to the obstacle solver (MGPCG_2_solid) and applies the reference's
 mask_* kernels at its call sites.  Control forces: ``control_num=N`` (the
 reference's add_control_force / control_num) enables ``control_force_para``
@@ -118,7 +118,7 @@ class LFM_Simulator:
     def __init__(self, *args, order=None, total_steps=None, n_passive=1, adj_kernel=None, mg_levels=0,
                  mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda",
                  mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
-                 overlap_projections=True, control_num=0):
+                 overlap_projections=True, control_num=0, BFECC_clamp=False, use_short_BFECC=False):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -179,6 +179,11 @@ class LFM_Simulator:
             self._fr = torch.zeros((N,), dtype=torch.float32, device=dev)
             self._fs = torch.zeros((N, d), dtype=torch.float32, device=dev)
         self._force_store = {}  # step -> (centres, radii, strengths, count) for checkpoint replays
+        # BFECC variants (hyperparameters BFECC_clamp / use_short_BFECC)
+        self.bfecc_clamp = bool(BFECC_clamp)
+        self.short_bfecc = bool(use_short_BFECC)
+        self._clamp_tmp = None
+        self._short_maps = None
         self._views()
 
     # Taichi-like attribute views -------------------------------------------
@@ -193,6 +198,36 @@ class LFM_Simulator:
         if self.control_num > 0:
             for a, nm in enumerate(["control_force_x", "control_force_y", "control_force_z"][: self.dim]):
                 setattr(self, nm, FieldView(self, lambda s, a=a: s.control_f.c[a]))
+
+    # BFECC variants -----------------------------------------------------------
+    def _bfecc_u(self, init, fwd, bwd, err, out):
+        """BFECC (2D/vortex/flowmap.py:440-463), clamp_u'd when BFECC_clamp."""
+        if not self.bfecc_clamp:
+            self.ops.bfecc_u(init, fwd, bwd, err, out)
+            return
+        if self._clamp_tmp is None:
+            self._clamp_tmp = (MacField.empty(self.n, self.device), torch.empty(self.n, device=self.device))
+        self.ops.bfecc_u_clamp(init, fwd, bwd, err, self._clamp_tmp[0], out)
+
+    def _bfecc_s(self, init, psi, phi, err, out):
+        """BFECC_scalar (2D/vortex/flowmap.py:465-481), clamp_u'd when BFECC_clamp."""
+        if not self.bfecc_clamp:
+            self.ops.bfecc_s(init, psi, phi, err, out)
+            return
+        if self._clamp_tmp is None:
+            self._clamp_tmp = (MacField.empty(self.n, self.device), torch.empty(self.n, device=self.device))
+        self.ops.bfecc_s_clamp(init, psi, phi, err, self._clamp_tmp[1], out)
+
+    def _one_step_maps(self, vel, dt):
+        """use_short_BFECC's psi_tem / phi_tem: identity maps marched one step
+        through vel over +dt and -dt."""
+        if self._short_maps is None:
+            self._short_maps = (FlowMap(self.n, self.device), FlowMap(self.n, self.device))
+        fwd, bwd = self._short_maps
+        for m, h in ((fwd, dt), (bwd, -dt)):
+            self.ops.reset_identity(m)
+            self.ops.march(vel, m, h, self.order)
+        return fwd, bwd
 
     # control forces -----------------------------------------------------------
     def _load_forces(self, para):
@@ -317,11 +352,11 @@ class LFM_Simulator:
         j = step % self.reinit_every
         if j == 0 and self.record and not self._replaying:
             self.passive_start[step] = [p.clone() for p in self.passive]
+            if self.checkpoint:
+                self.u_start[step // self.reinit_every] = self.u.clone()
         if write_passive and self.record and not self._replaying:
             # the reference writes the passive of every step to disk (:487-491)
             self.passive_steps[step] = self.passive_start.get(step) or [p.clone() for p in self.passive]
-            if self.checkpoint:
-                self.u_start[step // self.reinit_every] = self.u.clone()
         forces = None
         if control_force_para is not None:
             forces = self._load_forces(control_force_para)
@@ -357,8 +392,16 @@ class LFM_Simulator:
         self._mask_fwd(mid)
         # forward map phi/F marched with the midpoint velocity (dt sign flipped)
         ops.march(mid, self.phi_map, -dt, order)
-        # passive and full-step velocity advected by the midpoint velocity
-        ops.sl_advect(mid, dt, order, [self.u], [self._u_new], self.passive, self._passive_new)
+        if self.short_bfecc:
+            # BFECC from the step's start through one-step maps of the midpoint
+            # velocity (use_short_BFECC, 2D/vortex/lfm_midpoint_simulator.py:560-583)
+            fwd, bwd = self._one_step_maps(mid, dt)
+            for k in range(self.n_passive):
+                self._bfecc_s(self.passive[k], fwd.psi, bwd.psi, self._err_s, self._passive_new[k])
+            self._bfecc_u(self.u, fwd, bwd, self.err_u, self._u_new)
+        else:
+            # passive and full-step velocity advected by the midpoint velocity
+            ops.sl_advect(mid, dt, order, [self.u], [self._u_new], self.passive, self._passive_new)
         self.u, self._u_new = self._u_new, self.u
         self.passive, self._passive_new = self._passive_new, self.passive
         if forces is not None:
@@ -392,11 +435,11 @@ class LFM_Simulator:
         ops.march(self.u, self.psi_map, dt, order)
         for st in reversed(range(step - j, step)):
             ops.march(self.mid_u[st], self.psi_map, dt, order)
-        ops.bfecc_u(self.init_u, self.psi_map, self.phi_map, self.err_u, self.u)
+        self._bfecc_u(self.init_u, self.psi_map, self.phi_map, self.err_u, self.u)
         self.solver.Poisson(self.u)
         self._mask_fwd(self.u)
         for k in range(self.n_passive):
-            ops.bfecc_s(self.init_passive[k], self.psi_map.psi, self.phi_map.psi, self._err_s, self.passive[k])
+            self._bfecc_s(self.init_passive[k], self.psi_map.psi, self.phi_map.psi, self._err_s, self.passive[k])
             self.init_passive[k].copy_(self.passive[k])
         if self.record and not self.checkpoint:
             self._store_phi(step)
@@ -657,15 +700,28 @@ class LFM_Diff_Simulator(LFM_Simulator):
         # adjoint sources at the grid faces, from the current (end-of-step) state
         ops.adj_conv_source(self.u, self.adj_u, self.adj_kernel, self.r_u)
         ops.adj_scalar_source(self.passive, self.adj_passive, self.r_s)
-        # one back-trace through mid over -dt serves lam_u, u, s and lam_s
-        ops.sl_advect(mid, -dt, order, [self.adj_u, self.u], [self._adj_u_new, self._u_new],
-                      self.passive + self.adj_passive, self._passive_new + self._adj_passive_new)
+        if self.short_bfecc:
+            # BFECC of lam_u, lam_s, s and u through one-step maps of mid; the
+            # convective source then enters at -2 dt
+            # (use_short_BFECC, 2D/vortex/lfm_midpoint_diff_simulator.py:283-338)
+            fwd, bwd = self._one_step_maps(mid, -dt)
+            self._bfecc_u(self.adj_u, fwd, bwd, self.err_adj_u, self._adj_u_new)
+            for k in range(self.n_passive):
+                self._bfecc_s(self.adj_passive[k], fwd.psi, bwd.psi, self._err_s, self._adj_passive_new[k])
+                self._bfecc_s(self.passive[k], fwd.psi, bwd.psi, self._err_s, self._passive_new[k])
+            self._bfecc_u(self.u, fwd, bwd, self.err_adj_u, self._u_new)
+            src_dt = -2.0 * dt
+        else:
+            # one back-trace through mid over -dt serves lam_u, u, s and lam_s
+            ops.sl_advect(mid, -dt, order, [self.adj_u, self.u], [self._adj_u_new, self._u_new],
+                          self.passive + self.adj_passive, self._passive_new + self._adj_passive_new)
+            src_dt = -dt
         self.adj_u, self._adj_u_new = self._adj_u_new, self.adj_u
         self.u, self._u_new = self._u_new, self.u
         self.passive, self._passive_new = self._passive_new, self.passive
         self.adj_passive, self._adj_passive_new = self._adj_passive_new, self.adj_passive
-        # lam_u += -dt r_u - dt r_s  (2D/vortex/lfm_midpoint_diff_simulator.py:359-374)
-        ops.axpy(self.adj_u, -dt, self.r_u, self.adj_u, beta=-dt, z=self.r_s)
+        # lam_u += src_dt r_u - dt r_s  (2D/vortex/lfm_midpoint_diff_simulator.py:297-298,359-374)
+        ops.axpy(self.adj_u, src_dt, self.r_u, self.adj_u, beta=-dt, z=self.r_s)
         # psi/T march and source accumulation (AB3 at -2dt for r_u, -dt for r_s)
         ops.march(mid, self.psi_map, dt, order)
         ops.accumulate(self.r_u, self.psi_map, self.init_adj_u, self.ab_h1, self.ab_h2,
@@ -678,10 +734,10 @@ class LFM_Diff_Simulator(LFM_Simulator):
         self._mask_adj()
         if j == 0:
             fwd = self.flow_maps[step // self.reinit_every]
-            ops.bfecc_u(self.init_adj_u, fwd, self.psi_map, self.err_adj_u, self.adj_u)
+            self._bfecc_u(self.init_adj_u, fwd, self.psi_map, self.err_adj_u, self.adj_u)
             self.solver.Poisson_adj(self.adj_u)
             self.init_adj_u.copy_(self.adj_u)
-            ops.bfecc_u(self.init_u, fwd, self.psi_map, self.err_adj_u, self.u)
+            self._bfecc_u(self.init_u, fwd, self.psi_map, self.err_adj_u, self.u)
             self.solver.Poisson(self.u)
             self.init_u.copy_(self.u)
             self._mask_adj()
@@ -689,11 +745,11 @@ class LFM_Diff_Simulator(LFM_Simulator):
                 if self.control_num > 0:
                     # with control forces the passive is rebuilt through the
                     # interval's maps (2D/shape/lfm_midpoint_diff_simulator.py:339-343)
-                    ops.bfecc_s(self.init_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.passive[k])
+                    self._bfecc_s(self.init_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.passive[k])
                 else:
                     self.passive[k].copy_(self.passive_start[step][k])
                 self.init_passive[k].copy_(self.passive[k])
-                ops.bfecc_s(self.init_adj_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.adj_passive[k])
+                self._bfecc_s(self.init_adj_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.adj_passive[k])
                 self.init_adj_passive[k].copy_(self.adj_passive[k])
             self.ops.reset_identity(self.psi_map)
             if self.checkpoint:

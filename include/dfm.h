@@ -90,6 +90,20 @@ int dfm_bfecc_u(const dfm_grid* g, const dfm_mac* init, const dfm_flowmap* fwd, 
 int dfm_bfecc_s(const dfm_grid* g, const float* init, const float* psi, const float* phi, float* err, float* out,
                 void* stream);
 
+/* BFECC with BFECC_clamp = True: as dfm_bfecc_u, then every face value is
+ * clamped to the range of the first pullback T.init(psi) at the face's 2*dim
+ * axis neighbours on its own grid (clamp-to-edge indices).  tmp (caller
+ * scratch) holds that first pullback; out must not alias init, err or tmp.
+ * Replaces BFECC + clamp_u, 2D/vortex/flowmap.py:332-341,440-463,
+ * 3D/flowmap.py:323-334,405-434. */
+int dfm_bfecc_u_clamp(const dfm_grid* g, const dfm_mac* init, const dfm_flowmap* fwd, const dfm_flowmap* bwd,
+                      const dfm_mac* err, const dfm_mac* tmp, const dfm_mac* out, void* stream);
+
+/* BFECC_scalar with BFECC_clamp = True (cells).  Replaces BFECC_scalar +
+ * clamp_u, 2D/vortex/flowmap.py:465-481, 3D/flowmap.py:436-452. */
+int dfm_bfecc_s_clamp(const dfm_grid* g, const float* init, const float* psi, const float* phi, float* err, float* tmp,
+                      float* out, void* stream);
+
 /* accumulate_init_new (AB3 (23,-16,5)/12 with the reference's ind==1 form):
  * g = T_a . f(psi_a); init += dt * AB3(g, h1, h2); h2 = h1; h1 = g.
  * If f2 is non-NULL, additionally init += dt2 * (T_a . f2(psi_a))

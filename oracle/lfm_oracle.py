@@ -269,23 +269,40 @@ def advect_scalar(s0, psi, dx):
     return interp(2, s0, [0.5] * s0.ndim, psi, dx)
 
 
-def bfecc(init, fwd, bwd, dx):
-    """BFECC with BFECC_clamp=False (2D/vortex/flowmap.py:440-463):
+def clamp_extrema(c, ref):
+    """clamp_u (2D/vortex/flowmap.py:332-341, 3D/flowmap.py:323-334): c clamped
+    to [min, max] of ref at the 2*dim axis neighbours, indices clamped to the
+    array (sample's clamp-to-edge)."""
+    lo = np.full(ref.shape, np.inf)
+    hi = np.full(ref.shape, -np.inf)
+    for a in range(ref.ndim):
+        for d in (-1, 1):
+            nb = np.take(ref, np.clip(np.arange(ref.shape[a]) + d, 0, ref.shape[a] - 1), axis=a)
+            lo = np.minimum(lo, nb)
+            hi = np.maximum(hi, nb)
+    return np.minimum(np.maximum(c, lo), hi).astype(ref.dtype)
+
+
+def bfecc(init, fwd, bwd, dx, clamp=False):
+    """BFECC (2D/vortex/flowmap.py:440-463):
     u = T.init(psi); err = (F.u(phi) - init)/2; u -= T.err(psi),
-    where (T, psi) = `fwd` and (F, phi) = `bwd`."""
+    where (T, psi) = `fwd` and (F, phi) = `bwd`; with BFECC_clamp (clamp)
+    the result is clamp_u'd against the first pullback."""
     u = advect_u(init, fwd, dx)
     e = advect_u(u, bwd, dx)
     err = [(e[a] - init[a]) * 0.5 for a in range(len(init))]
     t = advect_u(err, fwd, dx)
-    return [u[a] - t[a] for a in range(len(init))]
+    out = [u[a] - t[a] for a in range(len(init))]
+    return [clamp_extrema(o, r) for o, r in zip(out, u)] if clamp else out
 
 
-def bfecc_scalar(init, psi, phi, dx):
-    """BFECC_scalar (2D/vortex/flowmap.py:465-481)."""
+def bfecc_scalar(init, psi, phi, dx, clamp=False):
+    """BFECC_scalar (2D/vortex/flowmap.py:465-481), clamp_u'd with BFECC_clamp."""
     s = advect_scalar(init, psi, dx)
     e = advect_scalar(s, phi, dx)
     err = (e - init) * 0.5
-    return s - advect_scalar(err, psi, dx)
+    out = s - advect_scalar(err, psi, dx)
+    return clamp_extrema(out, s) if clamp else out
 
 
 def accumulate(f, init, fmap, dx, dt):
@@ -645,9 +662,9 @@ class OracleLFM:
     LFM_Diff_Simulator.init_gradient / backtrack_flow_map
     (2D/vortex/lfm_midpoint_diff_simulator.py:126-503;
     3D/lfm_midpoint_diff_simulator.py:336-594) for the configuration with
-    viscosity 0, use_short_BFECC=False,
-    BFECC_clamp=False, backward_u=True and a final-time density loss (the
-    per-interval intermediate-target terms, q7, are off).
+    viscosity 0, backward_u=True and a final-time density loss (the
+    per-interval intermediate-target terms, q7, are off); BFECC_clamp and
+    use_short_BFECC are constructor flags.
 
     Storage follows the reference's disk cache semantics, kept in RAM:
     midpoint velocity per step, passive at the start of every step, forward
@@ -655,8 +672,10 @@ class OracleLFM:
     """
 
     def __init__(self, n, dx, dt, reinit_every, order, adj_kernel=None, project="exact", dtype=np.float64,
-                 mg_levels=None, mg_tol=1e-6, control_forces=False):
+                 mg_levels=None, mg_tol=1e-6, control_forces=False, bfecc_clamp=False, short_bfecc=False):
         self.control_forces = control_forces  # add_control_force: also switches the j == 0 passive rebuild
+        self.clamp = bfecc_clamp               # BFECC_clamp
+        self.short = short_bfecc               # use_short_BFECC
         self.n = tuple(n)
         self.dim = len(n)
         self.dx, self.dt, self.R, self.order = dx, dt, reinit_every, order
@@ -749,8 +768,17 @@ class OracleLFM:
         self._mask_fwd()
         self.mid[self.step_num] = [c.copy() for c in self.u]
         self.phi_map.march(self.u, -dt, order)
-        self.s = [sl_advect_s(self.u, s, dx, dt, order) for s in self.s]
-        self.u = sl_advect_u(self.u, ad_u, dx, dt, order)
+        if self.short:
+            # one-step maps through the midpoint velocity and BFECC from the
+            # step's start (use_short_BFECC, 2D/vortex/lfm_midpoint_simulator.py:560-583)
+            psi_t, phi_t = FlowMap(self.n, dx, self.dtype), FlowMap(self.n, dx, self.dtype)
+            psi_t.march(self.u, dt, order)
+            phi_t.march(self.u, -dt, order)
+            self.s = [bfecc_scalar(s, psi_t.psi, phi_t.psi, dx, self.clamp) for s in self.s]
+            self.u = bfecc(ad_u, psi_t, phi_t, dx, self.clamp)
+        else:
+            self.s = [sl_advect_s(self.u, s, dx, dt, order) for s in self.s]
+            self.u = sl_advect_u(self.u, ad_u, dx, dt, order)
         if f is not None:
             for a in range(self.dim):
                 self.u[a] = self.u[a] + dt * f[a]
@@ -763,10 +791,10 @@ class OracleLFM:
             self.psi_map.march(self.u, dt, order)
             for st in reversed(range(self.step_num - j, self.step_num)):
                 self.psi_map.march(self.mid[st], dt, order)
-            self.u = bfecc(self.init_u, self.psi_map, self.phi_map, dx)
+            self.u = bfecc(self.init_u, self.psi_map, self.phi_map, dx, self.clamp)
             self.project(self.u)
             self._mask_fwd()
-            self.s = [bfecc_scalar(si, self.psi_map.psi, self.phi_map.psi, dx) for si in self.init_s]
+            self.s = [bfecc_scalar(si, self.psi_map.psi, self.phi_map.psi, dx, self.clamp) for si in self.init_s]
             self.init_s = [s.copy() for s in self.s]
             self.maps[self.step_num // self.R] = self.phi_map.copy()
             self.phi_map.reset()
@@ -807,17 +835,34 @@ class OracleLFM:
                                                      self.lam_u)
         r_u = adj_conv_source(self.u, self.lam_u, dx, self.adj_kernel)
         r_s = [adj_scalar_source(s, l, dx) for s, l in zip(self.s, self.lam_s)]
-        self.lam_u = sl_advect_u(mid, self.lam_u, dx, -dt, order)
-        self.s = [sl_advect_s(mid, s, dx, -dt, order) for s in self.s]
-        self.u = sl_advect_u(mid, self.u, dx, -dt, order)
-        for a in range(self.dim):
-            self.lam_u[a] = self.lam_u[a] + (-dt) * r_u[a]
-        new_lam_s = []
-        for l, rs in zip(self.lam_s, r_s):
-            new_lam_s.append(sl_advect_s(mid, l, dx, -dt, order))
+        if self.short:
+            # use_short_BFECC (2D/vortex/lfm_midpoint_diff_simulator.py:283-342,
+            # 3D/lfm_midpoint_diff_simulator.py:440-496): one-step
+            # maps through mid, BFECC of lam_u, lam_s, s and u, and the
+            # convective source at -2 dt
+            psi_t, phi_t = FlowMap(self.n, dx, self.dtype), FlowMap(self.n, dx, self.dtype)
+            psi_t.march(mid, -dt, order)
+            phi_t.march(mid, dt, order)
+            lam = bfecc(self.lam_u, psi_t, phi_t, dx, self.clamp)
+            self.lam_u = [lam[a] + (-2 * dt) * r_u[a] for a in range(self.dim)]
+            self.lam_s = [bfecc_scalar(l, psi_t.psi, phi_t.psi, dx, self.clamp) for l in self.lam_s]
+            for rs in r_s:
+                for a in range(self.dim):
+                    self.lam_u[a] = self.lam_u[a] + (-dt) * rs[a]
+            self.s = [bfecc_scalar(s, psi_t.psi, phi_t.psi, dx, self.clamp) for s in self.s]
+            self.u = bfecc(self.u, psi_t, phi_t, dx, self.clamp)
+        else:
+            self.lam_u = sl_advect_u(mid, self.lam_u, dx, -dt, order)
+            self.s = [sl_advect_s(mid, s, dx, -dt, order) for s in self.s]
+            self.u = sl_advect_u(mid, self.u, dx, -dt, order)
             for a in range(self.dim):
-                self.lam_u[a] = self.lam_u[a] + (-dt) * rs[a]
-        self.lam_s = new_lam_s
+                self.lam_u[a] = self.lam_u[a] + (-dt) * r_u[a]
+            new_lam_s = []
+            for l, rs in zip(self.lam_s, r_s):
+                new_lam_s.append(sl_advect_s(mid, l, dx, -dt, order))
+                for a in range(self.dim):
+                    self.lam_u[a] = self.lam_u[a] + (-dt) * rs[a]
+            self.lam_s = new_lam_s
         self.psi_map.march(mid, dt, order)
         accumulate_ab3(r_u, self.init_lam_u, self.psi_map, self.H1, self.H2, self.R - 1 - j, dx, -2 * dt)
         for rs in r_s:
@@ -831,20 +876,20 @@ class OracleLFM:
         self._mask_adj()
         if j == 0:
             fwd = self.maps[self.step_num // self.R]
-            self.lam_u = bfecc(self.init_lam_u, fwd, self.psi_map, dx)
+            self.lam_u = bfecc(self.init_lam_u, fwd, self.psi_map, dx, self.clamp)
             self.project(self.lam_u)
             self.init_lam_u = [c.copy() for c in self.lam_u]
-            self.u = bfecc(self.init_u, fwd, self.psi_map, dx)
+            self.u = bfecc(self.init_u, fwd, self.psi_map, dx, self.clamp)
             self.project(self.u)
             self.init_u = [c.copy() for c in self.u]
             self._mask_adj()
             if self.control_forces:
                 # rebuilt through the interval's maps (2D/shape/lfm_midpoint_diff_simulator.py:339-343)
-                self.s = [bfecc_scalar(si, fwd.psi, self.psi_map.psi, dx) for si in self.init_s]
+                self.s = [bfecc_scalar(si, fwd.psi, self.psi_map.psi, dx, self.clamp) for si in self.init_s]
             else:
                 self.s = [s.copy() for s in self.s_start[self.step_num]]
             self.init_s = [s.copy() for s in self.s]
-            self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx) for l in self.init_lam_s]
+            self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx, self.clamp) for l in self.init_lam_s]
             self.init_lam_s = [l.copy() for l in self.lam_s]
             self.psi_map.reset()
         return self.step_num == 0

@@ -59,6 +59,37 @@ def test_forward_and_adjoint_match_oracle(dim, n, R, steps, device):
     assert rel_l2_fields(mac_to_np(sim.u), orc.u) < ADJ_TOL
 
 
+@pytest.mark.parametrize("dim,n,R,steps,clamp,short", [(2, (32, 32), 4, 8, True, True), (2, (32, 32), 4, 8, True, False),
+                                                     (2, (32, 32), 4, 8, False, True), (3, (16, 16, 16), 3, 6, True, True)])
+def test_bfecc_variants_match_oracle(dim, n, R, steps, clamp, short, device):
+    """BFECC_clamp and use_short_BFECC (the reference's plume stage-2
+    configuration uses both) through whole forward and adjoint trajectories."""
+    from dfm import LFM_Diff_Simulator, scenes
+
+    sc = scenes.scene_2d(n, seed=3, vortices=4) if dim == 2 else scenes.scene_3d(n, seed=3, tubes=6)
+    dx = sc["dx"]
+    dt = 0.5 * dx
+    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=steps, tol=1e-7, max_iters=200, BFECC_clamp=clamp,
+                             use_short_BFECC=short)
+    u0, s0, tgt = round32(sc["u0"]), round32(sc["density"]), round32(sc["target"])
+    sim.init(u0=u0, init_passive_np=s0)
+    orc = O.OracleLFM(n, dx, dt, R, sim.order, bfecc_clamp=clamp, short_bfecc=short)
+    orc.init(u0, [s0])
+    for _ in range(steps):
+        sim.step_midpoint()
+        orc.step()
+    assert rel_l2_fields(mac_to_np(sim.u), orc.u) < FWD_TOL
+    assert rel_l2(sim.passive[0].cpu().numpy(), orc.s[0]) < FWD_TOL
+    sim.init_gradient(target=tgt)
+    orc.init_gradient([tgt])
+    while not sim.backtrack_step_midpoint()[1]:
+        orc.backtrack_step()
+    orc.backtrack_step()
+    assert rel_l2_fields(mac_to_np(sim.adj_u), orc.lam_u) < ADJ_TOL
+    assert rel_l2(sim.adj_passive[0].cpu().numpy(), orc.lam_s[0]) < ADJ_TOL
+    assert rel_l2_fields(mac_to_np(sim.u), orc.u) < ADJ_TOL
+
+
 def test_reference_api_surface(device):
     from dfm import LFM_Diff_Simulator, scenes
 
