@@ -608,6 +608,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
                 setattr(self, nm, FieldView(self, lambda s, k=k: s._force_grads[k]))
         self.loss = 0.0
         self.final_ind = None
+        self._interval_targets = None  # step -> [target per passive], or a callable; see set_interval_targets
         names = ["adj_u_x", "adj_u_y", "adj_u_z"][: self.dim]
         for a, nm in enumerate(names):
             setattr(self, nm, FieldView(self, lambda s, a=a: s.adj_u.c[a]))
@@ -621,6 +622,38 @@ class LFM_Diff_Simulator(LFM_Simulator):
         ts = target if isinstance(target, (list, tuple)) else [target]
         for k, t in enumerate(ts[: self.n_passive]):
             self.target_passive[k].copy_(torch.as_tensor(t, dtype=torch.float32).to(self.device))
+
+    def set_interval_targets(self, targets):
+        """Intermediate density targets at reinitialisation-interval starts
+        (2D/vortex/lfm_midpoint_diff_simulator.py:478-497, 3D/lfm_midpoint_diff_simulator.py:586-591):
+        when the adjoint sweep reaches interval start `step`, lam_s += s - s*
+        (gradient factor 1, as the reference has it) and loss += sum (s* - s)^2.
+        `targets`: {step: array or [array per passive]}, a callable step ->
+        that or None, or None to turn the terms off (the default: final-time
+        loss only)."""
+        self._interval_targets = targets
+
+    def interval_targets_from_dir(self, log_target_dir=None):
+        """Targets read from a target run's passive series in the reference's
+        layout (<log_target_dir>/passive, as the reference's read of
+        "target_s*" finds them); returns a callable for set_interval_targets."""
+        import os
+
+        from . import rw_cache as W
+
+        d = os.path.join(log_target_dir or self.log_target_dir, "passive")
+        names = W.layout(self.dim, self.n_passive)["passive"][1]
+        cache = W.RWCache()
+        return lambda step: [np.asarray(cache.read(nm, d, step))[..., 0] for nm in names]
+
+    def _interval_target(self, step):
+        t = self._interval_targets
+        if t is None:
+            return None
+        t = t(step) if callable(t) else t.get(step)
+        if t is None:
+            return None
+        return t if isinstance(t, (list, tuple)) else [t]
 
     def init_gradient(self, theta=None, target=None, grad_passive=None, grad_u=None):
         """init_gradient (2D/vortex/lfm_midpoint_diff_simulator.py:126-207):
@@ -750,6 +783,15 @@ class LFM_Diff_Simulator(LFM_Simulator):
                     self.passive[k].copy_(self.passive_start[step][k])
                 self.init_passive[k].copy_(self.passive[k])
                 self._bfecc_s(self.init_adj_passive[k], fwd.psi, self.psi_map.psi, self._err_s, self.adj_passive[k])
+            targets = self._interval_target(step)
+            if targets is not None:
+                # intermediate targets: lam_s += s - s*, loss += sum (s* - s)^2
+                for k, tk in enumerate(targets[: self.n_passive]):
+                    d = self.passive[k] - torch.as_tensor(np.asarray(tk), dtype=torch.float32).to(self.device)
+                    self.adj_passive[k].add_(d)
+                    if self.loss is not None:
+                        self.loss += float(torch.sum(d.double() ** 2))
+            for k in range(self.n_passive):
                 self.init_adj_passive[k].copy_(self.adj_passive[k])
             self.ops.reset_identity(self.psi_map)
             if self.checkpoint:

@@ -689,6 +689,7 @@ class OracleLFM:
 
     mask = None
     bvel = None
+    interval_targets = None  # {interval-start step: [target per passive]}
 
     def set_mask(self, mask, bvel=None):
         """Obstacle mask on cells (solid where > 0 for the projection, >= 1 for
@@ -890,6 +891,14 @@ class OracleLFM:
                 self.s = [s.copy() for s in self.s_start[self.step_num]]
             self.init_s = [s.copy() for s in self.s]
             self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx, self.clamp) for l in self.init_lam_s]
+            tg = self.interval_targets.get(self.step_num) if self.interval_targets else None
+            if tg is not None:
+                # intermediate targets (q7): lam_s += s - s*, loss += sum (s* - s)^2
+                # (2D/vortex/lfm_midpoint_diff_simulator.py:478-497)
+                for k, t in enumerate(tg):
+                    d = self.s[k] - np.asarray(t, np.float64)
+                    self.lam_s[k] = self.lam_s[k] + d
+                    self.loss += float(np.sum(d ** 2))
             self.init_lam_s = [l.copy() for l in self.lam_s]
             self.psi_map.reset()
         return self.step_num == 0

@@ -208,3 +208,55 @@ def test_reference_file_layout_round_trip(dim, n, R, steps, tmp_path, device):
     assert a.loss == b.loss
     for x, y in zip(a.adj_u.c + [a.adj_passive[0]], b.adj_u.c + [b.adj_passive[0]]):
         assert torch.equal(x, y)
+
+
+def test_interval_targets_match_oracle(tmp_path, device):
+    """Intermediate targets at interval starts (the reference's q7 terms):
+    loss and adjoint against the oracle, with targets given directly and read
+    from a target run's passive series in the reference's layout."""
+    from dfm import LFM_Diff_Simulator, scenes
+
+    n, R, steps = (32, 32), 4, 8
+    sc = scenes.scene_2d(n, seed=4, vortices=4)
+    dx = sc["dx"]
+    rng = np.random.default_rng(6)
+    targets = {st: [round32(np.clip(sc["density"] + 0.2 * rng.standard_normal(n), 0, 1))] for st in (0, 4)}
+    sim = LFM_Diff_Simulator(*n, dx, 0.5 * dx, R, None, total_steps=steps, tol=1e-7, max_iters=200)
+    u0, s0, tgt = round32(sc["u0"]), round32(sc["density"]), round32(sc["target"])
+    sim.init(u0=u0, init_passive_np=s0)
+    orc = O.OracleLFM(n, dx, 0.5 * dx, R, sim.order)
+    orc.init(u0, [s0])
+    orc.interval_targets = targets
+    for _ in range(steps):
+        sim.step_midpoint()
+        orc.step()
+    sim.set_interval_targets(targets)
+    sim.init_gradient(target=tgt)
+    orc.init_gradient([tgt])
+    final_loss = orc.loss
+    while not sim.backtrack_step_midpoint()[1]:
+        orc.backtrack_step()
+    orc.backtrack_step()
+    assert orc.loss > 1.5 * final_loss  # the interval terms are on
+    assert abs(sim.loss - orc.loss) <= 1e-4 * orc.loss
+    assert rel_l2(sim.adj_passive[0].cpu().numpy(), orc.lam_s[0]) < ADJ_TOL
+    assert rel_l2_fields(mac_to_np(sim.adj_u), orc.lam_u) < ADJ_TOL
+    # the same targets as a target run's passive series on disk
+    from dfm import rw_cache as W
+
+    (tmp_path / "passive").mkdir()
+    cache = W.RWCache()
+    for st in range(5):
+        t = targets[st][0] if st in targets else np.zeros(n)
+        cache.write(W.as_channels(t, 2), "s1", str(tmp_path / "passive"), st)
+    cache.force_write("s1", str(tmp_path / "passive"))
+    b = LFM_Diff_Simulator(*n, dx, 0.5 * dx, R, None, str(tmp_path), total_steps=steps, tol=1e-7, max_iters=200)
+    b.init(u0=u0, init_passive_np=s0)
+    for _ in range(steps):
+        b.step_midpoint()
+    b.set_interval_targets(b.interval_targets_from_dir())
+    b.init_gradient(target=tgt)
+    while not b.backtrack_step_midpoint()[1]:
+        pass
+    assert b.loss == sim.loss
+    assert torch.equal(b.adj_passive[0], sim.adj_passive[0])
