@@ -90,7 +90,7 @@ def run_gpu(args):
     sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=args.mg_levels, mg_pre_post=2,
                              mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
                              mg_bf16_coarse=args.mg_bf16_coarse, mg_bottom_exact=args.mg_bottom_exact,
-                             mg_check_every=args.mg_check_every)
+                             mg_check_every=args.mg_check_every, overlap_march=False if args.overlap_march == "none" else args.overlap_march)
     u0, s0, tgt = sc["u0"], sc["density"], sc["target"]
 
     def forward_adjoint(steps):
@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--mg-rbgs-bottom", dest="mg_bottom_exact", action="store_false",
                     help="the reference's 50 RBGS sweeps on the coarsest level instead of the exact DCT solve")
     ap.add_argument("--mg-check-every", type=int, default=1, help="PCG iterations per convergence poll")
+    ap.add_argument("--overlap-march", default="both", choices=["none", "forward", "adjoint", "both"],
+                    help="which steps run their flow-map marches on a side stream beside the projections")
     ap.add_argument("--mg-levels", type=int, default=0,
                     help="MG levels (0: log2(n) - 2, the reference's count: 5 at 128^3, 6 at 256^3)")
     ap.add_argument("--batch", action="store_true",

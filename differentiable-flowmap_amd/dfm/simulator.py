@@ -40,6 +40,8 @@ are bitwise those of the first forward pass.
 """
 from __future__ import annotations
 
+import contextlib
+
 import numpy as np
 import torch
 
@@ -118,7 +120,8 @@ class LFM_Simulator:
     def __init__(self, *args, order=None, total_steps=None, n_passive=1, adj_kernel=None, mg_levels=0,
                  mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda",
                  mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
-                 overlap_projections=True, control_num=0, BFECC_clamp=False, use_short_BFECC=False):
+                 overlap_projections=True, control_num=0, BFECC_clamp=False, use_short_BFECC=False,
+                 overlap_march=True):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -149,6 +152,12 @@ class LFM_Simulator:
         self.overlap_projections = overlap_projections
         self._solver2 = None
         self._stream2 = None
+        # flow-map marches that only need the midpoint velocity run on a side
+        # stream beside the (latency-bound) projections: True / "both",
+        # "forward", "adjoint" or False
+        self.overlap_march = overlap_march
+        self._side = None
+        self._side_done = None
         dev = self.device
         self.u = MacField.empty(n, dev, zero=True)
         self._u_new = MacField.empty(n, dev, zero=True)
@@ -198,6 +207,34 @@ class LFM_Simulator:
         if self.control_num > 0:
             for a, nm in enumerate(["control_force_x", "control_force_y", "control_force_z"][: self.dim]):
                 setattr(self, nm, FieldView(self, lambda s, a=a: s.control_f.c[a]))
+
+    # side stream ---------------------------------------------------------------
+    @contextlib.contextmanager
+    def _on_side(self, where="both"):
+        """Queue the enclosed launches on the side stream (default priority),
+        after everything the current stream has queued so far; the current
+        stream waits for them at the next _join_side().  Without
+        overlap_march the launches stay on the current stream."""
+        if not self.overlap_march or (self.overlap_march != "both" and self.overlap_march is not True
+                                      and self.overlap_march != where):
+            yield
+            return
+        main = torch.cuda.current_stream()
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.device)
+        ready = torch.cuda.Event()
+        ready.record(main)
+        with torch.cuda.stream(self._side):
+            self._side.wait_event(ready)
+            yield
+        done = torch.cuda.Event()
+        done.record(self._side)
+        self._side_done = done
+
+    def _join_side(self):
+        if self._side_done is not None:
+            torch.cuda.current_stream().wait_event(self._side_done)
+            self._side_done = None
 
     # BFECC variants -----------------------------------------------------------
     def _bfecc_u(self, init, fwd, bwd, err, out):
@@ -390,8 +427,11 @@ class LFM_Simulator:
             ops.axpy(mid, 0.5 * dt, self.control_f, mid)
         self.solver.Poisson(mid)
         self._mask_fwd(mid)
-        # forward map phi/F marched with the midpoint velocity (dt sign flipped)
-        ops.march(mid, self.phi_map, -dt, order)
+        # forward map phi/F marched with the midpoint velocity (dt sign flipped);
+        # nothing before the reinitialisation reads it, so it runs beside the
+        # advection and the projection below
+        with self._on_side("forward"):
+            ops.march(mid, self.phi_map, -dt, order)
         if self.short_bfecc:
             # BFECC from the step's start through one-step maps of the midpoint
             # velocity (use_short_BFECC, 2D/vortex/lfm_midpoint_simulator.py:560-583)
@@ -409,9 +449,11 @@ class LFM_Simulator:
             # init_u only feeds the reinitialisation, which a replay skips
             ops.axpy(self.u, dt, self.control_f, self.u)
             if not self._replaying:
-                ops.pullback_u(self.control_f, self.phi_map, self.init_u, alpha=dt, beta=1.0, add=self.init_u)
+                with self._on_side("forward"):  # after the march
+                    ops.pullback_u(self.control_f, self.phi_map, self.init_u, alpha=dt, beta=1.0, add=self.init_u)
         self.solver.Poisson(self.u)
         self._mask_fwd(self.u)
+        self._join_side()
         if j == self.reinit_every - 1:
             if self._replaying:
                 # recomputation for the adjoint only needs the interval's (phi, F)
@@ -755,16 +797,21 @@ class LFM_Diff_Simulator(LFM_Simulator):
         self.adj_passive, self._adj_passive_new = self._adj_passive_new, self.adj_passive
         # lam_u += src_dt r_u - dt r_s  (2D/vortex/lfm_midpoint_diff_simulator.py:297-298,359-374)
         ops.axpy(self.adj_u, src_dt, self.r_u, self.adj_u, beta=-dt, z=self.r_s)
-        # psi/T march and source accumulation (AB3 at -2dt for r_u, -dt for r_s)
-        ops.march(mid, self.psi_map, dt, order)
-        ops.accumulate(self.r_u, self.psi_map, self.init_adj_u, self.ab_h1, self.ab_h2,
-                       self.reinit_every - 1 - j, -2.0 * dt, f2=self.r_s, dt2=-dt)
+        # psi/T march and source accumulation (AB3 at -2dt for r_u, -dt for r_s);
+        # they write only psi/T, init_adj_u (init_u) and the AB3 history, so
+        # they run on the side stream beside the two projections
+        with self._on_side("adjoint"):
+            ops.march(mid, self.psi_map, dt, order)
+            ops.accumulate(self.r_u, self.psi_map, self.init_adj_u, self.ab_h1, self.ab_h2,
+                           self.reinit_every - 1 - j, -2.0 * dt, f2=self.r_s, dt2=-dt)
+            if forces is not None:
+                # the step's force accumulated through (T, psi) (:296-300)
+                ops.pullback_u(self.control_f, self.psi_map, self.init_u, alpha=-dt, beta=1.0, add=self.init_u)
         if forces is not None:
-            # u takes the step's force back, accumulated through (T, psi) (:296-300)
-            ops.axpy(self.u, -dt, self.control_f, self.u)
-            ops.pullback_u(self.control_f, self.psi_map, self.init_u, alpha=-dt, beta=1.0, add=self.init_u)
+            ops.axpy(self.u, -dt, self.control_f, self.u)  # u takes the step's force back
         self._project_pair(self.adj_u, self.u)
         self._mask_adj()
+        self._join_side()
         if j == 0:
             fwd = self.flow_maps[step // self.reinit_every]
             self._bfecc_u(self.init_adj_u, fwd, self.psi_map, self.err_adj_u, self.adj_u)

@@ -141,11 +141,12 @@ def test_full_size_projection_and_uniform_march(n, device):
         assert float((fm.T_f[a][a] - 1).abs().max()) < 1e-6
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_overlapped_adjoint_projections_are_bitwise_sequential(dim, device):
+@pytest.mark.parametrize("dim,checkpoint", [(2, False), (3, False), (2, True)])
+def test_overlapped_adjoint_projections_are_bitwise_sequential(dim, checkpoint, device):
     """The adjoint step's two independent projections, queued on two MGPCG
-    handles and streams before either waits, give bitwise the results of the
-    two projections in sequence."""
+    handles and streams before either waits, and the flow-map marches (and
+    source accumulation) run on a side stream beside the projections, give
+    bitwise the results of everything in sequence on one stream."""
     from dfm import LFM_Diff_Simulator, scenes
 
     n, R, steps = ((32, 32), 4, 8) if dim == 2 else ((16, 16, 16), 3, 6)
@@ -154,7 +155,7 @@ def test_overlapped_adjoint_projections_are_bitwise_sequential(dim, device):
     out = []
     for overlap in (False, True):
         sim = LFM_Diff_Simulator(*n, dx, 0.5 * dx, R, None, total_steps=steps, tol=1e-7,
-                                 overlap_projections=overlap)
+                                 overlap_projections=overlap, overlap_march=overlap, checkpoint=checkpoint)
         sim.init(u0=round32(sc["u0"]), init_passive_np=round32(sc["density"]))
         for _ in range(steps):
             sim.step_midpoint()
