@@ -87,7 +87,7 @@ def run_gpu(args):
     dt = 0.5 * dx
     R = 10
     K, W = args.steps, args.warmup
-    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=args.mg_levels, mg_pre_post=2,
+    sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=args.mg_levels, mg_pre_post=args.mg_pre_post,
                              mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
                              mg_bf16_coarse=args.mg_bf16_coarse, mg_bottom_exact=args.mg_bottom_exact,
                              mg_check_every=args.mg_check_every, overlap_march=False if args.overlap_march == "none" else args.overlap_march)
@@ -284,6 +284,7 @@ def main():
     ap.add_argument("--mg-rbgs-bottom", dest="mg_bottom_exact", action="store_false",
                     help="the reference's 50 RBGS sweeps on the coarsest level instead of the exact DCT solve")
     ap.add_argument("--mg-check-every", type=int, default=1, help="PCG iterations per convergence poll")
+    ap.add_argument("--mg-pre-post", type=int, default=2, help="RBGS sweeps before and after each coarse correction")
     ap.add_argument("--overlap-march", default="both", choices=["none", "forward", "adjoint", "both"],
                     help="which steps run their flow-map marches on a side stream beside the projections")
     ap.add_argument("--mg-levels", type=int, default=0,

@@ -1361,6 +1361,11 @@ int pcg_end(Solver& s) {
       done = true;
       break;
     }
+    // the iteration budget counts the iterations the device performed, not
+    // those queued: queued ones past a convergence that the true-residual
+    // check then rejects were no-ops, and counting them would make a solve
+    // that ends at max_iters depend on how many the host queued at a time
+    P.it = s.h_flags[F_ITERS];
     if (s.h_flags[F_NTRUE] > P.refinements) {
       ++P.refinements;
       vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_ZR_INIT, s.p});

@@ -814,11 +814,11 @@ class LFM_Diff_Simulator(LFM_Simulator):
         self._join_side()
         if j == 0:
             fwd = self.flow_maps[step // self.reinit_every]
+            # BFECC of lam_u and u, then their two (independent) projections
             self._bfecc_u(self.init_adj_u, fwd, self.psi_map, self.err_adj_u, self.adj_u)
-            self.solver.Poisson_adj(self.adj_u)
-            self.init_adj_u.copy_(self.adj_u)
             self._bfecc_u(self.init_u, fwd, self.psi_map, self.err_adj_u, self.u)
-            self.solver.Poisson(self.u)
+            self._project_pair(self.adj_u, self.u)
+            self.init_adj_u.copy_(self.adj_u)
             self.init_u.copy_(self.u)
             self._mask_adj()
             for k in range(self.n_passive):
