@@ -90,7 +90,8 @@ def run_gpu(args):
     sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=args.mg_levels, mg_pre_post=args.mg_pre_post,
                              mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
                              mg_bf16_coarse=args.mg_bf16_coarse, mg_bottom_exact=args.mg_bottom_exact,
-                             mg_check_every=args.mg_check_every, overlap_march=False if args.overlap_march == "none" else args.overlap_march)
+                             mg_check_every=args.mg_check_every, overlap_march=False if args.overlap_march == "none" else args.overlap_march,
+                             mg_fused_sweeps=args.mg_fused_sweeps)
     u0, s0, tgt = sc["u0"], sc["density"], sc["target"]
 
     def forward_adjoint(steps):
@@ -285,6 +286,8 @@ def main():
                     help="the reference's 50 RBGS sweeps on the coarsest level instead of the exact DCT solve")
     ap.add_argument("--mg-check-every", type=int, default=1, help="PCG iterations per convergence poll")
     ap.add_argument("--mg-pre-post", type=int, default=2, help="RBGS sweeps before and after each coarse correction")
+    ap.add_argument("--mg-fused-sweeps", type=int, default=2, choices=[0, 1, 2],
+                    help="coarse-level smoothing: 2 LDS-tiled, 1 one launch per sweep, 0 two per sweep")
     ap.add_argument("--overlap-march", default="both", choices=["none", "forward", "adjoint", "both"],
                     help="which steps run their flow-map marches on a side stream beside the projections")
     ap.add_argument("--mg-levels", type=int, default=0,

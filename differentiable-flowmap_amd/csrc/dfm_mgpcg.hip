@@ -573,6 +573,7 @@ __global__ void __launch_bounds__(kB) k_update_p(int64_t N, const float* __restr
 template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restrict__ r, float* __restrict__ z,
                                                     const double* shift, const int* flags) {
+#pragma clang fp contract(off)
   if (is_done(flags)) return;
   float m = shift ? (float)*shift : 0.f;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
@@ -748,6 +749,7 @@ template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict__ r, const float* __restrict__ z,
                                                  Dims dc, float* __restrict__ rc, const double* shift,
                                                  const int* flags) {
+#pragma clang fp contract(off)
   if (is_done(flags)) return;
   float m = shift ? (float)*shift : 0.f;
   int64_t Nc = (int64_t)dc.n[0] * dc.n[1] * dc.n[2];
@@ -768,6 +770,230 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
         acc += (r[id] - m) - (nn * z[id] - s);
       }
   rc[cid] = acc * (1.0f / (float)(DIM - 1));
+}
+
+// ---------------------------------------------------- LDS-tiled levels ----
+// The coarse global levels are launch-latency bound: five launches per level
+// and V-cycle (first sweep, second sweep, restriction; two post sweeps).
+// Here one workgroup owns a tile of cells and keeps it in LDS with a halo
+// of kTH cells, enough for the dependency depth of the level's 2 x 2
+// half-sweeps: half-sweep t runs over the tile plus a halo shrinking by one
+// cell per half-sweep, so the interior values come out exactly as the
+// global half-sweeps produce them (same operations, same order, no
+// multiply-add contraction), with the redundant halo work done in LDS.
+//   k_tile_pre:  z from 0 by 2 RBGS sweeps (colour 0 first) + restriction;
+//   k_tile_post: 2 RBGS sweeps (colour 1 first) from z, the coarse
+//                correction added to the neighbour values of the first.
+// Levels without an obstacle mask, with 2 pre/post sweeps, whose extents are
+// multiples of the tile, use them.
+constexpr int kTH = 4;  // halo: the 4 half-sweeps of 2 sweeps
+template <int DIM> struct MgTile;
+template <> struct MgTile<3> { static constexpr int T = 8, E = 8 + 2 * kTH, EK = E, N = E * E * E; };
+template <> struct MgTile<2> { static constexpr int T = 16, E = 16 + 2 * kTH, EK = 1, N = E * E; };
+
+template <int DIM>
+struct TileFrame {
+  int o[3];  // global index of local (0, 0, 0)
+};
+
+template <int DIM>
+DFM_HD TileFrame<DIM> tile_frame(const Dims& d, uint32_t tile) {
+  using M = MgTile<DIM>;
+  const int t1 = d.n[1] / M::T, t2 = DIM == 3 ? d.n[2] / M::T : 1;
+  const int tk = (int)(tile % (uint32_t)t2), tij = (int)(tile / (uint32_t)t2);
+  const int tj = tij % t1, ti = tij / t1;
+  TileFrame<DIM> f;
+  f.o[0] = ti * M::T - kTH;
+  f.o[1] = tj * M::T - kTH;
+  f.o[2] = DIM == 3 ? tk * M::T - kTH : 0;
+  return f;
+}
+
+// Visit the in-grid cells of colour `colour` (colour < 0: every cell) in the
+// tile plus a halo of h, calling fn(i, j, k, l) with global indices and the
+// LDS slot.  3D: thread (lj, lk) of the 16 x 16 tile face walks its column
+// along i in steps of two (one colour), so no lane idles on the other colour
+// and no index is divided; 2D: the 24 x 24 box is enumerated with constant
+// divisors.
+template <int DIM, class F>
+DFM_HD void tile_pass(const Dims& d, const TileFrame<DIM>& f, int h, int colour, F&& fn) {
+  using M = MgTile<DIM>;
+  const int r0 = kTH - h, r1 = kTH + M::T + h;  // local range of the region on every axis
+  if constexpr (DIM == 3) {
+    static_assert(M::E * M::E == kB, "one thread per column of the tile face");
+    const int lj = threadIdx.x / M::E, lk = threadIdx.x % M::E;
+    const int j = f.o[1] + lj, k = f.o[2] + lk;
+    if (lj < r0 || lj >= r1 || lk < r0 || lk >= r1 || j < 0 || j >= d.n[1] || k < 0 || k >= d.n[2]) return;
+    int a0 = max(r0, -f.o[0]);
+    const int a1 = min(r1, d.n[0] - f.o[0]);
+    int step = 1;
+    if (colour >= 0) {
+      a0 += (colour - (f.o[0] + a0 + j + k)) & 1;
+      step = 2;
+    }
+    for (int li = a0; li < a1; li += step) fn(f.o[0] + li, j, k, (li * M::E + lj) * M::E + lk);
+  } else {
+    for (int t = threadIdx.x; t < M::E * M::E; t += kB) {
+      const int li = t / M::E, lj = t % M::E;
+      if (li < r0 || li >= r1 || lj < r0 || lj >= r1) continue;
+      const int i = f.o[0] + li, j = f.o[1] + lj;
+      if (i < 0 || i >= d.n[0] || j < 0 || j >= d.n[1]) continue;
+      if (colour >= 0 && ((i + j) & 1) != colour) continue;
+      fn(i, j, 0, t);
+    }
+  }
+}
+
+// Load the tile plus its full halo from global field(s) into LDS.  3D: each
+// thread's column of E cells is loaded with every load issued before any is
+// stored (one wave per SIMD leaves nothing else to hide the latency); cells
+// outside the grid get 0.  ld(i, j, k, slot, inside) is called per cell.
+template <int DIM, class L>
+DFM_HD void tile_load(const Dims& d, const TileFrame<DIM>& f, L&& ld) {
+  using M = MgTile<DIM>;
+  if constexpr (DIM == 3) {
+    const int lj = threadIdx.x / M::E, lk = threadIdx.x % M::E;
+    const int j = f.o[1] + lj, k = f.o[2] + lk;
+    const bool col = j >= 0 && j < d.n[1] && k >= 0 && k < d.n[2];
+#pragma unroll
+    for (int li = 0; li < M::E; ++li) {
+      const int i = f.o[0] + li;
+      ld(i, j, k, (li * M::E + lj) * M::E + lk, col && i >= 0 && i < d.n[0]);
+    }
+  } else {
+    for (int t = threadIdx.x; t < M::E * M::E; t += kB) {
+      const int i = f.o[0] + t / M::E, j = f.o[1] + t % M::E;
+      ld(i, j, 0, t, i >= 0 && i < d.n[0] && j >= 0 && j < d.n[1]);
+    }
+  }
+}
+
+// Neighbour sum of slot l (in-grid neighbours, for_nbrs order) of values
+// v(slot); returns the count in nn.
+template <int DIM, class V>
+DFM_HD float tile_nbr_sum(const Dims& d, int i, int j, int k, int l, float& nn, V&& v) {
+#pragma clang fp contract(off)
+  using M = MgTile<DIM>;
+  constexpr int SI = M::E * M::EK, SJ = M::EK;
+  float s = 0.f;
+  nn = 0.f;
+  if (i > 0) { s += v(l - SI); nn += 1.f; }
+  if (i < d.n[0] - 1) { s += v(l + SI); nn += 1.f; }
+  if (j > 0) { s += v(l - SJ); nn += 1.f; }
+  if (j < d.n[1] - 1) { s += v(l + SJ); nn += 1.f; }
+  if (DIM == 3) {
+    if (k > 0) { s += v(l - 1); nn += 1.f; }
+    if (k < d.n[2] - 1) { s += v(l + 1); nn += 1.f; }
+  }
+  return s;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kB) k_tile_pre(Dims d, const float* __restrict__ r, float* __restrict__ z, Dims dc,
+                                                 float* __restrict__ rc, const int* flags) {
+#pragma clang fp contract(off)
+  using M = MgTile<DIM>;
+  __shared__ float rl[M::N];
+  __shared__ float zl[M::N];
+  if (is_done(flags)) return;
+  const TileFrame<DIM> f = tile_frame<DIM>(d, blockIdx.x);
+  {
+    float rv[M::E];
+    int li = 0;
+    tile_load<DIM>(d, f, [&](int i, int j, int k, int, bool in) {
+      rv[li++ % M::E] = in ? r[cidx(d, i, j, k)] : 0.f;
+    });
+    li = 0;
+    tile_load<DIM>(d, f, [&](int, int, int, int l, bool) {
+      rl[l] = rv[li++ % M::E];
+      zl[l] = 0.f;
+    });
+  }
+  __syncthreads();
+  const float m = 0.f;
+  // first sweep from z = 0: colour 0 gets (r - 0) / count (k_first_sweep)
+  tile_pass<DIM>(d, f, kTH, 0, [&](int i, int j, int k, int l) {
+    zl[l] = (rl[l] - m) * inv_count(nbr_count<DIM>(d, i, j, k));
+  });
+  __syncthreads();
+  for (int t = 1; t < kTH; ++t) {  // half-sweeps 1..3: colours 1, 0, 1
+    tile_pass<DIM>(d, f, kTH - t, t & 1, [&](int i, int j, int k, int l) {
+      float nn;
+      const float s = tile_nbr_sum<DIM>(d, i, j, k, l, nn, [&](int q) { return zl[q]; });
+      zl[l] = (rl[l] - m + s) * inv_count(nn);
+    });
+    __syncthreads();
+  }
+  tile_pass<DIM>(d, f, 0, -1, [&](int i, int j, int k, int l) { z[cidx(d, i, j, k)] = zl[l]; });
+  // restriction of the tile's interior (k_restrict): one thread per coarse cell
+  constexpr int TC = M::T / 2, TCK = DIM == 3 ? TC : 1;
+  constexpr int SI = M::E * M::EK, SJ = M::EK;
+  const int kz = DIM == 3 ? 2 : 1;
+  for (int t = threadIdx.x; t < TC * TC * TCK; t += kB) {
+    const int ck = (f.o[2] + kTH) / 2 + (DIM == 3 ? t % TCK : 0);
+    const int cj = (f.o[1] + kTH) / 2 + (t / TCK) % TC;
+    const int ci = (f.o[0] + kTH) / 2 + t / (TCK * TC);
+    float acc = 0.f;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < kz; ++c) {
+          const int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
+          const int l = (i - f.o[0]) * SI + (j - f.o[1]) * SJ + (DIM == 3 ? k - f.o[2] : 0);
+          float nn;
+          const float s = tile_nbr_sum<DIM>(d, i, j, k, l, nn, [&](int q) { return zl[q]; });
+          acc += (rl[l] - m) - (nn * zl[l] - s);
+        }
+    rc[cidx(dc, ci, cj, DIM == 3 ? ck : 0)] = acc * (1.0f / (float)(DIM - 1));
+  }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kB) k_tile_post(Dims d, const float* __restrict__ r, const float* __restrict__ zin,
+                                                  const float* __restrict__ zc, Dims dc, float* __restrict__ zout,
+                                                  const int* flags) {
+#pragma clang fp contract(off)
+  using M = MgTile<DIM>;
+  __shared__ float rl[M::N];
+  __shared__ float zl[M::N];
+  __shared__ float pl[M::N];  // coarse correction at each cell (its parent's value)
+  if (is_done(flags)) return;
+  const TileFrame<DIM> f = tile_frame<DIM>(d, blockIdx.x);
+  {
+    float rv[M::E], zv[M::E], pv[M::E];
+    int li = 0;
+    tile_load<DIM>(d, f, [&](int i, int j, int k, int, bool in) {
+      const int q = li++ % M::E;
+      const uint32_t id = in ? cidx(d, i, j, k) : 0u;
+      rv[q] = in ? r[id] : 0.f;
+      zv[q] = in ? zin[id] : 0.f;
+      pv[q] = in ? zc[cidx(dc, i >> 1, j >> 1, DIM == 3 ? (k >> 1) : 0)] : 0.f;
+    });
+    li = 0;
+    tile_load<DIM>(d, f, [&](int, int, int, int l, bool) {
+      const int q = li++ % M::E;
+      rl[l] = rv[q];
+      zl[l] = zv[q];
+      pl[l] = pv[q];
+    });
+  }
+  __syncthreads();
+  const float m = 0.f;
+  // first half-sweep (colour 1): neighbour values are z + the coarse correction
+  tile_pass<DIM>(d, f, kTH - 1, 1, [&](int i, int j, int k, int l) {
+    float nn;
+    const float s = tile_nbr_sum<DIM>(d, i, j, k, l, nn, [&](int q) { return zl[q] + pl[q]; });
+    zl[l] = (rl[l] - m + s) * inv_count(nn);
+  });
+  __syncthreads();
+  for (int t = 1; t < kTH; ++t) {  // half-sweeps 1..3: colours 0, 1, 0
+    tile_pass<DIM>(d, f, kTH - 1 - t, (t + 1) & 1, [&](int i, int j, int k, int l) {
+      float nn;
+      const float s = tile_nbr_sum<DIM>(d, i, j, k, l, nn, [&](int q) { return zl[q]; });
+      zl[l] = (rl[l] - m + s) * inv_count(nn);
+    });
+    __syncthreads();
+  }
+  tile_pass<DIM>(d, f, 0, -1, [&](int i, int j, int k, int l) { zout[cidx(d, i, j, k)] = zl[l]; });
 }
 
 // Obstacle flags of the MG levels: level 0 = mask > 0 (the solver's
@@ -1056,6 +1282,7 @@ struct Solver {
   uint8_t* bm[kMaxLevels];
   double nfluid;
   int fused_sweeps;  // option: one launch per RBGS sweep on levels with z2
+  int tiled_sweeps;  // option: LDS-tiled pre/post smoothing on eligible coarse levels
 };
 
 // Largest LDS a workgroup may request on this device.
@@ -1192,6 +1419,23 @@ bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool fr
   return zr_done;
 }
 
+// Whether level l runs its pre/post smoothing in the LDS-tiled kernels:
+// a coarse global level (not 0, second buffer present), 2 pre/post sweeps,
+// extents multiples of the tile, fused sweeps enabled.
+inline bool tiled_level(const Solver& s, int l) {
+  if (l == 0 || !s.fused_sweeps || s.pre != 2 || !s.lv[l].z2 || s.tiled_sweeps == 0) return false;
+  const int T = s.G.dim == 3 ? MgTile<3>::T : MgTile<2>::T;
+  for (int a = 0; a < s.G.dim; ++a)
+    if (s.lv[l].n[a] % T) return false;
+  return true;
+}
+
+template <int DIM>
+inline unsigned tile_count(const Dims& d) {
+  const int T = MgTile<DIM>::T;
+  return (unsigned)((d.n[0] / T) * (d.n[1] / T) * (DIM == 3 ? d.n[2] / T : 1));
+}
+
 // apply_preconditioner (2D/vortex/mgpcg.py:182-200): z0 = M^-1 r0, followed by
 // z.r and its scalar step `zr`.  Levels below tail_lt use global kernels;
 // levels tail_lt.. run in one LDS kernel.  With an obstacle mask the bottom
@@ -1205,6 +1449,12 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   bool zr_done = false;
   for (int l = 0; l < Lg; ++l) {
     const double* sh = l == 0 ? shift0 : nullptr;
+    if (!MASK && tiled_level(s, l)) {
+      cur[l] = s.lv[l].z;
+      hipLaunchKernelGGL((k_tile_pre<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kB), 0, st, s.ld[l],
+                         (const float*)s.lv[l].r, cur[l], s.ld[l + 1], s.lv[l + 1].r, (const int*)s.flags);
+      continue;
+    }
     sweeps_global<DIM, MASK>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr, cur[l]);
     Lvl& F = s.lv[l];
     Lvl& C = s.lv[l + 1];
@@ -1233,6 +1483,14 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   }
   for (int l = Lg - 1; l >= 0; --l) {
     const double* sh = l == 0 ? shift0 : nullptr;
+    if (!MASK && tiled_level(s, l)) {
+      float* out = cur[l] == s.lv[l].z ? s.lv[l].z2 : s.lv[l].z;
+      hipLaunchKernelGGL((k_tile_post<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kB), 0, st, s.ld[l],
+                         (const float*)s.lv[l].r, (const float*)cur[l], (const float*)cur[l + 1], s.ld[l + 1], out,
+                         (const int*)s.flags);
+      cur[l] = out;
+      continue;
+    }
     const bool d0 = sweeps_global<DIM, MASK>(s, st, l, s.pre, 1, false, sh, cur[l + 1], l == 0 ? &zr : nullptr,
                                              cur[l]);
     if (l == 0) zr_done = d0;
@@ -1459,6 +1717,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->max_refine = 3;
   s->use_graph = 1;
   s->fused_sweeps = 1;
+  s->tiled_sweeps = 1;
   bool ok = true;
   for (int l = 0; l < levels && ok; ++l) {
     for (int a = 0; a < 3; ++a) s->lv[l].n[a] = a < g->dim ? (g->n[a] >> l) : 1;
@@ -1538,7 +1797,11 @@ extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
       drop_graph(*s);
       return DFM_OK;
     case DFM_MG_FUSED_SWEEPS:
+      // 0: two launches per sweep; 1: one launch per sweep; 2 (default): LDS-tiled
+      // pre/post smoothing where eligible, else one launch per sweep
+      if (value < 0 || value > 2) return set_error(DFM_E_ARG, "fused sweeps option must be 0, 1 or 2");
       s->fused_sweeps = value ? 1 : 0;
+      s->tiled_sweeps = value == 2 ? 1 : 0;
       drop_graph(*s);
       return DFM_OK;
     default:
@@ -1557,7 +1820,7 @@ extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
     case DFM_MG_GRAPH: *value = s->gexec ? 2 : s->use_graph; return DFM_OK;
     case DFM_MG_BOTTOM_EXACT: *value = (s->tail_lt < s->L) ? s->bottom_exact : 0; return DFM_OK;
     case DFM_MG_TAIL_LEVEL: *value = s->tail_lt; return DFM_OK;
-    case DFM_MG_FUSED_SWEEPS: *value = s->fused_sweeps; return DFM_OK;
+    case DFM_MG_FUSED_SWEEPS: *value = s->fused_sweeps + s->tiled_sweeps; return DFM_OK;
     default: return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
 }

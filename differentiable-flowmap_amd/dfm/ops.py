@@ -194,7 +194,7 @@ class MGPCG:
 
     def __init__(self, n, dx, levels=0, pre_post=2, bottom=50, tol=1e-6, relative=True, max_iters=400,
                  raise_on_noconv=False, bf16_coarse=False, tail=True, graph=True, check_every=1,
-                 bottom_exact=False):
+                 bottom_exact=False, fused_sweeps=2):
         self.n = tuple(n)
         self.g = L.make_grid(n, dx)
         self.lib = L.lib()
@@ -212,6 +212,8 @@ class MGPCG:
             self.set_option(L.MG_CHECK_EVERY, check_every)
         if bottom_exact:
             self.set_option(L.MG_BOTTOM_EXACT, 1)
+        if fused_sweeps != 2:
+            self.set_option(L.MG_FUSED_SWEEPS, fused_sweeps)
         self.tol, self.relative, self.max_iters = tol, relative, max_iters
         self.raise_on_noconv = raise_on_noconv
         self.last_iters = 0

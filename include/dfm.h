@@ -205,10 +205,14 @@ int dfm_mgpcg_pressure(void* handle, double** p);
  *      kernels.  get: the first tail level in use (L = no tail).  The batch
  *      executor's autotuner sets it per grid size. */
 #define DFM_MG_TAIL_LEVEL 7
-/*  DFM_MG_FUSED_SWEEPS: 1 = on the coarse global levels (at most 2^19 cells)
- *      run each red-black Gauss-Seidel sweep as one launch that recomputes
- *      the first colour's neighbour values (default; bitwise equal to the two
- *      half-sweeps); 0 = two launches per sweep. */
+/*  DFM_MG_FUSED_SWEEPS: smoothing launches of the coarse global levels (at
+ *      most 2^19 cells).  2 (default) = where the level has no obstacle mask,
+ *      2 pre/post sweeps and extents that are multiples of the tile (8^3 in
+ *      3D, 16^2 in 2D), its pre-smoothing with the restriction and its
+ *      post-smoothing with the prolongation each run as one LDS-tiled launch
+ *      (a halo of 4 cells recomputed per tile), else as 1; 1 = one launch per
+ *      red-black sweep, recomputing the first colour's neighbour values; 0 =
+ *      two launches per sweep.  All three give bitwise the same results. */
 #define DFM_MG_FUSED_SWEEPS 8
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);

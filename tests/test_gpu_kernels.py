@@ -357,12 +357,14 @@ def test_mgpcg_repeat_solves_bitwise_identical(n, levels, device):
 
 @pytest.mark.parametrize("n,levels,tail,masked", [((72, 56), 3, 0, False), ((256, 192), 5, 1, False),
                                                   ((40, 36, 20), 3, 0, False), ((128, 128, 128), 5, 1, False),
-                                                  ((96, 80), 4, 0, True), ((32, 32, 24), 3, 0, True)])
+                                                  ((96, 80), 4, 0, True), ((32, 32, 24), 3, 0, True),
+                                                  ((64, 48, 32), 3, 0, False), ((128, 96), 4, 0, False)])
 def test_mgpcg_fused_sweeps_bitwise(n, levels, tail, masked, device):
-    """One launch per red-black sweep on the coarse levels (the second colour
-    recomputes its neighbours' first-colour values) gives bitwise the fields
-    and iteration counts of two half-sweep launches, with and without the
-    LDS tail and with an obstacle mask."""
+    """The LDS-tiled pre/post smoothing of the coarse levels (option 2) and
+    one launch per red-black sweep (the second colour recomputes its
+    neighbours' first-colour values, option 1) give bitwise the fields and
+    iteration counts of two half-sweep launches (option 0), with and without
+    the LDS tail and with an obstacle mask."""
     from dfm import _lib as L
     from dfm.ops import MGPCG
 
@@ -374,7 +376,7 @@ def test_mgpcg_fused_sweeps_bitwise(n, levels, tail, masked, device):
         c = np.stack(np.meshgrid(*[(np.arange(m) + 0.5) / n[1] for m in n], indexing="ij"), -1)
         mask = (np.linalg.norm(c - 0.4, axis=-1) < 0.2).astype(np.float32)
     outs = []
-    for fused in (1, 0):
+    for fused in (2, 1, 0):
         s = MGPCG(n, 1.0 / n[1], levels=levels, tol=1e-7, max_iters=200)
         s.set_option(L.MG_TAIL, tail)
         s.set_option(L.MG_FUSED_SWEEPS, fused)
@@ -387,10 +389,11 @@ def test_mgpcg_fused_sweeps_bitwise(n, levels, tail, masked, device):
             s.Poisson(d)
             res.append((mac_to_np(d), s.status()[0]))
         outs.append(res)
-    for (fa, ia), (fb, ib) in zip(*outs):
-        assert ia == ib
-        for a, b in zip(fa, fb):
-            assert np.array_equal(a, b)
+    for other in outs[1:]:
+        for (fa, ia), (fb, ib) in zip(outs[0], other):
+            assert ia == ib
+            for a, b in zip(fa, fb):
+                assert np.array_equal(a, b)
 
 
 def test_mgpcg_solve_and_status(device):
