@@ -787,6 +787,7 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
 // Levels without an obstacle mask, with 2 pre/post sweeps, whose extents are
 // multiples of the tile, use them.
 constexpr int kTH = 4;  // halo: the 4 half-sweeps of 2 sweeps
+constexpr int kTB = 512;  // threads of a tiled workgroup: 2 per column of the 3D tile face
 template <int DIM> struct MgTile;
 template <> struct MgTile<3> { static constexpr int T = 8, E = 8 + 2 * kTH, EK = E, N = E * E * E; };
 template <> struct MgTile<2> { static constexpr int T = 16, E = 16 + 2 * kTH, EK = 1, N = E * E; };
@@ -811,17 +812,18 @@ DFM_HD TileFrame<DIM> tile_frame(const Dims& d, uint32_t tile) {
 
 // Visit the in-grid cells of colour `colour` (colour < 0: every cell) in the
 // tile plus a halo of h, calling fn(i, j, k, l) with global indices and the
-// LDS slot.  3D: thread (lj, lk) of the 16 x 16 tile face walks its column
-// along i in steps of two (one colour), so no lane idles on the other colour
-// and no index is divided; 2D: the 24 x 24 box is enumerated with constant
-// divisors.
+// LDS slot.  3D: the two threads of column (lj, lk) of the 16 x 16 tile face
+// walk it along i on one colour, alternating cells, so no lane idles on the
+// other colour and no index is divided; 2D: the 24 x 24 box is enumerated
+// with constant divisors.
 template <int DIM, class F>
 DFM_HD void tile_pass(const Dims& d, const TileFrame<DIM>& f, int h, int colour, F&& fn) {
   using M = MgTile<DIM>;
   const int r0 = kTH - h, r1 = kTH + M::T + h;  // local range of the region on every axis
   if constexpr (DIM == 3) {
-    static_assert(M::E * M::E == kB, "one thread per column of the tile face");
-    const int lj = threadIdx.x / M::E, lk = threadIdx.x % M::E;
+    static_assert(2 * M::E * M::E == kTB, "two threads per column of the tile face");
+    const int c = threadIdx.x % (M::E * M::E), half = threadIdx.x / (M::E * M::E);
+    const int lj = c / M::E, lk = c % M::E;
     const int j = f.o[1] + lj, k = f.o[2] + lk;
     if (lj < r0 || lj >= r1 || lk < r0 || lk >= r1 || j < 0 || j >= d.n[1] || k < 0 || k >= d.n[2]) return;
     int a0 = max(r0, -f.o[0]);
@@ -831,9 +833,10 @@ DFM_HD void tile_pass(const Dims& d, const TileFrame<DIM>& f, int h, int colour,
       a0 += (colour - (f.o[0] + a0 + j + k)) & 1;
       step = 2;
     }
-    for (int li = a0; li < a1; li += step) fn(f.o[0] + li, j, k, (li * M::E + lj) * M::E + lk);
+    // the column's cells alternate between its two threads
+    for (int li = a0 + half * step; li < a1; li += 2 * step) fn(f.o[0] + li, j, k, (li * M::E + lj) * M::E + lk);
   } else {
-    for (int t = threadIdx.x; t < M::E * M::E; t += kB) {
+    for (int t = threadIdx.x; t < M::E * M::E; t += kTB) {
       const int li = t / M::E, lj = t % M::E;
       if (li < r0 || li >= r1 || lj < r0 || lj >= r1) continue;
       const int i = f.o[0] + li, j = f.o[1] + lj;
@@ -845,23 +848,25 @@ DFM_HD void tile_pass(const Dims& d, const TileFrame<DIM>& f, int h, int colour,
 }
 
 // Load the tile plus its full halo from global field(s) into LDS.  3D: each
-// thread's column of E cells is loaded with every load issued before any is
-// stored (one wave per SIMD leaves nothing else to hide the latency); cells
-// outside the grid get 0.  ld(i, j, k, slot, inside) is called per cell.
+// thread's half column of E/2 cells is loaded with every load issued before
+// any is stored (two waves per SIMD leave little else to hide the latency);
+// cells outside the grid get 0.  ld(i, j, k, slot, inside) is called per cell.
 template <int DIM, class L>
 DFM_HD void tile_load(const Dims& d, const TileFrame<DIM>& f, L&& ld) {
   using M = MgTile<DIM>;
   if constexpr (DIM == 3) {
-    const int lj = threadIdx.x / M::E, lk = threadIdx.x % M::E;
+    const int c = threadIdx.x % (M::E * M::E), half = threadIdx.x / (M::E * M::E);
+    const int lj = c / M::E, lk = c % M::E;
     const int j = f.o[1] + lj, k = f.o[2] + lk;
     const bool col = j >= 0 && j < d.n[1] && k >= 0 && k < d.n[2];
 #pragma unroll
-    for (int li = 0; li < M::E; ++li) {
+    for (int q = 0; q < M::E / 2; ++q) {
+      const int li = half * (M::E / 2) + q;
       const int i = f.o[0] + li;
       ld(i, j, k, (li * M::E + lj) * M::E + lk, col && i >= 0 && i < d.n[0]);
     }
   } else {
-    for (int t = threadIdx.x; t < M::E * M::E; t += kB) {
+    for (int t = threadIdx.x; t < M::E * M::E; t += kTB) {
       const int i = f.o[0] + t / M::E, j = f.o[1] + t % M::E;
       ld(i, j, 0, t, i >= 0 && i < d.n[0] && j >= 0 && j < d.n[1]);
     }
@@ -889,7 +894,7 @@ DFM_HD float tile_nbr_sum(const Dims& d, int i, int j, int k, int l, float& nn, 
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kB) k_tile_pre(Dims d, const float* __restrict__ r, float* __restrict__ z, Dims dc,
+__global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const float* __restrict__ r, float* __restrict__ z, Dims dc,
                                                  float* __restrict__ rc, const int* flags) {
 #pragma clang fp contract(off)
   using M = MgTile<DIM>;
@@ -897,7 +902,7 @@ __global__ void __launch_bounds__(kB) k_tile_pre(Dims d, const float* __restrict
   __shared__ float zl[M::N];
   if (is_done(flags)) return;
   const TileFrame<DIM> f = tile_frame<DIM>(d, blockIdx.x);
-  {
+  if constexpr (DIM == 3) {  // stage in registers: every load issued before the first store
     float rv[M::E];
     int li = 0;
     tile_load<DIM>(d, f, [&](int i, int j, int k, int, bool in) {
@@ -906,6 +911,11 @@ __global__ void __launch_bounds__(kB) k_tile_pre(Dims d, const float* __restrict
     li = 0;
     tile_load<DIM>(d, f, [&](int, int, int, int l, bool) {
       rl[l] = rv[li++ % M::E];
+      zl[l] = 0.f;
+    });
+  } else {
+    tile_load<DIM>(d, f, [&](int i, int j, int, int l, bool in) {
+      rl[l] = in ? r[cidx(d, i, j, 0)] : 0.f;
       zl[l] = 0.f;
     });
   }
@@ -929,7 +939,7 @@ __global__ void __launch_bounds__(kB) k_tile_pre(Dims d, const float* __restrict
   constexpr int TC = M::T / 2, TCK = DIM == 3 ? TC : 1;
   constexpr int SI = M::E * M::EK, SJ = M::EK;
   const int kz = DIM == 3 ? 2 : 1;
-  for (int t = threadIdx.x; t < TC * TC * TCK; t += kB) {
+  for (int t = threadIdx.x; t < TC * TC * TCK; t += kTB) {
     const int ck = (f.o[2] + kTH) / 2 + (DIM == 3 ? t % TCK : 0);
     const int cj = (f.o[1] + kTH) / 2 + (t / TCK) % TC;
     const int ci = (f.o[0] + kTH) / 2 + t / (TCK * TC);
@@ -948,7 +958,7 @@ __global__ void __launch_bounds__(kB) k_tile_pre(Dims d, const float* __restrict
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kB) k_tile_post(Dims d, const float* __restrict__ r, const float* __restrict__ zin,
+__global__ void __launch_bounds__(kTB) k_tile_post(Dims d, const float* __restrict__ r, const float* __restrict__ zin,
                                                   const float* __restrict__ zc, Dims dc, float* __restrict__ zout,
                                                   const int* flags) {
 #pragma clang fp contract(off)
@@ -958,7 +968,7 @@ __global__ void __launch_bounds__(kB) k_tile_post(Dims d, const float* __restric
   __shared__ float pl[M::N];  // coarse correction at each cell (its parent's value)
   if (is_done(flags)) return;
   const TileFrame<DIM> f = tile_frame<DIM>(d, blockIdx.x);
-  {
+  if constexpr (DIM == 3) {  // stage in registers: every load issued before the first store
     float rv[M::E], zv[M::E], pv[M::E];
     int li = 0;
     tile_load<DIM>(d, f, [&](int i, int j, int k, int, bool in) {
@@ -966,7 +976,7 @@ __global__ void __launch_bounds__(kB) k_tile_post(Dims d, const float* __restric
       const uint32_t id = in ? cidx(d, i, j, k) : 0u;
       rv[q] = in ? r[id] : 0.f;
       zv[q] = in ? zin[id] : 0.f;
-      pv[q] = in ? zc[cidx(dc, i >> 1, j >> 1, DIM == 3 ? (k >> 1) : 0)] : 0.f;
+      pv[q] = in ? zc[cidx(dc, i >> 1, j >> 1, k >> 1)] : 0.f;
     });
     li = 0;
     tile_load<DIM>(d, f, [&](int, int, int, int l, bool) {
@@ -974,6 +984,13 @@ __global__ void __launch_bounds__(kB) k_tile_post(Dims d, const float* __restric
       rl[l] = rv[q];
       zl[l] = zv[q];
       pl[l] = pv[q];
+    });
+  } else {
+    tile_load<DIM>(d, f, [&](int i, int j, int, int l, bool in) {
+      const uint32_t id = in ? cidx(d, i, j, 0) : 0u;
+      rl[l] = in ? r[id] : 0.f;
+      zl[l] = in ? zin[id] : 0.f;
+      pl[l] = in ? zc[cidx(dc, i >> 1, j >> 1, 0)] : 0.f;
     });
   }
   __syncthreads();
@@ -1451,7 +1468,7 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
     const double* sh = l == 0 ? shift0 : nullptr;
     if (!MASK && tiled_level(s, l)) {
       cur[l] = s.lv[l].z;
-      hipLaunchKernelGGL((k_tile_pre<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kB), 0, st, s.ld[l],
+      hipLaunchKernelGGL((k_tile_pre<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kTB), 0, st, s.ld[l],
                          (const float*)s.lv[l].r, cur[l], s.ld[l + 1], s.lv[l + 1].r, (const int*)s.flags);
       continue;
     }
@@ -1485,7 +1502,7 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
     const double* sh = l == 0 ? shift0 : nullptr;
     if (!MASK && tiled_level(s, l)) {
       float* out = cur[l] == s.lv[l].z ? s.lv[l].z2 : s.lv[l].z;
-      hipLaunchKernelGGL((k_tile_post<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kB), 0, st, s.ld[l],
+      hipLaunchKernelGGL((k_tile_post<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kTB), 0, st, s.ld[l],
                          (const float*)s.lv[l].r, (const float*)cur[l], (const float*)cur[l + 1], s.ld[l + 1], out,
                          (const int*)s.flags);
       cur[l] = out;
