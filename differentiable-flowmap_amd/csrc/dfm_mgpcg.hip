@@ -935,24 +935,38 @@ __global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const float* __restric
     __syncthreads();
   }
   tile_pass<DIM>(d, f, 0, -1, [&](int i, int j, int k, int l) { z[cidx(d, i, j, k)] = zl[l]; });
-  // restriction of the tile's interior (k_restrict): one thread per coarse cell
-  constexpr int TC = M::T / 2, TCK = DIM == 3 ? TC : 1;
+  // restriction of the tile's interior (k_restrict): one thread per fine
+  // cell forms its residual term, then one thread per coarse cell adds its
+  // children's terms in k_restrict's order (i, j, k offsets, k fastest)
+  constexpr int TC = M::T / 2, TCK = DIM == 3 ? TC : 1, KZ = DIM == 3 ? 2 : 1;
   constexpr int SI = M::E * M::EK, SJ = M::EK;
-  const int kz = DIM == 3 ? 2 : 1;
-  for (int t = threadIdx.x; t < TC * TC * TCK; t += kTB) {
+  constexpr int NC = TC * TC * TCK, CH = 4 * KZ;
+  static_assert(NC * CH <= kTB, "one thread per fine cell of the tile");
+  __syncthreads();  // zl is final; rl slots of the interior are reused below
+  float term = 0.f;
+  const int t = threadIdx.x;
+  if (t < NC * CH) {
+    const int cc = t / CH, ch = t % CH;  // coarse cell, child (a, b, c) = (ch / (2 KZ), ch / KZ % 2, ch % KZ)
+    const int a = ch / (2 * KZ), b = (ch / KZ) % 2, c = ch % KZ;
+    const int ck = (f.o[2] + kTH) / 2 + (DIM == 3 ? cc % TCK : 0);
+    const int cj = (f.o[1] + kTH) / 2 + (cc / TCK) % TC;
+    const int ci = (f.o[0] + kTH) / 2 + cc / (TCK * TC);
+    const int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
+    const int l = (i - f.o[0]) * SI + (j - f.o[1]) * SJ + (DIM == 3 ? k - f.o[2] : 0);
+    float nn;
+    const float s = tile_nbr_sum<DIM>(d, i, j, k, l, nn, [&](int q) { return zl[q]; });
+    term = (rl[l] - m) - (nn * zl[l] - s);
+  }
+  __syncthreads();
+  float* terms = rl;  // NC * CH <= kTB floats
+  if (t < NC * CH) terms[t] = term;
+  __syncthreads();
+  if (t < NC) {
+    float acc = 0.f;
+    for (int q = 0; q < CH; ++q) acc += terms[t * CH + q];
     const int ck = (f.o[2] + kTH) / 2 + (DIM == 3 ? t % TCK : 0);
     const int cj = (f.o[1] + kTH) / 2 + (t / TCK) % TC;
     const int ci = (f.o[0] + kTH) / 2 + t / (TCK * TC);
-    float acc = 0.f;
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b)
-        for (int c = 0; c < kz; ++c) {
-          const int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
-          const int l = (i - f.o[0]) * SI + (j - f.o[1]) * SJ + (DIM == 3 ? k - f.o[2] : 0);
-          float nn;
-          const float s = tile_nbr_sum<DIM>(d, i, j, k, l, nn, [&](int q) { return zl[q]; });
-          acc += (rl[l] - m) - (nn * zl[l] - s);
-        }
     rc[cidx(dc, ci, cj, DIM == 3 ? ck : 0)] = acc * (1.0f / (float)(DIM - 1));
   }
 }
