@@ -230,7 +230,8 @@ def run_batch(args):
     from dfm.batch import BatchedLFM, make_problems
 
     probs = make_problems(count=32, seed=100, steps=args.batch_steps, reinit_every=12)
-    b = BatchedLFM(probs, workers=1)
+    b = BatchedLFM(probs, workers=1, overlap_march=args.batch_overlap in ("march", "both"),
+                   overlap_projections=args.batch_overlap in ("projections", "both"))
     tail = b.autotune_tail()
     b.run(12)  # untimed: builds every handle and graph
     seq, t_seq = b.run()
@@ -295,6 +296,8 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="the batched configuration: 32 uneven 2D instances, aggregate steps/s")
     ap.add_argument("--batch-steps", type=int, default=24)
+    ap.add_argument("--batch-overlap", default="projections", choices=["none", "projections", "march", "both"],
+                    help="per-instance side streams in the batch (the instances already overlap each other)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for --batch (1..32; 0 keeps the platform's setting: 16 measured slower than 4)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -63,7 +63,11 @@ class BatchedLFM:
     def __init__(self, problems, workers=4, device="cuda", **sim_kw):
         self.problems = list(problems)
         self.device = device
-        self.sim_kw = dict(tol=1e-6, max_iters=200)
+        # the instances already fill the hardware queues; a per-instance side
+        # stream for the marches only adds false dependencies between them
+        # (measured: 462 vs 568 instance-steps/s), the overlapped projection
+        # pair still helps (568 vs 544)
+        self.sim_kw = dict(tol=1e-6, max_iters=200, overlap_march=False)
         self.sim_kw.update(sim_kw)
         self.sims = {}
         self.set_workers(workers)
