@@ -50,6 +50,9 @@ constexpr double kEps = 1e-12;   // denominator guard (2D/vortex/mgpcg.py:204)
 
 enum Slot { S_ZR = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THRESH, S_MEAN, S_PAP, S_NFLUID, S_NSLOT };
 enum Flag { F_DONE = 0, F_ITERS, F_STATUS, F_NTRUE, F_NFLAG };  // F_NTRUE: failed true-residual checks
+// The scalars and the flags share one device block (scalars first) and one
+// pinned host mirror, so one copy reads both back.
+constexpr size_t kCtlBytes = sizeof(double) * S_NSLOT + sizeof(int) * F_NFLAG;
 
 struct Lvl {
   int n[3];
@@ -1604,7 +1607,7 @@ int pcg_queue(Solver& s, int k) {
   if (P.refinements < s.max_refine)
     hipLaunchKernelGGL((k_true_resid<DIM, MASK>), dim3(gridr(s.lv[0].size)), dim3(kB), 0, P.st, s.ld[0],
                        (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
-  if (hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, P.st) != hipSuccess)
+  if (hipMemcpyAsync(s.h_sc, s.sc, kCtlBytes, hipMemcpyDeviceToHost, P.st) != hipSuccess)
     return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
   return DFM_OK;
 }
@@ -1665,9 +1668,11 @@ int pcg_end(Solver& s) {
     int rc = pcg_queue<DIM, MASK>(s, k);
     if (rc != DFM_OK) return rc;
   }
-  bool rb = hipMemcpyAsync(s.h_sc, s.sc, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (!done) rb = rb && hipMemcpyAsync(s.h_flags, s.flags, sizeof(int) * F_NFLAG, hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (!rb || hipStreamSynchronize(st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: status readback failed");
+  // the scalars came back with the flags of the last synchronised batch; only
+  // a solve that stopped at max_iters after queueing a restart reads again
+  if (!done && (hipMemcpyAsync(s.h_sc, s.sc, kCtlBytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess))
+    return set_error(DFM_E_HIP, "mgpcg: status readback failed");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg launch", hipGetErrorString(e));
   s.last_iters = s.h_flags[F_ITERS];
@@ -1703,9 +1708,7 @@ void destroy(Solver* s) {
   if (s->ctr) (void)hipFree(s->ctr);
   for (int l = 0; l < kMaxLevels; ++l)
     if (s->bm[l]) (void)hipFree(s->bm[l]);
-  if (s->sc) (void)hipFree(s->sc);
-  if (s->flags) (void)hipFree(s->flags);
-  if (s->h_flags) (void)hipHostFree(s->h_flags);
+  if (s->sc) (void)hipFree(s->sc);  // flags live in the same block
   if (s->h_sc) (void)hipHostFree(s->h_sc);
   drop_graph(*s);
   if (s->cap) (void)hipStreamDestroy(s->cap);
@@ -1765,14 +1768,15 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
        hipMalloc(&s->Ap, sizeof(float) * N) == hipSuccess &&
        hipMalloc(&s->part, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
        hipMalloc(&s->ctr, sizeof(unsigned) * kCtrWords) == hipSuccess &&
-       hipMalloc(&s->sc, sizeof(double) * S_NSLOT) == hipSuccess &&
-       hipMalloc(&s->flags, sizeof(int) * F_NFLAG) == hipSuccess &&
-       hipHostMalloc(&s->h_flags, sizeof(int) * F_NFLAG, hipHostMallocDefault) == hipSuccess &&
-       hipHostMalloc(&s->h_sc, sizeof(double) * S_NSLOT, hipHostMallocDefault) == hipSuccess;
+       hipMalloc(&s->sc, kCtlBytes) == hipSuccess &&
+       hipHostMalloc(&s->h_sc, kCtlBytes, hipHostMallocDefault) == hipSuccess;
+  if (ok) {
+    s->flags = reinterpret_cast<int*>(reinterpret_cast<char*>(s->sc) + sizeof(double) * S_NSLOT);
+    s->h_flags = reinterpret_cast<int*>(reinterpret_cast<char*>(s->h_sc) + sizeof(double) * S_NSLOT);
+  }
   if (ok) ok = hipMemset(s->part, 0, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
                hipMemset(s->ctr, 0, sizeof(unsigned) * kCtrWords) == hipSuccess &&
-               hipMemset(s->sc, 0, sizeof(double) * S_NSLOT) == hipSuccess &&
-               hipMemset(s->flags, 0, sizeof(int) * F_NFLAG) == hipSuccess &&
+               hipMemset(s->sc, 0, kCtlBytes) == hipSuccess &&
                hipMemset(s->x, 0, sizeof(double) * N) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     destroy(s);
