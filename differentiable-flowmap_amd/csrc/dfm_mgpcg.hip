@@ -1732,6 +1732,9 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
     while ((1 << (lg + 1)) <= nmin) ++lg;
     levels = lg - 3 + 1;
     if (levels < 1) levels = 1;
+    // no more levels than the grid halves exactly into (e.g. 100 = 4 x 25: 3)
+    for (int a = 0; a < g->dim; ++a)
+      while (levels > 1 && g->n[a] % (1 << (levels - 1)) != 0) --levels;
   }
   if (levels > kMaxLevels) return set_error(DFM_E_ARG, "too many levels");
   for (int a = 0; a < g->dim; ++a)

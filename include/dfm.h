@@ -145,7 +145,8 @@ int dfm_axpy_mac(const dfm_grid* g, const dfm_mac* x, float alpha, const dfm_mac
 /* ---------------------------------------------------------------- MGPCG --
  * Handle owning the multigrid workspaces for one grid.  Levels, smoothing and
  * bottom sweeps follow MGPCG (2D/vortex/mgpcg.py:17-31): levels <= 0 picks
- * log2(min n) - 3 + 1.  Requires every n divisible by 2^(levels-1). */
+ * log2(min n) - 3 + 1, lowered until every n halves exactly that often;
+ * explicit levels require every n divisible by 2^(levels-1). */
 int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int bottom, void** handle);
 int dfm_mgpcg_destroy(void* handle);
 

@@ -396,6 +396,23 @@ def test_mgpcg_fused_sweeps_bitwise(n, levels, tail, masked, device):
                 assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n", [(100, 60), (48, 36, 20)])
+def test_mgpcg_auto_levels_on_grids_that_halve_few_times(n, device):
+    """Automatic levels stop where the grid no longer halves exactly (100 =
+    4 x 25 gives 3 levels), and the projection still matches the exact one."""
+    from dfm.ops import MGPCG
+
+    dim = len(n)
+    rng = np.random.default_rng(3)
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(dim)])
+    s = MGPCG(n, 1.0 / n[1], tol=1e-7, max_iters=300)
+    d = mac_to_dev(u, device)
+    s.Poisson(d)
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    assert rel_l2_fields(mac_to_np(d), ref) < 2e-6
+
+
 def test_mgpcg_solve_and_status(device):
     from dfm.ops import MGPCG
 
