@@ -62,6 +62,27 @@ DFM_HD int64_t xcd_block(int64_t bid, int64_t nblocks) {
   return (bid & 7) * (nb8 >> 3) + (bid >> 3);
 }
 
+// Grid-stride walk over [0, n) for kernels whose grid may be smaller than the
+// index space: the blocks on one XCD sweep one contiguous slab of it (n / 8,
+// rounded up to whole blocks), so the neighbouring planes a stencil reads are
+// the ones that XCD's L2 has just loaded.  for (id = first; id < end; id +=
+// stride).  Grids that are not a multiple of 8 blocks walk the plain way.
+template <typename I>
+struct SlabWalk {
+  I first, end, stride;
+};
+
+template <int BLOCK, typename I = uint32_t>
+__device__ inline SlabWalk<I> slab_walk(I n) {
+  const I G = gridDim.x;
+  if (G & 7) return SlabWalk<I>{(I)blockIdx.x * BLOCK + threadIdx.x, n, G * BLOCK};
+  I chunk = (n + 7) / 8;
+  chunk = (chunk + BLOCK - 1) / BLOCK * BLOCK;
+  const I lo = (I)(blockIdx.x & 7) * chunk;
+  const I hi = lo + chunk < n ? lo + chunk : n;
+  return SlabWalk<I>{lo + (I)(blockIdx.x >> 3) * BLOCK + threadIdx.x, hi, (G >> 3) * BLOCK};
+}
+
 // ----------------------------------------------------------- B-splines ----
 // Piecewise polynomials of the uniform B-splines of order 2 (quadratic,
 // support 1.5), 3 (cubic, support 2) and 4 (quartic, support 2.5), written

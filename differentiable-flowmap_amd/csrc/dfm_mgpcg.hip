@@ -38,6 +38,8 @@ namespace {
 
 constexpr int kB = 256;          // threads per block
 constexpr int kRedBlocks = 2048;  // most blocks of a reducing kernel (grid-stride beyond)
+constexpr int kUnroll = 4;        // cells per step of a long reducing stencil walk, loads in flight together
+constexpr unsigned kFluidBit = 1u << 31;  // nbr_load masks: marks a fluid cell
 // Coarse levels of at most this many cells run each RBGS sweep as one launch
 // (k_sweep2); their half-sweeps are launch-latency bound.
 constexpr int64_t kSweep2MaxCells = int64_t(1) << 19;
@@ -133,6 +135,27 @@ DFM_HD float nbr_count(const Dims& d, int i, int j, int k) {
   float nn = 0.f;
   for_nbrs<DIM, MASK>(d, i, j, k, cidx(d, i, j, k), [&](uint32_t, int, int, int) { nn += 1.f; });
   return nn;
+}
+
+// The 2*DIM neighbour values of cell (i, j, k) in for_nbrs order, loaded
+// before any is used (v[t] = 0 where absent); bit t of the result is set for
+// each neighbour present (in the grid and, MASK, fluid).  Walks that keep
+// several cells' loads in flight use it instead of for_nbrs.
+template <int DIM, bool MASK, typename T>
+DFM_HD unsigned nbr_load(const Dims& d, const T* __restrict__ f, int i, int j, int k, uint32_t id, T v[2 * DIM]) {
+  const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
+  const bool in[6] = {i > 0, i < d.n[0] - 1, j > 0, j < d.n[1] - 1, k > 0, k < d.n[2] - 1};
+  const uint32_t q[6] = {id - sx, id + sx, id - sy, id + sy, id - 1, id + 1};
+  unsigned m = 0;
+#pragma unroll
+  for (int t = 0; t < 2 * DIM; ++t) {
+    v[t] = T(0);
+    if (in[t] && is_fluid<MASK>(d, q[t])) {
+      v[t] = f[q[t]];
+      m |= 1u << t;
+    }
+  }
+  return m;
 }
 
 // Cell index -> (i, j, k); grids stay below 2^31 cells (checked at creation).
@@ -344,7 +367,8 @@ __global__ void __launch_bounds__(kB) k_div(Dims d, Faces u, float* __restrict__
   double acc = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
   const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) {
     int i, j, k;
     cell_of(d, id, i, j, k);
     // face grids: x (n0+1, n1, n2), y (n0, n1+1, n2), z (n0, n1, n2+1)
@@ -386,7 +410,8 @@ template <bool MASK>
 __global__ void __launch_bounds__(kB) k_load_b(int64_t N, const float* __restrict__ b, float* __restrict__ r,
                                                const uint8_t* __restrict__ bm, Red R) {
   double acc = 0;
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) {
     float v = (!MASK || bm[id] == 0) ? b[id] : 0.f;
     r[id] = v;
     acc += (double)v;
@@ -402,7 +427,8 @@ __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restri
                                                       Red R) {
   float m = (float)R.sc[S_MEAN];
   double acc = 0;
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) {
     float v = (!MASK || bm[id] == 0) ? r[id] - m : 0.f;
     r[id] = v;
     bkeep[id] = v;
@@ -415,47 +441,41 @@ __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restri
 // subtract_grad_p + apply_bc (2D/vortex/mgpcg.py:338-356, 3D/mgpcg.py:413-439;
 // obstacle form 2D/vortex/mgpcg_solid.py:433-461): interior faces
 // u_a -= p[I] - p[I - e_a]; wall faces and (MASK) faces of solid cells = 0.
-struct FaceDims {
-  Dims f[3];  // face grid of axis a (cells + e_a)
-};
-
+// One thread per cell updates the cell's low face on every axis (and the
+// wall face at the high end of an axis it closes), so p, in FP64, is read
+// once per cell rather than once per face grid.  Face (i, j, k) of axis a
+// lies in the (n + e_a) face grid at: x, the cell's own index; y,
+// id + i n2; z, id + i n1 + j.
 template <int DIM, bool MASK>
-__global__ void __launch_bounds__(kB) k_subgrad(Dims d, FaceDims fdims, Faces u, const double* __restrict__ p) {
-  int64_t nf[3];
-  const Dims* fd = fdims.f;
-  for (int a = 0; a < 3; ++a) nf[a] = (a < DIM) ? (int64_t)fd[a].n[0] * fd[a].n[1] * fd[a].n[2] : 0;
-  int64_t total = nf[0] + nf[1] + nf[2];
-  for (int64_t g = (int64_t)blockIdx.x * kB + threadIdx.x; g < total; g += (int64_t)gridDim.x * kB) {
-    int a = 0;
-    int64_t id = g;
-    if (id >= nf[0]) { id -= nf[0]; a = 1; if (id >= nf[1]) { id -= nf[1]; a = 2; } }
-    int i, j, k;
-    cell_of(fd[a], (uint32_t)id, i, j, k);
-    int I[3] = {i, j, k};
-    float* c = u.c[a];
-    int lo[3] = {i, j, k};
-    lo[a] -= 1;
-    bool open = !(I[a] == 0 || I[a] == d.n[a]);
-    uint32_t chi = 0, clo = 0;
-    if (open) {
-      chi = cidx(d, i, j, k);
-      clo = cidx(d, lo[0], lo[1], lo[2]);
-      open = is_fluid<MASK>(d, chi) && is_fluid<MASK>(d, clo);
-    }
-    if (!open) {
-      c[id] = 0.f;
-    } else {
-      // the pressure difference is formed in FP64: p grows like n |u| in
-      // these units, so an FP32 p would cost ~n ulps of velocity
-      c[id] = (float)((double)c[id] - (p[chi] - p[clo]));
-    }
+__global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const double* __restrict__ p) {
+  const int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
+  const uint32_t id = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  if (id >= N) return;
+  int i, j, k;
+  cell_of(d, id, i, j, k);
+  const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
+  const bool fl = is_fluid<MASK>(d, id);
+  const double pc = p[id];
+  // the pressure difference is formed in FP64: p grows like n |u| in these
+  // units, so an FP32 p would cost ~n ulps of velocity
+  auto face = [&](float* __restrict__ c, uint32_t f, bool inner, uint32_t lo, bool high_wall, uint32_t f_high) {
+    const bool open = inner && fl && is_fluid<MASK>(d, lo);
+    c[f] = open ? (float)((double)c[f] - (pc - p[lo])) : 0.f;
+    if (high_wall) c[f_high] = 0.f;
+  };
+  face(u.c[0], id, i > 0, id - sx, i == d.n[0] - 1, id + sx);
+  const uint32_t fy = id + (uint32_t)i * sy;
+  face(u.c[1], fy, j > 0, id - sy, j == d.n[1] - 1, fy + sy);
+  if (DIM == 3) {
+    const uint32_t fz = id + (uint32_t)i * (uint32_t)d.n[1] + (uint32_t)j;
+    face(u.c[2], fz, k > 0, id - 1, k == d.n[2] - 1, fz + 1);
   }
 }
 
 // Iterative-refinement residual r = b - A x evaluated in FP64 (x is FP64;
 // b is the recentred FP32 right-hand side); partial |r|^2 and sum r.
 // Solid cells: r = 0 (b and the operator row are 0 there).
-template <int DIM, bool MASK>
+template <int DIM, bool MASK, int U>
 __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restrict__ x, const float* __restrict__ b,
                                                    float* __restrict__ r, Red R, int if_converged) {
   // only after convergence on the recursive residual (uniform over the grid:
@@ -463,57 +483,109 @@ __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restr
   if (if_converged && !(R.flags[F_DONE] && R.flags[F_STATUS] == 1)) return;
   double rr = 0, rs = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
-    int i, j, k;
-    cell_of(d, id, i, j, k);
-    double v = 0.0;
-    if (is_fluid<MASK>(d, id)) {
-      double s = 0.0, nn = 0.0;
-      for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int, int, int) {
-        s += x[q];
-        nn += 1.0;
-      });
-      v = (double)b[id] - (nn * x[id] - s);
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  // U cells of the walk per step, all their loads issued before any is
+  // used; cells are still consumed in walk order, so U does not change
+  // the result
+  for (uint32_t id0 = w.first; id0 < w.end; id0 += U * w.stride) {
+    double xn[U][2 * DIM], xv[U];
+    float bv[U];
+    unsigned has[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t id = id0 + u * w.stride;
+      has[u] = 0;
+      xv[u] = 0.0;
+      bv[u] = 0.f;
+      if (id < w.end && is_fluid<MASK>(d, id)) {
+        int i, j, k;
+        cell_of(d, id, i, j, k);
+        has[u] = nbr_load<DIM, MASK>(d, x, i, j, k, id, xn[u]) | kFluidBit;
+        xv[u] = x[id];
+        bv[u] = b[id];
+      }
     }
-    r[id] = (float)v;
-    rr += v * v;
-    rs += v;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t id = id0 + u * w.stride;
+      if (id < w.end) {
+        double v = 0.0;
+        if (has[u]) {
+          double s = 0.0, nn = 0.0;
+#pragma unroll
+          for (int t = 0; t < 2 * DIM; ++t)
+            if (has[u] >> t & 1u) {
+              s += xn[u][t];
+              nn += 1.0;
+            }
+          v = (double)bv[u] - (nn * xv[u] - s);
+        }
+        r[id] = (float)v;
+        rr += v * v;
+        rs += v;
+      }
+    }
   }
   grid_reduce(R, FIN_TRUE, rr, rs);
 }
 
 // FP64 solution -> FP32 output
 __global__ void __launch_bounds__(kB) k_to_float(int64_t N, const double* __restrict__ x, float* __restrict__ y) {
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) y[id] = (float)x[id];
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) y[id] = (float)x[id];
 }
 
 // ------------------------------------------------------------ PCG steps ----
 // Ap = A p ; partial p.Ap.  MASK: solid rows are 0 (compute_Ap of the
 // obstacle solver updates fluid cells only, 2D/vortex/mgpcg_solid.py:172-178).
-template <int DIM, bool MASK>
+template <int DIM, bool MASK, int U>
 __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__ p, float* __restrict__ Ap,
                                                Red R) {
   if (is_done(R.flags)) return;
   double acc = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
-    int i, j, k;
-    cell_of(d, id, i, j, k);
-    float v = 0.f, pv = 0.f;
-    if (is_fluid<MASK>(d, id)) {
-      // formed in FP64 and rounded once: for smooth p, nn p - sum(p) cancels
-      // all but ~(pi/n)^2 of |p|, and the FP32 form's error then drifts the
-      // recursive residual away from b - A x by up to ~1e-2 |b| at 512^2
-      double s = 0.0, nn = 0.0;
-      for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int, int, int) {
-        s += (double)p[q];
-        nn += 1.0;
-      });
-      pv = p[id];
-      v = (float)(nn * (double)pv - s);
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  // U cells of the walk per step, all their loads issued before any is
+  // used; cells are still consumed in walk order, so U does not change
+  // the result
+  for (uint32_t id0 = w.first; id0 < w.end; id0 += U * w.stride) {
+    float pn[U][2 * DIM], pc[U];
+    unsigned has[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t id = id0 + u * w.stride;
+      has[u] = 0;
+      pc[u] = 0.f;
+      if (id < w.end && is_fluid<MASK>(d, id)) {
+        int i, j, k;
+        cell_of(d, id, i, j, k);
+        has[u] = nbr_load<DIM, MASK>(d, p, i, j, k, id, pn[u]) | kFluidBit;
+        pc[u] = p[id];
+      }
     }
-    Ap[id] = v;
-    acc += (double)pv * (double)v;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t id = id0 + u * w.stride;
+      if (id < w.end) {
+        float v = 0.f;
+        if (has[u]) {
+          // formed in FP64 and rounded once: for smooth p, nn p - sum(p)
+          // cancels all but ~(pi/n)^2 of |p|, and the FP32 form's error then
+          // drifts the recursive residual away from b - A x by up to
+          // ~1e-2 |b| at 512^2
+          double s = 0.0, nn = 0.0;
+#pragma unroll
+          for (int t = 0; t < 2 * DIM; ++t)
+            if (has[u] >> t & 1u) {
+              s += (double)pn[u][t];
+              nn += 1.0;
+            }
+          v = (float)(nn * (double)pc[u] - s);
+        }
+        Ap[id] = v;
+        acc += (double)pc[u] * (double)v;
+      }
+    }
   }
   grid_reduce(R, FIN_ALPHA, acc, 0.0);
 }
@@ -527,7 +599,8 @@ __global__ void __launch_bounds__(kB) k_update_xr(int64_t N, double* __restrict_
   const double alpha_d = R.sc[S_ALPHA];
   float alpha = (float)alpha_d;
   double rr = 0, rs = 0;
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) {
     x[id] = fma(alpha_d, (double)p[id], x[id]);
     float v = fmaf(-alpha, Ap[id], r[id]);
     r[id] = v;
@@ -548,7 +621,8 @@ __global__ void __launch_bounds__(kB) k_zr(int64_t N, const float* __restrict__ 
   if (is_done(R.flags)) return;
   float m = shift ? (float)*shift : 0.f;
   double acc = 0;
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) {
     const bool fl = !MASK || bm[id] == 0;
     float rv = fl ? r[id] - m : 0.f;
     if (shift && fl) r[id] = rv;
@@ -564,7 +638,8 @@ __global__ void __launch_bounds__(kB) k_update_p(int64_t N, const float* __restr
                                                  const double* sc, const int* flags) {
   if (is_done(flags)) return;
   float beta = (float)sc[S_BETA];
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB)
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride)
     p[id] = fmaf(beta, p[id], z[id]);
 }
 
@@ -675,7 +750,8 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
   const int inner = DIM == 3 ? d.n[2] : d.n[1];
   const uint32_t pairs = (uint32_t)(DIM == 3 ? d.n[0] * d.n[1] : d.n[0]) * d.fh.d;
   double acc = 0;
-  for (uint32_t t = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x; t < pairs; t += gridDim.x * kB) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>(pairs);
+  for (uint32_t t = w.first; t < w.end; t += w.stride) {
     int i, j, own, other;
     colour_pair<DIM>(d, t, phase, i, j, own, other);
     if (own < inner) {
@@ -1036,7 +1112,8 @@ __global__ void __launch_bounds__(kTB) k_tile_post(Dims d, const float* __restri
 __global__ void __launch_bounds__(kB) k_mask_level0(int64_t N, const float* __restrict__ mask,
                                                     uint8_t* __restrict__ bm, Red R) {
   double fluid = 0;
-  for (uint32_t id = blockIdx.x * kB + threadIdx.x; id < (uint32_t)N; id += gridDim.x * kB) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) {
     const uint8_t sld = mask[id] > 0.f ? 1 : 0;
     bm[id] = sld;
     fluid += sld ? 0.0 : 1.0;
@@ -1390,6 +1467,11 @@ inline unsigned gridr(int64_t n) {
   return (unsigned)(b < kRedBlocks ? b : kRedBlocks);
 }
 
+// Whether a reducing walk over n cells gives each thread more than kUnroll
+// cells (then the stencil walks keep kUnroll cells' loads in flight; below
+// that the unrolled form only costs occupancy).
+inline bool long_walk(int64_t n) { return n > (int64_t)kRedBlocks * kB * kUnroll; }
+
 inline Red red_of(const Solver& s, double tol = 0.0, int relative = 0) {
   const double n = s.masked ? (s.nfluid > 0 ? s.nfluid : 1.0) : (double)s.lv[0].size;
   return Red{s.part, s.ctr, s.sc, s.flags, 1.0 / n, tol, relative};
@@ -1547,7 +1629,10 @@ void pcg_iteration(Solver& s, hipStream_t st) {
   int64_t N = L0.size;
   unsigned gr = gridr(N);
   const Red R = red_of(s);
-  hipLaunchKernelGGL((k_Ap_dot<DIM, MASK>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
+  if (long_walk(N))
+    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, kUnroll>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
+  else
+    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, 1>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
   hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, (const float*)s.p, (const float*)s.Ap, R);
   vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_BETA, nullptr});
   hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, (const float*)s.z0_final, s.p, (const double*)s.sc,
@@ -1604,9 +1689,14 @@ int pcg_queue(Solver& s, int k) {
   P.it += k;
   // Converged on the recursive FP32 residual: measure the true residual
   // b - A x in FP64; if it misses the tolerance, PCG restarts from it.
-  if (P.refinements < s.max_refine)
-    hipLaunchKernelGGL((k_true_resid<DIM, MASK>), dim3(gridr(s.lv[0].size)), dim3(kB), 0, P.st, s.ld[0],
-                       (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
+  if (P.refinements < s.max_refine) {
+    if (long_walk(s.lv[0].size))
+      hipLaunchKernelGGL((k_true_resid<DIM, MASK, kUnroll>), dim3(gridr(s.lv[0].size)), dim3(kB), 0, P.st, s.ld[0],
+                         (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
+    else
+      hipLaunchKernelGGL((k_true_resid<DIM, MASK, 1>), dim3(gridr(s.lv[0].size)), dim3(kB), 0, P.st, s.ld[0],
+                         (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
+  }
   if (hipMemcpyAsync(s.h_sc, s.sc, kCtlBytes, hipMemcpyDeviceToHost, P.st) != hipSuccess)
     return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
   return DFM_OK;
@@ -1915,23 +2005,14 @@ extern "C" int dfm_mgpcg_project_end(void* handle) {
   Faces F = s->pend.F;
   Lvl& L0 = s->lv[0];
   const Dims& d0 = s->ld[0];
-  int64_t nf = 0;
-  for (int a = 0; a < s->G.dim; ++a) nf += (L0.size / L0.n[a]) * (L0.n[a] + 1);
-  unsigned gf = (unsigned)((nf + kB - 1) / kB);
-  if (gf > 65535u * 8u) gf = 65535u * 8u;
-  FaceDims fd;
-  for (int a = 0; a < 3; ++a) {
-    int n[3] = {L0.n[0], L0.n[1], L0.n[2]};
-    if (a < s->G.dim) n[a] += 1;
-    fd.f[a] = make_dims(n[0], n[1], n[2], s->G.dim);
-  }
+  const unsigned gc = grid1(L0.size);  // one thread per cell
   const double* px = s->x;
   if (s->G.dim == 2) {
-    if (m) hipLaunchKernelGGL((k_subgrad<2, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
-    else hipLaunchKernelGGL((k_subgrad<2, false>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+    if (m) hipLaunchKernelGGL((k_subgrad<2, true>), dim3(gc), dim3(kB), 0, st, d0, F, px);
+    else hipLaunchKernelGGL((k_subgrad<2, false>), dim3(gc), dim3(kB), 0, st, d0, F, px);
   } else {
-    if (m) hipLaunchKernelGGL((k_subgrad<3, true>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
-    else hipLaunchKernelGGL((k_subgrad<3, false>), dim3(gf), dim3(kB), 0, st, d0, fd, F, px);
+    if (m) hipLaunchKernelGGL((k_subgrad<3, true>), dim3(gc), dim3(kB), 0, st, d0, F, px);
+    else hipLaunchKernelGGL((k_subgrad<3, false>), dim3(gc), dim3(kB), 0, st, d0, F, px);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg subtract_grad_p", hipGetErrorString(e));
