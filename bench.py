@@ -91,7 +91,7 @@ def run_gpu(args):
                              mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
                              mg_bf16_coarse=args.mg_bf16_coarse, mg_bottom_exact=args.mg_bottom_exact,
                              mg_check_every=args.mg_check_every, overlap_march=False if args.overlap_march == "none" else args.overlap_march,
-                             mg_fused_sweeps=args.mg_fused_sweeps)
+                             mg_fused_sweeps=args.mg_fused_sweeps, mg_split_layout=not args.mg_natural_layout)
     u0, s0, tgt = sc["u0"], sc["density"], sc["target"]
 
     def forward_adjoint(steps):
@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--mg-pre-post", type=int, default=2, help="RBGS sweeps before and after each coarse correction")
     ap.add_argument("--mg-fused-sweeps", type=int, default=2, choices=[0, 1, 2],
                     help="coarse-level smoothing: 2 LDS-tiled, 1 one launch per sweep, 0 two per sweep")
+    ap.add_argument("--mg-natural-layout", action="store_true",
+                    help="finest MG level in natural cell order (default: colour-split red/black halves)")
     ap.add_argument("--overlap-march", default="both", choices=["none", "forward", "adjoint", "both"],
                     help="which steps run their flow-map marches on a side stream beside the projections")
     ap.add_argument("--mg-levels", type=int, default=0,

@@ -70,6 +70,13 @@ struct Dims {
   FastDiv f1;  // / n[1]
   FastDiv fh;  // / cell pairs along the innermost axis ((inner + 1) / 2)
   const uint8_t* bm;  // obstacle flags (1 = solid) of this level, or nullptr
+  // Colour-split storage (level 0 only; 0 = natural order): cell (i, j, k)
+  // of colour c = (i + j + k) & 1 and natural index id is stored at
+  // c * hh + id / 2, hh = cells / 2.  Needs an even innermost extent, so
+  // each row holds half its cells in each colour and id / 2 numbers a
+  // colour's cells densely.  A half-sweep then reads and writes only the
+  // cache lines of the colours it uses.
+  uint32_t hh;
 };
 
 inline Dims make_dims(int n0, int n1, int n2, int dim) {
@@ -82,6 +89,7 @@ inline Dims make_dims(int n0, int n1, int n2, int dim) {
   const int inner = dim == 3 ? n2 : n1;
   d.fh = make_fastdiv((uint32_t)((inner + 1) >> 1));
   d.bm = nullptr;
+  d.hh = 0;
   return d;
 }
 
@@ -92,12 +100,63 @@ DFM_HD uint32_t cidx(const Dims& d, int i, int j, int k) {
 
 template <bool MASK> DFM_HD bool is_fluid(const Dims& d, uint32_t id) { return !MASK || d.bm[id] == 0; }
 
+// Storage index of cell (i, j, k): natural, or colour-split (Dims::hh).
+template <bool SPLIT>
+DFM_HD uint32_t cell_at(const Dims& d, int i, int j, int k) {
+  const uint32_t id = cidx(d, i, j, k);
+  if (!SPLIT) return id;
+  return (((i + j + k) & 1) ? d.hh : 0u) + (id >> 1);
+}
+
+// Storage indices of the 2*DIM neighbours of colour-split cell (i, j, k) at
+// storage index id (order -x, +x, -y, +y, -z, +z) and whether each is in the
+// grid.  The neighbours have the other colour; along the innermost axis the
+// neighbour at natural index id_n +- 1 is at (id_n +- 1) / 2, which is the
+// cell's own half index plus (parity - 1) or parity.
+template <int DIM>
+DFM_HD void split_nbrs(const Dims& d, int i, int j, int k, uint32_t id, uint32_t q[6], bool in[6]) {
+  const uint32_t H = d.hh;
+  const bool c1 = id >= H;
+  const uint32_t m = c1 ? id - H : id;
+  const uint32_t o = (c1 ? 0u : H) + m;  // same half index, other colour
+  if (DIM == 3) {
+    const uint32_t hj = (uint32_t)d.n[2] >> 1, hi = (uint32_t)d.n[1] * hj, kp = (uint32_t)(k & 1);
+    q[0] = o - hi; q[1] = o + hi; q[2] = o - hj; q[3] = o + hj; q[4] = o - 1 + kp; q[5] = o + kp;
+    in[4] = k > 0;
+    in[5] = k < d.n[2] - 1;
+  } else {
+    const uint32_t hi = (uint32_t)d.n[1] >> 1, jp = (uint32_t)(j & 1);
+    q[0] = o - hi; q[1] = o + hi; q[2] = o - 1 + jp; q[3] = o + jp;
+    q[4] = q[5] = 0;
+    in[4] = in[5] = false;
+  }
+  in[0] = i > 0;
+  in[1] = i < d.n[0] - 1;
+  in[2] = j > 0;
+  in[3] = j < d.n[1] - 1;
+}
+
 // Visit the in-grid neighbours of cell (i, j, k) = id, restricted to fluid
 // cells when MASK (num_fluid_neighbors / neighbor_sum, 2D/vortex/mgpcg.py:92-115;
 // the obstacle form num_fluid_neighbors2 / neighbor_sum2,
 // 2D/vortex/mgpcg_solid.py:140-170), in the order -x, +x, -y, +y, -z, +z.
-template <int DIM, bool MASK, typename F>
+template <int DIM, bool MASK, bool SPLIT = false, typename F>
 DFM_HD void for_nbrs(const Dims& d, int i, int j, int k, uint32_t id, F&& fn) {
+  if constexpr (SPLIT) {
+    static_assert(!MASK, "the colour-split layout has no obstacle form");
+    uint32_t q[6];
+    bool in[6];
+    split_nbrs<DIM>(d, i, j, k, id, q, in);
+    if (in[0]) fn(q[0], i - 1, j, k);
+    if (in[1]) fn(q[1], i + 1, j, k);
+    if (in[2]) fn(q[2], i, j - 1, k);
+    if (in[3]) fn(q[3], i, j + 1, k);
+    if (DIM == 3) {
+      if (in[4]) fn(q[4], i, j, k - 1);
+      if (in[5]) fn(q[5], i, j, k + 1);
+    }
+    return;
+  }
   const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
   auto visit = [&](bool inside, uint32_t q, int a, int b, int c) {
     if (inside && is_fluid<MASK>(d, q)) fn(q, a, b, c);
@@ -113,11 +172,11 @@ DFM_HD void for_nbrs(const Dims& d, int i, int j, int k, uint32_t id, F&& fn) {
 }
 
 // neighbour sum and count
-template <int DIM, bool MASK = false>
+template <int DIM, bool MASK = false, bool SPLIT = false>
 DFM_HD float nbr_sum(const float* __restrict__ f, const Dims& d, int i, int j, int k, float& nn) {
   float s = 0.f;
   nn = 0.f;
-  for_nbrs<DIM, MASK>(d, i, j, k, cidx(d, i, j, k), [&](uint32_t q, int, int, int) {
+  for_nbrs<DIM, MASK, SPLIT>(d, i, j, k, cell_at<SPLIT>(d, i, j, k), [&](uint32_t q, int, int, int) {
     s += f[q];
     nn += 1.f;
   });
@@ -141,11 +200,17 @@ DFM_HD float nbr_count(const Dims& d, int i, int j, int k) {
 // before any is used (v[t] = 0 where absent); bit t of the result is set for
 // each neighbour present (in the grid and, MASK, fluid).  Walks that keep
 // several cells' loads in flight use it instead of for_nbrs.
-template <int DIM, bool MASK, typename T>
+template <int DIM, bool MASK, bool SPLIT, typename T>
 DFM_HD unsigned nbr_load(const Dims& d, const T* __restrict__ f, int i, int j, int k, uint32_t id, T v[2 * DIM]) {
-  const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
-  const bool in[6] = {i > 0, i < d.n[0] - 1, j > 0, j < d.n[1] - 1, k > 0, k < d.n[2] - 1};
-  const uint32_t q[6] = {id - sx, id + sx, id - sy, id + sy, id - 1, id + 1};
+  bool in[6];
+  uint32_t q[6];
+  if constexpr (SPLIT) {
+    split_nbrs<DIM>(d, i, j, k, id, q, in);
+  } else {
+    const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
+    in[0] = i > 0; in[1] = i < d.n[0] - 1; in[2] = j > 0; in[3] = j < d.n[1] - 1; in[4] = k > 0; in[5] = k < d.n[2] - 1;
+    q[0] = id - sx; q[1] = id + sx; q[2] = id - sy; q[3] = id + sy; q[4] = id - 1; q[5] = id + 1;
+  }
   unsigned m = 0;
 #pragma unroll
   for (int t = 0; t < 2 * DIM; ++t) {
@@ -166,6 +231,64 @@ DFM_HD void cell_of(const Dims& d, uint32_t id, int& i, int& j, int& k) {
   uint32_t q = fdiv(t, d.f1);
   j = (int)(t - q * (uint32_t)d.n[1]);
   i = (int)q;
+}
+
+// Pair t of cells along the innermost axis (cells 2h and 2h + 1 of a row):
+// (i, j) of the row, the innermost index `own` of its colour-`phase` cell and
+// `other` of the other cell.  Either may equal the row length (outside an
+// odd-sized row).
+template <int DIM>
+DFM_HD void colour_pair(const Dims& d, uint32_t t, int phase, int& i, int& j, int& own, int& other) {
+  const uint32_t half = d.fh.d;
+  uint32_t o = fdiv(t, d.fh);
+  int base = 2 * (int)(t - o * half);
+  int off;
+  if (DIM == 3) {
+    uint32_t q = fdiv(o, d.f1);
+    i = (int)q;
+    j = (int)(o - q * (uint32_t)d.n[1]);
+    off = (phase - i - j) & 1;
+  } else {
+    i = (int)o;
+    j = 0;
+    off = (phase - i) & 1;
+  }
+  own = base + off;
+  other = base + 1 - off;
+}
+
+// Colour-`phase` cell of pair index t; false when it falls outside an
+// odd-sized row.
+template <int DIM>
+DFM_HD bool colour_cell(const Dims& d, uint32_t t, int phase, int& i, int& j, int& k) {
+  int own, other;
+  colour_pair<DIM>(d, t, phase, i, j, own, other);
+  if (DIM == 3) {
+    k = own;
+    return k < d.n[2];
+  }
+  j = own;
+  k = 0;
+  return j < d.n[1];
+}
+
+// Storage index -> (i, j, k) in either layout.
+template <int DIM, bool SPLIT>
+DFM_HD void cell_of_at(const Dims& d, uint32_t id, int& i, int& j, int& k) {
+  if constexpr (SPLIT) {
+    const bool c1 = id >= d.hh;
+    (void)colour_cell<DIM>(d, c1 ? id - d.hh : id, c1 ? 1 : 0, i, j, k);
+  } else {
+    cell_of(d, id, i, j, k);
+  }
+}
+
+// Storage index of the cell with natural index id in the level's layout.
+DFM_HD uint32_t level0_at(const Dims& d, uint32_t id) {
+  if (!d.hh) return id;
+  int i, j, k;
+  cell_of(d, id, i, j, k);
+  return cell_at<true>(d, i, j, k);
 }
 
 // 1/nn for the neighbour counts that occur (1..6): one multiply instead of an
@@ -399,7 +522,7 @@ __global__ void __launch_bounds__(kB) k_div(Dims d, Faces u, float* __restrict__
       if (k == d.n[2] - 1 || solid) u.c[2][cidx(fz, i, j, k + 1)] = 0.f;
     }
     float bv = solid ? 0.f : -div;
-    b[id] = bv;
+    b[d.hh ? cell_at<true>(d, i, j, k) : id] = bv;  // level-0 layout
     acc += (double)bv;
   }
   grid_reduce(R, FIN_MEAN, acc, 0.0);
@@ -407,13 +530,13 @@ __global__ void __launch_bounds__(kB) k_div(Dims d, Faces u, float* __restrict__
 
 // copy b into r (for dfm_mgpcg_solve) with partial sum; solid cells get 0
 template <bool MASK>
-__global__ void __launch_bounds__(kB) k_load_b(int64_t N, const float* __restrict__ b, float* __restrict__ r,
+__global__ void __launch_bounds__(kB) k_load_b(Dims d, int64_t N, const float* __restrict__ b, float* __restrict__ r,
                                                const uint8_t* __restrict__ bm, Red R) {
   double acc = 0;
   const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
   for (uint32_t id = w.first; id < w.end; id += w.stride) {
     float v = (!MASK || bm[id] == 0) ? b[id] : 0.f;
-    r[id] = v;
+    r[level0_at(d, id)] = v;
     acc += (double)v;
   }
   grid_reduce(R, FIN_MEAN, acc, 0.0);
@@ -455,12 +578,15 @@ __global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const double* _
   cell_of(d, id, i, j, k);
   const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
   const bool fl = is_fluid<MASK>(d, id);
-  const double pc = p[id];
+  const bool split = d.hh != 0;  // level-0 layout of p
+  const uint32_t c1 = (uint32_t)((i + j + k) & 1);
+  auto at = [&](uint32_t lin, uint32_t colour) { return split ? (colour ? d.hh : 0u) + (lin >> 1) : lin; };
+  const double pc = p[at(id, c1)];
   // the pressure difference is formed in FP64: p grows like n |u| in these
   // units, so an FP32 p would cost ~n ulps of velocity
   auto face = [&](float* __restrict__ c, uint32_t f, bool inner, uint32_t lo, bool high_wall, uint32_t f_high) {
     const bool open = inner && fl && is_fluid<MASK>(d, lo);
-    c[f] = open ? (float)((double)c[f] - (pc - p[lo])) : 0.f;
+    c[f] = open ? (float)((double)c[f] - (pc - p[at(lo, c1 ^ 1u)])) : 0.f;
     if (high_wall) c[f_high] = 0.f;
   };
   face(u.c[0], id, i > 0, id - sx, i == d.n[0] - 1, id + sx);
@@ -475,7 +601,7 @@ __global__ void __launch_bounds__(kB) k_subgrad(Dims d, Faces u, const double* _
 // Iterative-refinement residual r = b - A x evaluated in FP64 (x is FP64;
 // b is the recentred FP32 right-hand side); partial |r|^2 and sum r.
 // Solid cells: r = 0 (b and the operator row are 0 there).
-template <int DIM, bool MASK, int U>
+template <int DIM, bool MASK, int U, bool SPLIT>
 __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restrict__ x, const float* __restrict__ b,
                                                    float* __restrict__ r, Red R, int if_converged) {
   // only after convergence on the recursive residual (uniform over the grid:
@@ -499,8 +625,8 @@ __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restr
       bv[u] = 0.f;
       if (id < w.end && is_fluid<MASK>(d, id)) {
         int i, j, k;
-        cell_of(d, id, i, j, k);
-        has[u] = nbr_load<DIM, MASK>(d, x, i, j, k, id, xn[u]) | kFluidBit;
+        cell_of_at<DIM, SPLIT>(d, id, i, j, k);
+        has[u] = nbr_load<DIM, MASK, SPLIT>(d, x, i, j, k, id, xn[u]) | kFluidBit;
         xv[u] = x[id];
         bv[u] = b[id];
       }
@@ -530,15 +656,23 @@ __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restr
 }
 
 // FP64 solution -> FP32 output
-__global__ void __launch_bounds__(kB) k_to_float(int64_t N, const double* __restrict__ x, float* __restrict__ y) {
+__global__ void __launch_bounds__(kB) k_to_float(Dims d, int64_t N, const double* __restrict__ x,
+                                                 float* __restrict__ y) {
   const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
-  for (uint32_t id = w.first; id < w.end; id += w.stride) y[id] = (float)x[id];
+  for (uint32_t id = w.first; id < w.end; id += w.stride) y[id] = (float)x[level0_at(d, id)];
+}
+
+// Level-0 pressure in natural cell order (from the colour-split layout)
+__global__ void __launch_bounds__(kB) k_unsplit(Dims d, int64_t N, const double* __restrict__ x,
+                                                double* __restrict__ y) {
+  const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
+  for (uint32_t id = w.first; id < w.end; id += w.stride) y[id] = x[level0_at(d, id)];
 }
 
 // ------------------------------------------------------------ PCG steps ----
 // Ap = A p ; partial p.Ap.  MASK: solid rows are 0 (compute_Ap of the
 // obstacle solver updates fluid cells only, 2D/vortex/mgpcg_solid.py:172-178).
-template <int DIM, bool MASK, int U>
+template <int DIM, bool MASK, int U, bool SPLIT>
 __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__ p, float* __restrict__ Ap,
                                                Red R) {
   if (is_done(R.flags)) return;
@@ -558,8 +692,8 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
       pc[u] = 0.f;
       if (id < w.end && is_fluid<MASK>(d, id)) {
         int i, j, k;
-        cell_of(d, id, i, j, k);
-        has[u] = nbr_load<DIM, MASK>(d, p, i, j, k, id, pn[u]) | kFluidBit;
+        cell_of_at<DIM, SPLIT>(d, id, i, j, k);
+        has[u] = nbr_load<DIM, MASK, SPLIT>(d, p, i, j, k, id, pn[u]) | kFluidBit;
         pc[u] = p[id];
       }
     }
@@ -648,7 +782,7 @@ __global__ void __launch_bounds__(kB) k_update_p(int64_t N, const float* __restr
 // colour-0 neighbours' r/nn.  `shift` (level 0) recentres r on the fly.
 // MASK: solid cells get z = 0 and are not neighbours (smooth of the obstacle
 // solver, 2D/vortex/mgpcg_solid.py:229-236).
-template <int DIM, bool MASK>
+template <int DIM, bool MASK, bool SPLIT>
 __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restrict__ r, float* __restrict__ z,
                                                     const double* shift, const int* flags) {
 #pragma clang fp contract(off)
@@ -662,77 +796,112 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
     return;
   }
   int i, j, k;
-  cell_of(d, id, i, j, k);
+  cell_of_at<DIM, SPLIT>(d, id, i, j, k);
   float nn = nbr_count<DIM, MASK>(d, i, j, k);
-  float rv = r[id] - m;
   if (((i + j + k) & 1) == 0) {
-    z[id] = rv * inv_count(nn);
+    z[id] = (r[id] - m) * inv_count(nn);
     return;
   }
+  // every neighbour value is loaded before the first is used
+  float rn[2 * DIM];
+  const unsigned has = nbr_load<DIM, MASK, SPLIT>(d, r, i, j, k, id, rn);
+  float rv = r[id] - m;
+  const int c3[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
   float s = 0.f;
-  for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
-    s += (r[q] - m) * inv_count(nbr_count<DIM, MASK>(d, a, b, c));
-  });
+#pragma unroll
+  for (int q = 0; q < 2 * DIM; ++q)
+    if (has >> q & 1u) s += (rn[q] - m) * inv_count(nbr_count<DIM, MASK>(d, c3[q][0], c3[q][1], c3[q][2]));
   z[id] = (rv + s) * inv_count(nn);
-}
-
-// Pair t of cells along the innermost axis (cells 2h and 2h + 1 of a row):
-// (i, j) of the row, the innermost index `own` of its colour-`phase` cell and
-// `other` of the other cell.  Either may equal the row length (outside an
-// odd-sized row).
-template <int DIM>
-DFM_HD void colour_pair(const Dims& d, uint32_t t, int phase, int& i, int& j, int& own, int& other) {
-  const uint32_t half = d.fh.d;
-  uint32_t o = fdiv(t, d.fh);
-  int base = 2 * (int)(t - o * half);
-  int off;
-  if (DIM == 3) {
-    uint32_t q = fdiv(o, d.f1);
-    i = (int)q;
-    j = (int)(o - q * (uint32_t)d.n[1]);
-    off = (phase - i - j) & 1;
-  } else {
-    i = (int)o;
-    j = 0;
-    off = (phase - i) & 1;
-  }
-  own = base + off;
-  other = base + 1 - off;
-}
-
-// Colour-`phase` cell of pair index t; false when it falls outside an
-// odd-sized row.
-template <int DIM>
-DFM_HD bool colour_cell(const Dims& d, uint32_t t, int phase, int& i, int& j, int& k) {
-  int own, other;
-  colour_pair<DIM>(d, t, phase, i, j, own, other);
-  if (DIM == 3) {
-    k = own;
-    return k < d.n[2];
-  }
-  j = own;
-  k = 0;
-  return j < d.n[1];
 }
 
 // Gauss-Seidel update of cell (i, j, k): (r - m + sum of neighbour z) / nn.
 // With zc, the neighbours' values are read as z + zc[coarse parent] (fused
 // prolongation; the cell's own value is overwritten, so it never needs it).
-template <int DIM, bool MASK>
+template <int DIM, bool MASK, bool SPLIT = false>
 DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* __restrict__ z,
                       const float* __restrict__ zc, const Dims& dc, float m, int i, int j, int k, uint32_t id) {
   float nn, s;
   if (zc) {
     s = 0.f;
     nn = 0.f;
-    for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
+    for_nbrs<DIM, MASK, SPLIT>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
       s += z[q] + zc[cidx(dc, a >> 1, b >> 1, DIM == 3 ? (c >> 1) : 0)];
       nn += 1.f;
     });
   } else {
-    s = nbr_sum<DIM, MASK>(z, d, i, j, k, nn);
+    s = nbr_sum<DIM, MASK, SPLIT>(z, d, i, j, k, nn);
   }
   return (r[id] - m + s) * inv_count(nn);
+}
+
+// One half-sweep of colour `phase` without an obstacle mask or a z.r
+// reduction, kSweepU pairs per thread (the pairs kB apart, so each step of
+// a wave stays coalesced): every load of a thread's pairs is issued before
+// the first store, which keeps several cache lines per lane in flight
+// (latency, not bandwidth, bounds a one-pair-per-thread half-sweep).  The
+// loads read the other colour's z, the coarse correction and r, none of
+// which the stores of this colour change.  Each cell is formed by gs_value's
+// operations in its order, so the result is bitwise the per-pair kernel's.
+#ifndef DFM_SWEEP_U
+#define DFM_SWEEP_U 2
+#endif
+constexpr int kSweepU = DFM_SWEEP_U;
+#ifndef DFM_ZR_U
+#define DFM_ZR_U 2
+#endif
+constexpr int kZrU = DFM_ZR_U;  // pairs per step of the z.r half-sweep's walk
+
+template <int DIM, bool SPLIT>
+__global__ void __launch_bounds__(kB) k_smooth_u(Dims d, const float* __restrict__ r, float* __restrict__ z,
+                                                 int phase, const double* shift, const float* __restrict__ zc,
+                                                 Dims dc, const int* flags) {
+  if (is_done(flags)) return;
+  const float m = shift ? (float)*shift : 0.f;
+  const int inner = DIM == 3 ? d.n[2] : d.n[1];
+  const uint32_t pairs = (uint32_t)(DIM == 3 ? d.n[0] * d.n[1] : d.n[0]) * d.fh.d;
+  const uint32_t t0 = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * (kB * kSweepU) + threadIdx.x;
+  float zn[kSweepU][2 * DIM], zp[kSweepU][2 * DIM], rv[kSweepU];
+  unsigned has[kSweepU];
+  uint32_t ids[kSweepU];
+#pragma unroll
+  for (int u = 0; u < kSweepU; ++u) {
+    const uint32_t t = t0 + u * kB;
+    has[u] = 0;
+    rv[u] = 0.f;
+    ids[u] = 0;
+    if (t < pairs) {
+      int i, j, own, other;
+      colour_pair<DIM>(d, t, phase, i, j, own, other);
+      if (own < inner) {
+        const int a = DIM == 3 ? j : own, k = DIM == 3 ? own : 0;
+        const uint32_t id = SPLIT ? (phase ? d.hh : 0u) + t : cidx(d, i, a, k);
+        ids[u] = id;
+        const unsigned nb = nbr_load<DIM, false, SPLIT>(d, (const float*)z, i, a, k, id, zn[u]);
+        has[u] = nb | kFluidBit;
+        rv[u] = r[id];
+        if (zc) {
+          const int c3[6][3] = {{i - 1, a, k}, {i + 1, a, k}, {i, a - 1, k}, {i, a + 1, k}, {i, a, k - 1}, {i, a, k + 1}};
+#pragma unroll
+          for (int q = 0; q < 2 * DIM; ++q)
+            zp[u][q] = (nb >> q & 1u) ? zc[cidx(dc, c3[q][0] >> 1, c3[q][1] >> 1, DIM == 3 ? (c3[q][2] >> 1) : 0)]
+                                      : 0.f;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kSweepU; ++u) {
+    if (has[u]) {
+      float sum = 0.f, nn = 0.f;
+#pragma unroll
+      for (int q = 0; q < 2 * DIM; ++q)
+        if (has[u] >> q & 1u) {
+          sum += zc ? zn[u][q] + zp[u][q] : zn[u][q];
+          nn += 1.f;
+        }
+      z[ids[u]] = (rv[u] - m + sum) * inv_count(nn);
+    }
+  }
 }
 
 // One half-sweep of colour `phase`; threads cover only that colour (the
@@ -741,7 +910,7 @@ DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* _
 // cells of each pair, forms z.r over them (the other colour's z is final) and
 // runs the scalar step `op` on the total (p = z as well when p is given).
 // MASK: solid cells are skipped (z and r stay 0 there).
-template <int DIM, bool ZR, bool MASK>
+template <int DIM, bool ZR, bool MASK, bool SPLIT>
 __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, float* __restrict__ z, int phase,
                                                const double* shift, const float* __restrict__ zc, Dims dc, Red R,
                                                int op, float* __restrict__ p) {
@@ -751,34 +920,87 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
   const uint32_t pairs = (uint32_t)(DIM == 3 ? d.n[0] * d.n[1] : d.n[0]) * d.fh.d;
   double acc = 0;
   const SlabWalk<uint32_t> w = slab_walk<kB>(pairs);
-  for (uint32_t t = w.first; t < w.end; t += w.stride) {
-    int i, j, own, other;
-    colour_pair<DIM>(d, t, phase, i, j, own, other);
-    if (own < inner) {
-      const int a = DIM == 3 ? j : own, k = DIM == 3 ? own : 0;
-      const uint32_t id = cidx(d, i, a, k);
-      if (is_fluid<MASK>(d, id)) {
-        const float zn = gs_value<DIM, MASK>(d, r, z, zc, dc, m, i, a, k, id);
-        z[id] = zn;
-        if (ZR) {
-          const float rv = r[id] - m;
-          if (shift) r[id] = rv;
-          if (p) p[id] = zn;
-          acc += (double)zn * (double)rv;
+  // U pairs of the walk per step, every load issued before the first store
+  // (the loads read the other colour's z, the coarse correction and r, none
+  // of which a store of this launch changes); pairs are consumed in walk
+  // order with gs_value's operations in its order
+  constexpr int U = ZR ? kZrU : 1;
+  constexpr unsigned kOwnBit = 1u << 30;  // the own cell lies in the row
+  for (uint32_t t0 = w.first; t0 < w.end; t0 += U * w.stride) {
+    float zn[U][2 * DIM], zp[U][2 * DIM], rown[U], zoth[U], roth[U];
+    uint32_t ido[U], idt[U];
+    unsigned has[U];  // kFluidBit: own cell fluid; kOwnBit; bits 0..: neighbours present
+    int ostate[U];    // other cell: 0 none (or not ZR), 1 solid, 2 fluid
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t t = t0 + u * w.stride;
+      has[u] = 0;
+      ostate[u] = 0;
+      rown[u] = zoth[u] = roth[u] = 0.f;
+      ido[u] = idt[u] = 0;
+      if (t >= w.end) continue;
+      int i, j, own, other;
+      colour_pair<DIM>(d, t, phase, i, j, own, other);
+      if (own < inner) {
+        const int a = DIM == 3 ? j : own, k = DIM == 3 ? own : 0;
+        // colour-split: pair t is half index t of both colours
+        const uint32_t id = SPLIT ? (phase ? d.hh : 0u) + t : cidx(d, i, a, k);
+        ido[u] = id;
+        has[u] = kOwnBit;
+        if (is_fluid<MASK>(d, id)) {
+          const unsigned nb = nbr_load<DIM, MASK, SPLIT>(d, (const float*)z, i, a, k, id, zn[u]);
+          has[u] |= nb | kFluidBit;
+          rown[u] = r[id];
+          if (zc) {
+            const int c3[6][3] = {{i - 1, a, k}, {i + 1, a, k}, {i, a - 1, k},
+                                  {i, a + 1, k}, {i, a, k - 1}, {i, a, k + 1}};
+#pragma unroll
+            for (int q = 0; q < 2 * DIM; ++q)
+              zp[u][q] = (nb >> q & 1u) ? zc[cidx(dc, c3[q][0] >> 1, c3[q][1] >> 1, DIM == 3 ? (c3[q][2] >> 1) : 0)]
+                                        : 0.f;
+          }
         }
-      } else if (ZR && p) {
-        p[id] = 0.f;
+      }
+      if (ZR && other < inner) {
+        const uint32_t id = SPLIT ? (phase ? 0u : d.hh) + t : DIM == 3 ? cidx(d, i, j, other) : cidx(d, i, other, 0);
+        idt[u] = id;
+        zoth[u] = z[id];
+        ostate[u] = 1;
+        if (is_fluid<MASK>(d, id)) {
+          ostate[u] = 2;
+          roth[u] = r[id];
+        }
       }
     }
-    if (ZR && other < inner) {
-      const uint32_t id = DIM == 3 ? cidx(d, i, j, other) : cidx(d, i, other, 0);
-      const float zo = z[id];
-      if (is_fluid<MASK>(d, id)) {
-        const float rv = r[id] - m;
-        if (shift) r[id] = rv;
-        acc += (double)zo * (double)rv;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (has[u] & kFluidBit) {
+        float sum = 0.f, nn = 0.f;
+#pragma unroll
+        for (int q = 0; q < 2 * DIM; ++q)
+          if (has[u] >> q & 1u) {
+            sum += zc ? zn[u][q] + zp[u][q] : zn[u][q];
+            nn += 1.f;
+          }
+        const float znew = (rown[u] - m + sum) * inv_count(nn);
+        z[ido[u]] = znew;
+        if (ZR) {
+          const float rv = rown[u] - m;
+          if (shift) r[ido[u]] = rv;
+          if (p) p[ido[u]] = znew;
+          acc += (double)znew * (double)rv;
+        }
+      } else if (ZR && (has[u] & kOwnBit) && p) {
+        p[ido[u]] = 0.f;
       }
-      if (p) p[id] = zo;
+      if (ZR && ostate[u]) {
+        if (ostate[u] == 2) {
+          const float rv = roth[u] - m;
+          if (shift) r[idt[u]] = rv;
+          acc += (double)zoth[u] * (double)rv;
+        }
+        if (p) p[idt[u]] = zoth[u];
+      }
     }
   }
   if (ZR) grid_reduce(R, op, acc, 0.0);
@@ -824,7 +1046,7 @@ __global__ void __launch_bounds__(kB) k_sweep2(Dims d, const float* __restrict__
 // of the fine residual r - A z, times 1/(dim-1).  One thread per coarse cell,
 // so the sum order is fixed.  MASK: fluid children only
 // (2D/vortex/mgpcg_solid.py:216-224).
-template <int DIM, bool MASK>
+template <int DIM, bool MASK, bool SPLIT>
 __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict__ r, const float* __restrict__ z,
                                                  Dims dc, float* __restrict__ rc, const double* shift,
                                                  const int* flags) {
@@ -836,18 +1058,35 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
   if (cid >= Nc) return;
   int ci, cj, ck;
   cell_of(dc, cid, ci, cj, ck);
+  // the children's values are all loaded before the first is used
+  constexpr int NCH = DIM == 3 ? 8 : 4, KZ = DIM == 3 ? 2 : 1;
+  float zn[NCH][2 * DIM], zv[NCH], rv[NCH];
+  unsigned has[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int i = 2 * ci + ch / (2 * KZ), j = 2 * cj + (ch / KZ) % 2, k = DIM == 3 ? 2 * ck + ch % KZ : 0;
+    const uint32_t id = cell_at<SPLIT>(d, i, j, k);
+    has[ch] = 0;
+    zv[ch] = rv[ch] = 0.f;
+    if (is_fluid<MASK>(d, id)) {
+      has[ch] = nbr_load<DIM, MASK, SPLIT>(d, z, i, j, k, id, zn[ch]) | kFluidBit;
+      zv[ch] = z[id];
+      rv[ch] = r[id];
+    }
+  }
   float acc = 0.f;
-  const int kz = DIM == 3 ? 2 : 1;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int c = 0; c < kz; ++c) {
-        int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
-        uint32_t id = cidx(d, i, j, k);
-        if (!is_fluid<MASK>(d, id)) continue;
-        float nn;
-        float s = nbr_sum<DIM, MASK>(z, d, i, j, k, nn);
-        acc += (r[id] - m) - (nn * z[id] - s);
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    if (!has[ch]) continue;
+    float s = 0.f, nn = 0.f;
+#pragma unroll
+    for (int q = 0; q < 2 * DIM; ++q)
+      if (has[ch] >> q & 1u) {
+        s += zn[ch][q];
+        nn += 1.f;
       }
+    acc += (rv[ch] - m) - (nn * zv[ch] - s);
+  }
   rc[cid] = acc * (1.0f / (float)(DIM - 1));
 }
 
@@ -1394,6 +1633,9 @@ struct Solver {
   double nfluid;
   int fused_sweeps;  // option: one launch per RBGS sweep on levels with z2
   int tiled_sweeps;  // option: LDS-tiled pre/post smoothing on eligible coarse levels
+  int split_opt;     // option: level 0 colour-split where eligible (set_layout)
+  double* xn;        // natural-order copy of a colour-split pressure (on request)
+  hipStream_t last_st;  // stream of the last solve
 };
 
 // Largest LDS a workgroup may request on this device.
@@ -1495,9 +1737,25 @@ struct ZrFuse {
 // half-sweep.  Levels with a second buffer run each full sweep as one k_sweep2
 // launch, alternating between lv[l].z and lv[l].z2; the others sweep in place.
 // On return z names the buffer holding the result.  Returns whether zr was done.
-template <int DIM, bool MASK>
-bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero, const double* shift,
-                   const float* zc_prolong, const ZrFuse* zr, float*& z) {
+// One global half-sweep of colour c (no z.r): several pairs per thread
+// without an obstacle mask, one per thread with it.
+template <int DIM, bool MASK, bool SPLIT>
+void half_sweep(Solver& s, hipStream_t st, const Dims& d, float* r, float* z, int c, const double* shift,
+                const float* zc, const Dims& dc, const Red& R) {
+  const uint32_t pairs = pairs_of(d, DIM);
+  if constexpr (!MASK) {
+    const unsigned g = (unsigned)((pairs + (uint32_t)(kB * kSweepU) - 1) / (uint32_t)(kB * kSweepU));
+    hipLaunchKernelGGL((k_smooth_u<DIM, SPLIT>), dim3(g), dim3(kB), 0, st, d, (const float*)r, z, c, shift, zc, dc,
+                       (const int*)s.flags);
+  } else {
+    hipLaunchKernelGGL((k_smooth<DIM, false, MASK, SPLIT>), dim3(grid1(pairs)), dim3(kB), 0, st, d, r, z, c, shift,
+                       zc, dc, R, 0, (float*)nullptr);
+  }
+}
+
+template <int DIM, bool MASK, bool SPLIT>
+bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero, const double* shift,
+                     const float* zc_prolong, const ZrFuse* zr, float*& z) {
   Lvl& L = s.lv[l];
   const Dims& d = s.ld[l];
   const Dims& dc = s.ld[l + 1 < s.L ? l + 1 : l];
@@ -1505,12 +1763,12 @@ bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool fr
   int k0 = 0;
   if (from_zero) {
     z = L.z;
-    hipLaunchKernelGGL((k_first_sweep<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, z, shift, s.flags);
+    hipLaunchKernelGGL((k_first_sweep<DIM, MASK, SPLIT>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, z, shift,
+                       s.flags);
     k0 = 1;
   }
   const uint32_t pairs = pairs_of(d, DIM);
-  const unsigned gs = grid1(pairs);
-  const bool fused = s.fused_sweeps && L.z2 && !zr;
+  const bool fused = !SPLIT && s.fused_sweeps && L.z2 && !zr;
   bool zr_done = false;
   for (int it = k0; it < sweeps; ++it) {
     const float* zc = (it == k0 && zc_prolong) ? zc_prolong : nullptr;
@@ -1521,18 +1779,30 @@ bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool fr
       z = zout;
       continue;
     }
-    hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, z, c1, shift, zc, dc, R,
-                       0, (float*)nullptr);
+    half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, c1, shift, zc, dc, R);
     if (zr && it == sweeps - 1) {
-      hipLaunchKernelGGL((k_smooth<DIM, true, MASK>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, z, 1 - c1,
+      hipLaunchKernelGGL((k_smooth<DIM, true, MASK, SPLIT>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, z, 1 - c1,
                          shift, (const float*)nullptr, dc, R, zr->op, zr->p);
       zr_done = true;
     } else {
-      hipLaunchKernelGGL((k_smooth<DIM, false, MASK>), dim3(gs), dim3(kB), 0, st, d, L.r, z, 1 - c1, shift,
-                         (const float*)nullptr, dc, R, 0, (float*)nullptr);
+      half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, 1 - c1, shift, (const float*)nullptr, dc, R);
     }
   }
   return zr_done;
+}
+
+// Level 0 in the colour-split layout takes the SPLIT kernels (never with an
+// obstacle mask); every other level is stored in natural order.
+inline bool split_level(const Solver& s, int l);
+
+template <int DIM, bool MASK>
+bool sweeps_global(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero, const double* shift,
+                   const float* zc_prolong, const ZrFuse* zr, float*& z) {
+  if constexpr (!MASK) {
+    if (split_level(s, l))
+      return sweeps_global_t<DIM, false, true>(s, st, l, sweeps, c1, from_zero, shift, zc_prolong, zr, z);
+  }
+  return sweeps_global_t<DIM, MASK, false>(s, st, l, sweeps, c1, from_zero, shift, zc_prolong, zr, z);
 }
 
 // Whether level l runs its pre/post smoothing in the LDS-tiled kernels:
@@ -1545,6 +1815,8 @@ inline bool tiled_level(const Solver& s, int l) {
     if (s.lv[l].n[a] % T) return false;
   return true;
 }
+
+inline bool split_level(const Solver& s, int l) { return l == 0 && s.ld[0].hh != 0; }
 
 template <int DIM>
 inline unsigned tile_count(const Dims& d) {
@@ -1574,8 +1846,17 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
     sweeps_global<DIM, MASK>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr, cur[l]);
     Lvl& F = s.lv[l];
     Lvl& C = s.lv[l + 1];
-    hipLaunchKernelGGL((k_restrict<DIM, MASK>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r, cur[l],
-                       s.ld[l + 1], C.r, sh, s.flags);
+    bool split_done = false;
+    if constexpr (!MASK) {
+      if (split_level(s, l)) {
+        hipLaunchKernelGGL((k_restrict<DIM, false, true>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r,
+                           cur[l], s.ld[l + 1], C.r, sh, s.flags);
+        split_done = true;
+      }
+    }
+    if (!split_done)
+      hipLaunchKernelGGL((k_restrict<DIM, MASK, false>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r, cur[l],
+                         s.ld[l + 1], C.r, sh, s.flags);
   }
   if (tail) {
     Lvl& T = s.lv[s.tail_lt];
@@ -1619,6 +1900,45 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
   }
 }
 
+// Ap = A p with p.Ap -> alpha, and the FP64 refinement residual b - A x
+// (after convergence only), in the level-0 layout and walk length's form.
+template <int DIM, bool MASK, bool SPLIT>
+void launch_Ap_dot_t(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
+  if (long_walk(s.lv[0].size))
+    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, kUnroll, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p,
+                       s.Ap, R);
+  else
+    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, 1, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap,
+                       R);
+}
+
+template <int DIM, bool MASK>
+void launch_Ap_dot(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
+  if constexpr (!MASK) {
+    if (split_level(s, 0)) return launch_Ap_dot_t<DIM, false, true>(s, st, gr, R);
+  }
+  launch_Ap_dot_t<DIM, MASK, false>(s, st, gr, R);
+}
+
+template <int DIM, bool MASK, bool SPLIT>
+void launch_true_resid_t(Solver& s, hipStream_t st) {
+  const unsigned gr = gridr(s.lv[0].size);
+  if (long_walk(s.lv[0].size))
+    hipLaunchKernelGGL((k_true_resid<DIM, MASK, kUnroll, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0],
+                       (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
+  else
+    hipLaunchKernelGGL((k_true_resid<DIM, MASK, 1, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const double*)s.x,
+                       (const float*)s.b, s.lv[0].r, red_of(s), 1);
+}
+
+template <int DIM, bool MASK>
+void launch_true_resid(Solver& s, hipStream_t st) {
+  if constexpr (!MASK) {
+    if (split_level(s, 0)) return launch_true_resid_t<DIM, false, true>(s, st);
+  }
+  launch_true_resid_t<DIM, MASK, false>(s, st);
+}
+
 // One PCG iteration once z, p and z.r are set (solve(), 2D/vortex/mgpcg.py:235-278):
 // Ap and p.Ap -> alpha; x/r update and r.r -> convergence flag and mean;
 // recentring V-cycle with z.r -> beta; p update.  Every kernel is a no-op once
@@ -1629,10 +1949,7 @@ void pcg_iteration(Solver& s, hipStream_t st) {
   int64_t N = L0.size;
   unsigned gr = gridr(N);
   const Red R = red_of(s);
-  if (long_walk(N))
-    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, kUnroll>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
-  else
-    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, 1>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap, R);
+  launch_Ap_dot<DIM, MASK>(s, st, gr, R);
   hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, (const float*)s.p, (const float*)s.Ap, R);
   vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_BETA, nullptr});
   hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, (const float*)s.z0_final, s.p, (const double*)s.sc,
@@ -1643,6 +1960,23 @@ void drop_graph(Solver& s) {
   if (s.gexec) (void)hipGraphExecDestroy(s.gexec);
   s.gexec = nullptr;
   s.gexec_iters = 0;
+}
+
+// Level 0 is stored colour-split (Dims::hh) unless the option is off, an
+// obstacle mask is set, level 0 sits in the LDS tail, or its innermost
+// extent is odd.  Decided at the start of each solve; a change drops the
+// captured iterations, whose kernels carry the layout.
+bool split_eligible(const Solver& s) {
+  const int inner = s.G.dim == 3 ? s.lv[0].n[2] : s.lv[0].n[1];
+  return s.split_opt && !s.masked && s.tail_lt > 0 && inner % 2 == 0;
+}
+
+void set_layout(Solver& s) {
+  const uint32_t hh = split_eligible(s) ? (uint32_t)(s.lv[0].size / 2) : 0u;
+  if (s.ld[0].hh != hh) {
+    s.ld[0].hh = hh;
+    drop_graph(s);
+  }
 }
 
 // Capture check_every PCG iterations into one executable graph (on a private
@@ -1689,14 +2023,7 @@ int pcg_queue(Solver& s, int k) {
   P.it += k;
   // Converged on the recursive FP32 residual: measure the true residual
   // b - A x in FP64; if it misses the tolerance, PCG restarts from it.
-  if (P.refinements < s.max_refine) {
-    if (long_walk(s.lv[0].size))
-      hipLaunchKernelGGL((k_true_resid<DIM, MASK, kUnroll>), dim3(gridr(s.lv[0].size)), dim3(kB), 0, P.st, s.ld[0],
-                         (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
-    else
-      hipLaunchKernelGGL((k_true_resid<DIM, MASK, 1>), dim3(gridr(s.lv[0].size)), dim3(kB), 0, P.st, s.ld[0],
-                         (const double*)s.x, (const float*)s.b, s.lv[0].r, red_of(s), 1);
-  }
+  if (P.refinements < s.max_refine) launch_true_resid<DIM, MASK>(s, P.st);
   if (hipMemcpyAsync(s.h_sc, s.sc, kCtlBytes, hipMemcpyDeviceToHost, P.st) != hipSuccess)
     return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
   return DFM_OK;
@@ -1791,6 +2118,7 @@ void destroy(Solver* s) {
     if (s->lv[l].z2) (void)hipFree(s->lv[l].z2);
   }
   if (s->x) (void)hipFree(s->x);
+  if (s->xn) (void)hipFree(s->xn);
   if (s->b) (void)hipFree(s->b);
   if (s->p) (void)hipFree(s->p);
   if (s->Ap) (void)hipFree(s->Ap);
@@ -1845,6 +2173,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->use_graph = 1;
   s->fused_sweeps = 1;
   s->tiled_sweeps = 1;
+  s->split_opt = 1;
   bool ok = true;
   for (int l = 0; l < levels && ok; ++l) {
     for (int a = 0; a < 3; ++a) s->lv[l].n[a] = a < g->dim ? (g->n[a] >> l) : 1;
@@ -1932,6 +2261,10 @@ extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
       s->tiled_sweeps = value == 2 ? 1 : 0;
       drop_graph(*s);
       return DFM_OK;
+    case DFM_MG_SPLIT_LAYOUT:
+      s->split_opt = value ? 1 : 0;
+      drop_graph(*s);
+      return DFM_OK;
     default:
       return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
@@ -1949,6 +2282,7 @@ extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
     case DFM_MG_BOTTOM_EXACT: *value = (s->tail_lt < s->L) ? s->bottom_exact : 0; return DFM_OK;
     case DFM_MG_TAIL_LEVEL: *value = s->tail_lt; return DFM_OK;
     case DFM_MG_FUSED_SWEEPS: *value = s->fused_sweeps + s->tiled_sweeps; return DFM_OK;
+    case DFM_MG_SPLIT_LAYOUT: *value = split_eligible(*s) ? 1 : 0; return DFM_OK;
     default: return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
 }
@@ -1963,6 +2297,8 @@ extern "C" int dfm_mgpcg_project_begin(void* handle, const dfm_mac* u, double to
   if (max_iters < 1) return set_error(DFM_E_ARG, "max_iters must be >= 1");
   if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  set_layout(*s);
+  s->last_st = st;
   Lvl& L0 = s->lv[0];
   const Dims& d0 = s->ld[0];
   Faces F{{u->c[0], u->c[1], u->c[2]}};
@@ -2034,21 +2370,23 @@ extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double to
   if (max_iters < 1) return set_error(DFM_E_ARG, "max_iters must be >= 1");
   if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  set_layout(*s);
+  s->last_st = st;
   Lvl& L0 = s->lv[0];
   unsigned gr = gridr(L0.size);
   int rc;
   if (s->masked) {
-    hipLaunchKernelGGL((k_load_b<true>), dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, (const uint8_t*)s->bm[0],
-                       red_of(*s));
+    hipLaunchKernelGGL((k_load_b<true>), dim3(gr), dim3(kB), 0, st, s->ld[0], L0.size, b, L0.r,
+                       (const uint8_t*)s->bm[0], red_of(*s));
     rc = s->G.dim == 2 ? pcg<2, true>(*s, st, tol, relative, max_iters) : pcg<3, true>(*s, st, tol, relative, max_iters);
   } else {
-    hipLaunchKernelGGL((k_load_b<false>), dim3(gr), dim3(kB), 0, st, L0.size, b, L0.r, (const uint8_t*)nullptr,
-                       red_of(*s));
+    hipLaunchKernelGGL((k_load_b<false>), dim3(gr), dim3(kB), 0, st, s->ld[0], L0.size, b, L0.r,
+                       (const uint8_t*)nullptr, red_of(*s));
     rc = s->G.dim == 2 ? pcg<2, false>(*s, st, tol, relative, max_iters)
                        : pcg<3, false>(*s, st, tol, relative, max_iters);
   }
   if (rc != DFM_OK && rc != DFM_E_NOCONV) return rc;
-  hipLaunchKernelGGL(k_to_float, dim3(gr), dim3(kB), 0, st, L0.size, s->x, x);
+  hipLaunchKernelGGL(k_to_float, dim3(gr), dim3(kB), 0, st, s->ld[0], L0.size, s->x, x);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: solution copy", hipGetErrorString(e));
   return rc;
@@ -2106,6 +2444,19 @@ extern "C" int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual) 
 extern "C" int dfm_mgpcg_pressure(void* handle, double** p) {
   Solver* s = static_cast<Solver*>(handle);
   if (!s || !p) return set_error(DFM_E_NULL, "dfm_mgpcg_pressure: NULL argument");
-  *p = s->x;
+  if (!s->ld[0].hh) {
+    *p = s->x;
+    return DFM_OK;
+  }
+  // colour-split level 0: reorder into natural order behind the last solve
+  const int64_t N = s->lv[0].size;
+  if (!s->xn && hipMalloc(&s->xn, sizeof(double) * N) != hipSuccess) {
+    s->xn = nullptr;
+    return set_error(DFM_E_HIP, "dfm_mgpcg_pressure: device allocation failed");
+  }
+  hipLaunchKernelGGL(k_unsplit, dim3(gridr(N)), dim3(kB), 0, s->last_st, s->ld[0], N, (const double*)s->x, s->xn);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(DFM_E_HIP, "dfm_mgpcg_pressure: reorder", hipGetErrorString(e));
+  *p = s->xn;
   return DFM_OK;
 }

@@ -88,6 +88,7 @@ MG_GRAPH = 5
 MG_BOTTOM_EXACT = 6
 MG_TAIL_LEVEL = 7
 MG_FUSED_SWEEPS = 8
+MG_SPLIT_LAYOUT = 9
 
 DFM_E_NOCONV = -4
 

@@ -194,7 +194,7 @@ class MGPCG:
 
     def __init__(self, n, dx, levels=0, pre_post=2, bottom=50, tol=1e-6, relative=True, max_iters=400,
                  raise_on_noconv=False, bf16_coarse=False, tail=True, graph=True, check_every=1,
-                 bottom_exact=False, fused_sweeps=2):
+                 bottom_exact=False, fused_sweeps=2, split_layout=True):
         self.n = tuple(n)
         self.g = L.make_grid(n, dx)
         self.lib = L.lib()
@@ -214,6 +214,8 @@ class MGPCG:
             self.set_option(L.MG_BOTTOM_EXACT, 1)
         if fused_sweeps != 2:
             self.set_option(L.MG_FUSED_SWEEPS, fused_sweeps)
+        if not split_layout:
+            self.set_option(L.MG_SPLIT_LAYOUT, 0)
         self.tol, self.relative, self.max_iters = tol, relative, max_iters
         self.raise_on_noconv = raise_on_noconv
         self.last_iters = 0

@@ -179,7 +179,10 @@ int dfm_mgpcg_project_end(void* handle);
 /* Iterations and relative residual sqrt(r.r / b.b) of the last solve. */
 int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
 
-/* Device pointer of the pressure (cells, FP64) from the last projection. */
+/* Device pointer of the pressure (cells, FP64, natural cell order) from the
+ * last projection.  With a colour-split finest level the pressure is first
+ * reordered into a second buffer by a kernel queued on the last solve's
+ * stream: read it on that stream (or after synchronising it). */
 int dfm_mgpcg_pressure(void* handle, double** p);
 
 /* Solver options (no counterpart in the reference; tuning of this build).
@@ -215,6 +218,14 @@ int dfm_mgpcg_pressure(void* handle, double** p);
  *      red-black sweep, recomputing the first colour's neighbour values; 0 =
  *      two launches per sweep.  All three give bitwise the same results. */
 #define DFM_MG_FUSED_SWEEPS 8
+/*  DFM_MG_SPLIT_LAYOUT: 1 (default) = store the finest level colour-split
+ *  (red cells in one half of each level-0 array, black in the other), so a
+ *  red-black half-sweep moves only the cache lines of the colours it uses.
+ *  Used when no obstacle mask is set, the finest level is not in the LDS
+ *  tail and its innermost extent is even; reading the option gives whether
+ *  the next solve will use it.  0 = natural order everywhere.  The results
+ *  agree to rounding (the FP64 reductions sum in another order). */
+#define DFM_MG_SPLIT_LAYOUT 9
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);
 

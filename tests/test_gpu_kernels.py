@@ -413,6 +413,86 @@ def test_mgpcg_auto_levels_on_grids_that_halve_few_times(n, device):
     assert rel_l2_fields(mac_to_np(d), ref) < 2e-6
 
 
+def _pressure_of(solver, n, device):
+    """The handle's FP64 pressure (dfm_mgpcg_pressure) copied into a tensor."""
+    import ctypes
+
+    from dfm import _lib as L
+
+    ptr = ctypes.c_void_p()
+    L.check(L.lib().dfm_mgpcg_pressure(solver.h, ctypes.byref(ptr)), "mgpcg_pressure")
+    out = torch.empty(int(np.prod(n)), dtype=torch.float64, device=device)
+    torch.cuda.synchronize()
+    # the HIP runtime libdfm.so is linked against (the one torch uses)
+    memcpy = L.lib().hipMemcpy
+    memcpy.restype = ctypes.c_int
+    memcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert memcpy(out.data_ptr(), ptr, out.numel() * 8, 3) == 0  # device to device
+    return out.cpu().numpy().reshape(n)
+
+
+@pytest.mark.parametrize("n,levels", [((64, 48, 32), 3), ((256, 192), 5), ((40, 36, 20), 3), ((192, 160), 4),
+                                      ((128, 128, 128), 5)])
+def test_mgpcg_split_layout_matches_natural(n, levels, device):
+    """The colour-split finest level (red cells in one half of each level-0
+    array, black in the other; the default where eligible) gives the
+    natural layout's projection up to rounding (its FP64 reductions sum in
+    another order), within one iteration, the exact projection, the same
+    pressure through dfm_mgpcg_pressure (natural order) and the same
+    solution through the b -> x entry."""
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    dim = len(n)
+    rng = np.random.default_rng(23 + n[0])
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(dim)])
+    b = rng.standard_normal(n)
+    b = round32(b - b.mean())
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    xe = O.neumann_solve_dct(b.astype(np.float64))
+    got = {}
+    for split in (True, False):
+        s = MGPCG(n, 1.0 / n[1], levels=levels, tol=1e-7, max_iters=200, split_layout=split)
+        assert s.get_option(L.MG_SPLIT_LAYOUT) == (1 if split else 0)
+        d = mac_to_dev(u, device)
+        s.Poisson(d)
+        it = s.status()[0]
+        p = _pressure_of(s, n, device)
+        x = torch.zeros(n, dtype=torch.float32, device=device)
+        s.solve(torch.tensor(b, dtype=torch.float32, device=device), x)
+        xs = x.cpu().numpy().astype(np.float64)
+        got[split] = (mac_to_np(d), it, p - p.mean(), xs - xs.mean())
+        assert rel_l2_fields(got[split][0], ref) < 2e-6
+        assert rel_l2(got[split][3], xe) < 1e-5
+    (fs, its, ps, xs), (fn, itn, pn, xn) = got[True], got[False]
+    assert abs(its - itn) <= 1
+    assert rel_l2_fields(fs, fn) < 1e-6
+    assert rel_l2(ps, pn) < 1e-5
+    assert rel_l2(xs, xn) < 1e-5
+
+
+def test_mgpcg_split_layout_eligibility(device):
+    """The split layout needs an even innermost extent, no obstacle mask and
+    a finest level outside the LDS tail (grids whose levels all fit in one
+    workgroup's LDS are solved there whole)."""
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    assert MGPCG((48, 40, 32), 1 / 40, levels=2).get_option(L.MG_SPLIT_LAYOUT) == 1
+    assert MGPCG((48, 40, 31), 1 / 40, levels=1).get_option(L.MG_SPLIT_LAYOUT) == 0
+    assert MGPCG((256, 200), 1 / 200, levels=2).get_option(L.MG_SPLIT_LAYOUT) == 1
+    assert MGPCG((256, 199), 1 / 199, levels=1).get_option(L.MG_SPLIT_LAYOUT) == 0
+    assert MGPCG((16, 16, 16), 1 / 16, levels=1).get_option(L.MG_SPLIT_LAYOUT) == 0  # level 0 in the tail
+    assert MGPCG((48, 40, 32), 1 / 40, levels=2, split_layout=False).get_option(L.MG_SPLIT_LAYOUT) == 0
+    s = MGPCG((256, 256), 1 / 256, levels=3)
+    assert s.get_option(L.MG_SPLIT_LAYOUT) == 1
+    s.set_mask(torch.zeros((256, 256), device=device))
+    assert s.get_option(L.MG_SPLIT_LAYOUT) == 0
+    s.set_mask(None)
+    assert s.get_option(L.MG_SPLIT_LAYOUT) == 1
+
+
 def test_mgpcg_solve_and_status(device):
     from dfm.ops import MGPCG
 
