@@ -834,14 +834,8 @@ DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* _
   return (r[id] - m + s) * inv_count(nn);
 }
 
-// One half-sweep of colour `phase` without an obstacle mask or a z.r
-// reduction, kSweepU pairs per thread (the pairs kB apart, so each step of
-// a wave stays coalesced): every load of a thread's pairs is issued before
-// the first store, which keeps several cache lines per lane in flight
-// (latency, not bandwidth, bounds a one-pair-per-thread half-sweep).  The
-// loads read the other colour's z, the coarse correction and r, none of
-// which the stores of this colour change.  Each cell is formed by gs_value's
-// operations in its order, so the result is bitwise the per-pair kernel's.
+// Pairs per thread of the global half-sweep kernels below, and pairs per
+// step of the z.r half-sweep's walk.
 #ifndef DFM_SWEEP_U
 #define DFM_SWEEP_U 2
 #endif
@@ -850,6 +844,57 @@ constexpr int kSweepU = DFM_SWEEP_U;
 #define DFM_ZR_U 2
 #endif
 constexpr int kZrU = DFM_ZR_U;  // pairs per step of the z.r half-sweep's walk
+
+// First half-sweep from z = 0 without an obstacle mask: the cells of colour
+// `phase` get (r - m) / nn (kSweepU pairs per thread, as k_smooth_u).  The
+// other colour then takes an ordinary half-sweep: its neighbour values are
+// exactly the (r - m) / nn that k_first_sweep forms in place, so the pair
+// gives k_first_sweep's result bitwise while each launch stays
+// bandwidth-bound (k_first_sweep re-forms six neighbour counts per cell).
+template <int DIM, bool SPLIT>
+__global__ void __launch_bounds__(kB) k_first_half(Dims d, const float* __restrict__ r, float* __restrict__ z,
+                                                   int phase, const double* shift, const int* flags) {
+#pragma clang fp contract(off)
+  if (is_done(flags)) return;
+  const float m = shift ? (float)*shift : 0.f;
+  const int inner = DIM == 3 ? d.n[2] : d.n[1];
+  const uint32_t pairs = (uint32_t)(DIM == 3 ? d.n[0] * d.n[1] : d.n[0]) * d.fh.d;
+  const uint32_t t0 = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * (kB * kSweepU) + threadIdx.x;
+  float rv[kSweepU], nn[kSweepU];
+  uint32_t ids[kSweepU];
+  bool has[kSweepU];
+#pragma unroll
+  for (int u = 0; u < kSweepU; ++u) {
+    const uint32_t t = t0 + u * kB;
+    has[u] = false;
+    rv[u] = nn[u] = 0.f;
+    ids[u] = 0;
+    if (t < pairs) {
+      int i, j, own, other;
+      colour_pair<DIM>(d, t, phase, i, j, own, other);
+      if (own < inner) {
+        const int a = DIM == 3 ? j : own, k = DIM == 3 ? own : 0;
+        const uint32_t id = SPLIT ? (phase ? d.hh : 0u) + t : cidx(d, i, a, k);
+        ids[u] = id;
+        has[u] = true;
+        rv[u] = r[id];
+        nn[u] = nbr_count<DIM>(d, i, a, k);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kSweepU; ++u)
+    if (has[u]) z[ids[u]] = (rv[u] - m) * inv_count(nn[u]);
+}
+
+// One half-sweep of colour `phase` without an obstacle mask or a z.r
+// reduction, kSweepU pairs per thread (the pairs kB apart, so each step of
+// a wave stays coalesced): every load of a thread's pairs is issued before
+// the first store, which keeps several cache lines per lane in flight
+// (latency, not bandwidth, bounds a one-pair-per-thread half-sweep).  The
+// loads read the other colour's z, the coarse correction and r, none of
+// which the stores of this colour change.  Each cell is formed by gs_value's
+// operations in its order, so the result is bitwise the per-pair kernel's.
 
 template <int DIM, bool SPLIT>
 __global__ void __launch_bounds__(kB) k_smooth_u(Dims d, const float* __restrict__ r, float* __restrict__ z,
@@ -1763,8 +1808,17 @@ bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool 
   int k0 = 0;
   if (from_zero) {
     z = L.z;
-    hipLaunchKernelGGL((k_first_sweep<DIM, MASK, SPLIT>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, z, shift,
-                       s.flags);
+    if constexpr (MASK) {
+      hipLaunchKernelGGL((k_first_sweep<DIM, MASK, SPLIT>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, z, shift,
+                         s.flags);
+    } else {
+      // colour c1 from zero, then an ordinary half-sweep of the other colour
+      // (k_first_sweep's result bitwise)
+      const unsigned g = (unsigned)((pairs_of(d, DIM) + (uint32_t)(kB * kSweepU) - 1) / (uint32_t)(kB * kSweepU));
+      hipLaunchKernelGGL((k_first_half<DIM, SPLIT>), dim3(g), dim3(kB), 0, st, d, (const float*)L.r, z, c1, shift,
+                         (const int*)s.flags);
+      half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, 1 - c1, shift, (const float*)nullptr, dc, R);
+    }
     k0 = 1;
   }
   const uint32_t pairs = pairs_of(d, DIM);
