@@ -196,10 +196,14 @@ DFM_HD float nbr_count(const Dims& d, int i, int j, int k) {
   return nn;
 }
 
-// The 2*DIM neighbour values of cell (i, j, k) in for_nbrs order, loaded
-// before any is used (v[t] = 0 where absent); bit t of the result is set for
-// each neighbour present (in the grid and, MASK, fluid).  Walks that keep
-// several cells' loads in flight use it instead of for_nbrs.
+// The 2*DIM neighbour values of cell (i, j, k) in for_nbrs order, for
+// walks that keep several cells' loads in flight: every neighbour is loaded
+// unconditionally into v (an absent one reads the cell's own, in-grid entry;
+// its value is not used), and bit t of the result is set for each neighbour
+// present (in the grid and, MASK, fluid).  The caller passes all of a step's
+// loaded values through pin() before using any: otherwise the compiler sinks
+// each load into the branch of its only, conditional, use and waits for it
+// there, one memory round trip per neighbour.
 template <int DIM, bool MASK, bool SPLIT, typename T>
 DFM_HD unsigned nbr_load(const Dims& d, const T* __restrict__ f, int i, int j, int k, uint32_t id, T v[2 * DIM]) {
   bool in[6];
@@ -214,13 +218,23 @@ DFM_HD unsigned nbr_load(const Dims& d, const T* __restrict__ f, int i, int j, i
   unsigned m = 0;
 #pragma unroll
   for (int t = 0; t < 2 * DIM; ++t) {
-    v[t] = T(0);
-    if (in[t] && is_fluid<MASK>(d, q[t])) {
-      v[t] = f[q[t]];
-      m |= 1u << t;
-    }
+    const uint32_t qs = in[t] ? q[t] : id;
+    v[t] = f[qs];
+    m |= (in[t] && is_fluid<MASK>(d, qs)) ? 1u << t : 0u;
   }
   return m;
+}
+
+// An empty asm that takes x: the load that produced x is then used
+// unconditionally here, so it is issued with the others before any wait.
+template <typename T>
+DFM_HD void pin(T& x) {
+  asm volatile("" : "+v"(x));
+}
+template <typename T, int N>
+DFM_HD void pin(T (&a)[N]) {
+#pragma unroll
+  for (int t = 0; t < N; ++t) pin(a[t]);
 }
 
 // Cell index -> (i, j, k); grids stay below 2^31 cells (checked at creation).
@@ -632,6 +646,10 @@ __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restr
       }
     }
 #pragma unroll
+    for (int u = 0; u < U; ++u) pin(xn[u]);
+    pin(xv);
+    pin(bv);
+#pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t id = id0 + u * w.stride;
       if (id < w.end) {
@@ -697,6 +715,9 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
         pc[u] = p[id];
       }
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) pin(pn[u]);
+    pin(pc);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t id = id0 + u * w.stride;
@@ -805,6 +826,7 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
   // every neighbour value is loaded before the first is used
   float rn[2 * DIM];
   const unsigned has = nbr_load<DIM, MASK, SPLIT>(d, r, i, j, k, id, rn);
+  pin(rn);
   float rv = r[id] - m;
   const int c3[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
   float s = 0.f;
@@ -924,15 +946,24 @@ __global__ void __launch_bounds__(kB) k_smooth_u(Dims d, const float* __restrict
         const unsigned nb = nbr_load<DIM, false, SPLIT>(d, (const float*)z, i, a, k, id, zn[u]);
         has[u] = nb | kFluidBit;
         rv[u] = r[id];
-        if (zc) {
+        if (zc) {  // the coarse parent of each neighbour (an absent one: of the cell)
           const int c3[6][3] = {{i - 1, a, k}, {i + 1, a, k}, {i, a - 1, k}, {i, a + 1, k}, {i, a, k - 1}, {i, a, k + 1}};
 #pragma unroll
-          for (int q = 0; q < 2 * DIM; ++q)
-            zp[u][q] = (nb >> q & 1u) ? zc[cidx(dc, c3[q][0] >> 1, c3[q][1] >> 1, DIM == 3 ? (c3[q][2] >> 1) : 0)]
-                                      : 0.f;
+          for (int q = 0; q < 2 * DIM; ++q) {
+            const bool pr = nb >> q & 1u;
+            zp[u][q] = zc[cidx(dc, (pr ? c3[q][0] : i) >> 1, (pr ? c3[q][1] : a) >> 1,
+                               DIM == 3 ? ((pr ? c3[q][2] : k) >> 1) : 0)];
+          }
         }
       }
     }
+  }
+#pragma unroll
+  for (int u = 0; u < kSweepU; ++u) pin(zn[u]);
+  pin(rv);
+  if (zc) {
+#pragma unroll
+    for (int u = 0; u < kSweepU; ++u) pin(zp[u]);
   }
 #pragma unroll
   for (int u = 0; u < kSweepU; ++u) {
@@ -996,13 +1027,15 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
           const unsigned nb = nbr_load<DIM, MASK, SPLIT>(d, (const float*)z, i, a, k, id, zn[u]);
           has[u] |= nb | kFluidBit;
           rown[u] = r[id];
-          if (zc) {
+          if (zc) {  // the coarse parent of each neighbour (an absent one: of the cell)
             const int c3[6][3] = {{i - 1, a, k}, {i + 1, a, k}, {i, a - 1, k},
                                   {i, a + 1, k}, {i, a, k - 1}, {i, a, k + 1}};
 #pragma unroll
-            for (int q = 0; q < 2 * DIM; ++q)
-              zp[u][q] = (nb >> q & 1u) ? zc[cidx(dc, c3[q][0] >> 1, c3[q][1] >> 1, DIM == 3 ? (c3[q][2] >> 1) : 0)]
-                                        : 0.f;
+            for (int q = 0; q < 2 * DIM; ++q) {
+              const bool pr = nb >> q & 1u;
+              zp[u][q] = zc[cidx(dc, (pr ? c3[q][0] : i) >> 1, (pr ? c3[q][1] : a) >> 1,
+                                 DIM == 3 ? ((pr ? c3[q][2] : k) >> 1) : 0)];
+            }
           }
         }
       }
@@ -1016,6 +1049,15 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
           roth[u] = r[id];
         }
       }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) pin(zn[u]);
+    pin(rown);
+    pin(zoth);
+    pin(roth);
+    if (zc) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) pin(zp[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -1119,6 +1161,10 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
       rv[ch] = r[id];
     }
   }
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) pin(zn[ch]);
+  pin(zv);
+  pin(rv);
   float acc = 0.f;
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
