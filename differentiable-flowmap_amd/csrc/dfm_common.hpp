@@ -302,15 +302,31 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     }
     return v;
   } else {
-    // x taps stay rolled (their weights computed per iteration), which is
-    // cheaper than the register pressure of a fully unrolled 3D loop
+    // the x weights are formed per slice (tap_w) rather than held in arrays
     Axis<K, GRAD> az;
     az.setup(p[2] * inv_dx, bl[2], S[2]);
     const float sx = clamp_coord<K>(p[0] * inv_dx, bl[0], S[0]);
     const int bx = support_base<K>(sx, S[0]);
     const float tx = sx - (float)(bx + Spline<K>::REF_OFF);
     float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-#pragma unroll 1
+    // Quadratic: all 27 taps (9 rows of 3, each one vector load) are
+    // requested before the first is used, one memory round trip per sample
+    // instead of one per x slice; the multiply-adds keep their order.  The
+    // wider splines keep one slice in flight at a time (register pressure).
+    float tv[K == 2 ? NT : 1][K == 2 ? NT : 1][K == 2 ? NT : 1];
+    if constexpr (K == 2) {
+#pragma unroll
+      for (int a = 0; a < NT; ++a) {
+        const int ia = tap_index<K>(bx, a, S[0]);
+#pragma unroll
+        for (int b = 0; b < NT; ++b) {
+          const float* row = f + ((int64_t)ia * S[1] + ay.idx[b]) * S[2];
+#pragma unroll
+          for (int c = 0; c < NT; ++c) tv[a][b][c] = row[az.idx[c]];
+        }
+      }
+    }
+#pragma unroll
     for (int a = 0; a < NT; ++a) {
       const float wxa = tap_w<K>(tx, a);
       const int ia = tap_index<K>(bx, a, S[0]);
@@ -321,7 +337,9 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
         float zv = 0.f, zd = 0.f;
 #pragma unroll
         for (int c = 0; c < NT; ++c) {
-          float val = row[az.idx[c]];
+          float val;
+          if constexpr (K == 2) val = tv[a][b][c];
+          else val = row[az.idx[c]];
           zv = fmaf(val, az.w[c], zv);
           if (GRAD) zd = fmaf(val, az.dw[c], zd);
         }
