@@ -62,6 +62,20 @@ DFM_HD int64_t xcd_block(int64_t bid, int64_t nblocks) {
   return (bid & 7) * (nb8 >> 3) + (bid >> 3);
 }
 
+// An empty asm that takes x: the load that produced x is then used
+// unconditionally at this point, so the compiler issues it together with
+// the loads before it instead of moving it next to its first use and
+// waiting for it there (one memory round trip per load).
+template <typename T>
+DFM_HD void pin(T& x) {
+  asm volatile("" : "+v"(x));
+}
+template <typename T, int N>
+DFM_HD void pin(T (&a)[N]) {
+#pragma unroll
+  for (int t = 0; t < N; ++t) pin(a[t]);
+}
+
 // Grid-stride walk over [0, n) for kernels whose grid may be smaller than the
 // index space: the blocks on one XCD sweep one contiguous slab of it (n / 8,
 // rounded up to whole blocks), so the neighbouring planes a stencil reads are

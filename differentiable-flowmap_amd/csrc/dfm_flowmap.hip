@@ -317,16 +317,27 @@ DFM_HD void dual_interp(const float* __restrict__ fd, const float* __restrict__ 
     wy[t] = axis == 1 ? ay.dw[t] : ay.w[t];
     if (DIM == 3) wz[t] = axis == 2 ? az.dw[t] : az.w[t];
   }
+  // Each slice's taps of both fields are loaded and pinned before the
+  // first is used (the compiler otherwise waits for every load before the
+  // next, keeping registers low); the multiply-adds keep their order.
   float d = 0.f, v = 0.f;
   if (DIM == 2) {
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
       int64_t rb = (int64_t)ax.idx[a] * S[1];
+      float td[NT], tv[NT];
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        td[b] = fd[rb + ay.idx[b]];
+        tv[b] = fv[rb + ay.idx[b]];
+      }
+      pin(td);
+      pin(tv);
       float rd = 0.f, rv = 0.f;
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
-        rd = fmaf(fd[rb + ay.idx[b]], wy[b], rd);
-        rv = fmaf(fv[rb + ay.idx[b]], ay.w[b], rv);
+        rd = fmaf(td[b], wy[b], rd);
+        rv = fmaf(tv[b], ay.w[b], rv);
       }
       d = fmaf(rd, wx[a], d);
       v = fmaf(rv, ax.w[a], v);
@@ -334,15 +345,29 @@ DFM_HD void dual_interp(const float* __restrict__ fd, const float* __restrict__ 
   } else {
 #pragma unroll 1
     for (int a = 0; a < NT; ++a) {
+      float td[NT][NT], tv[NT][NT];
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        const int64_t rb = ((int64_t)ax.idx[a] * S[1] + ay.idx[b]) * S[2];
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+          td[b][c] = fd[rb + az.idx[c]];
+          tv[b][c] = fv[rb + az.idx[c]];
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        pin(td[b]);
+        pin(tv[b]);
+      }
       float yd = 0.f, yv = 0.f;
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
-        int64_t rb = ((int64_t)ax.idx[a] * S[1] + ay.idx[b]) * S[2];
         float zd = 0.f, zv = 0.f;
 #pragma unroll
         for (int c = 0; c < NT; ++c) {
-          zd = fmaf(fd[rb + az.idx[c]], wz[c], zd);
-          zv = fmaf(fv[rb + az.idx[c]], az.w[c], zv);
+          zd = fmaf(td[b][c], wz[c], zd);
+          zv = fmaf(tv[b][c], az.w[c], zv);
         }
         yd = fmaf(zd, wy[b], yd);
         yv = fmaf(zv, ay.w[b], yv);

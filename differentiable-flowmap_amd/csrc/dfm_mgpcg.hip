@@ -225,18 +225,6 @@ DFM_HD unsigned nbr_load(const Dims& d, const T* __restrict__ f, int i, int j, i
   return m;
 }
 
-// An empty asm that takes x: the load that produced x is then used
-// unconditionally here, so it is issued with the others before any wait.
-template <typename T>
-DFM_HD void pin(T& x) {
-  asm volatile("" : "+v"(x));
-}
-template <typename T, int N>
-DFM_HD void pin(T (&a)[N]) {
-#pragma unroll
-  for (int t = 0; t < N; ++t) pin(a[t]);
-}
-
 // Cell index -> (i, j, k); grids stay below 2^31 cells (checked at creation).
 DFM_HD void cell_of(const Dims& d, uint32_t id, int& i, int& j, int& k) {
   uint32_t u = (uint32_t)id;
