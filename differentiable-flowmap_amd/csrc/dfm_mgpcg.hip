@@ -140,23 +140,8 @@ DFM_HD void split_nbrs(const Dims& d, int i, int j, int k, uint32_t id, uint32_t
 // cells when MASK (num_fluid_neighbors / neighbor_sum, 2D/vortex/mgpcg.py:92-115;
 // the obstacle form num_fluid_neighbors2 / neighbor_sum2,
 // 2D/vortex/mgpcg_solid.py:140-170), in the order -x, +x, -y, +y, -z, +z.
-template <int DIM, bool MASK, bool SPLIT = false, typename F>
+template <int DIM, bool MASK, typename F>
 DFM_HD void for_nbrs(const Dims& d, int i, int j, int k, uint32_t id, F&& fn) {
-  if constexpr (SPLIT) {
-    static_assert(!MASK, "the colour-split layout has no obstacle form");
-    uint32_t q[6];
-    bool in[6];
-    split_nbrs<DIM>(d, i, j, k, id, q, in);
-    if (in[0]) fn(q[0], i - 1, j, k);
-    if (in[1]) fn(q[1], i + 1, j, k);
-    if (in[2]) fn(q[2], i, j - 1, k);
-    if (in[3]) fn(q[3], i, j + 1, k);
-    if (DIM == 3) {
-      if (in[4]) fn(q[4], i, j, k - 1);
-      if (in[5]) fn(q[5], i, j, k + 1);
-    }
-    return;
-  }
   const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
   auto visit = [&](bool inside, uint32_t q, int a, int b, int c) {
     if (inside && is_fluid<MASK>(d, q)) fn(q, a, b, c);
@@ -172,11 +157,11 @@ DFM_HD void for_nbrs(const Dims& d, int i, int j, int k, uint32_t id, F&& fn) {
 }
 
 // neighbour sum and count
-template <int DIM, bool MASK = false, bool SPLIT = false>
+template <int DIM, bool MASK = false>
 DFM_HD float nbr_sum(const float* __restrict__ f, const Dims& d, int i, int j, int k, float& nn) {
   float s = 0.f;
   nn = 0.f;
-  for_nbrs<DIM, MASK, SPLIT>(d, i, j, k, cell_at<SPLIT>(d, i, j, k), [&](uint32_t q, int, int, int) {
+  for_nbrs<DIM, MASK>(d, i, j, k, cidx(d, i, j, k), [&](uint32_t q, int, int, int) {
     s += f[q];
     nn += 1.f;
   });
@@ -787,11 +772,12 @@ __global__ void __launch_bounds__(kB) k_update_p(int64_t N, const float* __restr
 }
 
 // -------------------------------------------------------------- V-cycle ----
-// First RBGS sweep from z = 0: colour 0 cells z = r/nn; colour 1 cells use the
-// colour-0 neighbours' r/nn.  `shift` (level 0) recentres r on the fly.
-// MASK: solid cells get z = 0 and are not neighbours (smooth of the obstacle
-// solver, 2D/vortex/mgpcg_solid.py:229-236).
-template <int DIM, bool MASK, bool SPLIT>
+// First RBGS sweep from z = 0 with an obstacle mask: colour 0 cells z = r/nn;
+// colour 1 cells use the colour-0 neighbours' r/nn.  `shift` (level 0)
+// recentres r on the fly.  Solid cells get z = 0 and are not neighbours
+// (smooth of the obstacle solver, 2D/vortex/mgpcg_solid.py:229-236).  Without
+// a mask the same sweep runs as k_first_half plus an ordinary half-sweep.
+template <int DIM, bool MASK>
 __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restrict__ r, float* __restrict__ z,
                                                     const double* shift, const int* flags) {
 #pragma clang fp contract(off)
@@ -805,7 +791,7 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
     return;
   }
   int i, j, k;
-  cell_of_at<DIM, SPLIT>(d, id, i, j, k);
+  cell_of(d, id, i, j, k);
   float nn = nbr_count<DIM, MASK>(d, i, j, k);
   if (((i + j + k) & 1) == 0) {
     z[id] = (r[id] - m) * inv_count(nn);
@@ -813,7 +799,7 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
   }
   // every neighbour value is loaded before the first is used
   float rn[2 * DIM];
-  const unsigned has = nbr_load<DIM, MASK, SPLIT>(d, r, i, j, k, id, rn);
+  const unsigned has = nbr_load<DIM, MASK, false>(d, r, i, j, k, id, rn);
   pin(rn);
   float rv = r[id] - m;
   const int c3[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
@@ -827,19 +813,19 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
 // Gauss-Seidel update of cell (i, j, k): (r - m + sum of neighbour z) / nn.
 // With zc, the neighbours' values are read as z + zc[coarse parent] (fused
 // prolongation; the cell's own value is overwritten, so it never needs it).
-template <int DIM, bool MASK, bool SPLIT = false>
+template <int DIM, bool MASK>
 DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* __restrict__ z,
                       const float* __restrict__ zc, const Dims& dc, float m, int i, int j, int k, uint32_t id) {
   float nn, s;
   if (zc) {
     s = 0.f;
     nn = 0.f;
-    for_nbrs<DIM, MASK, SPLIT>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
+    for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
       s += z[q] + zc[cidx(dc, a >> 1, b >> 1, DIM == 3 ? (c >> 1) : 0)];
       nn += 1.f;
     });
   } else {
-    s = nbr_sum<DIM, MASK, SPLIT>(z, d, i, j, k, nn);
+    s = nbr_sum<DIM, MASK>(z, d, i, j, k, nn);
   }
   return (r[id] - m + s) * inv_count(nn);
 }
@@ -1843,7 +1829,7 @@ bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool 
   if (from_zero) {
     z = L.z;
     if constexpr (MASK) {
-      hipLaunchKernelGGL((k_first_sweep<DIM, MASK, SPLIT>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, z, shift,
+      hipLaunchKernelGGL((k_first_sweep<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, L.r, z, shift,
                          s.flags);
     } else {
       // colour c1 from zero, then an ordinary half-sweep of the other colour
