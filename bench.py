@@ -30,9 +30,10 @@ METRIC = "fwd+adjoint sim steps/s at 128³ MAC grid; MGPCG HBM GB/s; grad rel-er
 HBM_PEAK_GBPS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
-# Algorithmic work per unit (flop / byte models; see DESIGN.md "Measurement").
-# 3D quadratic B-spline interpolation of one scalar: value + gradient ~480
-# flop, value only ~220 flop (64 taps, separable weights).
+# Algorithmic work per unit (flop / byte models).
+# 3D quadratic B-spline interpolation of one scalar in the reference's tap form
+# (a 4x4x4 loop, separable weights): value + gradient ~480 flop, value only
+# ~220 flop.  The kernels skip the 37 taps whose weight is exactly zero.
 FLOP_INTERP_GRAD_3D = 480.0
 FLOP_INTERP_VAL_3D = 220.0
 # MGPCG compulsory FP32 traffic: bytes per fine cell per PCG iteration (3D) and
