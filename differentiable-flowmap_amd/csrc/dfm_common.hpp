@@ -76,6 +76,14 @@ DFM_HD void pin(T (&a)[N]) {
   for (int t = 0; t < N; ++t) pin(a[t]);
 }
 
+// f + o for a 32-bit element offset o into a field of fewer than 2^30
+// values.  The byte offset stays 32-bit, so a load from a wave-uniform f is
+// one base register pair plus one offset register, with no 64-bit multiply
+// or add per tap.
+DFM_HD const float* at32(const float* f, uint32_t o) {
+  return reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (o << 2));
+}
+
 // Grid-stride walk over [0, n) for kernels whose grid may be smaller than the
 // index space: the blocks on one XCD sweep one contiguous slab of it (n / 8,
 // rounded up to whole blocks), so the neighbouring planes a stencil reads are
@@ -296,11 +304,11 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     float v = 0.f, gx = 0.f, gy = 0.f;
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
-      const float* row = f + (int64_t)ax.idx[a] * S[1];
+      const uint32_t row = __umul24((uint32_t)ax.idx[a], (uint32_t)S[1]);
       float rv = 0.f, rd = 0.f;
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
-        float val = row[ay.idx[b]];
+        float val = *at32(f, row + (uint32_t)ay.idx[b]);
         rv = fmaf(val, ay.w[b], rv);
         if (GRAD) rd = fmaf(val, ay.dw[b], rd);
       }
@@ -327,16 +335,20 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     // requested before the first is used, one memory round trip per sample
     // instead of one per x slice; the multiply-adds keep their order.  The
     // wider splines keep one slice in flight at a time (register pressure).
+    // Row offsets are 32-bit with 24-bit multiplies (a plane holds fewer than
+    // 2^24 values and a field fewer than 2^30); the quadratic z taps of a row
+    // are consecutive, so a row is one load at one offset.
     float tv[K == 2 ? NT : 1][K == 2 ? NT : 1][K == 2 ? NT : 1];
     if constexpr (K == 2) {
 #pragma unroll
       for (int a = 0; a < NT; ++a) {
-        const int ia = tap_index<K>(bx, a, S[0]);
+        const uint32_t ia = (uint32_t)tap_index<K>(bx, a, S[0]);
 #pragma unroll
         for (int b = 0; b < NT; ++b) {
-          const float* row = f + ((int64_t)ia * S[1] + ay.idx[b]) * S[2];
+          const uint32_t ro = __umul24(__umul24(ia, (uint32_t)S[1]) + (uint32_t)ay.idx[b], (uint32_t)S[2]);
+          const float* row = at32(f, ro + (uint32_t)az.idx[0]);
 #pragma unroll
-          for (int c = 0; c < NT; ++c) tv[a][b][c] = row[az.idx[c]];
+          for (int c = 0; c < NT; ++c) tv[a][b][c] = row[c];
         }
       }
     }

@@ -324,12 +324,12 @@ DFM_HD void dual_interp(const float* __restrict__ fd, const float* __restrict__ 
   if (DIM == 2) {
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
-      int64_t rb = (int64_t)ax.idx[a] * S[1];
+      const uint32_t rb = __umul24((uint32_t)ax.idx[a], (uint32_t)S[1]);
       float td[NT], tv[NT];
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
-        td[b] = fd[rb + ay.idx[b]];
-        tv[b] = fv[rb + ay.idx[b]];
+        td[b] = *at32(fd, rb + (uint32_t)ay.idx[b]);
+        tv[b] = *at32(fv, rb + (uint32_t)ay.idx[b]);
       }
       pin(td);
       pin(tv);
@@ -348,11 +348,11 @@ DFM_HD void dual_interp(const float* __restrict__ fd, const float* __restrict__ 
       float td[NT][NT], tv[NT][NT];
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
-        const int64_t rb = ((int64_t)ax.idx[a] * S[1] + ay.idx[b]) * S[2];
+        const uint32_t rb = __umul24(__umul24((uint32_t)ax.idx[a], (uint32_t)S[1]) + (uint32_t)ay.idx[b], (uint32_t)S[2]);
 #pragma unroll
         for (int c = 0; c < NT; ++c) {
-          td[b][c] = fd[rb + az.idx[c]];
-          tv[b][c] = fv[rb + az.idx[c]];
+          td[b][c] = *at32(fd, rb + (uint32_t)az.idx[c]);
+          tv[b][c] = *at32(fv, rb + (uint32_t)az.idx[c]);
         }
       }
 #pragma unroll
@@ -467,6 +467,13 @@ int check_grid(const dfm_grid* g, Geo& G) {
     if (a >= g->dim && g->n[a] != 1) return set_error(DFM_E_ARG, "2D grid must have n[2] == 1");
   }
   if (!(g->dx > 0.f)) return set_error(DFM_E_ARG, "dx must be positive");
+  // The samplers address a field with 32-bit offsets and 24-bit multiplies:
+  // every face or cell grid holds fewer than 2^30 values and, in 3D, its
+  // (x, y) plane fewer than 2^24.
+  const int64_t plane = (int64_t)(g->n[0] + 1) * (g->n[1] + 1);
+  const int64_t vol = g->dim == 3 ? plane * (g->n[2] + 1) : plane;
+  if (vol >= (int64_t(1) << 30) || (g->dim == 3 && plane >= (int64_t(1) << 24)))
+    return set_error(DFM_E_ARG, "grid too large (a face grid must hold fewer than 2^30 values, a 3D plane 2^24)");
   G.dim = g->dim;
   for (int a = 0; a < 3; ++a) G.n[a] = g->n[a];
   G.dx = g->dx;
