@@ -29,6 +29,7 @@
 #include "../../include/dfm.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 
@@ -1695,6 +1696,7 @@ struct Solver {
   int split_opt;     // option: level 0 colour-split where eligible (set_layout)
   double* xn;        // natural-order copy of a colour-split pressure (on request)
   hipStream_t last_st;  // stream of the last solve
+  int ready;         // partials, counters, control block and x zeroed (ensure_ready)
 };
 
 // Largest LDS a workgroup may request on this device.
@@ -2178,6 +2180,36 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   return pcg_end<DIM, MASK>(s);
 }
 
+// Zero-fills n 32-bit words; a kernel rather than a memset, so that a caller
+// capturing the first solve into a graph records kernel nodes only.
+__global__ void __launch_bounds__(kB) k_zero_words(uint32_t* __restrict__ p, uint64_t n) {
+  for (uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x; id < n; id += (uint64_t)gridDim.x * kB) p[id] = 0u;
+}
+
+// The first call on a handle that launches work zeroes the reduction
+// partials and counters, the control block and x, ordered on the caller's
+// stream.  Creation itself only allocates: it submits nothing and waits for
+// nothing, so it is safe while other threads capture graphs or keep their
+// streams busy.
+int ensure_ready(Solver& s, hipStream_t st) {
+  if (s.ready) return DFM_OK;
+  static_assert(kCtlBytes % 4 == 0, "control block is a whole number of words");
+  const struct {
+    void* p;
+    uint64_t words;
+  } bufs[4] = {{s.part, 2ull * kRedBlocks * 2}, {s.ctr, kCtrWords}, {s.sc, kCtlBytes / 4},
+               {s.x, 2ull * (uint64_t)s.lv[0].size}};
+  for (const auto& b : bufs) {
+    const uint64_t blocks = (b.words + kB - 1) / kB;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(kB), 0, st,
+                       static_cast<uint32_t*>(b.p), b.words);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: zeroing the handle's buffers", hipGetErrorString(e));
+  s.ready = 1;
+  return DFM_OK;
+}
+
 void destroy(Solver* s) {
   if (!s) return;
   for (int l = 0; l < s->L; ++l) {
@@ -2242,36 +2274,42 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->fused_sweeps = 1;
   s->tiled_sweeps = 1;
   s->split_opt = 1;
-  bool ok = true;
-  for (int l = 0; l < levels && ok; ++l) {
+  // Allocation only (ensure_ready zeroes on the first call that launches
+  // work); the first failing call is named in the error.
+  const char* failed = nullptr;
+  hipError_t err = hipSuccess;
+  auto need = [&](hipError_t e, const char* what) {
+    if (!failed && e != hipSuccess) {
+      failed = what;
+      err = e;
+    }
+    return failed == nullptr;
+  };
+  for (int l = 0; l < levels && !failed; ++l) {
     for (int a = 0; a < 3; ++a) s->lv[l].n[a] = a < g->dim ? (g->n[a] >> l) : 1;
     s->lv[l].size = (int64_t)s->lv[l].n[0] * s->lv[l].n[1] * s->lv[l].n[2];
     s->ld[l] = make_dims(s->lv[l].n[0], s->lv[l].n[1], s->lv[l].n[2], g->dim);
-    ok = hipMalloc(&s->lv[l].r, sizeof(float) * s->lv[l].size) == hipSuccess &&
-         hipMalloc(&s->lv[l].z, sizeof(float) * s->lv[l].size) == hipSuccess;
-    if (ok && l > 0 && s->lv[l].size <= kSweep2MaxCells)
-      ok = hipMalloc(&s->lv[l].z2, sizeof(float) * s->lv[l].size) == hipSuccess;
+    need(hipMalloc(&s->lv[l].r, sizeof(float) * s->lv[l].size), "hipMalloc (level residual)") &&
+        need(hipMalloc(&s->lv[l].z, sizeof(float) * s->lv[l].size), "hipMalloc (level correction)");
+    if (!failed && l > 0 && s->lv[l].size <= kSweep2MaxCells)
+      need(hipMalloc(&s->lv[l].z2, sizeof(float) * s->lv[l].size), "hipMalloc (level correction copy)");
   }
   int64_t N = s->lv[0].size;
-  ok = ok && hipMalloc(&s->x, sizeof(double) * N) == hipSuccess && hipMalloc(&s->b, sizeof(float) * N) == hipSuccess &&
-       hipMalloc(&s->p, sizeof(float) * N) == hipSuccess &&
-       hipMalloc(&s->Ap, sizeof(float) * N) == hipSuccess &&
-       hipMalloc(&s->part, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
-       hipMalloc(&s->ctr, sizeof(unsigned) * kCtrWords) == hipSuccess &&
-       hipMalloc(&s->sc, kCtlBytes) == hipSuccess &&
-       hipHostMalloc(&s->h_sc, kCtlBytes, hipHostMallocDefault) == hipSuccess;
-  if (ok) {
-    s->flags = reinterpret_cast<int*>(reinterpret_cast<char*>(s->sc) + sizeof(double) * S_NSLOT);
-    s->h_flags = reinterpret_cast<int*>(reinterpret_cast<char*>(s->h_sc) + sizeof(double) * S_NSLOT);
-  }
-  if (ok) ok = hipMemset(s->part, 0, sizeof(double) * 2 * kRedBlocks) == hipSuccess &&
-               hipMemset(s->ctr, 0, sizeof(unsigned) * kCtrWords) == hipSuccess &&
-               hipMemset(s->sc, 0, kCtlBytes) == hipSuccess &&
-               hipMemset(s->x, 0, sizeof(double) * N) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
-  if (!ok) {
+  need(hipMalloc(&s->x, sizeof(double) * N), "hipMalloc (x)") && need(hipMalloc(&s->b, sizeof(float) * N), "hipMalloc (b)") &&
+      need(hipMalloc(&s->p, sizeof(float) * N), "hipMalloc (p)") &&
+      need(hipMalloc(&s->Ap, sizeof(float) * N), "hipMalloc (Ap)") &&
+      need(hipMalloc(&s->part, sizeof(double) * 2 * kRedBlocks), "hipMalloc (partials)") &&
+      need(hipMalloc(&s->ctr, sizeof(unsigned) * kCtrWords), "hipMalloc (counters)") &&
+      need(hipMalloc(&s->sc, kCtlBytes), "hipMalloc (control block)") &&
+      need(hipHostMalloc(&s->h_sc, kCtlBytes, hipHostMallocDefault), "hipHostMalloc (control block)");
+  if (failed) {
+    char detail[160];
+    std::snprintf(detail, sizeof(detail), "%s: %s", failed, hipGetErrorString(err));
     destroy(s);
-    return set_error(DFM_E_HIP, "dfm_mgpcg_create: device allocation failed");
+    return set_error(DFM_E_HIP, "dfm_mgpcg_create: device allocation failed", detail);
   }
+  s->flags = reinterpret_cast<int*>(reinterpret_cast<char*>(s->sc) + sizeof(double) * S_NSLOT);
+  s->h_flags = reinterpret_cast<int*>(reinterpret_cast<char*>(s->h_sc) + sizeof(double) * S_NSLOT);
   setup_tail(*s);
   *handle = s;
   return DFM_OK;
@@ -2365,6 +2403,7 @@ extern "C" int dfm_mgpcg_project_begin(void* handle, const dfm_mac* u, double to
   if (max_iters < 1) return set_error(DFM_E_ARG, "max_iters must be >= 1");
   if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (int rc0 = ensure_ready(*s, st)) return rc0;
   set_layout(*s);
   s->last_st = st;
   Lvl& L0 = s->lv[0];
@@ -2438,6 +2477,7 @@ extern "C" int dfm_mgpcg_solve(void* handle, const float* b, float* x, double to
   if (max_iters < 1) return set_error(DFM_E_ARG, "max_iters must be >= 1");
   if (s->pend.active) return set_error(DFM_E_ARG, "mgpcg: a begun projection has not been ended on this handle");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (int rc0 = ensure_ready(*s, st)) return rc0;
   set_layout(*s);
   s->last_st = st;
   Lvl& L0 = s->lv[0];
@@ -2475,6 +2515,7 @@ extern "C" int dfm_mgpcg_set_mask(void* handle, const float* mask, void* stream)
   for (int l = 0; l < s->L; ++l)
     if (!s->bm[l] && hipMalloc(&s->bm[l], (size_t)s->lv[l].size) != hipSuccess)
       return set_error(DFM_E_HIP, "dfm_mgpcg_set_mask: device allocation failed");
+  if (int rc0 = ensure_ready(*s, st)) return rc0;
   Lvl& L0 = s->lv[0];
   Red R = red_of(*s);
   hipLaunchKernelGGL(k_mask_level0, dim3(gridr(L0.size)), dim3(kB), 0, st, L0.size, mask, s->bm[0], R);
@@ -2512,6 +2553,7 @@ extern "C" int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual) 
 extern "C" int dfm_mgpcg_pressure(void* handle, double** p) {
   Solver* s = static_cast<Solver*>(handle);
   if (!s || !p) return set_error(DFM_E_NULL, "dfm_mgpcg_pressure: NULL argument");
+  if (!s->ready) return set_error(DFM_E_ARG, "dfm_mgpcg_pressure: no solve has run on this handle");
   if (!s->ld[0].hh) {
     *p = s->x;
     return DFM_OK;

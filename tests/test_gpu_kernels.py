@@ -472,6 +472,36 @@ def test_mgpcg_split_layout_matches_natural(n, levels, device):
     assert rel_l2(xs, xn) < 1e-5
 
 
+def test_mgpcg_first_solve_on_side_stream(device):
+    """Creating a handle only allocates.  Its reduction counters, control
+    block and x are zeroed on the stream of the first call that launches
+    work, so the pressure accessor refuses a handle that has not solved yet,
+    and a first solve on a non-default stream projects like one on the
+    default stream."""
+    import ctypes
+
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    n = (96, 64)
+    rng = np.random.default_rng(5)
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(2)])
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    s = MGPCG(n, 1.0 / n[1], tol=1e-7, max_iters=200)
+    ptr = ctypes.c_void_p()
+    assert L.lib().dfm_mgpcg_pressure(s.h, ctypes.byref(ptr)) == -1
+    side = torch.cuda.Stream()
+    d = mac_to_dev(u, device)
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        s.Poisson(d)
+    side.synchronize()
+    assert rel_l2_fields(mac_to_np(d), ref) < 2e-6
+    p = _pressure_of(s, n, device)
+    assert np.isfinite(p).all() and np.abs(p).max() > 0
+
+
 def test_mgpcg_split_layout_eligibility(device):
     """The split layout needs an even innermost extent, no obstacle mask and
     a finest level outside the LDS tail (grids whose levels all fit in one
