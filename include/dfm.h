@@ -33,6 +33,9 @@ extern "C" {
 #define DFM_E_NOCONV (-4)     /* PCG reached max_iters without converging */
 #define DFM_E_NONFINITE (-5)  /* PCG residual became NaN/Inf */
 
+/* The flow-map entry points accept grids whose face grids hold fewer than
+ * 2^30 values and, in 3D, whose (x, y) planes hold fewer than 2^24 (the
+ * samplers address fields with 32-bit offsets); larger grids give DFM_E_ARG. */
 typedef struct dfm_grid {
   int32_t dim;  /* 2 or 3 */
   int32_t n[3]; /* cells per axis; n[2] = 1 in 2D */
@@ -146,7 +149,10 @@ int dfm_axpy_mac(const dfm_grid* g, const dfm_mac* x, float alpha, const dfm_mac
  * Handle owning the multigrid workspaces for one grid.  Levels, smoothing and
  * bottom sweeps follow MGPCG (2D/vortex/mgpcg.py:17-31): levels <= 0 picks
  * log2(min n) - 3 + 1, lowered until every n halves exactly that often;
- * explicit levels require every n divisible by 2^(levels-1). */
+ * explicit levels require every n divisible by 2^(levels-1).  Creation only
+ * allocates (it submits no work and waits for none); the handle's counters
+ * and control block are zeroed on the stream of the first call that launches
+ * work (project, solve or set_mask). */
 int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int bottom, void** handle);
 int dfm_mgpcg_destroy(void* handle);
 
@@ -182,7 +188,8 @@ int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
 /* Device pointer of the pressure (cells, FP64, natural cell order) from the
  * last projection.  With a colour-split finest level the pressure is first
  * reordered into a second buffer by a kernel queued on the last solve's
- * stream: read it on that stream (or after synchronising it). */
+ * stream: read it on that stream (or after synchronising it).  DFM_E_ARG
+ * before the handle's first solve. */
 int dfm_mgpcg_pressure(void* handle, double** p);
 
 /* Solver options (no counterpart in the reference; tuning of this build).
