@@ -83,6 +83,10 @@ DFM_HD void pin(T (&a)[N]) {
 DFM_HD const float* at32(const float* f, uint32_t o) {
   return reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + (o << 2));
 }
+// The same for a 32-bit byte offset.
+DFM_HD const float* at32b(const float* f, uint32_t ob) {
+  return reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + ob);
+}
 
 // Grid-stride walk over [0, n) for kernels whose grid may be smaller than the
 // index space: the blocks on one XCD sweep one contiguous slab of it (n / 8,
@@ -302,13 +306,16 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     Axis<K, GRAD> ax;
     ax.setup(p[0] * inv_dx, bl[0], S[0]);
     float v = 0.f, gx = 0.f, gy = 0.f;
+    // quadratic: tap (a, b) at o00 + a S1 + b
+    const uint32_t o00 = __umul24((uint32_t)ax.idx[0], (uint32_t)S[1]) + (uint32_t)ay.idx[0];
 #pragma unroll
     for (int a = 0; a < NT; ++a) {
       const uint32_t row = __umul24((uint32_t)ax.idx[a], (uint32_t)S[1]);
       float rv = 0.f, rd = 0.f;
 #pragma unroll
       for (int b = 0; b < NT; ++b) {
-        float val = *at32(f, row + (uint32_t)ay.idx[b]);
+        float val = K == 2 ? at32b(f, (o00 << 2) + ((uint32_t)a * (uint32_t)S[1] << 2))[b]
+                           : *at32(f, row + (uint32_t)ay.idx[b]);
         rv = fmaf(val, ay.w[b], rv);
         if (GRAD) rd = fmaf(val, ay.dw[b], rd);
       }
@@ -335,18 +342,20 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     // requested before the first is used, one memory round trip per sample
     // instead of one per x slice; the multiply-adds keep their order.  The
     // wider splines keep one slice in flight at a time (register pressure).
-    // Row offsets are 32-bit with 24-bit multiplies (a plane holds fewer than
-    // 2^24 values and a field fewer than 2^30); the quadratic z taps of a row
-    // are consecutive, so a row is one load at one offset.
+    // Quadratic taps are unclamped (the support base is), so tap (a, b, c)
+    // sits at o00 + (a S1 + b) S2 + c.  o00 is one 32-bit offset formed with
+    // 24-bit multiplies (a plane holds fewer than 2^24 values, a field fewer
+    // than 2^30); each of the 9 row loads (3 consecutive z taps) adds a
+    // wave-uniform row stride to it in 32 bits.
     float tv[K == 2 ? NT : 1][K == 2 ? NT : 1][K == 2 ? NT : 1];
     if constexpr (K == 2) {
+      const uint32_t S1 = (uint32_t)S[1], S2 = (uint32_t)S[2];
+      const uint32_t o00 = __umul24(__umul24((uint32_t)bx, S1) + (uint32_t)ay.idx[0], S2) + (uint32_t)az.idx[0];
 #pragma unroll
       for (int a = 0; a < NT; ++a) {
-        const uint32_t ia = (uint32_t)tap_index<K>(bx, a, S[0]);
 #pragma unroll
         for (int b = 0; b < NT; ++b) {
-          const uint32_t ro = __umul24(__umul24(ia, (uint32_t)S[1]) + (uint32_t)ay.idx[b], (uint32_t)S[2]);
-          const float* row = at32(f, ro + (uint32_t)az.idx[0]);
+          const float* row = at32b(f, (o00 << 2) + (((uint32_t)a * S1 + (uint32_t)b) * S2 << 2));
 #pragma unroll
           for (int c = 0; c < NT; ++c) tv[a][b][c] = row[c];
         }
