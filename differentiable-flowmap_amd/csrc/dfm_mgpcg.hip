@@ -48,7 +48,6 @@ constexpr unsigned kCtrGroups = 64;  // arrival-counter groups of a grid reducti
 constexpr unsigned kCtrStride = 32;  // counters 128 B apart
 constexpr unsigned kCtrWords = (kCtrGroups + 1) * kCtrStride;
 constexpr int kMaxLevels = 12;
-constexpr int kLdsCells = 8192;  // bottom solve in one workgroup up to this size
 constexpr double kEps = 1e-12;   // denominator guard (2D/vortex/mgpcg.py:204)
 
 enum Slot { S_ZR = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THRESH, S_MEAN, S_PAP, S_NFLUID, S_NSLOT };
