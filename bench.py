@@ -343,8 +343,8 @@ def main():
         print(json.dumps(run_batch(args)), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design note, not a benchmarked path.", file=sys.stderr)
+        print(f"bench.py: the simulator runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is not implemented.", file=sys.stderr)
         sys.exit(2)
     if args.steps % 10 or args.warmup % 10:
         sys.exit("--steps and --warmup must be multiples of the reinitialisation interval (10)")

@@ -192,7 +192,7 @@ int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
  * before the handle's first solve. */
 int dfm_mgpcg_pressure(void* handle, double** p);
 
-/* Solver options (no counterpart in the reference; tuning of this build).
+/* Solver options (no counterpart in the reference; tuning of this library).
  *  DFM_MG_BF16_COARSE: 1 = store levels >= 2 in BF16 (FP32 arithmetic) so
  *      the coarse V-cycle (up to 32^3) runs in one workgroup's LDS; 0 = FP32.
  *  DFM_MG_TAIL: 1 = run the coarse levels that fit in LDS in one kernel
