@@ -40,6 +40,26 @@ FLOP_INTERP_VAL_3D = 220.0
 # per solve (divergence, init, gradient subtraction).
 MG_BYTES_CELL_ITER_3D = 156.0
 MG_BYTES_CELL_SOLVE_3D = 64.0
+# Gathered 4-byte taps per face point of one RK4 march: 4 stages x 3 MAC
+# components x 27 support taps (the gradient reuses the value's taps).
+MARCH_TAPS_FACE = 4 * 3 * 27
+
+
+REINIT_EVERY = 10
+
+
+def plan_steps(steps, warmup, reinit_every=REINIT_EVERY):
+    """Step counts the run actually uses.  The adjoint sweep walks back
+    through whole reinitialisation intervals (each interval's forward map is
+    stored at its reinitialisation), so both counts are rounded up to whole
+    intervals: a warmup of 5 runs one interval of 10.  Returns
+    {"steps", "warmup", "steps_requested", "warmup_requested"}."""
+    if steps < 1:
+        raise ValueError(f"--steps must be at least 1 (got {steps})")
+    if warmup < 0:
+        raise ValueError(f"--warmup must be at least 0 (got {warmup})")
+    up = lambda k: -(-k // reinit_every) * reinit_every  # noqa: E731
+    return {"steps": up(steps), "warmup": up(warmup), "steps_requested": steps, "warmup_requested": warmup}
 
 
 def march_flop(n, order=4):
@@ -88,7 +108,7 @@ def run_gpu(args):
     sc = scenes.scene_3d(n, seed=0, tubes=16)
     dx = sc["dx"]
     dt = 0.5 * dx
-    R = 10
+    R = REINIT_EVERY
     K, W = args.steps, args.warmup
     sim = LFM_Diff_Simulator(*n, dx, dt, R, None, total_steps=max(K, W), mg_levels=args.mg_levels, mg_pre_post=args.mg_pre_post,
                              mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
@@ -123,26 +143,30 @@ def run_gpu(args):
 
     # dominant flow-map kernel by total device time
     models = {
-        "march": (march_flop(n), "dfm_march (k_march<3,4>)"),
-        "sl_advect": (None, "dfm_sl_advect (k_sl_advect<3,4>)"),
-        "adj_conv": (None, "dfm_adj_conv_source (k_adj_conv<3,3>)"),
+        "march": (march_flop(n), "dfm_march (k_march<3,4>)", MARCH_TAPS_FACE),
+        "sl_advect": (sl_flop(n, 2, 2), "dfm_sl_advect (k_sl_advect<3,4>)", None),
+        "adj_conv": (None, "dfm_adj_conv_source (k_adj_conv<3,3>)", None),
     }
     dom = max((k for k in models if k in summ), key=lambda k: summ[k][1])
     calls, tot_ms = summ[dom]
     avg_s = tot_ms / calls / 1e3
-    if dom == "march":
-        flop = models["march"][0]
-    elif dom == "sl_advect":
-        flop = sl_flop(n, 2, 2)
-    else:
+    flop = models[dom][0]
+    if flop is None:
         faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
         flop = faces * 3 * 216 * 4
     achieved = flop / avg_s / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, n),
                 "kernel": models[dom][1], "avg_launch_ms": round(avg_s * 1e3, 4), "calls": calls,
                 "flop_per_launch": flop,
-                "note": "FP32 vector roofline (gather/VALU-bound B-spline stencils; no MFMA on this path)"}
+                "note": "B-spline stencil gathers with no dense contraction, so no MFMA: priced against the FP32 "
+                        "vector peak with the reference's 64-tap flop count; the kernel executes only the 27 "
+                        "support taps per sample (executed_flop_frac), and its limit is the rate of gathered "
+                        "L1 loads (gathered_TBps)"}
+    if models[dom][2] is not None:
+        faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
+        roofline["executed_flop_frac"] = round(achieved * 27.0 / 64.0 / FP32_PEAK_TFLOPS, 4)
+        roofline["gathered_TBps"] = round(faces * models[dom][2] * 4.0 / avg_s / 1e12, 2)
     mg_calls, mg_ms = summ.get("mgpcg", (0, 0.0))
     mg_iters = cnt.get("mgpcg_iters", 0)
     cells = float(np.prod(n))
@@ -151,33 +175,70 @@ def run_gpu(args):
     mgpcg = {"achieved_GBps": round(mg_gbps, 1), "peak_GBps": HBM_PEAK_GBPS, "frac": round(mg_gbps / HBM_PEAK_GBPS, 4),
              "solves": mg_calls, "pcg_iters_total": mg_iters,
              "iters_per_solve": round(mg_iters / max(mg_calls, 1), 2), "ms_per_solve": round(mg_ms / max(mg_calls, 1), 3),
+             "not_converged": sim.noconv_count,
              "bytes_model": "156 B/cell/PCG-iter + 64 B/cell/solve (FP32 compulsory)"}
     # The same solver projecting the scene's initial velocity with nothing
     # beside it: inside the run the projections share the GPU with the
     # marches on the side stream, so their wall time there is not the
-    # solver's own.  Event-timed on the solve stream, 10 solves after 2.
+    # solver's own.
     from dfm.ops import MacField
 
     src = MacField([torch.as_tensor(np.ascontiguousarray(c), dtype=torch.float32, device="cuda") for c in u0])
+    mgpcg["alone"] = solve_alone(sim.solver, src, n)
+    mgpcg["alone"]["field"] = "the scene's initial velocity (unprojected), projected alone"
+    if not args.no_mg256 and args.size != 256:
+        mgpcg["alone_256"] = mgpcg_256_alone(args)
+    breakdown = {k: {"calls": v[0], "ms": round(v[1], 2)} for k, v in summ.items()}
+    return elapsed, roofline, mgpcg, breakdown
+
+
+def solve_alone(solver, src, n, reps=10):
+    """Event-timed projections of `src` by `solver` on the current stream
+    with nothing else queued: 2 untimed, then `reps` timed."""
+    import numpy as np
+    import torch
+
     work = src.clone()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    alone_ms, alone_its, reps = 0.0, 0, 10
+    ms, its, noconv0 = 0.0, 0, solver.noconv_count
     for rep in range(reps + 2):
         work.copy_(src)
         ev0.record()
-        sim.solver.Poisson(work)
+        solver.Poisson(work)
         ev1.record()
         torch.cuda.synchronize()
         if rep >= 2:
-            alone_ms += ev0.elapsed_time(ev1)
-            alone_its += sim.solver.last_iters
-    a_bytes = cells * (MG_BYTES_CELL_ITER_3D * alone_its + MG_BYTES_CELL_SOLVE_3D * reps)
-    a_gbps = a_bytes / (alone_ms / 1e3) / 1e9
-    mgpcg["alone"] = {"achieved_GBps": round(a_gbps, 1), "frac": round(a_gbps / HBM_PEAK_GBPS, 4),
-                      "ms_per_solve": round(alone_ms / reps, 3), "iters_per_solve": round(alone_its / reps, 2),
-                      "solves": reps, "field": "the scene's initial velocity (unprojected), projected alone"}
-    breakdown = {k: {"calls": v[0], "ms": round(v[1], 2)} for k, v in summ.items()}
-    return elapsed, roofline, mgpcg, breakdown
+            ms += ev0.elapsed_time(ev1)
+            its += solver.last_iters
+    cells = float(np.prod(n))
+    gbps = cells * (MG_BYTES_CELL_ITER_3D * its + MG_BYTES_CELL_SOLVE_3D * reps) / (ms / 1e3) / 1e9
+    return {"achieved_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4), "ms_per_solve": round(ms / reps, 3),
+            "iters_per_solve": round(its / reps, 2), "solves": reps, "not_converged": solver.noconv_count - noconv0}
+
+
+def mgpcg_256_alone(args):
+    """The solver at 256^3 (6 levels, the bandwidth-bound size: one FP32
+    field is 67 MB, past what the caches hold) projecting a seeded random
+    MAC field alone, with the run's solver options."""
+    import torch
+
+    from dfm.ops import MGPCG, MacField
+
+    n = (256, 256, 256)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1)
+    src = MacField.empty(n, "cuda")
+    for c in src.c:
+        c.copy_(torch.rand(c.shape, generator=gen, device="cuda") - 0.5)
+    solver = MGPCG(n, 1.0 / 256, levels=0, pre_post=args.mg_pre_post, bottom=50, tol=1e-6, max_iters=200,
+                   bf16_coarse=args.mg_bf16_coarse, tail=not args.mg_no_tail, bottom_exact=args.mg_bottom_exact,
+                   check_every=args.mg_check_every, fused_sweeps=args.mg_fused_sweeps,
+                   split_layout=not args.mg_natural_layout)
+    out = solve_alone(solver, src, n)
+    out["field"] = "seeded uniform random MAC field in [-0.5, 0.5), 256^3, projected alone"
+    del solver, src
+    torch.cuda.empty_cache()
+    return out
 
 
 def run_grad_check():
@@ -300,7 +361,7 @@ def run_copy_calibration():
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
 
 
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
@@ -308,6 +369,7 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad-check", action="store_true")
+    ap.add_argument("--no-mg256", action="store_true", help="skip the 256^3 MGPCG-alone measurement")
     ap.add_argument("--mg-no-tail", action="store_true", help="one launch per sweep on every MG level")
     ap.add_argument("--mg-bf16-coarse", action="store_true", help="BF16 storage of MG levels >= 2")
     ap.add_argument("--mg-rbgs-bottom", dest="mg_bottom_exact", action="store_false",
@@ -331,7 +393,15 @@ def main():
                     help="GPU_MAX_HW_QUEUES for --batch (1..32; 0 keeps the platform's setting: 16 measured slower than 4)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy used to calibrate the HBM traffic counters")
-    args = ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def main(argv=None):
+    args = parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: the simulator runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is not implemented.", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         run_copy_calibration()
         return
@@ -342,12 +412,11 @@ def main():
             configure_hw_queues(args.hw_queues)
         print(json.dumps(run_batch(args)), flush=True)
         return
-    if args.gpus != 1:
-        print(f"bench.py: the simulator runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is not implemented.", file=sys.stderr)
-        sys.exit(2)
-    if args.steps % 10 or args.warmup % 10:
-        sys.exit("--steps and --warmup must be multiples of the reinitialisation interval (10)")
+    try:
+        plan = plan_steps(args.steps, args.warmup)
+    except ValueError as e:
+        sys.exit(f"bench.py: {e}")
+    args.steps, args.warmup = plan["steps"], plan["warmup"]
     elapsed, roofline, mgpcg, breakdown = run_gpu(args)
     K = args.steps
     out = {
@@ -357,6 +426,8 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
+        "steps_requested": plan["steps_requested"],
+        "warmup_requested": plan["warmup_requested"],
         "ms_per_step": round(elapsed / K * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",

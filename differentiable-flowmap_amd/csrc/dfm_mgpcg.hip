@@ -48,7 +48,13 @@ constexpr unsigned kCtrGroups = 64;  // arrival-counter groups of a grid reducti
 constexpr unsigned kCtrStride = 32;  // counters 128 B apart
 constexpr unsigned kCtrWords = (kCtrGroups + 1) * kCtrStride;
 constexpr int kMaxLevels = 12;
-constexpr double kEps = 1e-12;   // denominator guard (2D/vortex/mgpcg.py:204)
+// alpha = z.r / p.Ap and beta = z.r' / z.r with a guard against a zero
+// denominator only.  The reference adds eps = 1e-12 to both denominators
+// (2D/vortex/mgpcg.py:204,240,272), which is harmless with its absolute stop
+// r.r < 1e-12 but not with a relative one: on a nearly divergence-free field
+// (b of order 1e-5 per cell) z.r falls below 1e-12 while r.r is still 1e-7
+// of b.b, the guard then dominates alpha, and PCG stalls at that level.
+DFM_HD double safe_ratio(double num, double den) { return den != 0.0 ? num / den : 0.0; }
 
 enum Slot { S_ZR = 0, S_ALPHA, S_BETA, S_RR, S_BB, S_THRESH, S_MEAN, S_PAP, S_NFLUID, S_NSLOT };
 enum Flag { F_DONE = 0, F_ITERS, F_STATUS, F_NTRUE, F_NFLAG };  // F_NTRUE: failed true-residual checks
@@ -351,7 +357,7 @@ DFM_HD void fin_op(const Red& R, int op, double a, double b) {
       break;
     case FIN_ALPHA:  // a = p.Ap
       sc[S_PAP] = a;
-      sc[S_ALPHA] = sc[S_ZR] / (a + kEps);
+      sc[S_ALPHA] = safe_ratio(sc[S_ZR], a);
       break;
     case FIN_RR: {  // a = r.r, b = sum r
       sc[S_RR] = a;
@@ -367,7 +373,7 @@ DFM_HD void fin_op(const Red& R, int op, double a, double b) {
       break;
     }
     case FIN_BETA: {  // a = z.r (new)
-      sc[S_BETA] = a / (sc[S_ZR] + kEps);
+      sc[S_BETA] = safe_ratio(a, sc[S_ZR]);
       sc[S_ZR] = a;
       break;
     }

@@ -6,6 +6,8 @@ Fields are float32 CUDA (HIP) tensors.  A MAC field is a list of face tensors
 """
 from __future__ import annotations
 
+import warnings
+
 import torch
 
 from . import _lib as L
@@ -187,13 +189,25 @@ class Ops:
                                       z.s if z is not None else None, out.s, self._st()), "axpy")
 
 
+class NoConvergenceWarning(RuntimeWarning):
+    """A PCG solve stopped at max_iters above its tolerance (DFM_E_NOCONV);
+    the gradient was still subtracted, so the field is only approximately
+    divergence-free."""
+
+
 class MGPCG:
     """Pressure projection on an all-Neumann box (the reference's
     ``solver.Poisson`` / ``Poisson_adj``, 2D/vortex/mgpcg.py:363-368,
-    3D/fast_mgpcg.py:241-323), backed by the libdfm MGPCG handle."""
+    3D/fast_mgpcg.py:241-323), backed by the libdfm MGPCG handle.
+
+    ``on_noconv`` says what a solve that reaches max_iters without converging
+    does: "warn" (default) issues a NoConvergenceWarning, "raise" raises the
+    DfmError, "count" only counts it.  Every such solve is counted in
+    ``noconv_count`` (the reference prints and carries on,
+    2D/vortex/mgpcg.py:280-282)."""
 
     def __init__(self, n, dx, levels=0, pre_post=2, bottom=50, tol=1e-6, relative=True, max_iters=400,
-                 raise_on_noconv=False, bf16_coarse=False, tail=True, graph=True, check_every=1,
+                 on_noconv="warn", bf16_coarse=False, tail=True, graph=True, check_every=1,
                  bottom_exact=False, fused_sweeps=2, split_layout=True):
         self.n = tuple(n)
         self.g = L.make_grid(n, dx)
@@ -217,7 +231,9 @@ class MGPCG:
         if not split_layout:
             self.set_option(L.MG_SPLIT_LAYOUT, 0)
         self.tol, self.relative, self.max_iters = tol, relative, max_iters
-        self.raise_on_noconv = raise_on_noconv
+        if on_noconv not in ("warn", "raise", "count"):
+            raise ValueError(f"on_noconv must be 'warn', 'raise' or 'count', not {on_noconv!r}")
+        self.on_noconv = on_noconv
         self.last_iters = 0
         self.noconv_count = 0
 
@@ -250,9 +266,15 @@ class MGPCG:
         return it.value, res.value
 
     def _done(self, rc, what):
-        if rc == L.DFM_E_NOCONV and not self.raise_on_noconv:
+        if rc == L.DFM_E_NOCONV:
             self.noconv_count += 1
-            rc = 0
+            timing.add_count("mgpcg_noconv")
+            if self.on_noconv != "raise":
+                if self.on_noconv == "warn":
+                    it, res = self.status()
+                    warnings.warn(f"{what}: PCG stopped at {it} iterations with relative residual {res:.3e} "
+                                  f"above tol {self.tol:g}", NoConvergenceWarning, stacklevel=3)
+                rc = 0
         L.check(rc, what)
         self.last_iters = self.status()[0]
 

@@ -39,9 +39,17 @@ gaussian_force         a force far from the walls integrates to
 gaussian_force_adjoint adj_strength / adj_center = central differences of
                        <f, lam> in s / c; penalty_* = central differences of
                        sum over forces of sum f_I^2 (f_I: force I alone)
-OracleLFM (whole step) composition of the above; the trajectory itself is
-                       **parity unpinned** against the reference (Taichi cannot
-                       run here) -- only its pieces are pinned.
+OracleLFM (whole step) tests/test_oracle_composition.py: the steady
+                       Taylor-Green flow and a passive it carries (exact
+                       characteristics) converge at second order at
+                       reinitialisations and first order inside an interval
+                       (2D and 3D); the adjoint's gap to central differences of
+                       the oracle's own loss shrinks under refinement (2D first
+                       order, 3D second); planted mistakes (dropped Jacobian,
+                       flipped time steps, dropped or mis-weighted adjoint
+                       sources) fail these pins.  Bit-level agreement with the
+                       reference's Taichi trajectories stays unpinned (Taichi
+                       cannot run here).
 =====================  ==========================================================
 """
 from __future__ import annotations

@@ -23,6 +23,22 @@ def gpu_available():
         return False
 
 
+@pytest.fixture(autouse=True)
+def _noconv_is_an_error(request):
+    """In GPU tests a projection that stops at max_iters above its tolerance
+    fails the test (the library warns by default)."""
+    if request.node.get_closest_marker("gpu") is None:
+        yield
+        return
+    import warnings
+
+    from dfm.ops import NoConvergenceWarning
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("error", NoConvergenceWarning)
+        yield
+
+
 @pytest.fixture(scope="session")
 def device():
     import torch
@@ -30,3 +46,19 @@ def device():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     return torch.device("cuda:0")
+
+
+def pytest_sessionfinish(session, exitstatus):
+    """Write the bounds checked through gpu_util.within() and the values they
+    saw to $DFM_PARITY_LOG (JSON), when that is set."""
+    path = os.environ.get("DFM_PARITY_LOG")
+    mod = sys.modules.get("gpu_util")
+    if not path or mod is None:
+        return
+    import json
+
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(mod.PARITY, f, indent=1)

@@ -5,7 +5,7 @@ the FP64 oracle, and the worker-count autotuner picks its fastest probe."""
 import numpy as np
 import pytest
 
-from gpu_util import rel_l2, round32
+from gpu_util import ADJ_TOL, FWD_TOL, rel_l2, round32, within
 
 from oracle import lfm_oracle as O
 
@@ -54,8 +54,8 @@ def test_instance_gradients_match_oracle(device):
             g[k] = sum(np.sum(l * (a - b)) for l, a, b in zip(orc.lam_u, faces(th + e), faces(th - e))) / 2e-6
         loss, grad = res[i]
         print(f"instance {i} n={n}: loss rel {abs(loss - orc.loss) / orc.loss:.2e}, grad rel-L2 {rel_l2(grad, g):.2e}")
-        assert abs(loss - orc.loss) <= 1e-4 * orc.loss
-        assert rel_l2(grad, g) < 1e-3
+        assert abs(loss - orc.loss) <= 1e-5 * orc.loss
+        within("grad", rel_l2(grad, g), ADJ_TOL)
 
 
 def test_autotune_keeps_fastest(device):
