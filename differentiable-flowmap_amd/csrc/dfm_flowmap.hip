@@ -468,12 +468,16 @@ int check_grid(const dfm_grid* g, Geo& G) {
   }
   if (!(g->dx > 0.f)) return set_error(DFM_E_ARG, "dx must be positive");
   // The samplers address a field with 32-bit offsets and 24-bit multiplies:
-  // every face or cell grid holds fewer than 2^30 values and, in 3D, its
-  // (x, y) plane fewer than 2^24.
+  // every face or cell grid holds fewer than 2^30 values; the multiplied
+  // operands (a 3D (x, y) plane, or in 2D each extent) stay below 2^24.
   const int64_t plane = (int64_t)(g->n[0] + 1) * (g->n[1] + 1);
   const int64_t vol = g->dim == 3 ? plane * (g->n[2] + 1) : plane;
-  if (vol >= (int64_t(1) << 30) || (g->dim == 3 && plane >= (int64_t(1) << 24)))
-    return set_error(DFM_E_ARG, "grid too large (a face grid must hold fewer than 2^30 values, a 3D plane 2^24)");
+  const int64_t lim24 = int64_t(1) << 24;
+  if (vol >= (int64_t(1) << 30) || (g->dim == 3 && plane >= lim24) ||
+      (g->dim == 2 && (g->n[0] + 1 >= lim24 || g->n[1] + 1 >= lim24)))
+    return set_error(DFM_E_ARG,
+                     "grid too large (a face grid must hold fewer than 2^30 values, a 3D plane and each 2D extent "
+                     "fewer than 2^24)");
   G.dim = g->dim;
   for (int a = 0; a < 3; ++a) G.n[a] = g->n[a];
   G.dx = g->dx;

@@ -34,8 +34,9 @@ extern "C" {
 #define DFM_E_NONFINITE (-5)  /* PCG residual became NaN/Inf */
 
 /* The flow-map entry points accept grids whose face grids hold fewer than
- * 2^30 values and, in 3D, whose (x, y) planes hold fewer than 2^24 (the
- * samplers address fields with 32-bit offsets); larger grids give DFM_E_ARG. */
+ * 2^30 values and, in 3D, whose (x, y) planes (in 2D, each extent + 1) stay
+ * below 2^24 (the samplers address fields with 32-bit offsets and 24-bit
+ * multiplies); larger grids give DFM_E_ARG. */
 typedef struct dfm_grid {
   int32_t dim;  /* 2 or 3 */
   int32_t n[3]; /* cells per axis; n[2] = 1 in 2D */

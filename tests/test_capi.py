@@ -62,11 +62,13 @@ def test_argument_validation_codes(lib):
     small = L.make_grid((2, 16), 1 / 16)
     assert lib.dfm_reset_identity(small, m, None) == -1
     # the samplers' 32-bit addressing: face grids below 2^30 values, 3D (x, y)
-    # planes below 2^24 (4095^2 cells -> 4096 x 4095 x-faces per plane)
-    for n in [(4095, 4095, 4), (1024, 1024, 1024), (32768, 32768)]:
+    # planes below 2^24 (4095^2 cells -> 4096 x 4095 x-faces per plane), and
+    # in 2D each extent + 1 below 2^24 (16777216 x 4: 84M values, under 2^30)
+    for n in [(4095, 4095, 4), (1024, 1024, 1024), (32768, 32768), (16777216, 4), (4, 16777215)]:
         assert lib.dfm_march(L.make_grid(n, 1e-3), u, m, 0.1, 2, None) == -1, n
         assert "too large" in lib.dfm_last_error().decode()
-    assert lib.dfm_march(L.make_grid((4094, 4094, 4), 1e-3), u, m, 0.1, 2, None) == -2  # size accepted
+    for n in [(4094, 4094, 4), (16777214, 4), (4, 16777214)]:
+        assert lib.dfm_march(L.make_grid(n, 1e-3), u, m, 0.1, 2, None) == -2, n  # size accepted
     h = ctypes.c_void_p()
     assert lib.dfm_mgpcg_create(g, 20, 2, 50, ctypes.byref(h)) == -1  # too many levels
     odd = L.make_grid((24, 20), 1 / 24)
