@@ -4,7 +4,16 @@
 #include "dfm_internal.hpp"
 #include "../../include/dfm.h"
 
+#include <mutex>
+
 using namespace dfm;
+
+// BFECC error field of dfm_bfecc_u_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends (DFM_NOALLOC_MAX_VALUES floats,
+// 32 MiB).  Host code never passes this name to a kernel: it is a host-side
+// stand-in, and the device address of each GPU's copy comes from
+// hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) float scratch[DFM_NOALLOC_MAX_VALUES];
 
 namespace {
 
@@ -639,6 +648,51 @@ extern "C" int dfm_bfecc_u(const dfm_grid* g, const dfm_mac* init, const dfm_flo
   if ((rc = dfm_pullback_u(g, out, bwd, 0.5f, -0.5f, init, err, stream))) return rc;
   // out = out - T.err(psi)
   return dfm_pullback_u(g, err, fwd, -1.0f, 1.0f, out, out, stream);
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the current GPU's copy of `scratch`, looked up once per
+// device (with that device current) and kept.
+int noalloc_scratch(float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(DFM_E_HIP, "dfm_bfecc_u_noalloc: hipGetDevice", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(DFM_E_ARG, "dfm_bfecc_u_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(DFM_E_HIP, "dfm_bfecc_u_noalloc: hipGetSymbolAddress(scratch)", hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return DFM_OK;
+}
+}  // namespace
+
+extern "C" int dfm_bfecc_u_noalloc(const dfm_grid* g, const dfm_mac* init, const dfm_flowmap* fwd,
+                                   const dfm_flowmap* bwd, const dfm_mac* out, void* stream) {
+  Geo G;
+  int rc = check_grid(g, G);
+  if (rc) return rc;
+  int64_t off[4] = {0, 0, 0, 0};
+  for (int a = 0; a < G.dim; ++a) off[a + 1] = off[a] + count_of_host(G, a);
+  if (off[G.dim] > (int64_t)DFM_NOALLOC_MAX_VALUES)
+    return set_error(DFM_E_ARG, "dfm_bfecc_u_noalloc: the grid's face values exceed DFM_NOALLOC_MAX_VALUES "
+                                "(use dfm_bfecc_u with a caller-provided error field)");
+  float* base = nullptr;
+  if ((rc = noalloc_scratch(&base))) return rc;
+  // the error field's components lie one after another in the scratch; every
+  // value is written by the second pullback before the third reads it, so
+  // nothing needs clearing between calls
+  dfm_mac err{};
+  for (int a = 0; a < G.dim; ++a) err.c[a] = base + off[a];
+  return dfm_bfecc_u(g, init, fwd, bwd, &err, out, stream);
 }
 
 extern "C" int dfm_bfecc_s(const dfm_grid* g, const float* init, const float* psi, const float* phi, float* err,

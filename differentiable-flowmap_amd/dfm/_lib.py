@@ -44,6 +44,7 @@ SIGNATURES = {
     "dfm_pullback_u": (c_int, [P(Grid), P(Mac), P(FlowMapS), c_float, c_float, P(Mac), P(Mac), c_void_p]),
     "dfm_pullback_s": (c_int, [P(Grid), c_void_p, c_void_p, c_float, c_float, c_void_p, c_void_p, c_void_p]),
     "dfm_bfecc_u": (c_int, [P(Grid), P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), P(Mac), c_void_p]),
+    "dfm_bfecc_u_noalloc": (c_int, [P(Grid), P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), c_void_p]),
     "dfm_bfecc_s": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dfm_bfecc_u_clamp": (c_int, [P(Grid), P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), P(Mac), P(Mac), c_void_p]),
     "dfm_bfecc_s_clamp": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -91,6 +92,7 @@ MG_FUSED_SWEEPS = 8
 MG_SPLIT_LAYOUT = 9
 
 DFM_E_NOCONV = -4
+NOALLOC_MAX_VALUES = 1 << 23
 
 _lib = None
 _lib_lock = threading.Lock()

@@ -90,6 +90,20 @@ int dfm_pullback_s(const dfm_grid* g, const float* src, const float* psi, float 
 int dfm_bfecc_u(const dfm_grid* g, const dfm_mac* init, const dfm_flowmap* fwd, const dfm_flowmap* bwd,
                 const dfm_mac* err, const dfm_mac* out, void* stream);
 
+/* dfm_bfecc_u without the err argument, allocating nothing: the error field
+ * lives in a library-owned device array of DFM_NOALLOC_MAX_VALUES floats
+ * (32 MiB per GPU, kept until the process ends), so a grid whose face values
+ * (sum over a of the face-grid sizes; 6,340,608 at 128^3) exceed that gives
+ * DFM_E_ARG.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (different streams or threads) race
+ * and must be ordered by the caller.  The first call on each device looks
+ * the array up (hipGetSymbolAddress) with that device current. */
+#define DFM_NOALLOC_MAX_VALUES (1 << 23)
+int dfm_bfecc_u_noalloc(const dfm_grid* g, const dfm_mac* init, const dfm_flowmap* fwd, const dfm_flowmap* bwd,
+                        const dfm_mac* out, void* stream);
+
 /* BFECC of a cell scalar.  Replaces BFECC_scalar, 2D/vortex/flowmap.py:465-481. */
 int dfm_bfecc_s(const dfm_grid* g, const float* init, const float* psi, const float* phi, float* err, float* out,
                 void* stream);
