@@ -778,6 +778,26 @@ __global__ void __launch_bounds__(kB) k_update_p(int64_t N, const float* __restr
 }
 
 // -------------------------------------------------------------- V-cycle ----
+// Storage of a level's r and z: FP32, or BF16 (uint16_t, round to nearest
+// even) with FP32 arithmetic -- the mixed-precision mode stores the levels
+// >= 2 that way, in global memory and in the LDS tail.
+DFM_HD float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+DFM_HD uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (uint16_t)(u >> 16);
+}
+template <typename T> struct Sto;
+template <> struct Sto<float> {
+  DFM_HD static float ld(const float* p, uint32_t i) { return p[i]; }
+  DFM_HD static void st(float* p, uint32_t i, float v) { p[i] = v; }
+};
+template <> struct Sto<uint16_t> {
+  DFM_HD static float ld(const uint16_t* p, uint32_t i) { return bf16_to_f32(p[i]); }
+  DFM_HD static void st(uint16_t* p, uint32_t i, float v) { p[i] = f32_to_bf16(v); }
+};
+using bf16_t = uint16_t;
+
 // First RBGS sweep from z = 0 with an obstacle mask: colour 0 cells z = r/nn;
 // colour 1 cells use the colour-0 neighbours' r/nn.  `shift` (level 0)
 // recentres r on the fly.  Solid cells get z = 0 and are not neighbours
@@ -819,15 +839,15 @@ __global__ void __launch_bounds__(kB) k_first_sweep(Dims d, const float* __restr
 // Gauss-Seidel update of cell (i, j, k): (r - m + sum of neighbour z) / nn.
 // With zc, the neighbours' values are read as z + zc[coarse parent] (fused
 // prolongation; the cell's own value is overwritten, so it never needs it).
-template <int DIM, bool MASK>
+template <int DIM, bool MASK, typename ZCT>
 DFM_HD float gs_value(const Dims& d, const float* __restrict__ r, const float* __restrict__ z,
-                      const float* __restrict__ zc, const Dims& dc, float m, int i, int j, int k, uint32_t id) {
+                      const ZCT* __restrict__ zc, const Dims& dc, float m, int i, int j, int k, uint32_t id) {
   float nn, s;
   if (zc) {
     s = 0.f;
     nn = 0.f;
     for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
-      s += z[q] + zc[cidx(dc, a >> 1, b >> 1, DIM == 3 ? (c >> 1) : 0)];
+      s += z[q] + Sto<ZCT>::ld(zc, cidx(dc, a >> 1, b >> 1, DIM == 3 ? (c >> 1) : 0));
       nn += 1.f;
     });
   } else {
@@ -892,9 +912,9 @@ __global__ void __launch_bounds__(kB) k_first_half(Dims d, const float* __restri
 // which the stores of this colour change.  Each cell is formed by gs_value's
 // operations in its order, so the result is bitwise the per-pair kernel's.
 
-template <int DIM, bool SPLIT>
+template <int DIM, bool SPLIT, typename ZCT = float>
 __global__ void __launch_bounds__(kB) k_smooth_u(Dims d, const float* __restrict__ r, float* __restrict__ z,
-                                                 int phase, const double* shift, const float* __restrict__ zc,
+                                                 int phase, const double* shift, const ZCT* __restrict__ zc,
                                                  Dims dc, const int* flags) {
   if (is_done(flags)) return;
   const float m = shift ? (float)*shift : 0.f;
@@ -925,8 +945,8 @@ __global__ void __launch_bounds__(kB) k_smooth_u(Dims d, const float* __restrict
 #pragma unroll
           for (int q = 0; q < 2 * DIM; ++q) {
             const bool pr = nb >> q & 1u;
-            zp[u][q] = zc[cidx(dc, (pr ? c3[q][0] : i) >> 1, (pr ? c3[q][1] : a) >> 1,
-                               DIM == 3 ? ((pr ? c3[q][2] : k) >> 1) : 0)];
+            zp[u][q] = Sto<ZCT>::ld(zc, cidx(dc, (pr ? c3[q][0] : i) >> 1, (pr ? c3[q][1] : a) >> 1,
+                                         DIM == 3 ? ((pr ? c3[q][2] : k) >> 1) : 0));
           }
         }
       }
@@ -1075,10 +1095,10 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
 // formed with the operations of the two half-sweeps in their order, so the
 // result is bitwise theirs.  zc (first sweep after prolongation only) is
 // added to the neighbour values of the first colour's update, as there.
-template <int DIM, bool MASK>
+template <int DIM, bool MASK, typename ZCT = float>
 __global__ void __launch_bounds__(kB) k_sweep2(Dims d, const float* __restrict__ r, const float* __restrict__ zin,
                                                float* __restrict__ zout, int c1, const double* shift,
-                                               const float* __restrict__ zc, Dims dc, const int* flags) {
+                                               const ZCT* __restrict__ zc, Dims dc, const int* flags) {
 #pragma clang fp contract(off)
   if (is_done(flags)) return;
   const float m = shift ? (float)*shift : 0.f;
@@ -1092,12 +1112,12 @@ __global__ void __launch_bounds__(kB) k_sweep2(Dims d, const float* __restrict__
   int i, j, k;
   cell_of(d, id, i, j, k);
   if (((i + j + k) & 1) == c1) {
-    zout[id] = gs_value<DIM, MASK>(d, r, zin, zc, dc, m, i, j, k, id);
+    zout[id] = gs_value<DIM, MASK, ZCT>(d, r, zin, zc, dc, m, i, j, k, id);
     return;
   }
   float s = 0.f, nn = 0.f;
   for_nbrs<DIM, MASK>(d, i, j, k, id, [&](uint32_t q, int a, int b, int c) {
-    s += gs_value<DIM, MASK>(d, r, zin, zc, dc, m, a, b, c, q);
+    s += gs_value<DIM, MASK, ZCT>(d, r, zin, zc, dc, m, a, b, c, q);
     nn += 1.f;
   });
   zout[id] = (r[id] - m + s) * inv_count(nn);
@@ -1107,9 +1127,9 @@ __global__ void __launch_bounds__(kB) k_sweep2(Dims d, const float* __restrict__
 // of the fine residual r - A z, times 1/(dim-1).  One thread per coarse cell,
 // so the sum order is fixed.  MASK: fluid children only
 // (2D/vortex/mgpcg_solid.py:216-224).
-template <int DIM, bool MASK, bool SPLIT>
+template <int DIM, bool MASK, bool SPLIT, typename CT = float>
 __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict__ r, const float* __restrict__ z,
-                                                 Dims dc, float* __restrict__ rc, const double* shift,
+                                                 Dims dc, CT* __restrict__ rc, const double* shift,
                                                  const int* flags) {
 #pragma clang fp contract(off)
   if (is_done(flags)) return;
@@ -1152,7 +1172,7 @@ __global__ void __launch_bounds__(kB) k_restrict(Dims d, const float* __restrict
       }
     acc += (rv[ch] - m) - (nn * zv[ch] - s);
   }
-  rc[cid] = acc * (1.0f / (float)(DIM - 1));
+  Sto<CT>::st(rc, cid, acc * (1.0f / (float)(DIM - 1)));
 }
 
 // ---------------------------------------------------- LDS-tiled levels ----
@@ -1276,9 +1296,9 @@ DFM_HD float tile_nbr_sum(const Dims& d, int i, int j, int k, int l, float& nn, 
   return s;
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const float* __restrict__ r, float* __restrict__ z, Dims dc,
-                                                 float* __restrict__ rc, const int* flags) {
+template <int DIM, typename FT = float, typename CT = float>
+__global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const FT* __restrict__ r, FT* __restrict__ z, Dims dc,
+                                                 CT* __restrict__ rc, const int* flags) {
 #pragma clang fp contract(off)
   using M = MgTile<DIM>;
   __shared__ float rl[M::N];
@@ -1289,7 +1309,7 @@ __global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const float* __restric
     float rv[M::E];
     int li = 0;
     tile_load<DIM>(d, f, [&](int i, int j, int k, int, bool in) {
-      rv[li++ % M::E] = in ? r[cidx(d, i, j, k)] : 0.f;
+      rv[li++ % M::E] = in ? Sto<FT>::ld(r, cidx(d, i, j, k)) : 0.f;
     });
     li = 0;
     tile_load<DIM>(d, f, [&](int, int, int, int l, bool) {
@@ -1298,7 +1318,7 @@ __global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const float* __restric
     });
   } else {
     tile_load<DIM>(d, f, [&](int i, int j, int, int l, bool in) {
-      rl[l] = in ? r[cidx(d, i, j, 0)] : 0.f;
+      rl[l] = in ? Sto<FT>::ld(r, cidx(d, i, j, 0)) : 0.f;
       zl[l] = 0.f;
     });
   }
@@ -1317,7 +1337,7 @@ __global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const float* __restric
     });
     __syncthreads();
   }
-  tile_pass<DIM>(d, f, 0, -1, [&](int i, int j, int k, int l) { z[cidx(d, i, j, k)] = zl[l]; });
+  tile_pass<DIM>(d, f, 0, -1, [&](int i, int j, int k, int l) { Sto<FT>::st(z, cidx(d, i, j, k), zl[l]); });
   // restriction of the tile's interior (k_restrict): one thread per fine
   // cell forms its residual term, then one thread per coarse cell adds its
   // children's terms in k_restrict's order (i, j, k offsets, k fastest)
@@ -1350,13 +1370,13 @@ __global__ void __launch_bounds__(kTB) k_tile_pre(Dims d, const float* __restric
     const int ck = (f.o[2] + kTH) / 2 + (DIM == 3 ? t % TCK : 0);
     const int cj = (f.o[1] + kTH) / 2 + (t / TCK) % TC;
     const int ci = (f.o[0] + kTH) / 2 + t / (TCK * TC);
-    rc[cidx(dc, ci, cj, DIM == 3 ? ck : 0)] = acc * (1.0f / (float)(DIM - 1));
+    Sto<CT>::st(rc, cidx(dc, ci, cj, DIM == 3 ? ck : 0), acc * (1.0f / (float)(DIM - 1)));
   }
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kTB) k_tile_post(Dims d, const float* __restrict__ r, const float* __restrict__ zin,
-                                                  const float* __restrict__ zc, Dims dc, float* __restrict__ zout,
+template <int DIM, typename FT = float, typename ZCT = float>
+__global__ void __launch_bounds__(kTB) k_tile_post(Dims d, const FT* __restrict__ r, const FT* __restrict__ zin,
+                                                  const ZCT* __restrict__ zc, Dims dc, FT* __restrict__ zout,
                                                   const int* flags) {
 #pragma clang fp contract(off)
   using M = MgTile<DIM>;
@@ -1371,9 +1391,9 @@ __global__ void __launch_bounds__(kTB) k_tile_post(Dims d, const float* __restri
     tile_load<DIM>(d, f, [&](int i, int j, int k, int, bool in) {
       const int q = li++ % M::E;
       const uint32_t id = in ? cidx(d, i, j, k) : 0u;
-      rv[q] = in ? r[id] : 0.f;
-      zv[q] = in ? zin[id] : 0.f;
-      pv[q] = in ? zc[cidx(dc, i >> 1, j >> 1, k >> 1)] : 0.f;
+      rv[q] = in ? Sto<FT>::ld(r, id) : 0.f;
+      zv[q] = in ? Sto<FT>::ld(zin, id) : 0.f;
+      pv[q] = in ? Sto<ZCT>::ld(zc, cidx(dc, i >> 1, j >> 1, k >> 1)) : 0.f;
     });
     li = 0;
     tile_load<DIM>(d, f, [&](int, int, int, int l, bool) {
@@ -1385,9 +1405,9 @@ __global__ void __launch_bounds__(kTB) k_tile_post(Dims d, const float* __restri
   } else {
     tile_load<DIM>(d, f, [&](int i, int j, int, int l, bool in) {
       const uint32_t id = in ? cidx(d, i, j, 0) : 0u;
-      rl[l] = in ? r[id] : 0.f;
-      zl[l] = in ? zin[id] : 0.f;
-      pl[l] = in ? zc[cidx(dc, i >> 1, j >> 1, 0)] : 0.f;
+      rl[l] = in ? Sto<FT>::ld(r, id) : 0.f;
+      zl[l] = in ? Sto<FT>::ld(zin, id) : 0.f;
+      pl[l] = in ? Sto<ZCT>::ld(zc, cidx(dc, i >> 1, j >> 1, 0)) : 0.f;
     });
   }
   __syncthreads();
@@ -1407,7 +1427,7 @@ __global__ void __launch_bounds__(kTB) k_tile_post(Dims d, const float* __restri
     });
     __syncthreads();
   }
-  tile_pass<DIM>(d, f, 0, -1, [&](int i, int j, int k, int l) { zout[cidx(d, i, j, k)] = zl[l]; });
+  tile_pass<DIM>(d, f, 0, -1, [&](int i, int j, int k, int l) { Sto<FT>::st(zout, cidx(d, i, j, k), zl[l]); });
 }
 
 // Obstacle flags of the MG levels: level 0 = mask > 0 (the solver's
@@ -1447,21 +1467,6 @@ __global__ void __launch_bounds__(kB) k_mask_coarsen(Dims d, const uint8_t* __re
 // mixed-precision mode): pre-smoothing from zero, restriction, the `bottom`
 // sweeps on the coarsest level, prolongation and post-smoothing -- the launch
 // sequence the reference issues per level collapsed into one kernel.
-DFM_HD float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-DFM_HD uint16_t f32_to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-  return (uint16_t)(u >> 16);
-}
-template <typename T> struct Lds;
-template <> struct Lds<float> {
-  DFM_HD static float ld(const float* p, int i) { return p[i]; }
-  DFM_HD static void st(float* p, int i, float v) { p[i] = v; }
-};
-template <> struct Lds<uint16_t> {
-  DFM_HD static float ld(const uint16_t* p, int i) { return bf16_to_f32(p[i]); }
-  DFM_HD static void st(uint16_t* p, int i, float v) { p[i] = f32_to_bf16(v); }
-};
 
 constexpr int kTailMax = 8;
 struct TailArgs {
@@ -1493,9 +1498,9 @@ DFM_HD void lds_dct_axis(const T* in, T* out, const int* n, int axis, bool inver
       // forward: k = coord (output frequency), m = sample; inverse: swapped
       int kf = inverse ? m : coord, xs = inverse ? coord : m;
       float c = (kf == 0 ? c0 : c1) * __cosf(pi_2n * (float)(kf * (2 * xs + 1)));
-      acc = fmaf(Lds<T>::ld(in, base + m * stride), c, acc);
+      acc = fmaf(Sto<T>::ld(in, base + m * stride), c, acc);
     }
-    Lds<T>::st(out, id, acc);
+    Sto<T>::st(out, id, acc);
   }
   __syncthreads();
 }
@@ -1518,8 +1523,8 @@ DFM_HD void lds_exact_solve(T* r, T* z, const int* n) {
     float lam = 2.0f - 2.0f * __cosf(3.14159265358979f * (float)i / (float)n[0]) +
                 2.0f - 2.0f * __cosf(3.14159265358979f * (float)j / (float)n[1]);
     if (DIM == 3) lam += 2.0f - 2.0f * __cosf(3.14159265358979f * (float)k / (float)n[2]);
-    float v = Lds<T>::ld(a, id);
-    Lds<T>::st(a, id, (i == 0 && j == 0 && k == 0) ? 0.f : v / lam);
+    float v = Sto<T>::ld(a, id);
+    Sto<T>::st(a, id, (i == 0 && j == 0 && k == 0) ? 0.f : v / lam);
   }
   __syncthreads();
   for (int ax = 0; ax < DIM; ++ax) {
@@ -1527,7 +1532,7 @@ DFM_HD void lds_exact_solve(T* r, T* z, const int* n) {
     T* t = a; a = b; b = t;
   }
   if (a != z) {
-    for (int id = threadIdx.x; id < total; id += blockDim.x) Lds<T>::st(z, id, Lds<T>::ld(a, id));
+    for (int id = threadIdx.x; id < total; id += blockDim.x) Sto<T>::st(z, id, Sto<T>::ld(a, id));
     __syncthreads();
   }
 }
@@ -1544,21 +1549,21 @@ DFM_HD void lds_half_sweep(T* r, T* z, const Dims& d, int colour) {
     if (!colour_cell<DIM>(d, t, colour, i, j, k)) continue;
     int id = (i * n[1] + j) * n[2] + k;
     float s = 0.f, nn = 0.f;
-    if (i > 0) { s += Lds<T>::ld(z, id - sx); nn += 1.f; }
-    if (i < n[0] - 1) { s += Lds<T>::ld(z, id + sx); nn += 1.f; }
-    if (j > 0) { s += Lds<T>::ld(z, id - sy); nn += 1.f; }
-    if (j < n[1] - 1) { s += Lds<T>::ld(z, id + sy); nn += 1.f; }
+    if (i > 0) { s += Sto<T>::ld(z, id - sx); nn += 1.f; }
+    if (i < n[0] - 1) { s += Sto<T>::ld(z, id + sx); nn += 1.f; }
+    if (j > 0) { s += Sto<T>::ld(z, id - sy); nn += 1.f; }
+    if (j < n[1] - 1) { s += Sto<T>::ld(z, id + sy); nn += 1.f; }
     if (DIM == 3) {
-      if (k > 0) { s += Lds<T>::ld(z, id - 1); nn += 1.f; }
-      if (k < n[2] - 1) { s += Lds<T>::ld(z, id + 1); nn += 1.f; }
+      if (k > 0) { s += Sto<T>::ld(z, id - 1); nn += 1.f; }
+      if (k < n[2] - 1) { s += Sto<T>::ld(z, id + 1); nn += 1.f; }
     }
-    Lds<T>::st(z, id, (Lds<T>::ld(r, id) + s) * inv_count(nn));
+    Sto<T>::st(z, id, (Sto<T>::ld(r, id) + s) * inv_count(nn));
   }
   __syncthreads();
 }
 
 template <int DIM, typename T>
-__global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restrict__ rin, float* __restrict__ zout,
+__global__ void __launch_bounds__(1024) k_tail(TailArgs A, const T* __restrict__ rin, T* __restrict__ zout,
                                                const double* shift, const int* flags) {
   if (is_done(flags)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1569,8 +1574,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
     T* r0 = smem + A.off[0];
     T* z0 = r0 + size_of(0);
     for (int id = threadIdx.x; id < size_of(0); id += blockDim.x) {
-      Lds<T>::st(r0, id, rin[id] - m);
-      Lds<T>::st(z0, id, 0.f);
+      Sto<T>::st(r0, id, Sto<T>::ld(rin, id) - m);
+      Sto<T>::st(z0, id, 0.f);
     }
   }
   __syncthreads();
@@ -1596,18 +1601,18 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
             int i = 2 * ci + a, j = 2 * cj + b, k = DIM == 3 ? 2 * ck + c : 0;
             int id = (i * n[1] + j) * n[2] + k;
             float s = 0.f, nn = 0.f;
-            if (i > 0) { s += Lds<T>::ld(z, id - n[1] * n[2]); nn += 1.f; }
-            if (i < n[0] - 1) { s += Lds<T>::ld(z, id + n[1] * n[2]); nn += 1.f; }
-            if (j > 0) { s += Lds<T>::ld(z, id - n[2]); nn += 1.f; }
-            if (j < n[1] - 1) { s += Lds<T>::ld(z, id + n[2]); nn += 1.f; }
+            if (i > 0) { s += Sto<T>::ld(z, id - n[1] * n[2]); nn += 1.f; }
+            if (i < n[0] - 1) { s += Sto<T>::ld(z, id + n[1] * n[2]); nn += 1.f; }
+            if (j > 0) { s += Sto<T>::ld(z, id - n[2]); nn += 1.f; }
+            if (j < n[1] - 1) { s += Sto<T>::ld(z, id + n[2]); nn += 1.f; }
             if (DIM == 3) {
-              if (k > 0) { s += Lds<T>::ld(z, id - 1); nn += 1.f; }
-              if (k < n[2] - 1) { s += Lds<T>::ld(z, id + 1); nn += 1.f; }
+              if (k > 0) { s += Sto<T>::ld(z, id - 1); nn += 1.f; }
+              if (k < n[2] - 1) { s += Sto<T>::ld(z, id + 1); nn += 1.f; }
             }
-            acc += Lds<T>::ld(r, id) - (nn * Lds<T>::ld(z, id) - s);
+            acc += Sto<T>::ld(r, id) - (nn * Sto<T>::ld(z, id) - s);
           }
-      Lds<T>::st(rc, cid, acc * A.inv_dm1);
-      Lds<T>::st(zc, cid, 0.f);
+      Sto<T>::st(rc, cid, acc * A.inv_dm1);
+      Sto<T>::st(zc, cid, 0.f);
     }
     __syncthreads();
   }
@@ -1636,7 +1641,7 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
       int k = id % n[2], t = id / n[2];
       int j = t % n[1], i = t / n[1];
       int cid = ((i >> 1) * nc[1] + (j >> 1)) * nc[2] + (DIM == 3 ? (k >> 1) : 0);
-      Lds<T>::st(z, id, Lds<T>::ld(z, id) + Lds<T>::ld(zc, cid));
+      Sto<T>::st(z, id, Sto<T>::ld(z, id) + Sto<T>::ld(zc, cid));
     }
     __syncthreads();
     for (int s = 0; s < A.pre; ++s) {
@@ -1645,7 +1650,7 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const float* __restri
     }
   }
   T* z0 = smem + A.off[0] + size_of(0);
-  for (int id = threadIdx.x; id < size_of(0); id += blockDim.x) zout[id] = Lds<T>::ld(z0, id);
+  for (int id = threadIdx.x; id < size_of(0); id += blockDim.x) Sto<T>::st(zout, id, Sto<T>::ld(z0, id));
 }
 
 // ------------------------------------------------------------- handle -----
@@ -1696,6 +1701,7 @@ struct Solver {
   int masked;
   uint8_t* bm[kMaxLevels];
   double nfluid;
+  int bf16_lo;       // first level stored in BF16 in global memory (L: none); see bf16_first_level
   int fused_sweeps;  // option: one launch per RBGS sweep on levels with z2
   int tiled_sweeps;  // option: LDS-tiled pre/post smoothing on eligible coarse levels
   int split_opt;     // option: level 0 colour-split where eligible (set_layout)
@@ -1807,22 +1813,29 @@ struct ZrFuse {
 // without an obstacle mask, one per thread with it.
 template <int DIM, bool MASK, bool SPLIT>
 void half_sweep(Solver& s, hipStream_t st, const Dims& d, float* r, float* z, int c, const double* shift,
-                const float* zc, const Dims& dc, const Red& R) {
+                const float* zc, const Dims& dc, const Red& R, bool zc_bf16 = false) {
   const uint32_t pairs = pairs_of(d, DIM);
   if constexpr (!MASK) {
     const unsigned g = (unsigned)((pairs + (uint32_t)(kB * kSweepU) - 1) / (uint32_t)(kB * kSweepU));
-    hipLaunchKernelGGL((k_smooth_u<DIM, SPLIT>), dim3(g), dim3(kB), 0, st, d, (const float*)r, z, c, shift, zc, dc,
-                       (const int*)s.flags);
+    if (zc_bf16)
+      hipLaunchKernelGGL((k_smooth_u<DIM, SPLIT, bf16_t>), dim3(g), dim3(kB), 0, st, d, (const float*)r, z, c, shift,
+                         reinterpret_cast<const bf16_t*>(zc), dc, (const int*)s.flags);
+    else
+      hipLaunchKernelGGL((k_smooth_u<DIM, SPLIT>), dim3(g), dim3(kB), 0, st, d, (const float*)r, z, c, shift, zc, dc,
+                         (const int*)s.flags);
   } else {
     hipLaunchKernelGGL((k_smooth<DIM, false, MASK, SPLIT>), dim3(grid1(pairs)), dim3(kB), 0, st, d, r, z, c, shift,
                        zc, dc, R, 0, (float*)nullptr);
   }
 }
 
+inline bool bf16_level(const Solver& s, int l) { return l >= s.bf16_lo; }
+
 template <int DIM, bool MASK, bool SPLIT>
 bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool from_zero, const double* shift,
                      const float* zc_prolong, const ZrFuse* zr, float*& z) {
   Lvl& L = s.lv[l];
+  const bool zc_bf16 = l + 1 < s.L && bf16_level(s, l + 1);  // MASK never stores BF16
   const Dims& d = s.ld[l];
   const Dims& dc = s.ld[l + 1 < s.L ? l + 1 : l];
   const Red R = red_of(s);
@@ -1849,12 +1862,17 @@ bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool 
     const float* zc = (it == k0 && zc_prolong) ? zc_prolong : nullptr;
     if (fused) {
       float* zout = z == L.z ? L.z2 : L.z;
-      hipLaunchKernelGGL((k_sweep2<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, (const float*)L.r,
-                         (const float*)z, zout, c1, shift, zc, dc, (const int*)s.flags);
+      if (zc && zc_bf16)
+        hipLaunchKernelGGL((k_sweep2<DIM, MASK, bf16_t>), dim3(grid1(L.size)), dim3(kB), 0, st, d, (const float*)L.r,
+                           (const float*)z, zout, c1, shift, reinterpret_cast<const bf16_t*>(zc), dc,
+                           (const int*)s.flags);
+      else
+        hipLaunchKernelGGL((k_sweep2<DIM, MASK>), dim3(grid1(L.size)), dim3(kB), 0, st, d, (const float*)L.r,
+                           (const float*)z, zout, c1, shift, zc, dc, (const int*)s.flags);
       z = zout;
       continue;
     }
-    half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, c1, shift, zc, dc, R);
+    half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, c1, shift, zc, dc, R, zc != nullptr && zc_bf16);
     if (zr && it == sweeps - 1) {
       hipLaunchKernelGGL((k_smooth<DIM, true, MASK, SPLIT>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, z, 1 - c1,
                          shift, (const float*)nullptr, dc, R, zr->op, zr->p);
@@ -1914,8 +1932,17 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
     const double* sh = l == 0 ? shift0 : nullptr;
     if (!MASK && tiled_level(s, l)) {
       cur[l] = s.lv[l].z;
-      hipLaunchKernelGGL((k_tile_pre<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kTB), 0, st, s.ld[l],
-                         (const float*)s.lv[l].r, cur[l], s.ld[l + 1], s.lv[l + 1].r, (const int*)s.flags);
+      const dim3 g(tile_count<DIM>(s.ld[l]));
+      if (bf16_level(s, l))
+        hipLaunchKernelGGL((k_tile_pre<DIM, bf16_t, bf16_t>), g, dim3(kTB), 0, st, s.ld[l],
+                           (const bf16_t*)s.lv[l].r, (bf16_t*)cur[l], s.ld[l + 1], (bf16_t*)s.lv[l + 1].r,
+                           (const int*)s.flags);
+      else if (bf16_level(s, l + 1))
+        hipLaunchKernelGGL((k_tile_pre<DIM, float, bf16_t>), g, dim3(kTB), 0, st, s.ld[l], (const float*)s.lv[l].r,
+                           cur[l], s.ld[l + 1], (bf16_t*)s.lv[l + 1].r, (const int*)s.flags);
+      else
+        hipLaunchKernelGGL((k_tile_pre<DIM>), g, dim3(kTB), 0, st, s.ld[l], (const float*)s.lv[l].r, cur[l],
+                           s.ld[l + 1], s.lv[l + 1].r, (const int*)s.flags);
       continue;
     }
     sweeps_global<DIM, MASK>(s, st, l, s.pre, 0, true, sh, nullptr, nullptr, cur[l]);
@@ -1924,20 +1951,30 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
     bool split_done = false;
     if constexpr (!MASK) {
       if (split_level(s, l)) {
-        hipLaunchKernelGGL((k_restrict<DIM, false, true>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r,
-                           cur[l], s.ld[l + 1], C.r, sh, s.flags);
+        if (bf16_level(s, l + 1))
+          hipLaunchKernelGGL((k_restrict<DIM, false, true, bf16_t>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l],
+                             F.r, cur[l], s.ld[l + 1], (bf16_t*)C.r, sh, s.flags);
+        else
+          hipLaunchKernelGGL((k_restrict<DIM, false, true>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r,
+                             cur[l], s.ld[l + 1], C.r, sh, s.flags);
         split_done = true;
       }
     }
-    if (!split_done)
-      hipLaunchKernelGGL((k_restrict<DIM, MASK, false>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r, cur[l],
-                         s.ld[l + 1], C.r, sh, s.flags);
+    if (!split_done) {
+      if (bf16_level(s, l + 1))
+        hipLaunchKernelGGL((k_restrict<DIM, MASK, false, bf16_t>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r,
+                           cur[l], s.ld[l + 1], (bf16_t*)C.r, sh, s.flags);
+      else
+        hipLaunchKernelGGL((k_restrict<DIM, MASK, false>), dim3(grid1(C.size)), dim3(kB), 0, st, s.ld[l], F.r,
+                           cur[l], s.ld[l + 1], C.r, sh, s.flags);
+    }
   }
   if (tail) {
     Lvl& T = s.lv[s.tail_lt];
     const double* sh = s.tail_lt == 0 ? shift0 : nullptr;
     if (s.tail_bf16)
-      hipLaunchKernelGGL((k_tail<DIM, uint16_t>), dim3(1), dim3(1024), s.tail_bytes, st, s.tail, T.r, T.z, sh, s.flags);
+      hipLaunchKernelGGL((k_tail<DIM, bf16_t>), dim3(1), dim3(1024), s.tail_bytes, st, s.tail, (const bf16_t*)T.r,
+                         (bf16_t*)T.z, sh, s.flags);
     else
       hipLaunchKernelGGL((k_tail<DIM, float>), dim3(1), dim3(1024), s.tail_bytes, st, s.tail, T.r, T.z, sh, s.flags);
     cur[s.tail_lt] = T.z;
@@ -1957,9 +1994,17 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
     const double* sh = l == 0 ? shift0 : nullptr;
     if (!MASK && tiled_level(s, l)) {
       float* out = cur[l] == s.lv[l].z ? s.lv[l].z2 : s.lv[l].z;
-      hipLaunchKernelGGL((k_tile_post<DIM>), dim3(tile_count<DIM>(s.ld[l])), dim3(kTB), 0, st, s.ld[l],
-                         (const float*)s.lv[l].r, (const float*)cur[l], (const float*)cur[l + 1], s.ld[l + 1], out,
-                         (const int*)s.flags);
+      const dim3 g(tile_count<DIM>(s.ld[l]));
+      if (bf16_level(s, l))
+        hipLaunchKernelGGL((k_tile_post<DIM, bf16_t, bf16_t>), g, dim3(kTB), 0, st, s.ld[l],
+                           (const bf16_t*)s.lv[l].r, (const bf16_t*)cur[l], (const bf16_t*)cur[l + 1], s.ld[l + 1],
+                           (bf16_t*)out, (const int*)s.flags);
+      else if (bf16_level(s, l + 1))
+        hipLaunchKernelGGL((k_tile_post<DIM, float, bf16_t>), g, dim3(kTB), 0, st, s.ld[l], (const float*)s.lv[l].r,
+                           (const float*)cur[l], (const bf16_t*)cur[l + 1], s.ld[l + 1], out, (const int*)s.flags);
+      else
+        hipLaunchKernelGGL((k_tile_post<DIM>), g, dim3(kTB), 0, st, s.ld[l], (const float*)s.lv[l].r,
+                           (const float*)cur[l], (const float*)cur[l + 1], s.ld[l + 1], out, (const int*)s.flags);
       cur[l] = out;
       continue;
     }
@@ -2046,7 +2091,25 @@ bool split_eligible(const Solver& s) {
   return s.split_opt && !s.masked && s.tail_lt > 0 && inner % 2 == 0;
 }
 
+// Mixed-precision storage: with DFM_MG_BF16_COARSE the levels >= 2 are kept
+// in BF16 where the kernels that touch them take that storage: the LDS tail
+// (always) and, when every global level between level 2 and the tail runs the
+// LDS-tiled pre/post kernels, those levels too (the restriction into the
+// first BF16 level and the prolongation out of it convert).  Otherwise only
+// the tail's levels are BF16.  No obstacle mask (its levels stay FP32).
+int bf16_first_level(const Solver& s) {
+  if (!s.bf16_coarse || s.masked || s.tail_lt >= s.L || s.tail_lt < 2) return s.L;
+  for (int l = 2; l < s.tail_lt; ++l)
+    if (!tiled_level(s, l)) return s.tail_lt;
+  return 2;
+}
+
 void set_layout(Solver& s) {
+  const int lo = bf16_first_level(s);
+  if (s.bf16_lo != lo) {
+    s.bf16_lo = lo;
+    drop_graph(s);
+  }
   const uint32_t hh = split_eligible(s) ? (uint32_t)(s.lv[0].size / 2) : 0u;
   if (s.ld[0].hh != hh) {
     s.ld[0].hh = hh;
@@ -2271,6 +2334,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->G.dx = g->dx;
   s->G.inv_dx = 1.0f / g->dx;
   s->L = levels;
+  s->bf16_lo = levels;
   s->pre = pre_post;
   s->bottom = bottom;
   s->check_every = 1;
@@ -2394,6 +2458,8 @@ extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
     case DFM_MG_TAIL_LEVEL: *value = s->tail_lt; return DFM_OK;
     case DFM_MG_FUSED_SWEEPS: *value = s->fused_sweeps + s->tiled_sweeps; return DFM_OK;
     case DFM_MG_SPLIT_LAYOUT: *value = split_eligible(*s) ? 1 : 0; return DFM_OK;
+    case DFM_MG_BF16_LEVEL: *value = bf16_first_level(*s); return DFM_OK;
+    case DFM_MG_LEVELS: *value = s->L; return DFM_OK;
     default: return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
 }

@@ -90,6 +90,8 @@ MG_BOTTOM_EXACT = 6
 MG_TAIL_LEVEL = 7
 MG_FUSED_SWEEPS = 8
 MG_SPLIT_LAYOUT = 9
+MG_BF16_LEVEL = 10
+MG_LEVELS = 11
 
 DFM_E_NOCONV = -4
 NOALLOC_MAX_VALUES = 1 << 23

@@ -208,7 +208,7 @@ class MGPCG:
 
     def __init__(self, n, dx, levels=0, pre_post=2, bottom=50, tol=1e-6, relative=True, max_iters=400,
                  on_noconv="warn", bf16_coarse=False, tail=True, graph=True, check_every=1,
-                 bottom_exact=False, fused_sweeps=2, split_layout=True):
+                 bottom_exact=False, fused_sweeps=2, split_layout=True, tail_level=0):
         self.n = tuple(n)
         self.g = L.make_grid(n, dx)
         self.lib = L.lib()
@@ -230,6 +230,8 @@ class MGPCG:
             self.set_option(L.MG_FUSED_SWEEPS, fused_sweeps)
         if not split_layout:
             self.set_option(L.MG_SPLIT_LAYOUT, 0)
+        if tail_level:
+            self.set_option(L.MG_TAIL_LEVEL, tail_level)
         self.tol, self.relative, self.max_iters = tol, relative, max_iters
         if on_noconv not in ("warn", "raise", "count"):
             raise ValueError(f"on_noconv must be 'warn', 'raise' or 'count', not {on_noconv!r}")

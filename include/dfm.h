@@ -208,8 +208,10 @@ int dfm_mgpcg_status(void* handle, int* iters, double* rel_residual);
 int dfm_mgpcg_pressure(void* handle, double** p);
 
 /* Solver options (no counterpart in the reference; tuning of this library).
- *  DFM_MG_BF16_COARSE: 1 = store levels >= 2 in BF16 (FP32 arithmetic) so
- *      the coarse V-cycle (up to 32^3) runs in one workgroup's LDS; 0 = FP32.
+ *  DFM_MG_BF16_COARSE: 1 = store levels >= 2 in BF16 (FP32 arithmetic):
+ *      in the one-workgroup LDS tail, which then holds up to 32^3, and in
+ *      the global coarse levels as DFM_MG_BF16_LEVEL says; 0 = FP32.  get:
+ *      whether the LDS tail is BF16.
  *  DFM_MG_TAIL: 1 = run the coarse levels that fit in LDS in one kernel
  *      (default); 0 = one launch per sweep on every level.  get: first tail level.
  *  DFM_MG_MAX_REFINE: FP64 iterative-refinement restarts (default 3).
@@ -248,6 +250,16 @@ int dfm_mgpcg_pressure(void* handle, double** p);
  *  the next solve will use it.  0 = natural order everywhere.  The results
  *  agree to rounding (the FP64 reductions sum in another order). */
 #define DFM_MG_SPLIT_LAYOUT 9
+/*  DFM_MG_BF16_LEVEL (get only): the first level the next solve stores in
+ *  BF16 in global memory (L = none).  With DFM_MG_BF16_COARSE set, no
+ *  obstacle mask and the LDS tail starting at level >= 2, that is level 2
+ *  when every global level from 2 to the tail runs the LDS-tiled pre/post
+ *  kernels (e.g. 64^3 at 256^3), else the tail's first level; the
+ *  restriction into it and the prolongation out of it convert, and the
+ *  arithmetic stays FP32. */
+#define DFM_MG_BF16_LEVEL 10
+/*  DFM_MG_LEVELS (get only): the handle's number of MG levels L. */
+#define DFM_MG_LEVELS 11
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);
 
