@@ -307,9 +307,10 @@ DFM_HD double wave_sum(double v) {
   return v;
 }
 
-// Block sum of up to two values; thread 0 gets the totals.
+// Block sum of up to two values over a BS-thread block; thread 0 gets the totals.
+template <int BS = kB>
 DFM_HD void block_sum2(double& a, double& b) {
-  __shared__ double sa[kB / 64], sb[kB / 64];
+  __shared__ double sa[BS / 64], sb[BS / 64];
   a = wave_sum(a);
   b = wave_sum(b);
   int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -317,7 +318,7 @@ DFM_HD void block_sum2(double& a, double& b) {
   __syncthreads();
   if (threadIdx.x == 0) {
     a = 0; b = 0;
-    for (int t = 0; t < kB / 64; ++t) { a += sa[t]; b += sb[t]; }
+    for (int t = 0; t < BS / 64; ++t) { a += sa[t]; b += sb[t]; }
   }
 }
 
@@ -343,7 +344,8 @@ DFM_HD void fin_op(const Red& R, int op, double a, double b) {
     case FIN_MEAN:  // a = sum r
       sc[S_MEAN] = a * R.inv_n;
       break;
-    case FIN_BNORM:  // a = b.b (after recentring)
+    case FIN_BNORM:  // a = b.b (after recentring, which k_recenter_norm wrote back)
+      sc[S_MEAN] = 0.0;
       sc[S_BB] = a;
       sc[S_RR] = a;
       sc[S_THRESH] = R.relative ? R.tol * R.tol * a : R.tol;
@@ -423,9 +425,11 @@ DFM_HD void wait_stores() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+template <int BS = kB>
 DFM_HD void grid_reduce(const Red& R, int op, double a, double b) {
+  static_assert(BS % 64 == 0 && kRedBlocks % BS == 0, "whole waves; the partials split evenly over the block");
   __shared__ int s_last;
-  block_sum2(a, b);
+  block_sum2<BS>(a, b);
   if (threadIdx.x == 0) {
     __hip_atomic_store(&R.part[blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&R.part[kRedBlocks + blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -448,11 +452,11 @@ DFM_HD void grid_reduce(const Red& R, int op, double a, double b) {
   if (!s_last) return;
   // all of a thread's partials are requested before any is summed (a rolled
   // loop would wait one memory latency per element)
-  constexpr int U = kRedBlocks / kB;
+  constexpr int U = kRedBlocks / BS;
   double va[U], vb[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const unsigned t = threadIdx.x + u * kB;
+    const unsigned t = threadIdx.x + u * BS;
     va[u] = t < gridDim.x ? __hip_atomic_load(&R.part[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
     vb[u] = t < gridDim.x ? __hip_atomic_load(&R.part[kRedBlocks + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                           : 0.0;
@@ -463,7 +467,7 @@ DFM_HD void grid_reduce(const Red& R, int op, double a, double b) {
     ta += va[u];
     tb += vb[u];
   }
-  block_sum2(ta, tb);
+  block_sum2<BS>(ta, tb);
   if (threadIdx.x == 0) fin_op(R, op, ta, tb);
 }
 
@@ -669,9 +673,26 @@ __global__ void __launch_bounds__(kB) k_unsplit(Dims d, int64_t N, const double*
 // ------------------------------------------------------------ PCG steps ----
 // Ap = A p ; partial p.Ap.  MASK: solid rows are 0 (compute_Ap of the
 // obstacle solver updates fluid cells only, 2D/vortex/mgpcg_solid.py:172-178).
+// A p of one cell from its loaded stencil: formed in FP64 and rounded once.
+// For smooth p, nn p - sum(p) cancels all but ~(pi/n)^2 of |p|, and the FP32
+// form's error then drifts the recursive residual away from b - A x by up to
+// ~1e-2 |b| at 512^2.  k_Ap_dot and k_update_xr both call it on the same
+// loads, so the value the update uses is bitwise the one alpha was formed
+// with, and A p is never stored.
+template <int DIM>
+DFM_HD float ap_value(unsigned has, const float* pn, float pc) {
+  double s = 0.0, nn = 0.0;
+#pragma unroll
+  for (int t = 0; t < 2 * DIM; ++t)
+    if (has >> t & 1u) {
+      s += (double)pn[t];
+      nn += 1.0;
+    }
+  return (float)(nn * (double)pc - s);
+}
+
 template <int DIM, bool MASK, int U, bool SPLIT>
-__global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__ p, float* __restrict__ Ap,
-                                               Red R) {
+__global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__ p, Red R) {
   if (is_done(R.flags)) return;
   double acc = 0;
   int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
@@ -701,22 +722,7 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
     for (int u = 0; u < U; ++u) {
       const uint32_t id = id0 + u * w.stride;
       if (id < w.end) {
-        float v = 0.f;
-        if (has[u]) {
-          // formed in FP64 and rounded once: for smooth p, nn p - sum(p)
-          // cancels all but ~(pi/n)^2 of |p|, and the FP32 form's error then
-          // drifts the recursive residual away from b - A x by up to
-          // ~1e-2 |b| at 512^2
-          double s = 0.0, nn = 0.0;
-#pragma unroll
-          for (int t = 0; t < 2 * DIM; ++t)
-            if (has[u] >> t & 1u) {
-              s += (double)pn[u][t];
-              nn += 1.0;
-            }
-          v = (float)(nn * (double)pc[u] - s);
-        }
-        Ap[id] = v;
+        const float v = has[u] ? ap_value<DIM>(has[u], pn[u], pc[u]) : 0.f;
         acc += (double)pc[u] * (double)v;
       }
     }
@@ -724,30 +730,72 @@ __global__ void __launch_bounds__(kB) k_Ap_dot(Dims d, const float* __restrict__
   grid_reduce(R, FIN_ALPHA, acc, 0.0);
 }
 
-// x += alpha p (x in FP64) ; r -= alpha Ap ; partial r.r and sum r.
-// (Solid cells hold p = Ap = r = 0 throughout, so they need no test here.)
-__global__ void __launch_bounds__(kB) k_update_xr(int64_t N, double* __restrict__ x, float* __restrict__ r,
-                                                  const float* __restrict__ p, const float* __restrict__ Ap,
-                                                  Red R) {
+// x += alpha p (x in FP64) ; r = (r - mean) - alpha A p with A p re-formed
+// from p's stencil (ap_value) ; partial r.r and sum r.  Without an obstacle
+// mask the stored r still carries the mean the last V-cycle removed on the
+// fly (sc[S_MEAN]; 0 right after the set-up's recentring): the V-cycle reads
+// r - mean and never writes it back, which saves the write-back in its last
+// half-sweep.  (r - mean) in FP32 is the value the write-back stored, so the
+// result is bitwise the same.  With a mask r is written back recentred, and
+// solid cells hold p = r = 0 throughout.
+template <int DIM, bool MASK, int U, bool SPLIT>
+__global__ void __launch_bounds__(kB) k_update_xr(Dims d, double* __restrict__ x, float* __restrict__ r,
+                                                  const float* __restrict__ p, Red R) {
   if (is_done(R.flags)) return;
   const double alpha_d = R.sc[S_ALPHA];
-  float alpha = (float)alpha_d;
+  const float alpha = (float)alpha_d;
+  const float m = MASK ? 0.f : (float)R.sc[S_MEAN];
   double rr = 0, rs = 0;
+  const int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
   const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
-  for (uint32_t id = w.first; id < w.end; id += w.stride) {
-    x[id] = fma(alpha_d, (double)p[id], x[id]);
-    float v = fmaf(-alpha, Ap[id], r[id]);
-    r[id] = v;
-    rr += (double)v * (double)v;
-    rs += (double)v;
+  // U cells of the walk per step, every load issued before any is used;
+  // cells are consumed in walk order (the sums do not depend on U)
+  for (uint32_t id0 = w.first; id0 < w.end; id0 += U * w.stride) {
+    float pn[U][2 * DIM], pc[U], rv[U];
+    double xv[U];
+    unsigned has[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t id = id0 + u * w.stride;
+      has[u] = 0;
+      pc[u] = rv[u] = 0.f;
+      xv[u] = 0.0;
+      if (id < w.end) {
+        xv[u] = x[id];
+        rv[u] = r[id];
+        if (is_fluid<MASK>(d, id)) {
+          int i, j, k;
+          cell_of_at<DIM, SPLIT>(d, id, i, j, k);
+          has[u] = nbr_load<DIM, MASK, SPLIT>(d, p, i, j, k, id, pn[u]) | kFluidBit;
+          pc[u] = p[id];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) pin(pn[u]);
+    pin(pc);
+    pin(rv);
+    pin(xv);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t id = id0 + u * w.stride;
+      if (id < w.end) {
+        const float ap = has[u] ? ap_value<DIM>(has[u], pn[u], pc[u]) : 0.f;
+        x[id] = fma(alpha_d, (double)pc[u], xv[u]);
+        const float v = fmaf(-alpha, ap, MASK ? rv[u] : rv[u] - m);
+        r[id] = v;
+        rr += (double)v * (double)v;
+        rs += (double)v;
+      }
+    }
   }
   grid_reduce(R, FIN_RR, rr, rs);
 }
 
-// r -= shift (written back) ; z.r and its scalar step `op` (FIN_ZR_INIT or
-// FIN_BETA); p = z when init_p.  Used when level 0 is not smoothed by the
-// global kernels (otherwise the last level-0 half-sweep does this).  Solid
-// cells keep r = 0.
+// z.r with r - shift and its scalar step `op` (FIN_ZR_INIT or FIN_BETA);
+// p = z when init_p; with a mask the recentred r is written back (see
+// k_update_xr).  Used when level 0 is not smoothed by the global kernels
+// (otherwise the last level-0 half-sweep does this).  Solid cells keep r = 0.
 template <bool MASK>
 __global__ void __launch_bounds__(kB) k_zr(int64_t N, const float* __restrict__ z, float* __restrict__ r,
                                            const double* shift, const uint8_t* __restrict__ bm, Red R, int op,
@@ -759,7 +807,7 @@ __global__ void __launch_bounds__(kB) k_zr(int64_t N, const float* __restrict__ 
   for (uint32_t id = w.first; id < w.end; id += w.stride) {
     const bool fl = !MASK || bm[id] == 0;
     float rv = fl ? r[id] - m : 0.f;
-    if (shift && fl) r[id] = rv;
+    if (MASK && shift && fl) r[id] = rv;
     float zv = z[id];
     if (init_p) p[id] = zv;
     acc += (double)zv * (double)rv;
@@ -1067,7 +1115,7 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
         z[ido[u]] = znew;
         if (ZR) {
           const float rv = rown[u] - m;
-          if (shift) r[ido[u]] = rv;
+          if (MASK && shift) r[ido[u]] = rv;
           if (p) p[ido[u]] = znew;
           acc += (double)znew * (double)rv;
         }
@@ -1077,7 +1125,7 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
       if (ZR && ostate[u]) {
         if (ostate[u] == 2) {
           const float rv = roth[u] - m;
-          if (shift) r[idt[u]] = rv;
+          if (MASK && shift) r[idt[u]] = rv;
           acc += (double)zoth[u] * (double)rv;
         }
         if (p) p[idt[u]] = zoth[u];
@@ -1671,7 +1719,7 @@ struct Solver {
   double* x;  // PCG iterate (pressure), FP64
   float* b;   // recentred right-hand side, kept for the FP64 refinement residual
   int max_refine;
-  float *p, *Ap;
+  float* p;
   double* part;   // 2 x kRedBlocks block partials
   unsigned* ctr;  // grid-reduction arrival counters (kCtrWords)
   double* sc;
@@ -2025,11 +2073,27 @@ void vcycle(Solver& s, hipStream_t st, const double* shift0, const ZrFuse& zr) {
 template <int DIM, bool MASK, bool SPLIT>
 void launch_Ap_dot_t(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
   if (long_walk(s.lv[0].size))
-    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, kUnroll, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p,
-                       s.Ap, R);
+    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, kUnroll, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, R);
   else
-    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, 1, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, s.Ap,
-                       R);
+    hipLaunchKernelGGL((k_Ap_dot<DIM, MASK, 1, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], (const float*)s.p, R);
+}
+
+template <int DIM, bool MASK, bool SPLIT>
+void launch_update_xr_t(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
+  if (long_walk(s.lv[0].size))
+    hipLaunchKernelGGL((k_update_xr<DIM, MASK, kUnroll, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], s.x, s.lv[0].r,
+                       (const float*)s.p, R);
+  else
+    hipLaunchKernelGGL((k_update_xr<DIM, MASK, 1, SPLIT>), dim3(gr), dim3(kB), 0, st, s.ld[0], s.x, s.lv[0].r,
+                       (const float*)s.p, R);
+}
+
+template <int DIM, bool MASK>
+void launch_update_xr(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
+  if constexpr (!MASK) {
+    if (split_level(s, 0)) return launch_update_xr_t<DIM, false, true>(s, st, gr, R);
+  }
+  launch_update_xr_t<DIM, MASK, false>(s, st, gr, R);
 }
 
 template <int DIM, bool MASK>
@@ -2070,7 +2134,7 @@ void pcg_iteration(Solver& s, hipStream_t st) {
   unsigned gr = gridr(N);
   const Red R = red_of(s);
   launch_Ap_dot<DIM, MASK>(s, st, gr, R);
-  hipLaunchKernelGGL(k_update_xr, dim3(gr), dim3(kB), 0, st, N, s.x, L0.r, (const float*)s.p, (const float*)s.Ap, R);
+  launch_update_xr<DIM, MASK>(s, st, gr, R);
   vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_BETA, nullptr});
   hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, (const float*)s.z0_final, s.p, (const double*)s.sc,
                      (const int*)s.flags);
@@ -2289,7 +2353,6 @@ void destroy(Solver* s) {
   if (s->xn) (void)hipFree(s->xn);
   if (s->b) (void)hipFree(s->b);
   if (s->p) (void)hipFree(s->p);
-  if (s->Ap) (void)hipFree(s->Ap);
   if (s->part) (void)hipFree(s->part);
   if (s->ctr) (void)hipFree(s->ctr);
   for (int l = 0; l < kMaxLevels; ++l)
@@ -2366,7 +2429,6 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   int64_t N = s->lv[0].size;
   need(hipMalloc(&s->x, sizeof(double) * N), "hipMalloc (x)") && need(hipMalloc(&s->b, sizeof(float) * N), "hipMalloc (b)") &&
       need(hipMalloc(&s->p, sizeof(float) * N), "hipMalloc (p)") &&
-      need(hipMalloc(&s->Ap, sizeof(float) * N), "hipMalloc (Ap)") &&
       need(hipMalloc(&s->part, sizeof(double) * 2 * kRedBlocks), "hipMalloc (partials)") &&
       need(hipMalloc(&s->ctr, sizeof(unsigned) * kCtrWords), "hipMalloc (counters)") &&
       need(hipMalloc(&s->sc, kCtlBytes), "hipMalloc (control block)") &&
