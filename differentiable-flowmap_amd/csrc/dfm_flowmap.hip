@@ -466,6 +466,346 @@ __global__ void __launch_bounds__(kBlock) k_axpy(Sets S, Axpy A) {
   A.out[s][i] = r;
 }
 
+// ------------------------------------------------ vector-Jacobian products ----
+// Transposes of the stages that are linear in a field (the pullbacks and the
+// semi-Lagrangian sample), with the gradients w.r.t. the map or the
+// advecting velocity, and the reverse of the march.  The transpose of a
+// B-spline sample scatters the incoming value with the sample's own tap
+// weights to the sample's own (clamped) taps by FP32 atomic adds, so the sums
+// are order-nondeterministic at rounding level.  A sample's derivative
+// w.r.t. its point is zero along an axis whose sample coordinate is clamped
+// at the wall band (the forward's value is constant there).
+
+// g[taps of the sample at p] += coef * weight (transpose of interp<K, false>).
+template <int K, int DIM>
+DFM_HD void scatter_sample(float* __restrict__ g, const int S[3], const float bl[3], const float p[3], float inv_dx,
+                           float coef) {
+  constexpr int NT = Spline<K>::NT;
+  Axis<K, false> ax, ay, az;
+  ax.setup(p[0] * inv_dx, bl[0], S[0]);
+  ay.setup(p[1] * inv_dx, bl[1], S[1]);
+  if (DIM == 3) az.setup(p[2] * inv_dx, bl[2], S[2]);
+#pragma unroll
+  for (int a = 0; a < NT; ++a) {
+    const float ca = coef * ax.w[a];
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      const float cb = ca * ay.w[b];
+      const int64_t row = (int64_t)ax.idx[a] * S[1] + ay.idx[b];
+      if (DIM == 2) {
+        atomicAdd(g + row, cb);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NT; ++c) atomicAdd(g + row * S[2] + az.idx[c], cb * az.w[c]);
+      }
+    }
+  }
+}
+
+// Gradient w.r.t. the point of the quadratic sample of f at p, zero along
+// clamped axes; returns the value.
+template <int DIM>
+DFM_HD float sample_dp(const float* __restrict__ f, const int S[3], const float bl[3], const float p[3], float inv_dx,
+                       float gr[3]) {
+  const float v = interp<2, true, DIM>(f, S, bl, p, inv_dx, gr);
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    const float raw = p[c] * inv_dx - bl[c];
+    if (!(raw > 1.0f && raw < (float)(S[c] - 2) - 1e-6f)) gr[c] = 0.f;
+  }
+  return v;
+}
+
+// Reverse of the march (k_march) for one point.  eval_vjp takes the incoming
+// gradients of one velocity sample u(q) (gv) and of k = J(q) Tq (gk, so
+// that the gradient of J[a][b] = du_a/dx_b is gk[a] Tq[b]), scatters them
+// into the velocity's gradient gu, and adds their derivative w.r.t. q to gq
+// (through the value's gradient and the quadratic spline's second
+// derivative, weights {1, -2, 1}; zero along an axis whose sample coordinate
+// is clamped at the wall band) and J^T gk to gTq.
+template <int DIM>
+DFM_HD void eval_vjp(const Geo& g, const Mac& u, const float q[3], const float gv[3], const float gk[3],
+                     const float Tq[3], float* const gu[3], float gq[3], float gTq[3]) {
+  constexpr float d2[3] = {1.0f, -2.0f, 1.0f};
+#pragma unroll 1
+  for (int a = 0; a < DIM; ++a) {
+    int S[3];
+    shape_of(g, a, S);
+    float bl[3] = {0.5f, 0.5f, 0.5f};
+    bl[a] = 0.0f;
+    Axis<2, true> ax[3];
+    float live[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const float raw = q[c] * g.inv_dx - bl[c];
+      ax[c].setup(q[c] * g.inv_dx, bl[c], S[c]);
+      live[c] = (raw > 1.0f && raw < (float)(S[c] - 2) - 1e-6f) ? 1.0f : 0.0f;
+    }
+    float gJ[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < DIM; ++b) gJ[b] = gk[a] * Tq[b];
+    float G[3] = {0.f, 0.f, 0.f}, H[3] = {0.f, 0.f, 0.f};
+    const int NZ = DIM == 3 ? 3 : 1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          const float w0 = ax[0].w[i], w1 = ax[1].w[j], w2 = DIM == 3 ? ax[2].w[k] : 1.0f;
+          const float e0 = ax[0].dw[i], e1 = ax[1].dw[j], e2 = DIM == 3 ? ax[2].dw[k] : 0.0f;
+          const float h0 = d2[i], h1 = d2[j], h2 = DIM == 3 ? d2[k] : 0.0f;
+          // first and second derivative weights of the tap (grid units)
+          const float D[3] = {e0 * w1 * w2, w0 * e1 * w2, w0 * w1 * e2};
+          const float Hm[3][3] = {{h0 * w1 * w2, e0 * e1 * w2, e0 * w1 * e2},
+                                  {e0 * e1 * w2, w0 * h1 * w2, w0 * e1 * e2},
+                                  {e0 * w1 * e2, w0 * e1 * e2, w0 * w1 * h2}};
+          int64_t id = (int64_t)ax[0].idx[i] * S[1] + ax[1].idx[j];
+          if (DIM == 3) id = id * S[2] + ax[2].idx[k];
+          float coef = gv[a] * (w0 * w1 * w2);
+#pragma unroll
+          for (int b = 0; b < DIM; ++b) coef = fmaf(gJ[b] * g.inv_dx, D[b], coef);
+          atomicAdd(gu[a] + id, coef);
+          const float f = u.c[a][id];
+#pragma unroll
+          for (int c = 0; c < DIM; ++c) {
+            G[c] = fmaf(f, D[c], G[c]);
+            float hc = 0.f;
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) hc = fmaf(gJ[b], Hm[c][b], hc);
+            H[c] = fmaf(f, hc, H[c]);
+          }
+        }
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const float J = G[c] * g.inv_dx;  // du_a / dx_c
+      gq[c] += live[c] * (gv[a] * J + H[c] * g.inv_dx * g.inv_dx);
+      gTq[c] = fmaf(J, gk[a], gTq[c]);
+    }
+  }
+}
+
+struct GuPtrs {
+  float* c[3];
+};
+
+// k_pullback_u is out_a = alpha T_a . src(psi_a) (+ beta add_a).  Per face
+// point: g_src_b += alpha T_a[b] g_out_a at psi_a; with GM.psi set, also
+// g_T_a[b] += alpha g_out_a src_b(psi_a) and g_psi_a += alpha g_out_a
+// sum_b T_a[b] grad src_b(psi_a).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_pullback_u_vjp(Geo g, Mac gout, Mac src, Sets S, MapPtrs M, float alpha,
+                                                          Epi G, MapPtrs GM) {
+  int64_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (gid >= S.off[S.nset]) return;
+  const int s = S.find(gid);
+  const int64_t i = gid - S.off[s], cnt = S.cnt[s];
+  const int a = S.grid[s];
+  const bool want_map = GM.psi[s] != nullptr;
+  float p[3] = {0.f, 0.f, 0.f}, gp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) p[c] = M.psi[s][c * cnt + i];
+  const float go = alpha * gout.c[a][i];
+#pragma unroll 1
+  for (int b = 0; b < DIM; ++b) {
+    int Sb[3];
+    shape_of(g, b, Sb);
+    float bl[3] = {0.5f, 0.5f, 0.5f};
+    bl[b] = 0.0f;
+    const float Tb = M.T[s][b * cnt + i];
+    scatter_sample<2, DIM>(G.out[b], Sb, bl, p, g.inv_dx, go * Tb);
+    if (want_map) {
+      float gr[3] = {0.f, 0.f, 0.f};
+      const float v = sample_dp<DIM>(src.c[b], Sb, bl, p, g.inv_dx, gr);
+      GM.T[s][b * cnt + i] += go * v;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) gp[c] = fmaf(go * Tb, gr[c], gp[c]);
+    }
+  }
+  if (want_map) {
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) GM.psi[s][c * cnt + i] += gp[c];
+  }
+}
+
+// k_pullback_s is out = alpha src(psi) (+ beta add): g_src += alpha g_out at
+// psi; with gpsi set, also gpsi += alpha g_out grad src(psi).
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_pullback_s_vjp(Geo g, const float* __restrict__ gout,
+                                                          const float* __restrict__ src,
+                                                          const float* __restrict__ psi, float alpha,
+                                                          float* __restrict__ gsrc, float* __restrict__ gpsi,
+                                                          int64_t cnt) {
+  int64_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= cnt) return;
+  float p[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) p[c] = psi[c * cnt + i];
+  int Sc[3];
+  shape_of(g, 3, Sc);
+  const float bl[3] = {0.5f, 0.5f, 0.5f};
+  const float go = alpha * gout[i];
+  scatter_sample<2, DIM>(gsrc, Sc, bl, p, g.inv_dx, go);
+  if (gpsi) {
+    float gr[3] = {0.f, 0.f, 0.f};
+    sample_dp<DIM>(src, Sc, bl, p, g.inv_dx, gr);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) gpsi[c * cnt + i] += go * gr[c];
+  }
+}
+
+struct SLVjpArgs {
+  int nvec, nscal;
+  Mac fsrc[kMaxVec];           // forward sources (read when the velocity gradient is wanted)
+  Mac gdst[kMaxVec];           // incoming gradients of the destinations
+  float* gsrc[kMaxVec][3];     // accumulated source gradients
+  const float* fs[kMaxScal];
+  const float* gds[kMaxScal];
+  float* gss[kMaxScal];
+  float* gvel[3];              // velocity gradient, or null
+};
+
+// Transpose of k_sl_advect (dst(x) = src(p(x)), p the back-trace of node x
+// through vel): the same back-trace, each destination's incoming value
+// scattered to its source at p; with the velocity gradient wanted, the
+// gradient at p is run back through the RK stages of the back-trace.
+template <int DIM, int ORDER>
+__global__ void __launch_bounds__(kBlock) k_sl_advect_vjp(Geo g, Mac vel, float dt, Sets S, SLVjpArgs A) {
+  int64_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (gid >= S.off[S.nset]) return;
+  int s = S.find(gid);
+  int64_t i = gid - S.off[s];
+  int grid = S.grid[s];
+  int sh[3];
+  shape_of(g, grid, sh);
+  int I, J, K;
+  unlin(sh, i, I, J, K);
+  float x[3] = {0.f, 0.f, 0.f}, p[3] = {0.f, 0.f, 0.f};
+  node_pos(g, grid, I, J, K, x);
+  backtrace<DIM, ORDER>(g, vel, dt, x, p);
+  float bl[3] = {0.5f, 0.5f, 0.5f};
+  if (grid < 3) bl[grid] = 0.0f;
+  const bool want_vel = A.gvel[0] != nullptr;
+  float gp[3] = {0.f, 0.f, 0.f};
+  const int nm = grid < 3 ? A.nvec : A.nscal;
+  for (int m = 0; m < nm; ++m) {
+    const float gi = grid < 3 ? A.gdst[m].c[grid][i] : A.gds[m][i];
+    float* dst = grid < 3 ? A.gsrc[m][grid] : A.gss[m];
+    scatter_sample<2, DIM>(dst, sh, bl, p, g.inv_dx, gi);
+    if (want_vel) {
+      float gr[3] = {0.f, 0.f, 0.f};
+      sample_dp<DIM>(grid < 3 ? A.fsrc[m].c[grid] : A.fs[m], sh, bl, p, g.inv_dx, gr);
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) gp[c] = fmaf(gi, gr[c], gp[c]);
+    }
+  }
+  if (!want_vel) return;
+  // back-trace stage points (as backtrace) and their reverse
+  constexpr int NS = ORDER == 2 ? 2 : 4;
+  float q[NS][3], v[3];
+  const float step[3] = {-0.5f * dt, -0.5f * dt, -dt};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) q[0][c] = x[c];
+#pragma unroll
+  for (int st = 0; st + 1 < NS; ++st) {
+    mac_eval<false, DIM>(g, vel, q[st], v, nullptr);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[st + 1][c] = c < DIM ? x[c] + step[st] * v[c] : 0.f;
+  }
+  float gv[NS][3];
+  const float wt4[4] = {1.f / 6.f, 2.f / 6.f, 2.f / 6.f, 1.f / 6.f};
+#pragma unroll
+  for (int st = 0; st < NS; ++st)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gv[st][c] = -dt * (ORDER == 2 ? (st == NS - 1 ? 1.0f : 0.0f) : wt4[st]) * gp[c];
+  const float zero[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int st = NS - 1; st >= 0; --st) {
+    float gq[3] = {0.f, 0.f, 0.f}, unused[3] = {0.f, 0.f, 0.f};
+    eval_vjp<DIM>(g, vel, q[st], gv[st], zero, zero, A.gvel, gq, unused);
+    if (st > 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gv[st - 1][c] += step[st - 1] * gq[c];
+    }
+  }
+}
+
+// gu += the velocity part of the march's vector-Jacobian product; the
+// incoming gradients (GM, w.r.t. the marched map) are replaced in place by
+// those w.r.t. the map before the march (M0).  Each point recomputes its RK
+// stage points from M0 and runs the stages backwards.
+template <int DIM, int ORDER>
+__global__ void __launch_bounds__(kBlock) k_march_vjp(Geo g, Mac u, float dt, Sets S, MapPtrs M0, MapPtrs GM,
+                                                     GuPtrs gu) {
+  int64_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (gid >= S.off[S.nset]) return;
+  const int s = S.find(gid);
+  const int64_t i = gid - S.off[s], cnt = S.cnt[s];
+  const bool has_t = M0.T[s] != nullptr;
+  constexpr int NS = ORDER == 2 ? 2 : 4;
+  float q[NS][3], Tq[NS][3];
+  float gp[3] = {0.f, 0.f, 0.f}, gT[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int st = 0; st < NS; ++st)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[st][c] = Tq[st][c] = 0.f;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    q[0][c] = M0.psi[s][c * cnt + i];
+    gp[c] = GM.psi[s][c * cnt + i];
+    if (has_t) {
+      Tq[0][c] = M0.T[s][c * cnt + i];
+      gT[c] = GM.T[s][c * cnt + i];
+    }
+  }
+  const float neg = -dt;
+  // forward: the stage points and transported columns of rk_point
+  float v[3], J[3][3];
+  const float step[3] = {0.5f * neg, 0.5f * neg, neg};
+#pragma unroll
+  for (int st = 0; st + 1 < NS; ++st) {
+    mac_eval<true, DIM>(g, u, q[st], v, J);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      float k = 0.f;
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) k = fmaf(J[a][b], Tq[st][b], k);
+      q[st + 1][a] = q[0][a] + step[st] * v[a];
+      Tq[st + 1][a] = Tq[0][a] + step[st] * k;
+    }
+  }
+  // backward
+  float gu_st[NS][3], gk_st[NS][3];
+  const float wt4[4] = {1.f / 6.f, 2.f / 6.f, 2.f / 6.f, 1.f / 6.f};
+#pragma unroll
+  for (int st = 0; st < NS; ++st)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float wgt = ORDER == 2 ? (st == NS - 1 ? 1.0f : 0.0f) : wt4[st];
+      gu_st[st][c] = neg * wgt * gp[c];
+      gk_st[st][c] = has_t ? neg * wgt * gT[c] : 0.f;
+    }
+#pragma unroll
+  for (int st = NS - 1; st >= 0; --st) {
+    float gq[3] = {0.f, 0.f, 0.f}, gTq[3] = {0.f, 0.f, 0.f};
+    eval_vjp<DIM>(g, u, q[st], gu_st[st], gk_st[st], Tq[st], gu.c, gq, gTq);
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      gp[c] += gq[c];
+      if (has_t) gT[c] += gTq[c];
+      if (st > 0) {
+        gu_st[st - 1][c] += step[st - 1] * gq[c];
+        if (has_t) gk_st[st - 1][c] += step[st - 1] * gTq[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    GM.psi[s][c * cnt + i] = gp[c];
+    if (has_t) GM.T[s][c * cnt + i] = gT[c];
+  }
+}
+
 // ------------------------------------------------------------ host glue ----
 int check_grid(const dfm_grid* g, Geo& G) {
   if (!g) return set_error(DFM_E_NULL, "grid is NULL");
@@ -850,4 +1190,154 @@ extern "C" int dfm_axpy_mac(const dfm_grid* g, const dfm_mac* x, float alpha, co
   int64_t total = S.off[S.nset];
   hipLaunchKernelGGL(k_axpy, dim3(blocks_for(total)), dim3(kBlock), 0, S_(stream), S, A);
   return launch_check("dfm_axpy_mac");
+}
+
+// ------------------------------------------------ vector-Jacobian products ----
+extern "C" int dfm_pullback_u_vjp(const dfm_grid* g, const dfm_mac* g_out, const dfm_mac* src, const dfm_flowmap* m,
+                                  float alpha, const dfm_mac* g_src, const dfm_flowmap* g_m, void* stream) {
+  Geo G;
+  int rc = check_grid(g, G);
+  if (rc) return rc;
+  if ((rc = check_mac(G, g_out, "pullback output gradient is NULL"))) return rc;
+  if ((rc = check_mac(G, g_src, "pullback source gradient is NULL"))) return rc;
+  if (!m) return set_error(DFM_E_NULL, "flowmap is NULL");
+  for (int a = 0; a < G.dim; ++a)
+    if (!m->psi_f[a] || !m->T_f[a]) return set_error(DFM_E_NULL, "pullback needs psi_f and T_f on every face axis");
+  if (g_m) {
+    if ((rc = check_mac(G, src, "the map gradient needs the pullback source"))) return rc;
+    for (int a = 0; a < G.dim; ++a)
+      if (!g_m->psi_f[a] || !g_m->T_f[a]) return set_error(DFM_E_NULL, "map gradient needs psi_f and T_f");
+  }
+  Sets S = face_sets(G);
+  MapPtrs M{}, GM{};
+  Epi E{};
+  for (int a = 0; a < G.dim; ++a) {
+    M.psi[a] = m->psi_f[a];
+    M.T[a] = m->T_f[a];
+    E.out[a] = g_src->c[a];
+    if (g_m) {
+      GM.psi[a] = g_m->psi_f[a];
+      GM.T[a] = g_m->T_f[a];
+    }
+  }
+  dim3 grid(blocks_for(S.off[S.nset])), blk(kBlock);
+  Mac GO = to_mac(g_out);
+  Mac SR = g_m ? to_mac(src) : Mac{};
+  DFM_DISPATCH_DIM(G, hipLaunchKernelGGL((k_pullback_u_vjp<2>), grid, blk, 0, S_(stream), G, GO, SR, S, M, alpha, E, GM),
+                   hipLaunchKernelGGL((k_pullback_u_vjp<3>), grid, blk, 0, S_(stream), G, GO, SR, S, M, alpha, E, GM));
+  return launch_check("dfm_pullback_u_vjp");
+}
+
+extern "C" int dfm_pullback_s_vjp(const dfm_grid* g, const float* g_out, const float* src, const float* psi,
+                                  float alpha, float* g_src, float* g_psi, void* stream) {
+  Geo G;
+  int rc = check_grid(g, G);
+  if (rc) return rc;
+  if (!g_out || !psi || !g_src) return set_error(DFM_E_NULL, "dfm_pullback_s_vjp: NULL pointer");
+  if (g_psi && !src) return set_error(DFM_E_NULL, "the map gradient needs the pullback source");
+  const int64_t cnt = count_of_host(G, 3);
+  dim3 grid(blocks_for(cnt)), blk(kBlock);
+  DFM_DISPATCH_DIM(G,
+                   hipLaunchKernelGGL((k_pullback_s_vjp<2>), grid, blk, 0, S_(stream), G, g_out, src, psi, alpha, g_src,
+                                      g_psi, cnt),
+                   hipLaunchKernelGGL((k_pullback_s_vjp<3>), grid, blk, 0, S_(stream), G, g_out, src, psi, alpha, g_src,
+                                      g_psi, cnt));
+  return launch_check("dfm_pullback_s_vjp");
+}
+
+extern "C" int dfm_sl_advect_vjp(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec,
+                                 const dfm_mac* src_vec, const dfm_mac* g_dst_vec, const dfm_mac* g_src_vec, int nscal,
+                                 const float* const* src_s, const float* const* g_dst_s, float* const* g_src_s,
+                                 const dfm_mac* g_vel, void* stream) {
+  Geo G;
+  int rc = check_grid(g, G);
+  if (rc) return rc;
+  if ((rc = check_mac(G, vel, "advection velocity is NULL"))) return rc;
+  if (order != 2 && order != 4) return set_error(DFM_E_ARG, "order must be 2 or 4");
+  if (nvec < 0 || nvec > kMaxVec || nscal < 0 || nscal > kMaxScal) return set_error(DFM_E_ARG, "too many sources");
+  SLVjpArgs A{};
+  A.nvec = nvec;
+  A.nscal = nscal;
+  if (g_vel) {
+    if ((rc = check_mac(G, g_vel, "velocity gradient is NULL"))) return rc;
+    for (int a = 0; a < 3; ++a) A.gvel[a] = a < G.dim ? g_vel->c[a] : nullptr;
+  }
+  for (int m = 0; m < nvec; ++m) {
+    if ((rc = check_mac(G, &g_dst_vec[m], "advected MAC gradient is NULL"))) return rc;
+    if ((rc = check_mac(G, &g_src_vec[m], "MAC source gradient is NULL"))) return rc;
+    A.gdst[m] = to_mac(&g_dst_vec[m]);
+    for (int a = 0; a < 3; ++a) A.gsrc[m][a] = g_src_vec[m].c[a];
+    if (g_vel) {
+      if (!src_vec) return set_error(DFM_E_NULL, "the velocity gradient needs the forward sources");
+      if ((rc = check_mac(G, &src_vec[m], "MAC source is NULL"))) return rc;
+      A.fsrc[m] = to_mac(&src_vec[m]);
+    }
+  }
+  for (int m = 0; m < nscal; ++m) {
+    if (!g_dst_s || !g_src_s || !g_dst_s[m] || !g_src_s[m]) return set_error(DFM_E_NULL, "scalar gradient is NULL");
+    A.gds[m] = g_dst_s[m];
+    A.gss[m] = g_src_s[m];
+    if (g_vel) {
+      if (!src_s || !src_s[m]) return set_error(DFM_E_NULL, "the velocity gradient needs the forward sources");
+      A.fs[m] = src_s[m];
+    }
+  }
+  Sets S = sets_init();
+  if (nvec > 0)
+    for (int a = 0; a < G.dim; ++a) sets_add(S, G, a);
+  if (nscal > 0) sets_add(S, G, 3);
+  if (S.nset == 0) return DFM_OK;
+  dim3 grid(blocks_for(S.off[S.nset])), blk(kBlock);
+  Mac V = to_mac(vel);
+  hipStream_t st = S_(stream);
+  if (G.dim == 2) {
+    if (order == 2) hipLaunchKernelGGL((k_sl_advect_vjp<2, 2>), grid, blk, 0, st, G, V, dt, S, A);
+    else hipLaunchKernelGGL((k_sl_advect_vjp<2, 4>), grid, blk, 0, st, G, V, dt, S, A);
+  } else {
+    if (order == 2) hipLaunchKernelGGL((k_sl_advect_vjp<3, 2>), grid, blk, 0, st, G, V, dt, S, A);
+    else hipLaunchKernelGGL((k_sl_advect_vjp<3, 4>), grid, blk, 0, st, G, V, dt, S, A);
+  }
+  return launch_check("dfm_sl_advect_vjp");
+}
+
+extern "C" int dfm_march_vjp(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m0, float dt, int order,
+                             const dfm_flowmap* gm, const dfm_mac* g_u, void* stream) {
+  Geo G;
+  int rc = check_grid(g, G);
+  if (rc) return rc;
+  if ((rc = check_mac(G, u, "velocity is NULL"))) return rc;
+  if ((rc = check_mac(G, g_u, "velocity gradient is NULL"))) return rc;
+  if (!m0 || !gm) return set_error(DFM_E_NULL, "flowmap is NULL");
+  if (order != 2 && order != 4) return set_error(DFM_E_ARG, "order must be 2 or 4");
+  Sets S = sets_init();
+  MapPtrs M{}, GM{};
+  for (int a = 0; a < G.dim; ++a)
+    if (m0->psi_f[a]) {
+      if (!gm->psi_f[a] || (m0->T_f[a] && !gm->T_f[a]))
+        return set_error(DFM_E_NULL, "map gradient lacks a set the map has");
+      M.psi[S.nset] = m0->psi_f[a];
+      M.T[S.nset] = m0->T_f[a];
+      GM.psi[S.nset] = gm->psi_f[a];
+      GM.T[S.nset] = gm->T_f[a];
+      sets_add(S, G, a);
+    }
+  if (m0->psi) {
+    if (!gm->psi) return set_error(DFM_E_NULL, "map gradient lacks a set the map has");
+    M.psi[S.nset] = m0->psi;
+    GM.psi[S.nset] = gm->psi;
+    sets_add(S, G, 3);
+  }
+  if (S.nset == 0) return DFM_OK;
+  Mac U = to_mac(u);
+  GuPtrs GU{{g_u->c[0], g_u->c[1], g_u->c[2]}};
+  dim3 grid(blocks_for(S.off[S.nset])), blk(kBlock);
+  hipStream_t st = S_(stream);
+  if (G.dim == 2) {
+    if (order == 2) hipLaunchKernelGGL((k_march_vjp<2, 2>), grid, blk, 0, st, G, U, dt, S, M, GM, GU);
+    else hipLaunchKernelGGL((k_march_vjp<2, 4>), grid, blk, 0, st, G, U, dt, S, M, GM, GU);
+  } else {
+    if (order == 2) hipLaunchKernelGGL((k_march_vjp<3, 2>), grid, blk, 0, st, G, U, dt, S, M, GM, GU);
+    else hipLaunchKernelGGL((k_march_vjp<3, 4>), grid, blk, 0, st, G, U, dt, S, M, GM, GU);
+  }
+  return launch_check("dfm_march_vjp");
 }

@@ -68,6 +68,17 @@ class FlowMap:
         self.T_f = [torch.empty((dim,) + face_shape(n, a), dtype=torch.float32, device=device) for a in range(dim)]
         self._s = None
 
+    @staticmethod
+    def wrap(psi, psi_f, T_f):
+        """A FlowMap over existing tensors (psi may be None; T_f entries may be
+        None for a map without Jacobian columns)."""
+        fm = FlowMap.__new__(FlowMap)
+        face0 = psi_f[0].shape[1:]  # n + e_0
+        fm.n = (face0[0] - 1,) + tuple(face0[1:])
+        fm.psi, fm.psi_f, fm.T_f = psi, list(psi_f), list(T_f)
+        fm._s = None
+        return fm
+
     @property
     def s(self):
         if self._s is None:
@@ -139,6 +150,37 @@ class Ops:
         with timing.region("sl_advect"):
             L.check(self.lib.dfm_sl_advect(self.g, vel.s, float(dt), int(order), nv, src_arr, dst_arr, ns,
                                            L.ptr_array(s_src), L.ptr_array(s_dst), self._st()), "sl_advect")
+
+    # vector-Jacobian products (the gradients accumulate into the g_* outputs)
+    def pullback_u_vjp(self, g_out: MacField, fm: FlowMap, g_src: MacField, alpha=1.0, src: MacField = None,
+                       g_fm: FlowMap = None):
+        """g_src += transpose of the pullback applied to g_out; with g_fm (and
+        src) also the map's psi_f / T_f gradients."""
+        L.check(self.lib.dfm_pullback_u_vjp(self.g, g_out.s, src.s if src is not None else None, fm.s, float(alpha),
+                                            g_src.s, g_fm.s if g_fm is not None else None, self._st()),
+                "pullback_u_vjp")
+
+    def pullback_s_vjp(self, g_out, psi, g_src, alpha=1.0, src=None, g_psi=None):
+        L.check(self.lib.dfm_pullback_s_vjp(self.g, g_out.data_ptr(), L.ptr(src), psi.data_ptr(), float(alpha),
+                                            g_src.data_ptr(), L.ptr(g_psi), self._st()), "pullback_s_vjp")
+
+    def sl_advect_vjp(self, vel: MacField, dt, order, g_vec_dst=(), g_vec_src=(), g_s_dst=(), g_s_src=(),
+                      vec_src=(), s_src=(), g_vel: MacField = None):
+        """Transpose of sl_advect w.r.t. its sources; with g_vel (and the
+        forward sources vec_src / s_src) also the advecting velocity's."""
+        nv, ns = len(g_vec_dst), len(g_s_dst)
+        mk = lambda ms: (L.Mac * max(1, nv))(*[m.s for m in ms])  # noqa: E731
+        L.check(self.lib.dfm_sl_advect_vjp(self.g, vel.s, float(dt), int(order), nv,
+                                           mk(vec_src) if vec_src else None, mk(g_vec_dst), mk(g_vec_src), ns,
+                                           L.ptr_array(s_src) if s_src else None, L.ptr_array(g_s_dst),
+                                           L.ptr_array(g_s_src), g_vel.s if g_vel is not None else None, self._st()),
+                "sl_advect_vjp")
+
+    def march_vjp(self, u: MacField, fm0: FlowMap, dt, order, g_fm: FlowMap, g_u: MacField):
+        """g_fm: gradient w.r.t. the marched map on entry, w.r.t. fm0 on
+        return; g_u accumulates the velocity's."""
+        L.check(self.lib.dfm_march_vjp(self.g, u.s, fm0.s, float(dt), int(order), g_fm.s, g_u.s, self._st()),
+                "march_vjp")
 
     def adj_conv_source(self, u: MacField, w: MacField, kern, r: MacField):
         with timing.region("adj_conv"):

@@ -154,6 +154,43 @@ int dfm_adj_conv_source(const dfm_grid* g, const dfm_mac* u, const dfm_mac* w, i
 int dfm_adj_scalar_source(const dfm_grid* g, int nscal, const float* const* s, const float* const* lam,
                           const dfm_mac* r, void* stream);
 
+/* ------------------------------------------- vector-Jacobian products ----
+ * Reverse-mode derivatives of the forward stages (no counterpart in the
+ * reference, which differentiates by its continuous adjoint; they back the
+ * per-stage torch.autograd.Functions).  The transpose of a B-spline sample
+ * scatters the incoming value with the sample's own weights to its own
+ * clamped taps by FP32 atomic adds: results agree to rounding from run to
+ * run.  Every output gradient is ACCUMULATED (the caller zeroes it first).
+ * A sample's derivative w.r.t. its point is zero along an axis where the
+ * sample coordinate is clamped at the wall band, as the forward's value is
+ * constant there.
+ *
+ * dfm_pullback_u_vjp: for dfm_pullback_u's alpha (T_a . src(psi_a)) part,
+ * g_src_b += alpha T_a[b] g_out_a at psi_a; with g_m non-NULL (psi_f / T_f
+ * sets; src then required) also the map's gradients g_T_a[b] += alpha g_out_a
+ * src_b(psi_a) and g_psi_a += alpha g_out_a sum_b T_a[b] grad src_b(psi_a). */
+int dfm_pullback_u_vjp(const dfm_grid* g, const dfm_mac* g_out, const dfm_mac* src, const dfm_flowmap* m, float alpha,
+                       const dfm_mac* g_src, const dfm_flowmap* g_m, void* stream);
+/* dfm_pullback_s_vjp: for dfm_pullback_s's alpha src(psi) part, g_src +=
+ * alpha g_out at psi; with g_psi non-NULL ((dim, *n), src then required)
+ * also g_psi += alpha g_out grad src(psi). */
+int dfm_pullback_s_vjp(const dfm_grid* g, const float* g_out, const float* src, const float* psi, float alpha,
+                       float* g_src, float* g_psi, void* stream);
+/* dfm_sl_advect_vjp: for dfm_sl_advect (dst = src at the back-traced node),
+ * g_src_* += the transpose of the sample applied to g_dst_*; with g_vel
+ * non-NULL (src_vec / src_s then required, the forward sources) also the
+ * advecting velocity's gradient through the RK back-trace. */
+int dfm_sl_advect_vjp(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec, const dfm_mac* src_vec,
+                      const dfm_mac* g_dst_vec, const dfm_mac* g_src_vec, int nscal, const float* const* src_s,
+                      const float* const* g_dst_s, float* const* g_src_s, const dfm_mac* g_vel, void* stream);
+/* dfm_march_vjp: the vector-Jacobian product of dfm_march.  m0 is the map
+ * before the march (read only); gm holds on entry the gradients w.r.t. the
+ * marched map's psi / T sets (the same sets as m0) and on return those
+ * w.r.t. m0's (replaced in place, not accumulated); g_u accumulates the
+ * velocity's.  The RK stages are recomputed from m0. */
+int dfm_march_vjp(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m0, float dt, int order,
+                  const dfm_flowmap* gm, const dfm_mac* g_u, void* stream);
+
 /* out_a = x_a + alpha * y_a + beta * z_a on every face grid (z may be NULL
  * when beta == 0).  Replaces add_fields / scale_field on MAC fields,
  * 2D/vortex/taichi_utils.py:105-113. */
