@@ -110,9 +110,11 @@ __device__ inline SlabWalk<I> slab_walk(I n) {
 }
 
 // ----------------------------------------------------------- B-splines ----
-// Piecewise polynomials of the uniform B-splines of order 2 (quadratic,
-// support 1.5), 3 (cubic, support 2) and 4 (quartic, support 2.5), written
-// from their closed forms; derivative sign follows x.
+// Piecewise polynomials of the uniform B-splines of order 1 (linear, support
+// 1), 2 (quadratic, support 1.5), 3 (cubic, support 2) and 4 (quartic,
+// support 2.5), written from their closed forms; derivative sign follows x
+// (the linear one's derivative at x = 0 is -1, as N_1's derivative in
+// 2D/vortex/taichi_utils.py:213-218).
 //
 // Only the taps inside the support are visited: NT = K + 1 consecutive nodes
 // starting at base(s).  The reference loops over 4 / 6 / 8 taps per axis
@@ -125,6 +127,22 @@ __device__ inline SlabWalk<I> slab_walk(I n) {
 // sample's offset t from the support's reference node REF = base + REF_OFF
 // (tap x = t + REF_OFF - tap index); the pieces are those of w(x) below.
 template <int K> struct Spline;
+
+template <> struct Spline<1> {
+  static constexpr int NT = 2;       // nodes floor(s) .. floor(s)+1
+  static constexpr int REF_OFF = 0;  // t = s - floor(s) in [0, 1)
+  DFM_HD static int base(float s) { return (int)floorf(s); }
+  DFM_HD static float w(float x) { return fmaxf(0.0f, 1.0f - fabsf(x)); }
+  DFM_HD static float dw(float x) { return fabsf(x) < 1.0f ? (x < 0.0f ? 1.0f : -1.0f) : 0.0f; }
+  DFM_HD static void weights(float t, float* wt) {
+    wt[0] = 1.0f - t;
+    wt[1] = t;
+  }
+  DFM_HD static void dweights(float, float* d) {
+    d[0] = -1.0f;
+    d[1] = 1.0f;
+  }
+};
 
 template <> struct Spline<2> {
   static constexpr int NT = 3;       // nodes round(s)-1 .. round(s)+1
@@ -240,12 +258,21 @@ template <> struct Spline<4> {
 };
 
 // Clamped sample coordinate s in [1, S-2-1e-6] (the reference's clamp,
-// 2D/vortex/taichi_utils.py:420-421) and the first node of the support.  For
-// the quadratic spline s in [1, S-2] puts the 3 nodes inside [0, S-1]; the
-// base is clamped anyway so that no input (NaN included) can leave the field.
-// Wider splines clamp every node index to the field, as sample() does.
+// 2D/vortex/taichi_utils.py:420-421; [0, S-1-1e-6] for the linear spline,
+// :225-226) and the first node of the support.  For the quadratic spline s in
+// [1, S-2] puts the 3 nodes inside [0, S-1]; the base is clamped anyway so
+// that no input (NaN included) can leave the field.  The other splines clamp
+// every node index to the field, as sample() does.
+template <int K> DFM_HD float clamp_lo() { return K == 1 ? 0.0f : 1.0f; }
+template <int K> DFM_HD float clamp_hi(int S) { return (float)(S - (K == 1 ? 1 : 2)) - 1e-6f; }
 template <int K> DFM_HD float clamp_coord(float p_over_dx, float bl, int S) {
-  return fmaxf(1.0f, fminf(p_over_dx - bl, (float)(S - 2) - 1e-6f));
+  return fmaxf(clamp_lo<K>(), fminf(p_over_dx - bl, clamp_hi<K>(S)));
+}
+// 1 where the sample coordinate p/dx - bl is strictly inside the clamp range
+// (the sample then moves with p), 0 where it is clamped.
+template <int K> DFM_HD float clamp_live(float p_over_dx, float bl, int S) {
+  const float raw = p_over_dx - bl;
+  return (raw > clamp_lo<K>() && raw < clamp_hi<K>(S)) ? 1.0f : 0.0f;
 }
 template <int K> DFM_HD int support_base(float s, int S) {
   int b = Spline<K>::base(s);
@@ -405,11 +432,11 @@ struct Mac {
   const float* c[3];
 };
 
-// u(p) for the MAC field (quadratic), optionally J[a][b] = d u_a / d x_b.
-// The component loop stays rolled; its results are stored through
-// wave-uniform branches (a dynamically indexed val[a] / J[a] would live in
-// scratch memory).
-template <bool GRAD, int DIM>
+// u(p) for the MAC field (spline order K, quadratic by default), optionally
+// J[a][b] = d u_a / d x_b.  The component loop stays rolled; its results are
+// stored through wave-uniform branches (a dynamically indexed val[a] / J[a]
+// would live in scratch memory).
+template <bool GRAD, int DIM, int K = 2>
 DFM_HD void mac_eval(const Geo& g, const Mac& u, const float p[3], float val[3], float J[3][3]) {
 #pragma unroll 1
   for (int a = 0; a < DIM; ++a) {
@@ -417,7 +444,7 @@ DFM_HD void mac_eval(const Geo& g, const Mac& u, const float p[3], float val[3],
     shape_of(g, a, S);
     const float bl[3] = {a == 0 ? 0.0f : 0.5f, a == 1 ? 0.0f : 0.5f, a == 2 ? 0.0f : 0.5f};
     float gr[3] = {0.f, 0.f, 0.f};
-    const float v = interp<2, GRAD, DIM>(u.c[a], S, bl, p, g.inv_dx, gr);
+    const float v = interp<K, GRAD, DIM>(u.c[a], S, bl, p, g.inv_dx, gr);
     if (a == 0) {
       val[0] = v;
       if (GRAD) { J[0][0] = gr[0]; J[0][1] = gr[1]; J[0][2] = gr[2]; }

@@ -5,6 +5,7 @@
 #include "../../include/dfm.h"
 
 #include <mutex>
+#include <type_traits>
 
 using namespace dfm;
 
@@ -166,7 +167,7 @@ struct Epi {
   float* out[4];
 };
 
-template <int DIM>
+template <int DIM, int K>
 __global__ void __launch_bounds__(kBlock) k_pullback_u(Geo g, Mac src, Sets S, MapPtrs M, Epi E) {
   int64_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (gid >= S.off[S.nset]) return;
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_pullback_u(Geo g, Mac src, Sets S, M
   float p[3] = {0.f, 0.f, 0.f}, v[3];
 #pragma unroll
   for (int c = 0; c < DIM; ++c) p[c] = M.psi[s][c * cnt + i];
-  mac_eval<false, DIM>(g, src, p, v, nullptr);
+  mac_eval<false, DIM, K>(g, src, p, v, nullptr);
   float d = 0.f;
 #pragma unroll
   for (int c = 0; c < DIM; ++c) d = fmaf(M.T[s][c * cnt + i], v[c], d);
@@ -185,7 +186,7 @@ __global__ void __launch_bounds__(kBlock) k_pullback_u(Geo g, Mac src, Sets S, M
   E.out[s][i] = r;
 }
 
-template <int DIM>
+template <int DIM, int K>
 __global__ void __launch_bounds__(kBlock) k_pullback_s(Geo g, const float* __restrict__ src,
                                                        const float* __restrict__ psi, float alpha, float beta,
                                                        const float* add, float* out, int64_t cnt) {
@@ -197,7 +198,7 @@ __global__ void __launch_bounds__(kBlock) k_pullback_s(Geo g, const float* __res
   int S[3];
   shape_of(g, 3, S);
   const float bl[3] = {0.5f, 0.5f, 0.5f};
-  float v = interp<2, false, DIM>(src, S, bl, p, g.inv_dx, nullptr);
+  float v = interp<K, false, DIM>(src, S, bl, p, g.inv_dx, nullptr);
   float r = alpha * v;
   if (beta != 0.f) r = fmaf(beta, add[i], r);
   out[i] = r;
@@ -283,7 +284,9 @@ DFM_HD void backtrace(const Geo& g, const Mac& vel, float dt, const float x[3], 
   for (int a = 0; a < DIM; ++a) p[a] = x[a] - (v1[a] + 2.0f * v2[a] + 2.0f * v3[a] + v4[a]) / 6.0f * dt;
 }
 
-template <int DIM, int ORDER>
+// The back-trace samples the velocity quadratically; the sources are sampled
+// with the spline of order KS.
+template <int DIM, int ORDER, int KS>
 __global__ void __launch_bounds__(kBlock) k_sl_advect(Geo g, Mac vel, float dt, Sets S, SLArgs A) {
   int64_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (gid >= S.off[S.nset]) return;
@@ -300,9 +303,9 @@ __global__ void __launch_bounds__(kBlock) k_sl_advect(Geo g, Mac vel, float dt, 
   float bl[3] = {0.5f, 0.5f, 0.5f};
   if (grid < 3) {
     bl[grid] = 0.0f;
-    for (int m = 0; m < A.nvec; ++m) A.dst[m][grid][i] = interp<2, false, DIM>(A.src[m].c[grid], sh, bl, p, g.inv_dx, nullptr);
+    for (int m = 0; m < A.nvec; ++m) A.dst[m][grid][i] = interp<KS, false, DIM>(A.src[m].c[grid], sh, bl, p, g.inv_dx, nullptr);
   } else {
-    for (int m = 0; m < A.nscal; ++m) A.sdst[m][i] = interp<2, false, DIM>(A.ssrc[m], sh, bl, p, g.inv_dx, nullptr);
+    for (int m = 0; m < A.nscal; ++m) A.sdst[m][i] = interp<KS, false, DIM>(A.ssrc[m], sh, bl, p, g.inv_dx, nullptr);
   }
 }
 
@@ -502,17 +505,14 @@ DFM_HD void scatter_sample(float* __restrict__ g, const int S[3], const float bl
   }
 }
 
-// Gradient w.r.t. the point of the quadratic sample of f at p, zero along
+// Gradient w.r.t. the point of the order-K sample of f at p, zero along
 // clamped axes; returns the value.
-template <int DIM>
+template <int K, int DIM>
 DFM_HD float sample_dp(const float* __restrict__ f, const int S[3], const float bl[3], const float p[3], float inv_dx,
                        float gr[3]) {
-  const float v = interp<2, true, DIM>(f, S, bl, p, inv_dx, gr);
+  const float v = interp<K, true, DIM>(f, S, bl, p, inv_dx, gr);
 #pragma unroll
-  for (int c = 0; c < DIM; ++c) {
-    const float raw = p[c] * inv_dx - bl[c];
-    if (!(raw > 1.0f && raw < (float)(S[c] - 2) - 1e-6f)) gr[c] = 0.f;
-  }
+  for (int c = 0; c < DIM; ++c) gr[c] *= clamp_live<K>(p[c] * inv_dx, bl[c], S[c]);
   return v;
 }
 
@@ -537,9 +537,8 @@ DFM_HD void eval_vjp(const Geo& g, const Mac& u, const float q[3], const float g
     float live[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-      const float raw = q[c] * g.inv_dx - bl[c];
       ax[c].setup(q[c] * g.inv_dx, bl[c], S[c]);
-      live[c] = (raw > 1.0f && raw < (float)(S[c] - 2) - 1e-6f) ? 1.0f : 0.0f;
+      live[c] = clamp_live<2>(q[c] * g.inv_dx, bl[c], S[c]);
     }
     float gJ[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -593,7 +592,7 @@ struct GuPtrs {
 // point: g_src_b += alpha T_a[b] g_out_a at psi_a; with GM.psi set, also
 // g_T_a[b] += alpha g_out_a src_b(psi_a) and g_psi_a += alpha g_out_a
 // sum_b T_a[b] grad src_b(psi_a).
-template <int DIM>
+template <int DIM, int K>
 __global__ void __launch_bounds__(kBlock) k_pullback_u_vjp(Geo g, Mac gout, Mac src, Sets S, MapPtrs M, float alpha,
                                                           Epi G, MapPtrs GM) {
   int64_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -613,10 +612,10 @@ __global__ void __launch_bounds__(kBlock) k_pullback_u_vjp(Geo g, Mac gout, Mac 
     float bl[3] = {0.5f, 0.5f, 0.5f};
     bl[b] = 0.0f;
     const float Tb = M.T[s][b * cnt + i];
-    scatter_sample<2, DIM>(G.out[b], Sb, bl, p, g.inv_dx, go * Tb);
+    scatter_sample<K, DIM>(G.out[b], Sb, bl, p, g.inv_dx, go * Tb);
     if (want_map) {
       float gr[3] = {0.f, 0.f, 0.f};
-      const float v = sample_dp<DIM>(src.c[b], Sb, bl, p, g.inv_dx, gr);
+      const float v = sample_dp<K, DIM>(src.c[b], Sb, bl, p, g.inv_dx, gr);
       GM.T[s][b * cnt + i] += go * v;
 #pragma unroll
       for (int c = 0; c < DIM; ++c) gp[c] = fmaf(go * Tb, gr[c], gp[c]);
@@ -630,7 +629,7 @@ __global__ void __launch_bounds__(kBlock) k_pullback_u_vjp(Geo g, Mac gout, Mac 
 
 // k_pullback_s is out = alpha src(psi) (+ beta add): g_src += alpha g_out at
 // psi; with gpsi set, also gpsi += alpha g_out grad src(psi).
-template <int DIM>
+template <int DIM, int K>
 __global__ void __launch_bounds__(kBlock) k_pullback_s_vjp(Geo g, const float* __restrict__ gout,
                                                           const float* __restrict__ src,
                                                           const float* __restrict__ psi, float alpha,
@@ -645,10 +644,10 @@ __global__ void __launch_bounds__(kBlock) k_pullback_s_vjp(Geo g, const float* _
   shape_of(g, 3, Sc);
   const float bl[3] = {0.5f, 0.5f, 0.5f};
   const float go = alpha * gout[i];
-  scatter_sample<2, DIM>(gsrc, Sc, bl, p, g.inv_dx, go);
+  scatter_sample<K, DIM>(gsrc, Sc, bl, p, g.inv_dx, go);
   if (gpsi) {
     float gr[3] = {0.f, 0.f, 0.f};
-    sample_dp<DIM>(src, Sc, bl, p, g.inv_dx, gr);
+    sample_dp<K, DIM>(src, Sc, bl, p, g.inv_dx, gr);
 #pragma unroll
     for (int c = 0; c < DIM; ++c) gpsi[c * cnt + i] += go * gr[c];
   }
@@ -669,7 +668,7 @@ struct SLVjpArgs {
 // through vel): the same back-trace, each destination's incoming value
 // scattered to its source at p; with the velocity gradient wanted, the
 // gradient at p is run back through the RK stages of the back-trace.
-template <int DIM, int ORDER>
+template <int DIM, int ORDER, int KS>
 __global__ void __launch_bounds__(kBlock) k_sl_advect_vjp(Geo g, Mac vel, float dt, Sets S, SLVjpArgs A) {
   int64_t gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (gid >= S.off[S.nset]) return;
@@ -691,10 +690,10 @@ __global__ void __launch_bounds__(kBlock) k_sl_advect_vjp(Geo g, Mac vel, float 
   for (int m = 0; m < nm; ++m) {
     const float gi = grid < 3 ? A.gdst[m].c[grid][i] : A.gds[m][i];
     float* dst = grid < 3 ? A.gsrc[m][grid] : A.gss[m];
-    scatter_sample<2, DIM>(dst, sh, bl, p, g.inv_dx, gi);
+    scatter_sample<KS, DIM>(dst, sh, bl, p, g.inv_dx, gi);
     if (want_vel) {
       float gr[3] = {0.f, 0.f, 0.f};
-      sample_dp<DIM>(grid < 3 ? A.fsrc[m].c[grid] : A.fs[m], sh, bl, p, g.inv_dx, gr);
+      sample_dp<KS, DIM>(grid < 3 ? A.fsrc[m].c[grid] : A.fs[m], sh, bl, p, g.inv_dx, gr);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) gp[c] = fmaf(gi, gr[c], gp[c]);
     }
@@ -873,6 +872,22 @@ Sets face_sets(const Geo& G) {
     }                                     \
   } while (0)
 
+// Calls f(std::integral_constant<int, K>) for the sampling spline of order
+// kern: 1 (linear, N_1), 2 (quadratic, N_2, the simulator's) or 3 (cubic, N_3).
+int check_kern(int kern) {
+  if (kern < 1 || kern > 3)
+    return set_error(DFM_E_ARG, "interpolation kernel must be 1 (linear), 2 (quadratic) or 3 (cubic)");
+  return DFM_OK;
+}
+template <class F> int with_kern(int kern, F&& f) {
+  switch (kern) {
+    case 1: f(std::integral_constant<int, 1>{}); return DFM_OK;
+    case 2: f(std::integral_constant<int, 2>{}); return DFM_OK;
+    case 3: f(std::integral_constant<int, 3>{}); return DFM_OK;
+    default: return check_kern(kern);
+  }
+}
+
 extern "C" int dfm_version(void) { return 1; }
 
 extern "C" int dfm_reset_identity(const dfm_grid* g, const dfm_flowmap* m, void* stream) {
@@ -935,11 +950,12 @@ extern "C" int dfm_march(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap*
   return launch_check("dfm_march");
 }
 
-extern "C" int dfm_pullback_u(const dfm_grid* g, const dfm_mac* src, const dfm_flowmap* m, float alpha, float beta,
-                              const dfm_mac* add, const dfm_mac* out, void* stream) {
+extern "C" int dfm_pullback_u_k(const dfm_grid* g, int kern, const dfm_mac* src, const dfm_flowmap* m, float alpha,
+                                float beta, const dfm_mac* add, const dfm_mac* out, void* stream) {
   Geo G;
   int rc = check_grid(g, G);
   if (rc) return rc;
+  if ((rc = check_kern(kern))) return rc;
   if ((rc = check_mac(G, src, "pullback source is NULL"))) return rc;
   if ((rc = check_mac(G, out, "pullback output is NULL"))) return rc;
   if (beta != 0.f && (rc = check_mac(G, add, "pullback add field is NULL"))) return rc;
@@ -960,34 +976,59 @@ extern "C" int dfm_pullback_u(const dfm_grid* g, const dfm_mac* src, const dfm_f
   int64_t total = S.off[S.nset];
   dim3 grid(blocks_for(total)), blk(kBlock);
   Mac U = to_mac(src);
-  DFM_DISPATCH_DIM(G, hipLaunchKernelGGL((k_pullback_u<2>), grid, blk, 0, S_(stream), G, U, S, M, E),
-                   hipLaunchKernelGGL((k_pullback_u<3>), grid, blk, 0, S_(stream), G, U, S, M, E));
-  return launch_check("dfm_pullback_u");
+  rc = with_kern(kern, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    DFM_DISPATCH_DIM(G, hipLaunchKernelGGL((k_pullback_u<2, K>), grid, blk, 0, S_(stream), G, U, S, M, E),
+                     hipLaunchKernelGGL((k_pullback_u<3, K>), grid, blk, 0, S_(stream), G, U, S, M, E));
+  });
+  return rc ? rc : launch_check("dfm_pullback_u");
+}
+
+extern "C" int dfm_pullback_u(const dfm_grid* g, const dfm_mac* src, const dfm_flowmap* m, float alpha, float beta,
+                              const dfm_mac* add, const dfm_mac* out, void* stream) {
+  return dfm_pullback_u_k(g, 2, src, m, alpha, beta, add, out, stream);
+}
+
+extern "C" int dfm_pullback_s_k(const dfm_grid* g, int kern, const float* src, const float* psi, float alpha,
+                                float beta, const float* add, float* out, void* stream) {
+  Geo G;
+  int rc = check_grid(g, G);
+  if (rc) return rc;
+  if ((rc = check_kern(kern))) return rc;
+  if (!src || !psi || !out || (beta != 0.f && !add)) return set_error(DFM_E_NULL, "dfm_pullback_s: NULL pointer");
+  int64_t cnt = count_of_host(G, 3);
+  dim3 grid(blocks_for(cnt)), blk(kBlock);
+  rc = with_kern(kern, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    DFM_DISPATCH_DIM(G,
+                     hipLaunchKernelGGL((k_pullback_s<2, K>), grid, blk, 0, S_(stream), G, src, psi, alpha, beta, add,
+                                        out, cnt),
+                     hipLaunchKernelGGL((k_pullback_s<3, K>), grid, blk, 0, S_(stream), G, src, psi, alpha, beta, add,
+                                        out, cnt));
+  });
+  return rc ? rc : launch_check("dfm_pullback_s");
 }
 
 extern "C" int dfm_pullback_s(const dfm_grid* g, const float* src, const float* psi, float alpha, float beta,
                               const float* add, float* out, void* stream) {
-  Geo G;
-  int rc = check_grid(g, G);
-  if (rc) return rc;
-  if (!src || !psi || !out || (beta != 0.f && !add)) return set_error(DFM_E_NULL, "dfm_pullback_s: NULL pointer");
-  int64_t cnt = count_of_host(G, 3);
-  dim3 grid(blocks_for(cnt)), blk(kBlock);
-  DFM_DISPATCH_DIM(G,
-                   hipLaunchKernelGGL((k_pullback_s<2>), grid, blk, 0, S_(stream), G, src, psi, alpha, beta, add, out, cnt),
-                   hipLaunchKernelGGL((k_pullback_s<3>), grid, blk, 0, S_(stream), G, src, psi, alpha, beta, add, out, cnt));
-  return launch_check("dfm_pullback_s");
+  return dfm_pullback_s_k(g, 2, src, psi, alpha, beta, add, out, stream);
+}
+
+extern "C" int dfm_bfecc_u_k(const dfm_grid* g, int kern, const dfm_mac* init, const dfm_flowmap* fwd,
+                             const dfm_flowmap* bwd, const dfm_mac* err, const dfm_mac* out, void* stream) {
+  int rc;
+  if ((rc = check_kern(kern))) return rc;
+  // out = T.init(psi)
+  if ((rc = dfm_pullback_u_k(g, kern, init, fwd, 1.0f, 0.0f, nullptr, out, stream))) return rc;
+  // err = 0.5 F.out(phi) - 0.5 init
+  if ((rc = dfm_pullback_u_k(g, kern, out, bwd, 0.5f, -0.5f, init, err, stream))) return rc;
+  // out = out - T.err(psi)
+  return dfm_pullback_u_k(g, kern, err, fwd, -1.0f, 1.0f, out, out, stream);
 }
 
 extern "C" int dfm_bfecc_u(const dfm_grid* g, const dfm_mac* init, const dfm_flowmap* fwd, const dfm_flowmap* bwd,
                            const dfm_mac* err, const dfm_mac* out, void* stream) {
-  int rc;
-  // out = T.init(psi)
-  if ((rc = dfm_pullback_u(g, init, fwd, 1.0f, 0.0f, nullptr, out, stream))) return rc;
-  // err = 0.5 F.out(phi) - 0.5 init
-  if ((rc = dfm_pullback_u(g, out, bwd, 0.5f, -0.5f, init, err, stream))) return rc;
-  // out = out - T.err(psi)
-  return dfm_pullback_u(g, err, fwd, -1.0f, 1.0f, out, out, stream);
+  return dfm_bfecc_u_k(g, 2, init, fwd, bwd, err, out, stream);
 }
 
 namespace {
@@ -1035,12 +1076,18 @@ extern "C" int dfm_bfecc_u_noalloc(const dfm_grid* g, const dfm_mac* init, const
   return dfm_bfecc_u(g, init, fwd, bwd, &err, out, stream);
 }
 
+extern "C" int dfm_bfecc_s_k(const dfm_grid* g, int kern, const float* init, const float* psi, const float* phi,
+                             float* err, float* out, void* stream) {
+  int rc;
+  if ((rc = check_kern(kern))) return rc;
+  if ((rc = dfm_pullback_s_k(g, kern, init, psi, 1.0f, 0.0f, nullptr, out, stream))) return rc;
+  if ((rc = dfm_pullback_s_k(g, kern, out, phi, 0.5f, -0.5f, init, err, stream))) return rc;
+  return dfm_pullback_s_k(g, kern, err, psi, -1.0f, 1.0f, out, out, stream);
+}
+
 extern "C" int dfm_bfecc_s(const dfm_grid* g, const float* init, const float* psi, const float* phi, float* err,
                            float* out, void* stream) {
-  int rc;
-  if ((rc = dfm_pullback_s(g, init, psi, 1.0f, 0.0f, nullptr, out, stream))) return rc;
-  if ((rc = dfm_pullback_s(g, out, phi, 0.5f, -0.5f, init, err, stream))) return rc;
-  return dfm_pullback_s(g, err, psi, -1.0f, 1.0f, out, out, stream);
+  return dfm_bfecc_s_k(g, 2, init, psi, phi, err, out, stream);
 }
 
 extern "C" int dfm_accumulate(const dfm_grid* g, const dfm_mac* f, const dfm_flowmap* m, const dfm_mac* init,
@@ -1076,12 +1123,13 @@ extern "C" int dfm_accumulate(const dfm_grid* g, const dfm_mac* f, const dfm_flo
   return launch_check("dfm_accumulate");
 }
 
-extern "C" int dfm_sl_advect(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec,
-                             const dfm_mac* src_vec, const dfm_mac* dst_vec, int nscal, const float* const* src_s,
-                             float* const* dst_s, void* stream) {
+extern "C" int dfm_sl_advect_k(const dfm_grid* g, int kern, const dfm_mac* vel, float dt, int order, int nvec,
+                               const dfm_mac* src_vec, const dfm_mac* dst_vec, int nscal, const float* const* src_s,
+                               float* const* dst_s, void* stream) {
   Geo G;
   int rc = check_grid(g, G);
   if (rc) return rc;
+  if ((rc = check_kern(kern))) return rc;
   if ((rc = check_mac(G, vel, "advection velocity is NULL"))) return rc;
   if (order != 2 && order != 4) return set_error(DFM_E_ARG, "order must be 2 or 4");
   if (nvec < 0 || nvec > kMaxVec || nscal < 0 || nscal > kMaxScal) return set_error(DFM_E_ARG, "too many sources");
@@ -1108,14 +1156,23 @@ extern "C" int dfm_sl_advect(const dfm_grid* g, const dfm_mac* vel, float dt, in
   dim3 grid(blocks_for(total)), blk(kBlock);
   Mac V = to_mac(vel);
   hipStream_t st = S_(stream);
-  if (G.dim == 2) {
-    if (order == 2) hipLaunchKernelGGL((k_sl_advect<2, 2>), grid, blk, 0, st, G, V, dt, S, A);
-    else hipLaunchKernelGGL((k_sl_advect<2, 4>), grid, blk, 0, st, G, V, dt, S, A);
-  } else {
-    if (order == 2) hipLaunchKernelGGL((k_sl_advect<3, 2>), grid, blk, 0, st, G, V, dt, S, A);
-    else hipLaunchKernelGGL((k_sl_advect<3, 4>), grid, blk, 0, st, G, V, dt, S, A);
-  }
-  return launch_check("dfm_sl_advect");
+  rc = with_kern(kern, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    if (G.dim == 2) {
+      if (order == 2) hipLaunchKernelGGL((k_sl_advect<2, 2, K>), grid, blk, 0, st, G, V, dt, S, A);
+      else hipLaunchKernelGGL((k_sl_advect<2, 4, K>), grid, blk, 0, st, G, V, dt, S, A);
+    } else {
+      if (order == 2) hipLaunchKernelGGL((k_sl_advect<3, 2, K>), grid, blk, 0, st, G, V, dt, S, A);
+      else hipLaunchKernelGGL((k_sl_advect<3, 4, K>), grid, blk, 0, st, G, V, dt, S, A);
+    }
+  });
+  return rc ? rc : launch_check("dfm_sl_advect");
+}
+
+extern "C" int dfm_sl_advect(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec,
+                             const dfm_mac* src_vec, const dfm_mac* dst_vec, int nscal, const float* const* src_s,
+                             float* const* dst_s, void* stream) {
+  return dfm_sl_advect_k(g, 2, vel, dt, order, nvec, src_vec, dst_vec, nscal, src_s, dst_s, stream);
 }
 
 extern "C" int dfm_adj_conv_source(const dfm_grid* g, const dfm_mac* u, const dfm_mac* w, int kern,
@@ -1193,11 +1250,13 @@ extern "C" int dfm_axpy_mac(const dfm_grid* g, const dfm_mac* x, float alpha, co
 }
 
 // ------------------------------------------------ vector-Jacobian products ----
-extern "C" int dfm_pullback_u_vjp(const dfm_grid* g, const dfm_mac* g_out, const dfm_mac* src, const dfm_flowmap* m,
-                                  float alpha, const dfm_mac* g_src, const dfm_flowmap* g_m, void* stream) {
+extern "C" int dfm_pullback_u_vjp(const dfm_grid* g, int kern, const dfm_mac* g_out, const dfm_mac* src,
+                                  const dfm_flowmap* m, float alpha, const dfm_mac* g_src, const dfm_flowmap* g_m,
+                                  void* stream) {
   Geo G;
   int rc = check_grid(g, G);
   if (rc) return rc;
+  if ((rc = check_kern(kern))) return rc;
   if ((rc = check_mac(G, g_out, "pullback output gradient is NULL"))) return rc;
   if ((rc = check_mac(G, g_src, "pullback source gradient is NULL"))) return rc;
   if (!m) return set_error(DFM_E_NULL, "flowmap is NULL");
@@ -1223,35 +1282,44 @@ extern "C" int dfm_pullback_u_vjp(const dfm_grid* g, const dfm_mac* g_out, const
   dim3 grid(blocks_for(S.off[S.nset])), blk(kBlock);
   Mac GO = to_mac(g_out);
   Mac SR = g_m ? to_mac(src) : Mac{};
-  DFM_DISPATCH_DIM(G, hipLaunchKernelGGL((k_pullback_u_vjp<2>), grid, blk, 0, S_(stream), G, GO, SR, S, M, alpha, E, GM),
-                   hipLaunchKernelGGL((k_pullback_u_vjp<3>), grid, blk, 0, S_(stream), G, GO, SR, S, M, alpha, E, GM));
-  return launch_check("dfm_pullback_u_vjp");
+  rc = with_kern(kern, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    DFM_DISPATCH_DIM(
+        G, hipLaunchKernelGGL((k_pullback_u_vjp<2, K>), grid, blk, 0, S_(stream), G, GO, SR, S, M, alpha, E, GM),
+        hipLaunchKernelGGL((k_pullback_u_vjp<3, K>), grid, blk, 0, S_(stream), G, GO, SR, S, M, alpha, E, GM));
+  });
+  return rc ? rc : launch_check("dfm_pullback_u_vjp");
 }
 
-extern "C" int dfm_pullback_s_vjp(const dfm_grid* g, const float* g_out, const float* src, const float* psi,
+extern "C" int dfm_pullback_s_vjp(const dfm_grid* g, int kern, const float* g_out, const float* src, const float* psi,
                                   float alpha, float* g_src, float* g_psi, void* stream) {
   Geo G;
   int rc = check_grid(g, G);
   if (rc) return rc;
+  if ((rc = check_kern(kern))) return rc;
   if (!g_out || !psi || !g_src) return set_error(DFM_E_NULL, "dfm_pullback_s_vjp: NULL pointer");
   if (g_psi && !src) return set_error(DFM_E_NULL, "the map gradient needs the pullback source");
   const int64_t cnt = count_of_host(G, 3);
   dim3 grid(blocks_for(cnt)), blk(kBlock);
-  DFM_DISPATCH_DIM(G,
-                   hipLaunchKernelGGL((k_pullback_s_vjp<2>), grid, blk, 0, S_(stream), G, g_out, src, psi, alpha, g_src,
-                                      g_psi, cnt),
-                   hipLaunchKernelGGL((k_pullback_s_vjp<3>), grid, blk, 0, S_(stream), G, g_out, src, psi, alpha, g_src,
-                                      g_psi, cnt));
-  return launch_check("dfm_pullback_s_vjp");
+  rc = with_kern(kern, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    DFM_DISPATCH_DIM(G,
+                     hipLaunchKernelGGL((k_pullback_s_vjp<2, K>), grid, blk, 0, S_(stream), G, g_out, src, psi, alpha,
+                                        g_src, g_psi, cnt),
+                     hipLaunchKernelGGL((k_pullback_s_vjp<3, K>), grid, blk, 0, S_(stream), G, g_out, src, psi, alpha,
+                                        g_src, g_psi, cnt));
+  });
+  return rc ? rc : launch_check("dfm_pullback_s_vjp");
 }
 
-extern "C" int dfm_sl_advect_vjp(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec,
+extern "C" int dfm_sl_advect_vjp(const dfm_grid* g, int kern, const dfm_mac* vel, float dt, int order, int nvec,
                                  const dfm_mac* src_vec, const dfm_mac* g_dst_vec, const dfm_mac* g_src_vec, int nscal,
                                  const float* const* src_s, const float* const* g_dst_s, float* const* g_src_s,
                                  const dfm_mac* g_vel, void* stream) {
   Geo G;
   int rc = check_grid(g, G);
   if (rc) return rc;
+  if ((rc = check_kern(kern))) return rc;
   if ((rc = check_mac(G, vel, "advection velocity is NULL"))) return rc;
   if (order != 2 && order != 4) return set_error(DFM_E_ARG, "order must be 2 or 4");
   if (nvec < 0 || nvec > kMaxVec || nscal < 0 || nscal > kMaxScal) return set_error(DFM_E_ARG, "too many sources");
@@ -1290,14 +1358,17 @@ extern "C" int dfm_sl_advect_vjp(const dfm_grid* g, const dfm_mac* vel, float dt
   dim3 grid(blocks_for(S.off[S.nset])), blk(kBlock);
   Mac V = to_mac(vel);
   hipStream_t st = S_(stream);
-  if (G.dim == 2) {
-    if (order == 2) hipLaunchKernelGGL((k_sl_advect_vjp<2, 2>), grid, blk, 0, st, G, V, dt, S, A);
-    else hipLaunchKernelGGL((k_sl_advect_vjp<2, 4>), grid, blk, 0, st, G, V, dt, S, A);
-  } else {
-    if (order == 2) hipLaunchKernelGGL((k_sl_advect_vjp<3, 2>), grid, blk, 0, st, G, V, dt, S, A);
-    else hipLaunchKernelGGL((k_sl_advect_vjp<3, 4>), grid, blk, 0, st, G, V, dt, S, A);
-  }
-  return launch_check("dfm_sl_advect_vjp");
+  rc = with_kern(kern, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    if (G.dim == 2) {
+      if (order == 2) hipLaunchKernelGGL((k_sl_advect_vjp<2, 2, K>), grid, blk, 0, st, G, V, dt, S, A);
+      else hipLaunchKernelGGL((k_sl_advect_vjp<2, 4, K>), grid, blk, 0, st, G, V, dt, S, A);
+    } else {
+      if (order == 2) hipLaunchKernelGGL((k_sl_advect_vjp<3, 2, K>), grid, blk, 0, st, G, V, dt, S, A);
+      else hipLaunchKernelGGL((k_sl_advect_vjp<3, 4, K>), grid, blk, 0, st, G, V, dt, S, A);
+    }
+  });
+  return rc ? rc : launch_check("dfm_sl_advect_vjp");
 }
 
 extern "C" int dfm_march_vjp(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m0, float dt, int order,

@@ -52,10 +52,18 @@ SIGNATURES = {
                                c_float, c_void_p]),
     "dfm_sl_advect": (c_int, [P(Grid), P(Mac), c_float, c_int, c_int, P(Mac), P(Mac), c_int, P(c_void_p),
                               P(c_void_p), c_void_p]),
-    "dfm_pullback_u_vjp": (c_int, [P(Grid), P(Mac), P(Mac), P(FlowMapS), c_float, P(Mac), P(FlowMapS), c_void_p]),
+    "dfm_pullback_u_k": (c_int, [P(Grid), c_int, P(Mac), P(FlowMapS), c_float, c_float, P(Mac), P(Mac), c_void_p]),
+    "dfm_pullback_s_k": (c_int, [P(Grid), c_int, c_void_p, c_void_p, c_float, c_float, c_void_p, c_void_p, c_void_p]),
+    "dfm_bfecc_u_k": (c_int, [P(Grid), c_int, P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), P(Mac), c_void_p]),
+    "dfm_bfecc_s_k": (c_int, [P(Grid), c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dfm_sl_advect_k": (c_int, [P(Grid), c_int, P(Mac), c_float, c_int, c_int, P(Mac), P(Mac), c_int, P(c_void_p),
+                                P(c_void_p), c_void_p]),
+    "dfm_pullback_u_vjp": (c_int, [P(Grid), c_int, P(Mac), P(Mac), P(FlowMapS), c_float, P(Mac), P(FlowMapS),
+                                   c_void_p]),
     "dfm_march_vjp": (c_int, [P(Grid), P(Mac), P(FlowMapS), c_float, c_int, P(FlowMapS), P(Mac), c_void_p]),
-    "dfm_pullback_s_vjp": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_void_p]),
-    "dfm_sl_advect_vjp": (c_int, [P(Grid), P(Mac), c_float, c_int, c_int, P(Mac), P(Mac), P(Mac), c_int, P(c_void_p),
+    "dfm_pullback_s_vjp": (c_int, [P(Grid), c_int, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p,
+                                   c_void_p]),
+    "dfm_sl_advect_vjp": (c_int, [P(Grid), c_int, P(Mac), c_float, c_int, c_int, P(Mac), P(Mac), P(Mac), c_int, P(c_void_p),
                                   P(c_void_p), P(c_void_p), P(Mac), c_void_p]),
     "dfm_adj_conv_source": (c_int, [P(Grid), P(Mac), P(Mac), c_int, P(Mac), c_void_p]),
     "dfm_adj_scalar_source": (c_int, [P(Grid), c_int, P(c_void_p), P(c_void_p), P(Mac), c_void_p]),

@@ -98,14 +98,14 @@ class PullbackU(torch.autograd.Function):
     flow-map pullback of the impulse, 2D/vortex/flowmap.py:187-231)."""
 
     @staticmethod
-    def forward(ctx, ops, alpha, *t):
+    def forward(ctx, ops, alpha, kern, *t):
         d = ops.dim
         src = MacField([x.detach().contiguous() for x in t[:d]])
         psi_f = [x.detach().contiguous() for x in t[d:2 * d]]
         T_f = [x.detach().contiguous() for x in t[2 * d:3 * d]]
         out = MacField([torch.empty_like(c) for c in src.c])
-        ops.pullback_u(src, FlowMap.wrap(None, psi_f, T_f), out, alpha)
-        ctx.ops, ctx.alpha = ops, float(alpha)
+        ops.pullback_u(src, FlowMap.wrap(None, psi_f, T_f), out, alpha, kern=kern)
+        ctx.ops, ctx.alpha, ctx.kern = ops, float(alpha), int(kern)
         ctx.save_for_backward(*src.c, *psi_f, *T_f)
         return tuple(out.c)
 
@@ -116,13 +116,13 @@ class PullbackU(torch.autograd.Function):
         src = MacField(sv[:d])
         fm = FlowMap.wrap(None, list(sv[d:2 * d]), list(sv[2 * d:3 * d]))
         gsrc = MacField([torch.zeros_like(c) for c in src.c])
-        want_map = any(ctx.needs_input_grad[2 + d:])
+        want_map = any(ctx.needs_input_grad[3 + d:])
         gfm = FlowMap.wrap(None, [torch.zeros_like(x) for x in fm.psi_f],
                            [torch.zeros_like(x) for x in fm.T_f]) if want_map else None
         gout = MacField([_zeros_like_or(x, c) for x, c in zip(g, src.c)])
-        ctx.ops.pullback_u_vjp(gout, fm, gsrc, ctx.alpha, src=src if want_map else None, g_fm=gfm)
+        ctx.ops.pullback_u_vjp(gout, fm, gsrc, ctx.alpha, src=src if want_map else None, g_fm=gfm, kern=ctx.kern)
         mg = tuple(gfm.psi_f) + tuple(gfm.T_f) if want_map else (None,) * (2 * d)
-        return (None, None) + tuple(gsrc.c) + mg
+        return (None, None, None) + tuple(gsrc.c) + mg
 
 
 class PullbackS(torch.autograd.Function):
@@ -130,11 +130,11 @@ class PullbackS(torch.autograd.Function):
     pullback of a passive scalar)."""
 
     @staticmethod
-    def forward(ctx, ops, alpha, src, psi):
+    def forward(ctx, ops, alpha, kern, src, psi):
         src, psi = src.detach().contiguous(), psi.detach().contiguous()
         out = torch.empty_like(src)
-        ops.pullback_s(src, psi, out, alpha)
-        ctx.ops, ctx.alpha = ops, float(alpha)
+        ops.pullback_s(src, psi, out, alpha, kern=kern)
+        ctx.ops, ctx.alpha, ctx.kern = ops, float(alpha), int(kern)
         ctx.save_for_backward(src, psi)
         return out
 
@@ -142,10 +142,10 @@ class PullbackS(torch.autograd.Function):
     def backward(ctx, g):
         src, psi = ctx.saved_tensors
         gsrc = torch.zeros_like(src)
-        gpsi = torch.zeros_like(psi) if ctx.needs_input_grad[3] else None
+        gpsi = torch.zeros_like(psi) if ctx.needs_input_grad[4] else None
         ctx.ops.pullback_s_vjp(_zeros_like_or(g, src), psi, gsrc, ctx.alpha, src=src if gpsi is not None else None,
-                               g_psi=gpsi)
-        return None, None, gsrc, gpsi
+                               g_psi=gpsi, kern=ctx.kern)
+        return None, None, None, gsrc, gpsi
 
 
 class SLAdvect(torch.autograd.Function):
@@ -154,15 +154,16 @@ class SLAdvect(torch.autograd.Function):
     advect_scalar_grid, 2D/vortex/flowmap.py:59-81)."""
 
     @staticmethod
-    def forward(ctx, ops, dt, order, nvec, nscal, *t):
+    def forward(ctx, ops, dt, order, kern, nvec, nscal, *t):
         d = ops.dim
         vel = MacField([x.detach().contiguous() for x in t[:d]])
         vsrc = [MacField([x.detach().contiguous() for x in t[d + m * d:d + (m + 1) * d]]) for m in range(nvec)]
         ssrc = [x.detach().contiguous() for x in t[d + nvec * d:]]
         vdst = [MacField([torch.empty_like(c) for c in m.c]) for m in vsrc]
         sdst = [torch.empty_like(x) for x in ssrc]
-        ops.sl_advect(vel, dt, order, vsrc, vdst, ssrc, sdst)
+        ops.sl_advect(vel, dt, order, vsrc, vdst, ssrc, sdst, kern=kern)
         ctx.ops, ctx.dt, ctx.order, ctx.nvec, ctx.nscal = ops, float(dt), int(order), nvec, nscal
+        ctx.kern = int(kern)
         ctx.save_for_backward(*vel.c, *[c for m in vsrc for c in m.c], *ssrc)
         return tuple(c for m in vdst for c in m.c) + tuple(sdst)
 
@@ -177,11 +178,12 @@ class SLAdvect(torch.autograd.Function):
         gsd = [_zeros_like_or(g[nv * d + k], ssrc[k]) for k in range(ns)]
         gvs = [MacField([torch.zeros_like(c) for c in m.c]) for m in vsrc]
         gss = [torch.zeros_like(x) for x in ssrc]
-        want_vel = any(ctx.needs_input_grad[5:5 + d])
+        want_vel = any(ctx.needs_input_grad[6:6 + d])
         gvel = MacField([torch.zeros_like(c) for c in vel.c]) if want_vel else None
         ctx.ops.sl_advect_vjp(vel, ctx.dt, ctx.order, gvd, gvs, gsd, gss,
-                              vec_src=vsrc if want_vel else (), s_src=ssrc if want_vel else (), g_vel=gvel)
-        return (None,) * 5 + (tuple(gvel.c) if want_vel else (None,) * d) + \
+                              vec_src=vsrc if want_vel else (), s_src=ssrc if want_vel else (), g_vel=gvel,
+                              kern=ctx.kern)
+        return (None,) * 6 + (tuple(gvel.c) if want_vel else (None,) * d) + \
             tuple(c for m in gvs for c in m.c) + tuple(gss)
 
 
@@ -196,18 +198,19 @@ def march(ops, u, psi_f, T_f, dt, order, psi=None):
     return list(out[:d]), list(out[d:2 * d]), (out[2 * d] if psi is not None else None)
 
 
-def pullback_u(ops, src, psi_f, T_f, alpha=1.0):
-    return list(PullbackU.apply(ops, float(alpha), *src, *psi_f, *T_f))
+def pullback_u(ops, src, psi_f, T_f, alpha=1.0, kern=2):
+    """Differentiable pullback (kern: sampling spline, 1 linear / 2 quadratic / 3 cubic)."""
+    return list(PullbackU.apply(ops, float(alpha), int(kern), *src, *psi_f, *T_f))
 
 
-def pullback_s(ops, src, psi, alpha=1.0):
-    return PullbackS.apply(ops, float(alpha), src, psi)
+def pullback_s(ops, src, psi, alpha=1.0, kern=2):
+    return PullbackS.apply(ops, float(alpha), int(kern), src, psi)
 
 
-def sl_advect(ops, vel, dt, order, vec_src=(), s_src=()):
+def sl_advect(ops, vel, dt, order, vec_src=(), s_src=(), kern=2):
     """Differentiable sl_advect: returns ([advected MAC fields], [scalars])."""
     d = ops.dim
-    out = SLAdvect.apply(ops, float(dt), int(order), len(vec_src), len(s_src), *vel,
+    out = SLAdvect.apply(ops, float(dt), int(order), int(kern), len(vec_src), len(s_src), *vel,
                          *[c for m in vec_src for c in m], *s_src)
     nv = len(vec_src)
     return [list(out[m * d:(m + 1) * d]) for m in range(nv)], list(out[nv * d:])

@@ -115,20 +115,23 @@ class Ops:
         with timing.region("march"):
             L.check(self.lib.dfm_march(self.g, u.s, fm.s, float(dt), int(order), self._st()), "march")
 
-    def pullback_u(self, src: MacField, fm: FlowMap, out: MacField, alpha=1.0, beta=0.0, add: MacField = None):
-        L.check(self.lib.dfm_pullback_u(self.g, src.s, fm.s, float(alpha), float(beta),
-                                        add.s if add is not None else None, out.s, self._st()), "pullback_u")
+    # kern: sampling spline of the advected quantity, 1 linear / 2 quadratic
+    # (the reference's) / 3 cubic; the velocity moving points stays quadratic
+    def pullback_u(self, src: MacField, fm: FlowMap, out: MacField, alpha=1.0, beta=0.0, add: MacField = None,
+                   kern=2):
+        L.check(self.lib.dfm_pullback_u_k(self.g, int(kern), src.s, fm.s, float(alpha), float(beta),
+                                          add.s if add is not None else None, out.s, self._st()), "pullback_u")
 
-    def pullback_s(self, src, psi, out, alpha=1.0, beta=0.0, add=None):
-        L.check(self.lib.dfm_pullback_s(self.g, src.data_ptr(), psi.data_ptr(), float(alpha), float(beta),
-                                        L.ptr(add), out.data_ptr(), self._st()), "pullback_s")
+    def pullback_s(self, src, psi, out, alpha=1.0, beta=0.0, add=None, kern=2):
+        L.check(self.lib.dfm_pullback_s_k(self.g, int(kern), src.data_ptr(), psi.data_ptr(), float(alpha),
+                                          float(beta), L.ptr(add), out.data_ptr(), self._st()), "pullback_s")
 
-    def bfecc_u(self, init: MacField, fwd: FlowMap, bwd: FlowMap, err: MacField, out: MacField):
-        L.check(self.lib.dfm_bfecc_u(self.g, init.s, fwd.s, bwd.s, err.s, out.s, self._st()), "bfecc_u")
+    def bfecc_u(self, init: MacField, fwd: FlowMap, bwd: FlowMap, err: MacField, out: MacField, kern=2):
+        L.check(self.lib.dfm_bfecc_u_k(self.g, int(kern), init.s, fwd.s, bwd.s, err.s, out.s, self._st()), "bfecc_u")
 
-    def bfecc_s(self, init, psi, phi, err, out):
-        L.check(self.lib.dfm_bfecc_s(self.g, init.data_ptr(), psi.data_ptr(), phi.data_ptr(), err.data_ptr(),
-                                     out.data_ptr(), self._st()), "bfecc_s")
+    def bfecc_s(self, init, psi, phi, err, out, kern=2):
+        L.check(self.lib.dfm_bfecc_s_k(self.g, int(kern), init.data_ptr(), psi.data_ptr(), phi.data_ptr(),
+                                       err.data_ptr(), out.data_ptr(), self._st()), "bfecc_s")
 
     def bfecc_u_clamp(self, init: MacField, fwd: FlowMap, bwd: FlowMap, err: MacField, tmp: MacField, out: MacField):
         L.check(self.lib.dfm_bfecc_u_clamp(self.g, init.s, fwd.s, bwd.s, err.s, tmp.s, out.s, self._st()),
@@ -143,34 +146,35 @@ class Ops:
         L.check(self.lib.dfm_accumulate(self.g, f.s, fm.s, init.s, h1.s, h2.s, int(ind), float(dt),
                                         f2.s if f2 is not None else None, float(dt2), self._st()), "accumulate")
 
-    def sl_advect(self, vel: MacField, dt, order, vec_src=(), vec_dst=(), s_src=(), s_dst=()):
+    def sl_advect(self, vel: MacField, dt, order, vec_src=(), vec_dst=(), s_src=(), s_dst=(), kern=2):
         nv, ns = len(vec_src), len(s_src)
         src_arr = (L.Mac * max(1, nv))(*[m.s for m in vec_src])
         dst_arr = (L.Mac * max(1, nv))(*[m.s for m in vec_dst])
         with timing.region("sl_advect"):
-            L.check(self.lib.dfm_sl_advect(self.g, vel.s, float(dt), int(order), nv, src_arr, dst_arr, ns,
-                                           L.ptr_array(s_src), L.ptr_array(s_dst), self._st()), "sl_advect")
+            L.check(self.lib.dfm_sl_advect_k(self.g, int(kern), vel.s, float(dt), int(order), nv, src_arr, dst_arr, ns,
+                                             L.ptr_array(s_src), L.ptr_array(s_dst), self._st()), "sl_advect")
 
     # vector-Jacobian products (the gradients accumulate into the g_* outputs)
     def pullback_u_vjp(self, g_out: MacField, fm: FlowMap, g_src: MacField, alpha=1.0, src: MacField = None,
-                       g_fm: FlowMap = None):
+                       g_fm: FlowMap = None, kern=2):
         """g_src += transpose of the pullback applied to g_out; with g_fm (and
         src) also the map's psi_f / T_f gradients."""
-        L.check(self.lib.dfm_pullback_u_vjp(self.g, g_out.s, src.s if src is not None else None, fm.s, float(alpha),
-                                            g_src.s, g_fm.s if g_fm is not None else None, self._st()),
-                "pullback_u_vjp")
+        L.check(self.lib.dfm_pullback_u_vjp(self.g, int(kern), g_out.s, src.s if src is not None else None, fm.s,
+                                            float(alpha), g_src.s, g_fm.s if g_fm is not None else None,
+                                            self._st()), "pullback_u_vjp")
 
-    def pullback_s_vjp(self, g_out, psi, g_src, alpha=1.0, src=None, g_psi=None):
-        L.check(self.lib.dfm_pullback_s_vjp(self.g, g_out.data_ptr(), L.ptr(src), psi.data_ptr(), float(alpha),
-                                            g_src.data_ptr(), L.ptr(g_psi), self._st()), "pullback_s_vjp")
+    def pullback_s_vjp(self, g_out, psi, g_src, alpha=1.0, src=None, g_psi=None, kern=2):
+        L.check(self.lib.dfm_pullback_s_vjp(self.g, int(kern), g_out.data_ptr(), L.ptr(src), psi.data_ptr(),
+                                            float(alpha), g_src.data_ptr(), L.ptr(g_psi), self._st()),
+                "pullback_s_vjp")
 
     def sl_advect_vjp(self, vel: MacField, dt, order, g_vec_dst=(), g_vec_src=(), g_s_dst=(), g_s_src=(),
-                      vec_src=(), s_src=(), g_vel: MacField = None):
+                      vec_src=(), s_src=(), g_vel: MacField = None, kern=2):
         """Transpose of sl_advect w.r.t. its sources; with g_vel (and the
         forward sources vec_src / s_src) also the advecting velocity's."""
         nv, ns = len(g_vec_dst), len(g_s_dst)
         mk = lambda ms: (L.Mac * max(1, nv))(*[m.s for m in ms])  # noqa: E731
-        L.check(self.lib.dfm_sl_advect_vjp(self.g, vel.s, float(dt), int(order), nv,
+        L.check(self.lib.dfm_sl_advect_vjp(self.g, int(kern), vel.s, float(dt), int(order), nv,
                                            mk(vec_src) if vec_src else None, mk(g_vec_dst), mk(g_vec_src), ns,
                                            L.ptr_array(s_src) if s_src else None, L.ptr_array(g_s_dst),
                                            L.ptr_array(g_s_src), g_vel.s if g_vel is not None else None, self._st()),

@@ -121,7 +121,8 @@ class LFM_Simulator:
                  mg_pre_post=2, mg_bottom=50, tol=1e-6, max_iters=400, record=True, device="cuda",
                  mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
                  overlap_projections=True, control_num=0, BFECC_clamp=False, use_short_BFECC=False,
-                 overlap_march=True, mg_fused_sweeps=2, mg_split_layout=True, mg_on_noconv="warn", mg_tail_level=0):
+                 overlap_march=True, mg_fused_sweeps=2, mg_split_layout=True, mg_on_noconv="warn", mg_tail_level=0,
+                 interp_kern=2):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -189,6 +190,16 @@ class LFM_Simulator:
             self._fr = torch.zeros((N,), dtype=torch.float32, device=dev)
             self._fs = torch.zeros((N, d), dtype=torch.float32, device=dev)
         self._force_store = {}  # step -> (centres, radii, strengths, count) for checkpoint replays
+        # B-spline order sampling the advected fields (impulse, passives and
+        # their adjoints) in the BFECC pullbacks and the semi-Lagrangian
+        # advection: 2 = N_2, the reference's; 1 = N_1 (linear) and 3 = N_3
+        # (cubic) are optional variants.  The map march, the back-trace
+        # velocity and accumulate_init stay quadratic.
+        if interp_kern not in (1, 2, 3):
+            raise ValueError(f"interp_kern must be 1, 2 or 3, not {interp_kern!r}")
+        if BFECC_clamp and interp_kern != 2:
+            raise ValueError("BFECC_clamp is implemented for the quadratic sampling kernel only (interp_kern=2)")
+        self.interp_kern = int(interp_kern)
         # BFECC variants (hyperparameters BFECC_clamp / use_short_BFECC)
         self.bfecc_clamp = bool(BFECC_clamp)
         self.short_bfecc = bool(use_short_BFECC)
@@ -241,7 +252,7 @@ class LFM_Simulator:
     def _bfecc_u(self, init, fwd, bwd, err, out):
         """BFECC (2D/vortex/flowmap.py:440-463), clamp_u'd when BFECC_clamp."""
         if not self.bfecc_clamp:
-            self.ops.bfecc_u(init, fwd, bwd, err, out)
+            self.ops.bfecc_u(init, fwd, bwd, err, out, kern=self.interp_kern)
             return
         if self._clamp_tmp is None:
             self._clamp_tmp = (MacField.empty(self.n, self.device), torch.empty(self.n, device=self.device))
@@ -250,7 +261,7 @@ class LFM_Simulator:
     def _bfecc_s(self, init, psi, phi, err, out):
         """BFECC_scalar (2D/vortex/flowmap.py:465-481), clamp_u'd when BFECC_clamp."""
         if not self.bfecc_clamp:
-            self.ops.bfecc_s(init, psi, phi, err, out)
+            self.ops.bfecc_s(init, psi, phi, err, out, kern=self.interp_kern)
             return
         if self._clamp_tmp is None:
             self._clamp_tmp = (MacField.empty(self.n, self.device), torch.empty(self.n, device=self.device))
@@ -427,7 +438,7 @@ class LFM_Simulator:
         ops, order, step = self.ops, self.order, self.step_num[None]
         mid = self._mid_buffer(step)
         # half-step self-advection of u, then projection -> midpoint velocity
-        ops.sl_advect(self.u, 0.5 * dt, order, [self.u], [mid])
+        ops.sl_advect(self.u, 0.5 * dt, order, [self.u], [mid], kern=self.interp_kern)
         if forces is not None:
             # half-step control force before the first projection (:527-537)
             self._force_field(forces)
@@ -448,7 +459,8 @@ class LFM_Simulator:
             self._bfecc_u(self.u, fwd, bwd, self.err_u, self._u_new)
         else:
             # passive and full-step velocity advected by the midpoint velocity
-            ops.sl_advect(mid, dt, order, [self.u], [self._u_new], self.passive, self._passive_new)
+            ops.sl_advect(mid, dt, order, [self.u], [self._u_new], self.passive, self._passive_new,
+                          kern=self.interp_kern)
         self.u, self._u_new = self._u_new, self.u
         self.passive, self._passive_new = self._passive_new, self.passive
         if forces is not None:
@@ -796,7 +808,8 @@ class LFM_Diff_Simulator(LFM_Simulator):
         else:
             # one back-trace through mid over -dt serves lam_u, u, s and lam_s
             ops.sl_advect(mid, -dt, order, [self.adj_u, self.u], [self._adj_u_new, self._u_new],
-                          self.passive + self.adj_passive, self._passive_new + self._adj_passive_new)
+                          self.passive + self.adj_passive, self._passive_new + self._adj_passive_new,
+                          kern=self.interp_kern)
             src_dt = -dt
         self.adj_u, self._adj_u_new = self._adj_u_new, self.adj_u
         self.u, self._u_new = self._u_new, self.u

@@ -141,6 +141,28 @@ int dfm_accumulate(const dfm_grid* g, const dfm_mac* f, const dfm_flowmap* m, co
 int dfm_sl_advect(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec, const dfm_mac* src_vec,
                   const dfm_mac* dst_vec, int nscal, const float* const* src_s, float* const* dst_s, void* stream);
 
+/* ---------------------------------------- interpolation-kernel variants --
+ * The pullbacks, BFECC and semi-Lagrangian advection above sample the
+ * advected quantity with the quadratic B-spline N_2, as the reference's
+ * simulators do.  These variants take the sampling spline as kern: 1 = the
+ * linear N_1 of interp_1 / interp_u_MAC_1 (2D/vortex/taichi_utils.py:210-247,
+ * 647-652; 3D/taichi_utils.py:262-325; sample coordinate clamped to
+ * [0, n-1-1e-6]), 2 = quadratic (the entry points above), 3 = the cubic N_3
+ * of interp_grad_3 (2D/vortex/taichi_utils.py:450-484).  The velocity that
+ * moves points (the march and the semi-Lagrangian back-trace) stays
+ * quadratic.  Any other kern gives DFM_E_ARG. */
+int dfm_pullback_u_k(const dfm_grid* g, int kern, const dfm_mac* src, const dfm_flowmap* m, float alpha, float beta,
+                     const dfm_mac* add, const dfm_mac* out, void* stream);
+int dfm_pullback_s_k(const dfm_grid* g, int kern, const float* src, const float* psi, float alpha, float beta,
+                     const float* add, float* out, void* stream);
+int dfm_bfecc_u_k(const dfm_grid* g, int kern, const dfm_mac* init, const dfm_flowmap* fwd, const dfm_flowmap* bwd,
+                  const dfm_mac* err, const dfm_mac* out, void* stream);
+int dfm_bfecc_s_k(const dfm_grid* g, int kern, const float* init, const float* psi, const float* phi, float* err,
+                  float* out, void* stream);
+int dfm_sl_advect_k(const dfm_grid* g, int kern, const dfm_mac* vel, float dt, int order, int nvec,
+                    const dfm_mac* src_vec, const dfm_mac* dst_vec, int nscal, const float* const* src_s,
+                    float* const* dst_s, void* stream);
+
 /* Adjoint convective source r_a(X_a) = ((grad u)^T w)_a with the B-spline of
  * order kern (4 = quartic as in 2D, 3 = cubic as in 3D).
  * Replaces calculate_nabla_u_w, 2D/vortex/flowmap.py:361-377, 3D/flowmap.py:359-380. */
@@ -163,24 +185,26 @@ int dfm_adj_scalar_source(const dfm_grid* g, int nscal, const float* const* s, c
  * run.  Every output gradient is ACCUMULATED (the caller zeroes it first).
  * A sample's derivative w.r.t. its point is zero along an axis where the
  * sample coordinate is clamped at the wall band, as the forward's value is
- * constant there.
+ * constant there.  kern is the sampling spline of the forward call (1, 2 or
+ * 3, as the _k variants; 2 for the plain entry points).
  *
  * dfm_pullback_u_vjp: for dfm_pullback_u's alpha (T_a . src(psi_a)) part,
  * g_src_b += alpha T_a[b] g_out_a at psi_a; with g_m non-NULL (psi_f / T_f
  * sets; src then required) also the map's gradients g_T_a[b] += alpha g_out_a
  * src_b(psi_a) and g_psi_a += alpha g_out_a sum_b T_a[b] grad src_b(psi_a). */
-int dfm_pullback_u_vjp(const dfm_grid* g, const dfm_mac* g_out, const dfm_mac* src, const dfm_flowmap* m, float alpha,
-                       const dfm_mac* g_src, const dfm_flowmap* g_m, void* stream);
+int dfm_pullback_u_vjp(const dfm_grid* g, int kern, const dfm_mac* g_out, const dfm_mac* src, const dfm_flowmap* m,
+                       float alpha, const dfm_mac* g_src, const dfm_flowmap* g_m, void* stream);
 /* dfm_pullback_s_vjp: for dfm_pullback_s's alpha src(psi) part, g_src +=
  * alpha g_out at psi; with g_psi non-NULL ((dim, *n), src then required)
  * also g_psi += alpha g_out grad src(psi). */
-int dfm_pullback_s_vjp(const dfm_grid* g, const float* g_out, const float* src, const float* psi, float alpha,
-                       float* g_src, float* g_psi, void* stream);
+int dfm_pullback_s_vjp(const dfm_grid* g, int kern, const float* g_out, const float* src, const float* psi,
+                       float alpha, float* g_src, float* g_psi, void* stream);
 /* dfm_sl_advect_vjp: for dfm_sl_advect (dst = src at the back-traced node),
  * g_src_* += the transpose of the sample applied to g_dst_*; with g_vel
  * non-NULL (src_vec / src_s then required, the forward sources) also the
  * advecting velocity's gradient through the RK back-trace. */
-int dfm_sl_advect_vjp(const dfm_grid* g, const dfm_mac* vel, float dt, int order, int nvec, const dfm_mac* src_vec,
+int dfm_sl_advect_vjp(const dfm_grid* g, int kern, const dfm_mac* vel, float dt, int order, int nvec,
+                      const dfm_mac* src_vec,
                       const dfm_mac* g_dst_vec, const dfm_mac* g_src_vec, int nscal, const float* const* src_s,
                       const float* const* g_dst_s, float* const* g_src_s, const dfm_mac* g_vel, void* stream);
 /* dfm_march_vjp: the vector-Jacobian product of dfm_march.  m0 is the map

@@ -22,9 +22,12 @@
 namespace {
 
 // B-spline kernels N_k and derivatives dN_k.
+// linear:    2D/vortex/taichi_utils.py:210-218
 // quadratic: 2D/vortex/taichi_utils.py:381-400 (3D/taichi_utils.py:356-375)
 // cubic:     2D/vortex/taichi_utils.py:356-377 (3D/taichi_utils.py:410-431)
 // quartic:   2D/vortex/taichi_utils.py:325-353 (3D/taichi_utils.py:379-407)
+template <class R> R N1(R x) { return R(1) - std::fabs(x); }
+template <class R> R dN1(R x) { return x < R(0) ? R(1) : R(-1); }
 template <class R> R N2(R x) {
   R a = std::fabs(x);
   if (a < R(0.5)) return R(0.75) - a * a;
@@ -65,13 +68,13 @@ template <class R> R dN4(R x) {
   else if (a < R(2.5)) r = R(-125) / R(48) + R(25) / R(8) * a - R(1.25) * a2 + a3 / R(6);
   return x < R(0) ? -r : r;
 }
-template <class R> R Nk(int k, R x) { return k == 2 ? N2(x) : (k == 3 ? N3(x) : N4(x)); }
-template <class R> R dNk(int k, R x) { return k == 2 ? dN2(x) : (k == 3 ? dN3(x) : dN4(x)); }
+template <class R> R Nk(int k, R x) { return k == 1 ? N1(x) : k == 2 ? N2(x) : (k == 3 ? N3(x) : N4(x)); }
+template <class R> R dNk(int k, R x) { return k == 1 ? dN1(x) : k == 2 ? dN2(x) : (k == 3 ? dN3(x) : dN4(x)); }
 
-// Tap window [lo, hi] relative to floor(s): quadratic -1..2, cubic -2..3,
-// quartic -3..4 (2D/vortex/taichi_utils.py:433,470,506).
+// Tap window [lo, hi] relative to floor(s): linear 0..1, quadratic -1..2,
+// cubic -2..3, quartic -3..4 (2D/vortex/taichi_utils.py:230-234,433,470,506).
 inline void window(int k, int& lo, int& hi) {
-  if (k == 2) { lo = -1; hi = 2; } else if (k == 3) { lo = -2; hi = 3; } else { lo = -3; hi = 4; }
+  if (k == 1) { lo = 0; hi = 1; } else if (k == 2) { lo = -1; hi = 2; } else if (k == 3) { lo = -2; hi = 3; } else { lo = -3; hi = 4; }
 }
 
 template <class R> struct Fld {
@@ -91,14 +94,17 @@ template <class R> R sample(const Fld<R>& f, R u, R v, R w) {
 // interp_grad_{2,3,4}: value and gradient of a B-spline interpolant of field f
 // whose node (0,..) sits at BL*dx.  The sample coordinate is clamped to
 // [1, dim-2-eps] per axis before the weights are formed
-// (2D/vortex/taichi_utils.py:413-520, 3D/taichi_utils.py:433-536).
+// (2D/vortex/taichi_utils.py:413-520, 3D/taichi_utils.py:433-536); the
+// linear interp_grad_1 clamps to [0, dim-1-eps] (2D/vortex/taichi_utils.py:249-281).
 template <class R>
 void interp(int kern, int dim, const Fld<R>& f, const R* bl, const R* p, R dx, R* val, R* grad) {
   const R eps = R(1e-6);
+  const R lo_s = kern == 1 ? R(0) : R(1);
+  const int hi_off = kern == 1 ? 1 : 2;
   R s[3] = {0, 0, 0}, fl[3] = {0, 0, 0};
   for (int a = 0; a < dim; ++a) {
     R u = p[a] / dx - bl[a];
-    s[a] = std::max(R(1), std::min(u, R(f.s[a] - 2) - eps));
+    s[a] = std::max(lo_s, std::min(u, R(f.s[a] - hi_off) - eps));
     fl[a] = std::floor(s[a]);
   }
   int lo, hi;
@@ -162,7 +168,7 @@ void march_point(const Mac<R>& U, int order, R dx, R dt, R* psi, R* T) {
   const int d = U.dim;
   R neg = -dt;
   R J[3][3], u1[3], u2[3], u3[3], u4[3], k1[3], k2[3], k3[3], k4[3];
-  R p1[3], p2[3], p3[3], T1[3], T2[3], T3[3];
+  R p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0}, p3[3] = {0, 0, 0}, T1[3] = {0, 0, 0}, T2[3] = {0, 0, 0}, T3[3] = {0, 0, 0};
   auto JT = [&](R (*Jm)[3], const R* t, R* out) {
     for (int a = 0; a < d; ++a) {
       out[a] = 0;
@@ -411,27 +417,29 @@ void march_points(int dim, int order, const int* n, R dx, R dt, const R* ux, con
 }
 
 // out[i] = T[i] . u0(psi[i])   (advect_u: 2D/vortex/flowmap.py:83-97,
-// 3D/flowmap.py:51-69; accumulate_init reuses it with F, phi, f)
+// 3D/flowmap.py:51-69; accumulate_init reuses it with F, phi, f); u0 sampled
+// with the B-spline of order kern (2 in the reference's simulators)
 template <class R>
-void pullback_points(int dim, const int* n, R dx, const R* ux, const R* uy, const R* uz, const R* psi, const R* T,
-                     int64_t np, R* out) {
+void pullback_points(int kern, int dim, const int* n, R dx, const R* ux, const R* uy, const R* uz, const R* psi,
+                     const R* T, int64_t np, R* out) {
   Mac<R> U;
   const R* c[3] = {ux, uy, uz};
   U.init(dim, n, c);
 #pragma omp parallel for
   for (int64_t i = 0; i < np; ++i) {
     R u[3];
-    U.eval(2, psi + i * dim, dx, u, nullptr);
+    U.eval(kern, psi + i * dim, dx, u, nullptr);
     R s = 0;
     for (int a = 0; a < dim; ++a) s += T[i * dim + a] * u[a];
     out[i] = s;
   }
 }
 
-// back-trace points x by vel, then sample src (shape/bl given) quadratically
+// back-trace points x by vel (quadratic), then sample src (shape/bl given)
+// with the B-spline of order kern
 template <class R>
-void sl_points(int dim, int order, const int* n, R dx, R dt, const R* ux, const R* uy, const R* uz, const R* src,
-               const int* src_shape, const R* src_bl, const R* x, int64_t np, R* out) {
+void sl_points(int kern, int dim, int order, const int* n, R dx, R dt, const R* ux, const R* uy, const R* uz,
+               const R* src, const int* src_shape, const R* src_bl, const R* x, int64_t np, R* out) {
   Mac<R> U;
   const R* c[3] = {ux, uy, uz};
   U.init(dim, n, c);
@@ -441,7 +449,7 @@ void sl_points(int dim, int order, const int* n, R dx, R dt, const R* ux, const 
   for (int64_t i = 0; i < np; ++i) {
     R p[3];
     backtrace(U, order, dx, dt, x + i * dim, p);
-    interp(2, dim, S, src_bl, p, dx, out + i, (R*)nullptr);
+    interp(kern, dim, S, src_bl, p, dx, out + i, (R*)nullptr);
   }
 }
 
@@ -456,14 +464,14 @@ void sl_points(int dim, int order, const int* n, R dx, R dt, const R* ux, const 
                                      const R* uz, R* psi, R* T, int64_t np) {                                     \
     march_points<R>(dim, order, n, dx, dt, ux, uy, uz, psi, T, np);                                              \
   }                                                                                                                \
-  extern "C" void oracle_pullback_##SFX(int dim, const int* n, R dx, const R* ux, const R* uy, const R* uz,       \
-                                        const R* psi, const R* T, int64_t np, R* out) {                           \
-    pullback_points<R>(dim, n, dx, ux, uy, uz, psi, T, np, out);                                                 \
+  extern "C" void oracle_pullback_##SFX(int kern, int dim, const int* n, R dx, const R* ux, const R* uy,         \
+                                        const R* uz, const R* psi, const R* T, int64_t np, R* out) {              \
+    pullback_points<R>(kern, dim, n, dx, ux, uy, uz, psi, T, np, out);                                           \
   }                                                                                                                \
-  extern "C" void oracle_sl_##SFX(int dim, int order, const int* n, R dx, R dt, const R* ux, const R* uy,         \
-                                  const R* uz, const R* src, const int* src_shape, const R* src_bl, const R* x,   \
-                                  int64_t np, R* out) {                                                            \
-    sl_points<R>(dim, order, n, dx, dt, ux, uy, uz, src, src_shape, src_bl, x, np, out);                         \
+  extern "C" void oracle_sl_##SFX(int kern, int dim, int order, const int* n, R dx, R dt, const R* ux,           \
+                                  const R* uy, const R* uz, const R* src, const int* src_shape, const R* src_bl,  \
+                                  const R* x, int64_t np, R* out) {                                                \
+    sl_points<R>(kern, dim, order, n, dx, dt, ux, uy, uz, src, src_shape, src_bl, x, np, out);                   \
   }                                                                                                                \
   extern "C" int oracle_mgpcg_##SFX(int dim, const int* n, int levels, int pre, int bottom, const R* b, R* x,     \
                                     int max_iters, double tol, int relative, int* iters, double* res) {           \
@@ -473,5 +481,5 @@ void sl_points(int dim, int order, const int* n, R dx, R dt, const R* ux, const 
 DFM_ORACLE_EXPORTS(double, d)
 DFM_ORACLE_EXPORTS(float, f)
 
-// B-spline kernel (k = 2,3,4) or its derivative at x, for the kernel tests.
+// B-spline kernel (k = 1,2,3,4) or its derivative at x, for the kernel tests.
 extern "C" double oracle_bspline_d(int k, int deriv, double x) { return deriv ? dNk<double>(k, x) : Nk<double>(k, x); }

@@ -185,8 +185,8 @@ def march(u, psi, T, dx, dt, order):
         *ptrs, _p(psi), _p(T) if T is not None else None, ctypes.c_int64(psi.size // dim))
 
 
-def pullback(u0, psi, T, dx):
-    """T . u0(psi) per point."""
+def pullback(u0, psi, T, dx, kern=2):
+    """T . u0(psi) per point (u0 sampled with the B-spline of order kern)."""
     dtp = u0[0].dtype
     dim = len(u0)
     comps, ptrs = _ucomps(u0)
@@ -194,13 +194,14 @@ def pullback(u0, psi, T, dx):
     T = np.ascontiguousarray(T, dtype=dtp)
     out = np.empty(psi.shape[:-1], dtype=dtp)
     getattr(lib(), f"oracle_pullback_{_sfx(dtp)}")(
-        ctypes.c_int(dim), _p(_shape3(_cells(u0))), _real(dtp, dx), *ptrs, _p(psi), _p(T),
+        ctypes.c_int(kern), ctypes.c_int(dim), _p(_shape3(_cells(u0))), _real(dtp, dx), *ptrs, _p(psi), _p(T),
         ctypes.c_int64(psi.size // dim), _p(out))
     return out
 
 
-def sl_points(vel, src, src_bl, x, dx, dt, order):
-    """Back-trace x through vel over dt (RK order), sample src there."""
+def sl_points(vel, src, src_bl, x, dx, dt, order, kern=2):
+    """Back-trace x through vel over dt (RK order), sample src there with the
+    B-spline of order kern."""
     dtp = vel[0].dtype
     dim = len(vel)
     comps, ptrs = _ucomps(vel)
@@ -209,25 +210,25 @@ def sl_points(vel, src, src_bl, x, dx, dt, order):
     out = np.empty(x.shape[:-1], dtype=dtp)
     blv = np.ascontiguousarray(list(src_bl) + [0.5] * (3 - dim), dtype=dtp)
     getattr(lib(), f"oracle_sl_{_sfx(dtp)}")(
-        ctypes.c_int(dim), ctypes.c_int(order), _p(_shape3(_cells(vel))), _real(dtp, dx), _real(dtp, dt),
+        ctypes.c_int(kern), ctypes.c_int(dim), ctypes.c_int(order), _p(_shape3(_cells(vel))), _real(dtp, dx), _real(dtp, dt),
         *ptrs, _p(src), _p(_shape3(src.shape)), _p(blv), _p(x), ctypes.c_int64(x.size // dim), _p(out))
     return out
 
 
 # ---------------------------------------------------------- field kernels ---
-def sl_advect_u(vel, src, dx, dt, order):
+def sl_advect_u(vel, src, dx, dt, order, kern=2):
     """advect_u_grid (2D/vortex/flowmap.py:59-69,233-291; 3D/flowmap.py:202-282):
     component a of src sampled at the back-traced face-a positions."""
     n = _cells(vel)
     dim = len(vel)
-    return [sl_points(vel, src[a], mac_bl(dim, a), face_coords(n, a, dx, vel[0].dtype), dx, dt, order)
+    return [sl_points(vel, src[a], mac_bl(dim, a), face_coords(n, a, dx, vel[0].dtype), dx, dt, order, kern)
             for a in range(dim)]
 
 
-def sl_advect_s(vel, s, dx, dt, order):
+def sl_advect_s(vel, s, dx, dt, order, kern=2):
     """advect_scalar_grid (2D/vortex/flowmap.py:71-81,293-329)."""
     n = _cells(vel)
-    return sl_points(vel, s, [0.5] * len(vel), cell_coords(n, dx, vel[0].dtype), dx, dt, order)
+    return sl_points(vel, s, [0.5] * len(vel), cell_coords(n, dx, vel[0].dtype), dx, dt, order, kern)
 
 
 class FlowMap:
@@ -267,14 +268,14 @@ class FlowMap:
         return m
 
 
-def advect_u(u0, fmap, dx):
+def advect_u(u0, fmap, dx, kern=2):
     """Impulse pullback u1_a = T_a . u0(psi_a) (2D/vortex/flowmap.py:83-97)."""
-    return [pullback(u0, fmap.psi_f[a], fmap.T_f[a], dx) for a in range(len(u0))]
+    return [pullback(u0, fmap.psi_f[a], fmap.T_f[a], dx, kern) for a in range(len(u0))]
 
 
-def advect_scalar(s0, psi, dx):
-    """s1 = s0(psi), quadratic (2D/vortex/flowmap.py:99-107)."""
-    return interp(2, s0, [0.5] * s0.ndim, psi, dx)
+def advect_scalar(s0, psi, dx, kern=2):
+    """s1 = s0(psi), quadratic in the reference (2D/vortex/flowmap.py:99-107)."""
+    return interp(kern, s0, [0.5] * s0.ndim, psi, dx)
 
 
 def clamp_extrema(c, ref):
@@ -291,25 +292,25 @@ def clamp_extrema(c, ref):
     return np.minimum(np.maximum(c, lo), hi).astype(ref.dtype)
 
 
-def bfecc(init, fwd, bwd, dx, clamp=False):
+def bfecc(init, fwd, bwd, dx, clamp=False, kern=2):
     """BFECC (2D/vortex/flowmap.py:440-463):
     u = T.init(psi); err = (F.u(phi) - init)/2; u -= T.err(psi),
     where (T, psi) = `fwd` and (F, phi) = `bwd`; with BFECC_clamp (clamp)
     the result is clamp_u'd against the first pullback."""
-    u = advect_u(init, fwd, dx)
-    e = advect_u(u, bwd, dx)
+    u = advect_u(init, fwd, dx, kern)
+    e = advect_u(u, bwd, dx, kern)
     err = [(e[a] - init[a]) * 0.5 for a in range(len(init))]
-    t = advect_u(err, fwd, dx)
+    t = advect_u(err, fwd, dx, kern)
     out = [u[a] - t[a] for a in range(len(init))]
     return [clamp_extrema(o, r) for o, r in zip(out, u)] if clamp else out
 
 
-def bfecc_scalar(init, psi, phi, dx, clamp=False):
+def bfecc_scalar(init, psi, phi, dx, clamp=False, kern=2):
     """BFECC_scalar (2D/vortex/flowmap.py:465-481), clamp_u'd with BFECC_clamp."""
-    s = advect_scalar(init, psi, dx)
-    e = advect_scalar(s, phi, dx)
+    s = advect_scalar(init, psi, dx, kern)
+    e = advect_scalar(s, phi, dx, kern)
     err = (e - init) * 0.5
-    out = s - advect_scalar(err, psi, dx)
+    out = s - advect_scalar(err, psi, dx, kern)
     return clamp_extrema(out, s) if clamp else out
 
 
@@ -680,8 +681,14 @@ class OracleLFM:
     """
 
     def __init__(self, n, dx, dt, reinit_every, order, adj_kernel=None, project="exact", dtype=np.float64,
-                 mg_levels=None, mg_tol=1e-6, control_forces=False, bfecc_clamp=False, short_bfecc=False):
+                 mg_levels=None, mg_tol=1e-6, control_forces=False, bfecc_clamp=False, short_bfecc=False,
+                 interp_kern=2):
         self.control_forces = control_forces  # add_control_force: also switches the j == 0 passive rebuild
+        # B-spline order sampling the advected fields in the BFECC pullbacks and
+        # the semi-Lagrangian advection (2 = the reference's; 1 = linear N_1,
+        # 3 = cubic N_3 as optional variants); the map march, the back-trace
+        # velocity and accumulate_init stay quadratic
+        self.kern = interp_kern
         self.clamp = bfecc_clamp               # BFECC_clamp
         self.short = short_bfecc               # use_short_BFECC
         self.n = tuple(n)
@@ -767,7 +774,7 @@ class OracleLFM:
         dx, dt, order = self.dx, self.dt, self.order
         self.s_start[self.step_num] = [s.copy() for s in self.s]
         ad_u = [c.copy() for c in self.u]
-        self.u = sl_advect_u(ad_u, ad_u, dx, 0.5 * dt, order)
+        self.u = sl_advect_u(ad_u, ad_u, dx, 0.5 * dt, order, self.kern)
         f = None
         if force is not None:
             f = self._force(force)
@@ -783,11 +790,11 @@ class OracleLFM:
             psi_t, phi_t = FlowMap(self.n, dx, self.dtype), FlowMap(self.n, dx, self.dtype)
             psi_t.march(self.u, dt, order)
             phi_t.march(self.u, -dt, order)
-            self.s = [bfecc_scalar(s, psi_t.psi, phi_t.psi, dx, self.clamp) for s in self.s]
-            self.u = bfecc(ad_u, psi_t, phi_t, dx, self.clamp)
+            self.s = [bfecc_scalar(s, psi_t.psi, phi_t.psi, dx, self.clamp, self.kern) for s in self.s]
+            self.u = bfecc(ad_u, psi_t, phi_t, dx, self.clamp, self.kern)
         else:
-            self.s = [sl_advect_s(self.u, s, dx, dt, order) for s in self.s]
-            self.u = sl_advect_u(self.u, ad_u, dx, dt, order)
+            self.s = [sl_advect_s(self.u, s, dx, dt, order, self.kern) for s in self.s]
+            self.u = sl_advect_u(self.u, ad_u, dx, dt, order, self.kern)
         if f is not None:
             for a in range(self.dim):
                 self.u[a] = self.u[a] + dt * f[a]
@@ -800,10 +807,10 @@ class OracleLFM:
             self.psi_map.march(self.u, dt, order)
             for st in reversed(range(self.step_num - j, self.step_num)):
                 self.psi_map.march(self.mid[st], dt, order)
-            self.u = bfecc(self.init_u, self.psi_map, self.phi_map, dx, self.clamp)
+            self.u = bfecc(self.init_u, self.psi_map, self.phi_map, dx, self.clamp, self.kern)
             self.project(self.u)
             self._mask_fwd()
-            self.s = [bfecc_scalar(si, self.psi_map.psi, self.phi_map.psi, dx, self.clamp) for si in self.init_s]
+            self.s = [bfecc_scalar(si, self.psi_map.psi, self.phi_map.psi, dx, self.clamp, self.kern) for si in self.init_s]
             self.init_s = [s.copy() for s in self.s]
             self.maps[self.step_num // self.R] = self.phi_map.copy()
             self.phi_map.reset()
@@ -852,23 +859,23 @@ class OracleLFM:
             psi_t, phi_t = FlowMap(self.n, dx, self.dtype), FlowMap(self.n, dx, self.dtype)
             psi_t.march(mid, -dt, order)
             phi_t.march(mid, dt, order)
-            lam = bfecc(self.lam_u, psi_t, phi_t, dx, self.clamp)
+            lam = bfecc(self.lam_u, psi_t, phi_t, dx, self.clamp, self.kern)
             self.lam_u = [lam[a] + (-2 * dt) * r_u[a] for a in range(self.dim)]
-            self.lam_s = [bfecc_scalar(l, psi_t.psi, phi_t.psi, dx, self.clamp) for l in self.lam_s]
+            self.lam_s = [bfecc_scalar(l, psi_t.psi, phi_t.psi, dx, self.clamp, self.kern) for l in self.lam_s]
             for rs in r_s:
                 for a in range(self.dim):
                     self.lam_u[a] = self.lam_u[a] + (-dt) * rs[a]
-            self.s = [bfecc_scalar(s, psi_t.psi, phi_t.psi, dx, self.clamp) for s in self.s]
-            self.u = bfecc(self.u, psi_t, phi_t, dx, self.clamp)
+            self.s = [bfecc_scalar(s, psi_t.psi, phi_t.psi, dx, self.clamp, self.kern) for s in self.s]
+            self.u = bfecc(self.u, psi_t, phi_t, dx, self.clamp, self.kern)
         else:
-            self.lam_u = sl_advect_u(mid, self.lam_u, dx, -dt, order)
-            self.s = [sl_advect_s(mid, s, dx, -dt, order) for s in self.s]
-            self.u = sl_advect_u(mid, self.u, dx, -dt, order)
+            self.lam_u = sl_advect_u(mid, self.lam_u, dx, -dt, order, self.kern)
+            self.s = [sl_advect_s(mid, s, dx, -dt, order, self.kern) for s in self.s]
+            self.u = sl_advect_u(mid, self.u, dx, -dt, order, self.kern)
             for a in range(self.dim):
                 self.lam_u[a] = self.lam_u[a] + (-dt) * r_u[a]
             new_lam_s = []
             for l, rs in zip(self.lam_s, r_s):
-                new_lam_s.append(sl_advect_s(mid, l, dx, -dt, order))
+                new_lam_s.append(sl_advect_s(mid, l, dx, -dt, order, self.kern))
                 for a in range(self.dim):
                     self.lam_u[a] = self.lam_u[a] + (-dt) * rs[a]
             self.lam_s = new_lam_s
@@ -885,20 +892,20 @@ class OracleLFM:
         self._mask_adj()
         if j == 0:
             fwd = self.maps[self.step_num // self.R]
-            self.lam_u = bfecc(self.init_lam_u, fwd, self.psi_map, dx, self.clamp)
+            self.lam_u = bfecc(self.init_lam_u, fwd, self.psi_map, dx, self.clamp, self.kern)
             self.project(self.lam_u)
             self.init_lam_u = [c.copy() for c in self.lam_u]
-            self.u = bfecc(self.init_u, fwd, self.psi_map, dx, self.clamp)
+            self.u = bfecc(self.init_u, fwd, self.psi_map, dx, self.clamp, self.kern)
             self.project(self.u)
             self.init_u = [c.copy() for c in self.u]
             self._mask_adj()
             if self.control_forces:
                 # rebuilt through the interval's maps (2D/shape/lfm_midpoint_diff_simulator.py:339-343)
-                self.s = [bfecc_scalar(si, fwd.psi, self.psi_map.psi, dx, self.clamp) for si in self.init_s]
+                self.s = [bfecc_scalar(si, fwd.psi, self.psi_map.psi, dx, self.clamp, self.kern) for si in self.init_s]
             else:
                 self.s = [s.copy() for s in self.s_start[self.step_num]]
             self.init_s = [s.copy() for s in self.s]
-            self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx, self.clamp) for l in self.init_lam_s]
+            self.lam_s = [bfecc_scalar(l, fwd.psi, self.psi_map.psi, dx, self.clamp, self.kern) for l in self.init_lam_s]
             tg = self.interval_targets.get(self.step_num) if self.interval_targets else None
             if tg is not None:
                 # intermediate targets (q7): lam_s += s - s*, loss += sum (s* - s)^2

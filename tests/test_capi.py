@@ -110,3 +110,25 @@ def test_poisson_solver_shim_host_checks():
         ps.build()
     with pytest.raises(ValueError):
         PoissonSolver([2, 2])
+
+
+def test_interp_kernel_argument_validation(lib):
+    """The sampling-kernel variants accept kern 1, 2, 3 only (DFM_E_ARG, no
+    device work, even when there is nothing to advect)."""
+    from dfm import _lib as L
+
+    g = L.make_grid((16, 16), 1 / 16)
+    fake = ctypes.c_void_p(256)
+    mac = L.Mac()
+    mac.c[0] = mac.c[1] = fake.value
+    m = L.FlowMapS()
+    for a in range(2):
+        m.psi_f[a] = m.T_f[a] = fake.value
+    for kern in (0, 4, -1):
+        assert lib.dfm_pullback_u_k(g, kern, mac, m, 1.0, 0.0, None, mac, None) == -1
+        assert "interpolation kernel" in lib.dfm_last_error().decode()
+        assert lib.dfm_pullback_s_k(g, kern, fake, fake, 1.0, 0.0, None, fake, None) == -1
+        assert lib.dfm_bfecc_u_k(g, kern, mac, m, m, mac, mac, None) == -1
+        assert lib.dfm_bfecc_s_k(g, kern, fake, fake, fake, fake, fake, None) == -1
+        assert lib.dfm_sl_advect_k(g, kern, mac, 0.1, 2, 0, None, None, 0, None, None, None) == -1
+    assert lib.dfm_sl_advect_k(g, 3, mac, 0.1, 2, 0, None, None, 0, None, None, None) == 0  # nothing to advect

@@ -602,3 +602,56 @@ def test_short_bfecc_steps_move_fields_the_semi_lagrangian_way(clamp):
         dl[short] = [a - b for a, b in zip(o.lam_u, lam0)]
     c = cos(dl[True], dl[False])
     assert c > 0.8  # reversed maps: 0.10
+
+
+# ------------------------------------------------- linear (N_1) variant -----
+def test_linear_bspline_matches_scipy_basis_element():
+    """N_1 = the degree-1 B-spline (hat on knots -1, 0, 1) inside its support;
+    its derivative is -1 on [0, 1) and +1 on (-1, 0) (the reference's dN_1
+    takes -1 at x = 0)."""
+    x = np.linspace(-0.999, 0.999, 801)
+    ref = BSpline.basis_element([-1, 0, 1], extrapolate=False)(x)
+    np.testing.assert_allclose(bspline(1, x), ref, atol=1e-15)
+    xs = x[np.abs(x) > 1e-9]
+    np.testing.assert_allclose(bspline(1, xs, 1), BSpline.basis_element([-1, 0, 1]).derivative()(xs), atol=1e-12)
+    assert bspline(1, [0.0], 1)[0] == -1.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_linear_interp_matches_scipy_regular_grid(dim):
+    """interp(1, ...) is multilinear interpolation on the node lattice at
+    BL*dx: scipy's RegularGridInterpolator at interior points; outside the
+    box the coordinate clamps to [0, n-1-1e-6] (the nearest face value)."""
+    from scipy.interpolate import RegularGridInterpolator
+
+    rng = np.random.default_rng(5 + dim)
+    shape = (9, 7, 6)[:dim]
+    dx = 0.1
+    bl = [0.0, 0.5, 0.5][:dim]
+    f = rng.standard_normal(shape)
+    axes = [(np.arange(s) + b) * dx for s, b in zip(shape, bl)]
+    rgi = RegularGridInterpolator(axes, f, method="linear")
+    lo = np.array([a[0] for a in axes])
+    hi = np.array([a[-1] for a in axes])
+    pts = lo + (hi - lo) * rng.uniform(0.001, 0.999, size=(200, dim))
+    np.testing.assert_allclose(O.interp(1, f, bl, pts, dx), rgi(pts), atol=1e-12)
+    # outside: clamped to the box (the reference's s = max(0, min(u, n-1-eps)))
+    out = lo + (hi - lo) * rng.uniform(-0.5, 1.5, size=(200, dim))
+    np.testing.assert_allclose(O.interp(1, f, bl, out, dx), rgi(np.clip(out, lo, hi)), atol=1e-5)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_linear_interp_reproduces_affine_fields_and_gradient(dim):
+    """A multilinear interpolant is exact on affine data, with the exact
+    gradient inside a cell."""
+    rng = np.random.default_rng(17 + dim)
+    shape = (8, 9, 7)[:dim]
+    dx = 0.25
+    bl = [0.5] * dim
+    c = rng.standard_normal(dim + 1)
+    X = np.meshgrid(*[(np.arange(s) + 0.5) * dx for s in shape], indexing="ij")
+    f = c[0] + sum(c[1 + d] * X[d] for d in range(dim))
+    pts = 0.5 * dx + rng.uniform(0.01, 0.99, size=(100, dim)) * (np.array(shape) - 1) * dx
+    v, g = O.interp(1, f, bl, pts, dx, grad=True)
+    np.testing.assert_allclose(v, c[0] + pts @ c[1:], atol=1e-12)
+    np.testing.assert_allclose(g, np.broadcast_to(c[1:], g.shape), atol=1e-12)
