@@ -51,18 +51,18 @@ def planted(name, wrap):
 
 
 def _pullback_without_jacobian(old):
-    def advect_u(u0, fmap, dx):
+    def advect_u(u0, fmap, dx, kern=2):
         out = []
         for a in range(len(u0)):
             eye = np.zeros_like(fmap.T_f[a])
             eye[..., a] = 1.0
-            out.append(O.pullback(u0, fmap.psi_f[a], eye, dx))
+            out.append(O.pullback(u0, fmap.psi_f[a], eye, dx, kern))
         return out
     return advect_u
 
 
 def _passive_advected_backwards(old):
-    return lambda vel, s, dx, dt, order: old(vel, s, dx, -dt, order)
+    return lambda vel, s, dx, dt, order, kern=2: old(vel, s, dx, -dt, order, kern)
 
 
 @contextlib.contextmanager
