@@ -312,33 +312,44 @@ def run_cpu_baseline(size):
 def run_batch(args):
     """Config 5: 32 independent 2D forward+adjoint problems, grids from
     {128,192,256,384,512}^2 by seed, 4-12 vortices each, theta gradient per
-    instance.  Sequential run, worker-count autotune, concurrent run."""
+    instance.  The batched native engine (one launch per stage over all
+    instances, batched MGPCG, graph-replayed segments; LDS-tail size and
+    first-check iterations autotuned), timed beside the per-instance
+    simulators run one after another and on concurrent worker threads; the
+    per-instance gradients are compared with the one-after-another run."""
     import numpy as np
 
-    from dfm.batch import BatchedLFM, make_problems
+    from dfm.batch import BatchedLFM, BatchEngine, make_problems
 
     probs = make_problems(count=32, seed=100, steps=args.batch_steps, reinit_every=12)
-    b = BatchedLFM(probs, workers=1, overlap_march=args.batch_overlap in ("march", "both"),
-                   overlap_projections=args.batch_overlap in ("projections", "both"))
-    tail = b.autotune_tail()
+    eng = BatchEngine(probs, tol=1e-6, max_iters=200)
+    tune = eng.autotune()
+    res, t = eng.run()
+    st = eng.stats()
+    steps_total = eng.instance_steps()
+    # per-instance simulators: one after another (the reference gradients),
+    # then on worker threads
+    b = BatchedLFM(probs, workers=1)
     b.run(12)  # untimed: builds every handle and graph
     seq, t_seq = b.run()
-    tune = b.autotune(candidates=(1, 2, 4, 8, 16), probe_steps=12)
-    res, t = b.run()
-    steps_total = b.instance_steps()
+    b.set_workers(8)
+    b.run(12)
+    _, t_thr = b.run()
     per = []
     for p, (l0, g0), (l1, g1) in zip(probs, seq, res):
         per.append({"n": p["n"][0], "vortices": int(p["theta"].size // 4), "loss": round(float(l1), 6),
                     "grad_norm": round(float(np.linalg.norm(g1)), 6),
-                    "grad_rel_diff_vs_sequential": float(np.linalg.norm(g1 - g0) / max(np.linalg.norm(g0), 1e-30))})
+                    "grad_rel_err_vs_single": float(np.linalg.norm(g1 - g0) / max(np.linalg.norm(g0), 1e-30))})
     return {
         "metric": "aggregate fwd+adjoint instance-steps/s, 32 uneven 2D instances (128^2..512^2)",
         "value": round(steps_total / t, 2), "unit": "steps/s", "n_gpus": 1, "steps": args.batch_steps,
-        "ms_total": round(t * 1e3, 1), "sequential_value": round(steps_total / t_seq, 2),
-        "speedup_vs_sequential": round(t_seq / t, 3), "workers": b.workers,
-        "autotune_s": {str(k): round(v, 4) for k, v in tune.items()},
-        "mg_tail_level_ms": {str(n[0]): {str(k): round(v, 3) for k, v in t.items()} for n, t in tail.items()},
-        "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+        "ms_total": round(t * 1e3, 1),
+        "sequential_value": round(steps_total / t_seq, 2), "thread_pool_value": round(steps_total / t_thr, 2),
+        "speedup_vs_sequential": round(t_seq / t, 3), "speedup_vs_thread_pool": round(t_thr / t, 3),
+        "engine": {"options": eng.options, "projections": st["projections"], "max_pcg_iters": st["max_iters"],
+                   "noconv": st["noconv"],
+                   "autotune_s": {k: {str(a): round(v, 4) for a, v in d.items()} for k, d in tune.items()}},
+        "max_grad_rel_err_vs_single": max(q["grad_rel_err_vs_single"] for q in per),
         "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded vortices / disk / Gaussian blobs)",
         "config": {"workload": "32 independent 2D forward+adjoint LFM problems, RK2, reinit every 12, "
                                "MGPCG tol 1e-6, theta gradient", "instances": len(probs)},
@@ -387,8 +398,6 @@ def parse_args(argv=None):
     ap.add_argument("--batch", action="store_true",
                     help="the batched configuration: 32 uneven 2D instances, aggregate steps/s")
     ap.add_argument("--batch-steps", type=int, default=24)
-    ap.add_argument("--batch-overlap", default="projections", choices=["none", "projections", "march", "both"],
-                    help="per-instance side streams in the batch (the instances already overlap each other)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for --batch (1..32; 0 keeps the platform's setting: 16 measured slower than 4)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -52,6 +52,17 @@ SIGNATURES = {
                                c_float, c_void_p]),
     "dfm_sl_advect": (c_int, [P(Grid), P(Mac), c_float, c_int, c_int, P(Mac), P(Mac), c_int, P(c_void_p),
                               P(c_void_p), c_void_p]),
+    "dfm_batch_create": (c_int, [c_void_p, c_int, c_int, c_int, ctypes.c_double, c_int, P(c_void_p)]),
+    "dfm_batch_destroy": (c_int, [c_void_p]),
+    "dfm_batch_field": (c_int, [c_void_p, c_int, c_int, P(c_void_p), P(c_void_p)]),
+    "dfm_batch_copy": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "dfm_batch_set_option": (c_int, [c_void_p, c_int, c_int]),
+    "dfm_batch_init": (c_int, [c_void_p, c_void_p]),
+    "dfm_batch_forward": (c_int, [c_void_p, c_int, c_void_p]),
+    "dfm_batch_init_gradient": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "dfm_batch_backward": (c_int, [c_void_p, c_int, c_void_p]),
+    "dfm_batch_stats": (c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
+    "dfm_batch_levels": (c_int, [c_void_p, c_int, P(c_int), P(c_int)]),
     "dfm_pullback_u_k": (c_int, [P(Grid), c_int, P(Mac), P(FlowMapS), c_float, c_float, P(Mac), P(Mac), c_void_p]),
     "dfm_pullback_s_k": (c_int, [P(Grid), c_int, c_void_p, c_void_p, c_float, c_float, c_void_p, c_void_p, c_void_p]),
     "dfm_bfecc_u_k": (c_int, [P(Grid), c_int, P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), P(Mac), c_void_p]),
@@ -105,6 +116,14 @@ MG_FUSED_SWEEPS = 8
 MG_SPLIT_LAYOUT = 9
 MG_BF16_LEVEL = 10
 MG_LEVELS = 11
+
+# dfm_batch fields and options
+BATCH_U, BATCH_S, BATCH_TARGET, BATCH_ADJ_U, BATCH_ADJ_S = 0, 1, 2, 3, 4
+BATCH_GRAPH, BATCH_FIRST_ITERS, BATCH_CHUNK, BATCH_TAIL_CELLS = 0, 1, 2, 3
+
+
+class BatchInstance(ctypes.Structure):
+    _fields_ = [("n", c_int * 2), ("dx", c_float), ("dt", c_float)]
 
 DFM_E_NOCONV = -4
 NOALLOC_MAX_VALUES = 1 << 23

@@ -199,3 +199,218 @@ class BatchedLFM:
         best = min(times, key=times.get)
         self.set_workers(best)
         return times
+
+
+class BatchEngine:
+    """The batched native form of ``BatchedLFM``: every problem in one
+    libdfm batch engine (dfm_batch_*), so each stage of a step is one launch
+    over all instances, the projections are one batched MGPCG with
+    per-instance FP64 reductions and convergence flags, and the launches
+    between two convergence checks replay from captured hipGraphs.
+
+    Problems must share reinit_every and steps (as ``make_problems`` makes
+    them).  ``run`` returns the same (loss, dL/dtheta) per instance as
+    ``BatchedLFM.run``; the first run captures the graphs, later runs replay
+    them."""
+
+    def __init__(self, problems, tol=1e-6, max_iters=200, graphs=True, first_iters=None, chunk=None,
+                 tail_cells=None, device="cuda"):
+        import torch
+
+        from . import _lib as L
+
+        self.problems = list(problems)
+        R = {p["R"] for p in self.problems}
+        steps = {p["steps"] for p in self.problems}
+        if len(R) != 1 or len(steps) != 1:
+            raise ValueError("BatchEngine needs one reinit_every and one step count for all problems")
+        self.R, self.steps = R.pop(), steps.pop()
+        self.device = torch.device(device)
+        self.lib = L.lib()
+        n = len(self.problems)
+        arr = (L.BatchInstance * n)()
+        for i, p in enumerate(self.problems):
+            arr[i].n[0], arr[i].n[1] = int(p["n"][0]), int(p["n"][1])
+            arr[i].dx, arr[i].dt = float(p["dx"]), float(p["dt"])
+        h = L.c_void_p()
+        L.check(self.lib.dfm_batch_create(L.ctypes.cast(arr, L.c_void_p), n, self.R, self.steps, float(tol),
+                                          int(max_iters), L.ctypes.byref(h)), "batch_create")
+        self.h = h
+        if not graphs:
+            self.set_option(L.BATCH_GRAPH, 0)
+        if first_iters is not None:
+            self.set_option(L.BATCH_FIRST_ITERS, first_iters)
+        if chunk is not None:
+            self.set_option(L.BATCH_CHUNK, chunk)
+        if tail_cells is not None:
+            self.set_option(L.BATCH_TAIL_CELLS, tail_cells)
+        self.options = {"first_iters": first_iters or 12, "chunk": chunk or 4, "tail_cells": tail_cells or 16384}
+        # per-instance device inputs, uploaded once
+        self.theta = [torch.tensor(p["theta"], dtype=torch.float32, device=self.device) for p in self.problems]
+        self.density = [torch.tensor(p["density"], dtype=torch.float32, device=self.device) for p in self.problems]
+        self.target = [torch.tensor(p["target"], dtype=torch.float32, device=self.device) for p in self.problems]
+        self.grids = [L.make_grid(p["n"], p["dx"]) for p in self.problems]
+        self.loss = np.zeros(n, dtype=np.float64)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.dfm_batch_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def set_option(self, option, value):
+        from . import _lib as L
+
+        L.check(self.lib.dfm_batch_set_option(self.h, int(option), int(value)), "batch_set_option")
+
+    def _mac(self, field, i):
+        from . import _lib as L
+
+        c0, c1 = L.c_void_p(), L.c_void_p()
+        L.check(self.lib.dfm_batch_field(self.h, field, i, L.ctypes.byref(c0), L.ctypes.byref(c1)), "batch_field")
+        m = L.Mac()
+        m.c[0], m.c[1] = c0.value, c1.value
+        return m
+
+    def get(self, field, i):
+        """Copy of instance i's field: [x faces, y faces] for MAC fields, one
+        tensor for cell fields."""
+        import torch
+
+        from . import _lib as L
+
+        n = self.problems[i]["n"]
+        if field in (L.BATCH_U, L.BATCH_ADJ_U):
+            out = [torch.empty((n[0] + 1, n[1]), device=self.device), torch.empty((n[0], n[1] + 1), device=self.device)]
+            L.check(self.lib.dfm_batch_copy(self.h, field, i, out[0].data_ptr(), out[1].data_ptr(), 0,
+                                            L.stream_handle()), "batch_copy")
+        else:
+            out = torch.empty(tuple(n), device=self.device)
+            L.check(self.lib.dfm_batch_copy(self.h, field, i, out.data_ptr(), None, 0, L.stream_handle()),
+                    "batch_copy")
+        return out
+
+    def put(self, field, i, value):
+        from . import _lib as L
+
+        if isinstance(value, (list, tuple)):
+            v = [t.float().contiguous() for t in value]
+            L.check(self.lib.dfm_batch_copy(self.h, field, i, v[0].data_ptr(), v[1].data_ptr(), 1,
+                                            L.stream_handle()), "batch_copy")
+        else:
+            v = value.float().contiguous()
+            L.check(self.lib.dfm_batch_copy(self.h, field, i, v.data_ptr(), None, 1, L.stream_handle()),
+                    "batch_copy")
+
+    def start(self, u0=None):
+        """Initial state of every instance: u = the vortex field of theta (or
+        u0[i], MAC tensors), s = density, then the projected init."""
+        from . import _lib as L
+
+        st = L.stream_handle()
+        for i, p in enumerate(self.problems):
+            if u0 is not None:
+                self.put(L.BATCH_U, i, u0[i])
+            else:
+                L.check(self.lib.dfm_vortex_velocity_2d(self.grids[i], self.theta[i].data_ptr(),
+                                                        self.theta[i].numel() // 4, self._mac(L.BATCH_U, i), st),
+                        "vortex_velocity_2d")
+            self.put(L.BATCH_S, i, self.density[i])
+            self.put(L.BATCH_TARGET, i, self.target[i])
+        L.check(self.lib.dfm_batch_init(self.h, st), "batch_init")
+
+    def forward(self, steps=None):
+        from . import _lib as L
+
+        L.check(self.lib.dfm_batch_forward(self.h, int(self.steps if steps is None else steps), L.stream_handle()),
+                "batch_forward")
+
+    def init_gradient(self):
+        from . import _lib as L
+
+        L.check(self.lib.dfm_batch_init_gradient(self.h, self.loss.ctypes.data_as(L.c_void_p), L.stream_handle()),
+                "batch_init_gradient")
+        return self.loss.copy()
+
+    def backward(self, steps=None):
+        from . import _lib as L
+
+        L.check(self.lib.dfm_batch_backward(self.h, int(self.steps if steps is None else steps), L.stream_handle()),
+                "batch_backward")
+
+    def dtheta(self, i):
+        """dL/dtheta of instance i from its adjoint velocity at t = 0 (the
+        analytic vortex VJP, as calculate_dtheta)."""
+        import torch
+
+        from . import _lib as L
+
+        out = torch.empty_like(self.theta[i])
+        L.check(self.lib.dfm_vortex_velocity_2d_vjp(self.grids[i], self.theta[i].data_ptr(),
+                                                    self.theta[i].numel() // 4, self._mac(L.BATCH_ADJ_U, i),
+                                                    out.data_ptr(), L.stream_handle()), "vortex_velocity_2d_vjp")
+        return out
+
+    def run(self, steps=None):
+        """Forward, terminal adjoint, backward and the theta gradients of every
+        instance; returns ([(loss, dL/dtheta)], seconds)."""
+        import torch
+
+        if steps is not None and steps != self.steps:
+            raise ValueError("BatchEngine runs the whole horizon it was created for")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.start()
+        self.forward()
+        loss = self.init_gradient()
+        self.backward()
+        grads = [self.dtheta(i) for i in range(len(self.problems))]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return [(float(loss[i]), grads[i].double().cpu().numpy()) for i in range(len(self.problems))], dt
+
+    def instance_steps(self, steps=None):
+        return len(self.problems) * self.steps
+
+    def autotune(self, tails=(1024, 4096, 16384), first_iters=(8, 12, 16)):
+        """Pick the LDS-tail size (the largest level, in cells, that an
+        instance's one-workgroup tail holds; finer levels are batched global
+        launches) and then the PCG iterations run before the first
+        convergence check, each by timing a whole run (after one untimed run
+        that captures the graphs); returns {knob: {value: seconds}}."""
+        from . import _lib as L
+
+        out = {"tail_cells": {}, "first_iters": {}}
+        for t in tails:
+            self.set_option(L.BATCH_TAIL_CELLS, t)
+            self.run()
+            out["tail_cells"][t] = self.run()[1]
+        best_t = min(out["tail_cells"], key=out["tail_cells"].get)
+        self.set_option(L.BATCH_TAIL_CELLS, best_t)
+        for f in first_iters:
+            self.set_option(L.BATCH_FIRST_ITERS, f)
+            self.run()
+            out["first_iters"][f] = self.run()[1]
+        best_f = min(out["first_iters"], key=out["first_iters"].get)
+        self.set_option(L.BATCH_FIRST_ITERS, best_f)
+        self.options.update(tail_cells=best_t, first_iters=best_f)
+        self.run()  # recapture with the chosen schedule
+        return out
+
+    def stats(self):
+        from . import _lib as L
+
+        a, b, c = L.c_int(), L.c_int(), L.c_int()
+        L.check(self.lib.dfm_batch_stats(self.h, L.ctypes.byref(a), L.ctypes.byref(b), L.ctypes.byref(c)),
+                "batch_stats")
+        return {"projections": a.value, "max_iters": b.value, "noconv": c.value}
+
+    def levels(self, i):
+        from . import _lib as L
+
+        a, b = L.c_int(), L.c_int()
+        L.check(self.lib.dfm_batch_levels(self.h, i, L.ctypes.byref(a), L.ctypes.byref(b)), "batch_levels")
+        return a.value, b.value

@@ -14,7 +14,7 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include")
 LIB = os.path.join(HERE, "libdfm.so")
 
-SOURCES = ["dfm_flowmap.hip", "dfm_mgpcg.hip", "dfm_init.hip", "dfm_mask.hip", "dfm_force.hip", "dfm_poisson_tiled.hip", "dfm_fields.hip", "dfm_clamp.hip", "dfm_errors.cpp"]
+SOURCES = ["dfm_flowmap.hip", "dfm_mgpcg.hip", "dfm_init.hip", "dfm_mask.hip", "dfm_force.hip", "dfm_poisson_tiled.hip", "dfm_fields.hip", "dfm_clamp.hip", "dfm_batch.hip", "dfm_errors.cpp"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 (v_pk_*_f32)
 # instructions; ordinary float code is scheduled as scalar VALU ops instead.

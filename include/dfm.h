@@ -221,6 +221,78 @@ int dfm_march_vjp(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m0, fl
 int dfm_axpy_mac(const dfm_grid* g, const dfm_mac* x, float alpha, const dfm_mac* y, float beta, const dfm_mac* z,
                  const dfm_mac* out, void* stream);
 
+/* ------------------------------------------------ batched 2D instances --
+ * Forward + adjoint of many independent 2D problems of uneven size in one
+ * engine: the reference's per-problem LFM_Simulator.step_flow_map /
+ * LFM_Diff_Simulator.backtrack_flow_map (2D/vortex/lfm_midpoint_simulator.py:
+ * 517-661, 2D/vortex/lfm_midpoint_diff_simulator.py:260-503; RK2, quartic
+ * convective source, plain BFECC, every midpoint velocity and forward map
+ * kept), with every stage one launch over all instances, a batched MGPCG
+ * (per-instance FP64 scalars, convergence flags and level counts,
+ * int(log2(min n)) - 2 as 2D/vortex/mgpcg.py:28) and the launches between
+ * two convergence checks replayed from captured hipGraphs.  No counterpart
+ * in the reference, which runs one problem per process.  Extents must be
+ * 8..8192 and divisible by 2^levels.  The engine owns its device memory
+ * (allocated at creation). */
+typedef struct {
+  int n[2];  /* cells per axis */
+  float dx;
+  float dt;
+} dfm_batch_instance;
+typedef struct dfm_batch dfm_batch;
+
+int dfm_batch_create(const dfm_batch_instance* inst, int count, int reinit_every, int total_steps, double tol,
+                     int max_iters, dfm_batch** out);
+int dfm_batch_destroy(dfm_batch* b);
+
+/* Device pointers of instance inst's field in the engine: MAC fields give
+ * the x- and y-face arrays ((n0+1) x n1 and n0 x (n1+1), row-major), cell
+ * fields c0 only (n0 x n1).  The caller writes u (unprojected initial
+ * velocity), s (density) and target before dfm_batch_init, and reads u, s,
+ * adj_u and adj_s. */
+#define DFM_BATCH_U 0
+#define DFM_BATCH_S 1
+#define DFM_BATCH_TARGET 2
+#define DFM_BATCH_ADJ_U 3
+#define DFM_BATCH_ADJ_S 4
+int dfm_batch_field(dfm_batch* b, int field, int inst, float** c0, float** c1);
+/* Copies instance inst's field between the engine and caller device buffers
+ * of the same shapes (c1: the y faces of a MAC field, unused for cells):
+ * to_engine = 1 writes the engine's field, 0 reads it.  Asynchronous on
+ * stream. */
+int dfm_batch_copy(dfm_batch* b, int field, int inst, float* c0, float* c1, int to_engine, void* stream);
+
+/* DFM_BATCH_GRAPH: 1 = replay captured graphs (default), 0 = launch eagerly.
+ * DFM_BATCH_FIRST_ITERS: PCG iterations inside the segment that starts a
+ * projection (default 12); DFM_BATCH_CHUNK: iterations per further replay
+ * until every instance has converged (default 4).  DFM_BATCH_TAIL_CELLS:
+ * the largest level (in cells) an instance's one-workgroup LDS tail holds
+ * (default 16384; each instance's finer levels run as batched global
+ * launches), the batched form's tile-shape knob; setting it re-plans the
+ * levels and reallocates the level arrays (between runs only).  Setting an
+ * option drops the captured graphs. */
+#define DFM_BATCH_GRAPH 0
+#define DFM_BATCH_FIRST_ITERS 1
+#define DFM_BATCH_CHUNK 2
+#define DFM_BATCH_TAIL_CELLS 3
+int dfm_batch_set_option(dfm_batch* b, int option, int value);
+
+/* LFM_Simulator.init after the fields are written: project u, init_u = u,
+ * init_s = s, reset the forward map. */
+int dfm_batch_init(dfm_batch* b, void* stream);
+/* step_midpoint, `steps` times (at most total_steps in all). */
+int dfm_batch_forward(dfm_batch* b, int steps, void* stream);
+/* init_gradient after the last forward step: adj_s = 2 (s - target),
+ * adj_u = 0; loss[inst] = sum (s - target)^2 (host array of count, or NULL). */
+int dfm_batch_init_gradient(dfm_batch* b, double* loss, void* stream);
+/* backtrack_step_midpoint, `steps` times. */
+int dfm_batch_backward(dfm_batch* b, int steps, void* stream);
+/* Projections run, the most PCG iterations any instance took, and how many
+ * instance solves stopped without converging (max_iters or non-finite). */
+int dfm_batch_stats(dfm_batch* b, int* projections, int* max_iters, int* noconv);
+/* Multigrid levels of an instance and the first level its LDS tail holds. */
+int dfm_batch_levels(dfm_batch* b, int inst, int* levels, int* tail_level);
+
 /* ---------------------------------------------------------------- MGPCG --
  * Handle owning the multigrid workspaces for one grid.  Levels, smoothing and
  * bottom sweeps follow MGPCG (2D/vortex/mgpcg.py:17-31): levels <= 0 picks
