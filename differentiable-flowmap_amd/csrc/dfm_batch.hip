@@ -103,23 +103,14 @@ DFM_HD MapV map_of(float* base, const Inst& I) {
   return m;
 }
 
-// Launch domain: blk[i] .. blk[i+1] are instance i's blocks.
+// Launch domain: blk[i] .. blk[i+1] are instance i's blocks; bt[b] =
+// (instance, first block of that instance) for every block b, so a block
+// finds its instance with one load.
 struct Dom {
   const int* blk;
+  const int2* bt;
   int count;
 };
-
-// Largest instance whose first block is <= b (instances with no blocks are
-// skipped, since they share their first block with the next instance).
-DFM_HD int find_inst(const Dom& D, int b) {
-  int lo = 0, hi = D.count;
-  while (hi - lo > 1) {
-    const int m = (lo + hi) >> 1;
-    if (D.blk[m] <= b) lo = m;
-    else hi = m;
-  }
-  return lo;
-}
 
 struct Ctx {
   const Inst* I;
@@ -132,17 +123,36 @@ struct Ctx {
 
 DFM_HD bool inst_done(const Ctx& C, int i) { return C.fl[i * F_COUNT + F_DONE] != 0; }
 
-// Point index within its instance of this thread (block-aligned domains).
+// Each thread handles PPT points of its instance, BS apart: the block's
+// set-up latency (instance lookup, instance record) is paid once per
+// PPT * BS points, and the points' loads are in flight together.
+constexpr int PPT = 4;
 struct Where {
   int i;
-  int64_t t;
+  int t0;
 };
 DFM_HD Where where(const Dom& D) {
-  Where w;
-  w.i = find_inst(D, blockIdx.x);
-  w.t = (int64_t)(blockIdx.x - D.blk[w.i]) * BS + threadIdx.x;
-  return w;
+  const int2 e = D.bt[blockIdx.x];
+  return Where{e.x, (int)(blockIdx.x - e.y) * (BS * PPT) + (int)threadIdx.x};
 }
+
+// q = t / d, r = t % d for 0 <= t < 2^24, d >= 1: a float quotient estimate
+// (off by at most a few units) corrected exactly in integers, instead of
+// the long integer-division sequence.
+DFM_HD void divmod(int t, int d, int& q, int& r) {
+  q = (int)((float)t * __frcp_rn((float)d));
+  r = t - q * d;
+  while (r < 0) {
+    --q;
+    r += d;
+  }
+  while (r >= d) {
+    ++q;
+    r -= d;
+  }
+}
+// (i, j) of row-major index t in a grid whose rows hold `cols` entries.
+DFM_HD void rc_of(int t, int cols, int& i, int& j) { divmod(t, cols, i, j); }
 
 // ------------------------------------------------------------ reductions ----
 DFM_HD double wave_sum(double v) {
@@ -260,8 +270,8 @@ DFM_HD void inst_reduce(const Ctx& C, const Dom& D, int i, int op, double a, dou
 // ------------------------------------------------------- flow-map stages ----
 // Face points [0, F0) x faces, [F0, F0 + F1) y faces, then (ALL domains)
 // cells [F0 + F1, F0 + F1 + C).
-DFM_HD int point_grid(const Inst& I, int64_t t, int64_t& k) {
-  const int64_t F0 = faces0(I), F1 = faces1(I);
+DFM_HD int point_grid(const Inst& I, int t, int& k) {
+  const int F0 = (int)faces0(I), F1 = (int)faces1(I);
   if (t < F0) {
     k = t;
     return 0;
@@ -273,7 +283,17 @@ DFM_HD int point_grid(const Inst& I, int64_t t, int64_t& k) {
   k = t - F0 - F1;
   return 3;
 }
-DFM_HD int64_t npoints(const Inst& I, bool with_cells) { return faces0(I) + faces1(I) + (with_cells ? cells(I) : 0); }
+DFM_HD int npoints(const Inst& I, bool with_cells) {
+  return (int)(faces0(I) + faces1(I) + (with_cells ? cells(I) : 0));
+}
+DFM_HD void node_of(const Geo& g, int grid, int k, float x[3]) {
+  int sh[3];
+  shape_of(g, grid, sh);
+  int a, b;
+  rc_of(k, sh[1], a, b);
+  x[2] = 0.f;
+  node_pos(g, grid, a, b, 0, x);
+}
 
 struct SLB {
   const float* vel;      // MAC pool of the advecting velocity
@@ -288,26 +308,29 @@ struct SLB {
 __global__ void __launch_bounds__(BS) k_b_sl(Ctx C, Dom D, SLB A) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, A.nscal > 0)) return;
-  int64_t k;
-  const int grid = point_grid(I, w.t, k);
-  if (grid < 3 && A.nvec == 0) return;
+  const int N = npoints(I, A.nscal > 0);
   const Geo g = geo_of(I);
-  int sh[3];
-  shape_of(g, grid, sh);
-  int a, b, c;
-  unlin(sh, k, a, b, c);
-  float x[3] = {0.f, 0.f, 0.f}, p[3] = {0.f, 0.f, 0.f};
-  node_pos(g, grid, a, b, c, x);
-  backtrace<2, 2>(g, mac_of(A.vel, I), A.dt_scale * I.dt, x, p);
-  float bl[3] = {0.5f, 0.5f, 0.5f};
-  if (grid < 3) {
-    bl[grid] = 0.0f;
-    for (int m = 0; m < A.nvec; ++m)
-      mac_c(A.vdst[m], I, grid)[k] = interp<2, false, 2>(mac_of(A.vsrc[m], I).c[grid], sh, bl, p, g.inv_dx, nullptr);
-  } else {
-    for (int m = 0; m < A.nscal; ++m)
-      A.sdst[m][I.oc + k] = interp<2, false, 2>(A.ssrc[m] + I.oc, sh, bl, p, g.inv_dx, nullptr);
+  const Mac vel = mac_of(A.vel, I);
+  const float dt = A.dt_scale * I.dt;
+#pragma unroll 1
+  for (int u = 0, t = w.t0; u < PPT && t < N; ++u, t += BS) {
+    int k;
+    const int grid = point_grid(I, t, k);
+    if (grid < 3 && A.nvec == 0) continue;
+    int sh[3];
+    shape_of(g, grid, sh);
+    float x[3], p[3] = {0.f, 0.f, 0.f};
+    node_of(g, grid, k, x);
+    backtrace<2, 2>(g, vel, dt, x, p);
+    float bl[3] = {0.5f, 0.5f, 0.5f};
+    if (grid < 3) {
+      bl[grid] = 0.0f;
+      for (int m = 0; m < A.nvec; ++m)
+        mac_c(A.vdst[m], I, grid)[k] = interp<2, false, 2>(mac_of(A.vsrc[m], I).c[grid], sh, bl, p, g.inv_dx, nullptr);
+    } else {
+      for (int m = 0; m < A.nscal; ++m)
+        A.sdst[m][I.oc + k] = interp<2, false, 2>(A.ssrc[m] + I.oc, sh, bl, p, g.inv_dx, nullptr);
+    }
   }
 }
 
@@ -315,60 +338,61 @@ __global__ void __launch_bounds__(BS) k_b_sl(Ctx C, Dom D, SLB A) {
 __global__ void __launch_bounds__(BS) k_b_march(Ctx C, Dom D, const float* vel, float* map, float dt_scale) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, true)) return;
-  int64_t k;
-  const int grid = point_grid(I, w.t, k);
+  const int N = npoints(I, true);
   const Geo g = geo_of(I);
   const MapV M = map_of(map, I);
   const Mac u = mac_of(vel, I);
   const float dt = dt_scale * I.dt;
-  float p[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
-  if (grid < 3) {
-    const int64_t cnt = grid ? faces1(I) : faces0(I);
-    float* ps = M.pf[grid];
-    float* ts = M.tf[grid];
-    p[0] = ps[k];
-    p[1] = ps[cnt + k];
-    T[0] = ts[k];
-    T[1] = ts[cnt + k];
-    rk_point<2, 2, true>(g, u, dt, p, T);
-    ps[k] = p[0];
-    ps[cnt + k] = p[1];
-    ts[k] = T[0];
-    ts[cnt + k] = T[1];
-  } else {
-    const int64_t cnt = cells(I);
-    p[0] = M.psi[k];
-    p[1] = M.psi[cnt + k];
-    rk_point<2, 2, false>(g, u, dt, p, T);
-    M.psi[k] = p[0];
-    M.psi[cnt + k] = p[1];
+#pragma unroll 1
+  for (int uu = 0, t = w.t0; uu < PPT && t < N; ++uu, t += BS) {
+    int k;
+    const int grid = point_grid(I, t, k);
+    float p[3] = {0.f, 0.f, 0.f}, T[3] = {0.f, 0.f, 0.f};
+    if (grid < 3) {
+      const int cnt = (int)(grid ? faces1(I) : faces0(I));
+      float* ps = M.pf[grid];
+      float* ts = M.tf[grid];
+      p[0] = ps[k];
+      p[1] = ps[cnt + k];
+      T[0] = ts[k];
+      T[1] = ts[cnt + k];
+      rk_point<2, 2, true>(g, u, dt, p, T);
+      ps[k] = p[0];
+      ps[cnt + k] = p[1];
+      ts[k] = T[0];
+      ts[cnt + k] = T[1];
+    } else {
+      const int cnt = (int)cells(I);
+      p[0] = M.psi[k];
+      p[1] = M.psi[cnt + k];
+      rk_point<2, 2, false>(g, u, dt, p, T);
+      M.psi[k] = p[0];
+      M.psi[cnt + k] = p[1];
+    }
   }
 }
 
 __global__ void __launch_bounds__(BS) k_b_reset(Ctx C, Dom D, float* map) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, true)) return;
-  int64_t k;
-  const int grid = point_grid(I, w.t, k);
+  const int N = npoints(I, true);
   const Geo g = geo_of(I);
   const MapV M = map_of(map, I);
-  int sh[3];
-  shape_of(g, grid, sh);
-  int a, b, c;
-  unlin(sh, k, a, b, c);
-  float x[3] = {0.f, 0.f, 0.f};
-  node_pos(g, grid, a, b, c, x);
-  if (grid < 3) {
-    const int64_t cnt = grid ? faces1(I) : faces0(I);
-    M.pf[grid][k] = x[0];
-    M.pf[grid][cnt + k] = x[1];
-    M.tf[grid][k] = grid == 0 ? 1.f : 0.f;
-    M.tf[grid][cnt + k] = grid == 1 ? 1.f : 0.f;
-  } else {
-    M.psi[k] = x[0];
-    M.psi[cells(I) + k] = x[1];
+  for (int u = 0, t = w.t0; u < PPT && t < N; ++u, t += BS) {
+    int k;
+    const int grid = point_grid(I, t, k);
+    float x[3];
+    node_of(g, grid, k, x);
+    if (grid < 3) {
+      const int cnt = (int)(grid ? faces1(I) : faces0(I));
+      M.pf[grid][k] = x[0];
+      M.pf[grid][cnt + k] = x[1];
+      M.tf[grid][k] = grid == 0 ? 1.f : 0.f;
+      M.tf[grid][cnt + k] = grid == 1 ? 1.f : 0.f;
+    } else {
+      M.psi[k] = x[0];
+      M.psi[cells(I) + k] = x[1];
+    }
   }
 }
 
@@ -383,34 +407,39 @@ struct PBB {
 __global__ void __launch_bounds__(BS) k_b_pullback_u(Ctx C, Dom D, PBB A) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, false)) return;
-  int64_t k;
-  const int grid = point_grid(I, w.t, k);
+  const int N = npoints(I, false);
   const Geo g = geo_of(I);
   const MapV M = map_of(const_cast<float*>(A.map), I);
-  const int64_t cnt = grid ? faces1(I) : faces0(I);
-  float p[3] = {M.pf[grid][k], M.pf[grid][cnt + k], 0.f}, v[3];
-  mac_eval<false, 2>(g, mac_of(A.src, I), p, v, nullptr);
-  const float d = fmaf(M.tf[grid][cnt + k], v[1], M.tf[grid][k] * v[0]);
-  float r = A.alpha * d;
-  if (A.beta != 0.f) r = fmaf(A.beta, mac_of(A.add, I).c[grid][k], r);
-  mac_c(A.out, I, grid)[k] = r;
+  const Mac src = mac_of(A.src, I);
+#pragma unroll 1
+  for (int u = 0, t = w.t0; u < PPT && t < N; ++u, t += BS) {
+    int k;
+    const int grid = point_grid(I, t, k);
+    const int cnt = (int)(grid ? faces1(I) : faces0(I));
+    float p[3] = {M.pf[grid][k], M.pf[grid][cnt + k], 0.f}, v[3];
+    mac_eval<false, 2>(g, src, p, v, nullptr);
+    const float d = fmaf(M.tf[grid][cnt + k], v[1], M.tf[grid][k] * v[0]);
+    float r = A.alpha * d;
+    if (A.beta != 0.f) r = fmaf(A.beta, mac_of(A.add, I).c[grid][k], r);
+    mac_c(A.out, I, grid)[k] = r;
+  }
 }
 __global__ void __launch_bounds__(BS) k_b_pullback_s(Ctx C, Dom D, PBB A) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  const int64_t cnt = cells(I);
-  if (w.t >= cnt) return;
-  const int64_t k = w.t;
+  const int cnt = (int)cells(I);
   const Geo g = geo_of(I);
   const MapV M = map_of(const_cast<float*>(A.map), I);
-  const float p[3] = {M.psi[k], M.psi[cnt + k], 0.f};
   int S[3];
   shape_of(g, 3, S);
   const float bl[3] = {0.5f, 0.5f, 0.5f};
-  float r = A.alpha * interp<2, false, 2>(A.src + I.oc, S, bl, p, g.inv_dx, nullptr);
-  if (A.beta != 0.f) r = fmaf(A.beta, A.add[I.oc + k], r);
-  A.out[I.oc + k] = r;
+#pragma unroll 1
+  for (int u = 0, k = w.t0; u < PPT && k < cnt; ++u, k += BS) {
+    const float p[3] = {M.psi[k], M.psi[cnt + k], 0.f};
+    float r = A.alpha * interp<2, false, 2>(A.src + I.oc, S, bl, p, g.inv_dx, nullptr);
+    if (A.beta != 0.f) r = fmaf(A.beta, A.add[I.oc + k], r);
+    A.out[I.oc + k] = r;
+  }
 }
 
 // accumulate_init_new (AB3) of f through the map, plus f2 at dt2.
@@ -427,31 +456,34 @@ struct AccB {
 __global__ void __launch_bounds__(BS) k_b_accumulate(Ctx C, Dom D, AccB A) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, false)) return;
-  int64_t k;
-  const int grid = point_grid(I, w.t, k);
+  const int N = npoints(I, false);
   const Geo g = geo_of(I);
   const MapV M = map_of(const_cast<float*>(A.map), I);
-  const int64_t cnt = grid ? faces1(I) : faces0(I);
-  const float p[3] = {M.pf[grid][k], M.pf[grid][cnt + k], 0.f};
-  const float T0 = M.tf[grid][k], T1 = M.tf[grid][cnt + k];
-  float v[3];
-  mac_eval<false, 2>(g, mac_of(A.f, I), p, v, nullptr);
-  const float gv = fmaf(T1, v[1], T0 * v[0]);
-  float* init = mac_c(A.init, I, grid) + k;
-  float* h1 = mac_c(A.h1, I, grid) + k;
-  float* h2 = mac_c(A.h2, I, grid) + k;
   const float dt = A.dt_scale * I.dt, dt2 = A.dt2_scale * I.dt;
-  float acc = *init;
-  const float hp = *h1;
-  if (A.ind == 0) acc += gv * dt;
-  else if (A.ind == 1) acc += (23.0f * gv - 16.0f * gv + 5.0f * hp) / 12.0f * dt;
-  else acc += (23.0f * gv - 16.0f * hp + 5.0f * *h2) / 12.0f * dt;
-  mac_eval<false, 2>(g, mac_of(A.f2, I), p, v, nullptr);
-  acc += fmaf(T1, v[1], T0 * v[0]) * dt2;
-  *init = acc;
-  *h2 = hp;
-  *h1 = gv;
+#pragma unroll 1
+  for (int u = 0, t = w.t0; u < PPT && t < N; ++u, t += BS) {
+    int k;
+    const int grid = point_grid(I, t, k);
+    const int cnt = (int)(grid ? faces1(I) : faces0(I));
+    const float p[3] = {M.pf[grid][k], M.pf[grid][cnt + k], 0.f};
+    const float T0 = M.tf[grid][k], T1 = M.tf[grid][cnt + k];
+    float v[3];
+    mac_eval<false, 2>(g, mac_of(A.f, I), p, v, nullptr);
+    const float gv = fmaf(T1, v[1], T0 * v[0]);
+    float* init = mac_c(A.init, I, grid) + k;
+    float* h1 = mac_c(A.h1, I, grid) + k;
+    float* h2 = mac_c(A.h2, I, grid) + k;
+    float acc = *init;
+    const float hp = *h1;
+    if (A.ind == 0) acc += gv * dt;
+    else if (A.ind == 1) acc += (23.0f * gv - 16.0f * gv + 5.0f * hp) / 12.0f * dt;
+    else acc += (23.0f * gv - 16.0f * hp + 5.0f * *h2) / 12.0f * dt;
+    mac_eval<false, 2>(g, mac_of(A.f2, I), p, v, nullptr);
+    acc += fmaf(T1, v[1], T0 * v[0]) * dt2;
+    *init = acc;
+    *h2 = hp;
+    *h1 = gv;
+  }
 }
 
 // Both adjoint sources at the face points: r_u = ((grad u)^T lam_u) with the
@@ -460,35 +492,34 @@ __global__ void __launch_bounds__(BS) k_b_sources(Ctx C, Dom D, const float* u, 
                                                   const float* lam_s, float* r_u, float* r_s) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, false)) return;
-  int64_t k;
-  const int grid = point_grid(I, w.t, k);
+  const int N = npoints(I, false);
   const Geo g = geo_of(I);
-  int sh[3];
-  shape_of(g, grid, sh);
-  int a, b, c;
-  unlin(sh, k, a, b, c);
-  float x[3] = {0.f, 0.f, 0.f};
-  node_pos(g, grid, a, b, c, x);
   const Mac U = mac_of(u, I), W = mac_of(lam, I);
-  float r = 0.f;
+#pragma unroll 1
+  for (int uu = 0, t = w.t0; uu < PPT && t < N; ++uu, t += BS) {
+    int k;
+    const int grid = point_grid(I, t, k);
+    float x[3];
+    node_of(g, grid, k, x);
+    float r = 0.f;
 #pragma unroll
-  for (int cc = 0; cc < 2; ++cc) {
+    for (int cc = 0; cc < 2; ++cc) {
+      int Sc[3];
+      shape_of(g, cc, Sc);
+      float bl[3] = {0.5f, 0.5f, 0.5f};
+      bl[cc] = 0.0f;
+      float d, v;
+      dual_interp<4, 2>(U.c[cc], W.c[cc], Sc, bl, x, g.inv_dx, grid, d, v);
+      r = fmaf(d, v, r);
+    }
+    mac_c(r_u, I, grid)[k] = r;
     int Sc[3];
-    shape_of(g, cc, Sc);
-    float bl[3] = {0.5f, 0.5f, 0.5f};
-    bl[cc] = 0.0f;
+    shape_of(g, 3, Sc);
+    const float bl[3] = {0.5f, 0.5f, 0.5f};
     float d, v;
-    dual_interp<4, 2>(U.c[cc], W.c[cc], Sc, bl, x, g.inv_dx, grid, d, v);
-    r = fmaf(d, v, r);
+    dual_interp<3, 2>(s + I.oc, lam_s + I.oc, Sc, bl, x, g.inv_dx, grid, d, v);
+    mac_c(r_s, I, grid)[k] = d * v;
   }
-  mac_c(r_u, I, grid)[k] = r;
-  int Sc[3];
-  shape_of(g, 3, Sc);
-  const float bl[3] = {0.5f, 0.5f, 0.5f};
-  float d, v;
-  dual_interp<3, 2>(s + I.oc, lam_s + I.oc, Sc, bl, x, g.inv_dx, grid, d, v);
-  mac_c(r_s, I, grid)[k] = d * v;
 }
 
 // out = x + (alpha dt) y + (beta dt) z on MAC pools (z may be null).
@@ -496,32 +527,41 @@ __global__ void __launch_bounds__(BS) k_b_axpy(Ctx C, Dom D, const float* x, con
                                                float alpha_dt, float beta_dt) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, false)) return;
-  const int64_t o = I.of + w.t;
-  float r = fmaf(alpha_dt * I.dt, y[o], x[o]);
-  if (z) r = fmaf(beta_dt * I.dt, z[o], r);
-  out[o] = r;
+  const int N = npoints(I, false);
+  const float a = alpha_dt * I.dt, b = beta_dt * I.dt;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int t = w.t0 + u * BS;
+    if (t < N) {
+      const int64_t o = I.of + t;
+      float r = fmaf(a, y[o], x[o]);
+      if (z) r = fmaf(b, z[o], r);
+      out[o] = r;
+    }
+  }
 }
 
-// Copies: up to 3 MAC pools (face points) and 3 cell pools in one launch.
+// Copies: up to 3 MAC pools (face points) and 3 cell pools in one launch; a
+// null source writes zeros.
 struct CopyB {
   int nmac, ncell;
   const float* msrc[3];
   float* mdst[3];
   const float* csrc[3];
   float* cdst[3];
-  float czero;  // cells: dst = src * (1 - czero) (czero = 1 zeroes)
 };
 __global__ void __launch_bounds__(BS) k_b_copy(Ctx C, Dom D, CopyB A) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  const int64_t F = faces0(I) + faces1(I);
-  if (w.t >= npoints(I, A.ncell > 0)) return;
-  if (w.t < F) {
-    for (int m = 0; m < A.nmac; ++m) A.mdst[m][I.of + w.t] = A.msrc[m] ? A.msrc[m][I.of + w.t] : 0.f;
-  } else {
-    const int64_t k = w.t - F;
-    for (int m = 0; m < A.ncell; ++m) A.cdst[m][I.oc + k] = A.csrc[m] ? A.csrc[m][I.oc + k] : 0.f;
+  const int F = (int)(faces0(I) + faces1(I));
+  const int N = npoints(I, A.ncell > 0);
+  for (int u = 0, t = w.t0; u < PPT && t < N; ++u, t += BS) {
+    if (t < F) {
+      for (int m = 0; m < A.nmac; ++m) A.mdst[m][I.of + t] = A.msrc[m] ? A.msrc[m][I.of + t] : 0.f;
+    } else {
+      const int k = t - F;
+      for (int m = 0; m < A.ncell; ++m) A.cdst[m][I.oc + k] = A.csrc[m] ? A.csrc[m][I.oc + k] : 0.f;
+    }
   }
 }
 
@@ -529,12 +569,17 @@ __global__ void __launch_bounds__(BS) k_b_copy(Ctx C, Dom D, CopyB A) {
 __global__ void __launch_bounds__(BS) k_b_terminal(Ctx C, Dom D, const float* s, const float* tgt, float* lam_s) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
+  const int N = (int)cells(I);
   double l = 0;
-  if (w.t < cells(I)) {
-    const int64_t o = I.oc + w.t;
-    const float d = s[o] - tgt[o];
-    lam_s[o] = 2.0f * d;
-    l = (double)d * (double)d;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int t = w.t0 + u * BS;
+    if (t < N) {
+      const int64_t o = I.oc + t;
+      const float d = s[o] - tgt[o];
+      lam_s[o] = 2.0f * d;
+      l += (double)d * (double)d;
+    }
   }
   inst_reduce<BS>(C, D, w.i, FIN_LOSS, l, 0.0);
 }
@@ -542,7 +587,8 @@ __global__ void __launch_bounds__(BS) k_b_terminal(Ctx C, Dom D, const float* s,
 // ------------------------------------------------------------------ MGPCG ----
 // Level-l arrays of an instance: natural row-major (i n1 + j), n >> l.
 struct MG {
-  float* z;  // level pools (levels 0..T): z, r
+  float* z;   // level pools (levels 0..T): z (down legs and tail), r
+  float* zo;  // z after the up leg of a global level (levels 0..T-1)
   float* r;
   float* p;
   float* Ap;
@@ -552,7 +598,7 @@ struct MG {
 // Neumann 5-point operator pieces of cell (i, j) on an a x b grid.
 DFM_HD int nnbr(int i, int j, int a, int b) { return (i > 0) + (i < a - 1) + (j > 0) + (j < b - 1); }
 DFM_HD float nsum(const float* f, int i, int j, int a, int b) {
-  const int64_t id = (int64_t)i * b + j;
+  const int id = i * b + j;
   float s = 0.f;
   if (i > 0) s += f[id - b];
   if (i < a - 1) s += f[id + b];
@@ -561,21 +607,28 @@ DFM_HD float nsum(const float* f, int i, int j, int a, int b) {
   return s;
 }
 DFM_HD float inv_nn(int nn) { return nn == 4 ? 0.25f : (nn == 3 ? (1.0f / 3.0f) : (nn == 2 ? 0.5f : (nn == 1 ? 1.f : 0.f))); }
+DFM_HD float mean_of(const Ctx& C, int inst) { return (float)C.sc[(int64_t)inst * S_COUNT + S_MEAN]; }
 
 // b = -div(u) (wall faces read as 0), partial sum -> mean; level-0 r = b.
 __global__ void __launch_bounds__(BS) k_b_div(Ctx C, Dom D, const float* u, MG M) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
+  const int N = (int)cells(I);
+  const Mac U = mac_of(u, I);
   double s = 0;
-  if (w.t < cells(I)) {
-    const int i = (int)(w.t / I.n1), j = (int)(w.t % I.n1);
-    const Mac U = mac_of(u, I);
-    const int64_t fx = (int64_t)i * I.n1 + j, fy = (int64_t)i * (I.n1 + 1) + j;
-    const float uxl = i > 0 ? U.c[0][fx] : 0.f, uxh = i < I.n0 - 1 ? U.c[0][fx + I.n1] : 0.f;
-    const float uyl = j > 0 ? U.c[1][fy] : 0.f, uyh = j < I.n1 - 1 ? U.c[1][fy + 1] : 0.f;
-    const float b = -((uxh - uxl) + (uyh - uyl));
-    M.r[I.om[0] + w.t] = b;
-    s = b;
+#pragma unroll
+  for (int uu = 0; uu < PPT; ++uu) {
+    const int t = w.t0 + uu * BS;
+    if (t < N) {
+      int i, j;
+      rc_of(t, I.n1, i, j);
+      const int fx = i * I.n1 + j, fy = i * (I.n1 + 1) + j;
+      const float uxl = i > 0 ? U.c[0][fx] : 0.f, uxh = i < I.n0 - 1 ? U.c[0][fx + I.n1] : 0.f;
+      const float uyl = j > 0 ? U.c[1][fy] : 0.f, uyh = j < I.n1 - 1 ? U.c[1][fy + 1] : 0.f;
+      const float b = -((uxh - uxl) + (uyh - uyl));
+      M.r[I.om[0] + t] = b;
+      s += b;
+    }
   }
   inst_reduce<BS>(C, D, w.i, FIN_MEAN, s, 0.0);
 }
@@ -584,113 +637,310 @@ __global__ void __launch_bounds__(BS) k_b_div(Ctx C, Dom D, const float* u, MG M
 __global__ void __launch_bounds__(BS) k_b_recenter(Ctx C, Dom D, MG M) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
+  const int N = (int)cells(I);
+  const float m = mean_of(C, w.i);
   double rr = 0;
-  if (w.t < cells(I)) {
-    const float m = (float)C.sc[(int64_t)w.i * S_COUNT + S_MEAN];
-    float* r = M.r + I.om[0] + w.t;
-    const float v = *r - m;
-    *r = v;
-    M.x[I.oc + w.t] = 0.0;
-    rr = (double)v * v;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int t = w.t0 + u * BS;
+    if (t < N) {
+      float* r = M.r + I.om[0] + t;
+      const float v = *r - m;
+      *r = v;
+      M.x[I.oc + t] = 0.0;
+      rr += (double)v * v;
+    }
   }
   inst_reduce<BS>(C, D, w.i, FIN_BNORM, rr, 0.0);
 }
 
-// The level-l residual right-hand side of a cell: level 0 carries the
-// deferred mean of the PCG residual.
-DFM_HD float rhs(const MG& M, const Inst& I, const Ctx& C, int inst, int l, int64_t id) {
-  const float v = M.r[I.om[l] + id];
-  return l == 0 ? v - (float)C.sc[(int64_t)inst * S_COUNT + S_MEAN] : v;
-}
+// Global multigrid levels: LDS-tiled legs.  A block owns a TS x TS tile of
+// an instance's level and loads the tile plus a kTH-cell halo (cells of the
+// grid only) into LDS; every half-sweep updates the whole region, reading a
+// neighbour outside the region as 0.  The down leg starts from z = 0 and
+// the up leg from the exact prolonged z, so after the leg's four
+// half-sweeps only cells within 2 (down) or 3 (up) of the region's edge
+// can differ from the global red-black sweep; with kTH = 4 the tile's
+// cells, and for the restriction their neighbours, are the global sweep's
+// values, and each leg is one launch instead of four.
+constexpr int kTH = 4;
 
-// First pre-smoothing sweep from z = 0, both colours: red z = r/nn, black z
-// = (r + sum of the red neighbours' r/nn)/nn.
-__global__ void __launch_bounds__(BS) k_b_pre1(Ctx C, Dom D, MG M, int l) {
-  const Where w = where(D);
-  if (inst_done(C, w.i)) return;
-  const Inst I = C.I[w.i];
-  const int a = n_of(I, l, 0), b = n_of(I, l, 1);
-  if (w.t >= (int64_t)a * b) return;
-  const int i = (int)(w.t / b), j = (int)(w.t % b);
-  const float m = l == 0 ? (float)C.sc[(int64_t)w.i * S_COUNT + S_MEAN] : 0.f;
-  const float* r = M.r + I.om[l];
-  float z;
-  if (((i + j) & 1) == 0) {
-    z = (r[w.t] - m) * inv_nn(nnbr(i, j, a, b));
-  } else {
+template <int R>
+__device__ void tile_half(float* zt, const float* rt, int i0, int j0, int a, int b, int c) {
+  constexpr int HR = R / 2;
+  for (int t = threadIdx.x; t < R * HR; t += BS) {
+    const int li = t / HR;
+    const int i = i0 + li;
+    const int lj = 2 * (t - li * HR) + ((c + i + j0) & 1);
+    const int j = j0 + lj;
+    if (i < 0 || i >= a || j < 0 || j >= b) continue;
+    const int id = li * R + lj;
     float s = 0.f;
-    if (i > 0) s += (r[w.t - b] - m) * inv_nn(nnbr(i - 1, j, a, b));
-    if (i < a - 1) s += (r[w.t + b] - m) * inv_nn(nnbr(i + 1, j, a, b));
-    if (j > 0) s += (r[w.t - 1] - m) * inv_nn(nnbr(i, j - 1, a, b));
-    if (j < b - 1) s += (r[w.t + 1] - m) * inv_nn(nnbr(i, j + 1, a, b));
-    z = (r[w.t] - m + s) * inv_nn(nnbr(i, j, a, b));
+    if (i > 0 && li > 0) s += zt[id - R];
+    if (i < a - 1 && li < R - 1) s += zt[id + R];
+    if (j > 0 && lj > 0) s += zt[id - 1];
+    if (j < b - 1 && lj < R - 1) s += zt[id + 1];
+    zt[id] = (rt[id] + s) * inv_nn(nnbr(i, j, a, b));
   }
-  M.z[I.om[l] + w.t] = z;
 }
 
-// Half-sweep of colour `phase` on level l (one thread per cell of that
-// colour; rows have an even length, so each row holds b/2 of them).  With
-// prolong, the neighbours (of the other colour) are read with the coarse
-// correction z_{l+1}[I/2] added (the prolongation of those cells is never
-// written: they are overwritten by the next half-sweep).
-__global__ void __launch_bounds__(BS) k_b_half(Ctx C, Dom D, MG M, int l, int phase, int prolong) {
-  const Where w = where(D);
-  if (inst_done(C, w.i)) return;
-  const Inst I = C.I[w.i];
-  const int a = n_of(I, l, 0), b = n_of(I, l, 1), hb = b >> 1;
-  if (w.t >= (int64_t)a * hb) return;
-  const int i = (int)(w.t / hb);
-  const int j = 2 * (int)(w.t % hb) + ((i + phase) & 1);
-  const int64_t id = (int64_t)i * b + j;
+// Tile of block b of a tiled domain: instance, tile row / column.
+DFM_HD void tile_of(const Dom& D, int TS, const Inst& I, int l, int& ti, int& tj) {
+  const int2 e = D.bt[blockIdx.x];
+  const int tiles_j = (n_of(I, l, 1) + TS - 1) / TS;
+  rc_of((int)blockIdx.x - e.y, tiles_j, ti, tj);
+}
+
+// Down leg on level l: two red-black sweeps from z = 0, then r_{l+1} = the
+// restriction (sum over 4 children) of r - A z; z written back.
+template <int TS>
+__global__ void __launch_bounds__(BS) k_b_down(Ctx C, Dom D, MG M, int l) {
+  constexpr int R = TS + 2 * kTH;
+  __shared__ float zt[R * R], rt[R * R];
+  const int inst = D.bt[blockIdx.x].x;
+  if (inst_done(C, inst)) return;
+  const Inst I = C.I[inst];
+  const int a = n_of(I, l, 0), b = n_of(I, l, 1);
+  int ti, tj;
+  tile_of(D, TS, I, l, ti, tj);
+  const int i0 = ti * TS - kTH, j0 = tj * TS - kTH;
+  const float m = l == 0 ? mean_of(C, inst) : 0.f;
+  const float* r = M.r + I.om[l];
+  for (int t = threadIdx.x; t < R * R; t += BS) {
+    const int li = t / R, lj = t - li * R, i = i0 + li, j = j0 + lj;
+    rt[t] = (i >= 0 && i < a && j >= 0 && j < b) ? r[i * b + j] - m : 0.f;
+    zt[t] = 0.f;
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int h = 0; h < 4; ++h) {
+    tile_half<R>(zt, rt, i0, j0, a, b, h & 1);
+    __syncthreads();
+  }
+  // restriction of the tile's residual (the tile origin is even, so its
+  // coarse cells are whole)
+  constexpr int CT = TS / 2;
+  const int cb = b >> 1;
+  float* rc = M.r + I.om[l + 1];
+  for (int t = threadIdx.x; t < CT * CT; t += BS) {
+    const int ci = t / CT, cj = t - ci * CT;
+    const int ic = ti * CT + ci, jc = tj * CT + cj;
+    if (2 * ic >= a || 2 * jc >= b) continue;
+    float sres = 0.f;
+#pragma unroll
+    for (int di = 0; di < 2; ++di)
+#pragma unroll
+      for (int dj = 0; dj < 2; ++dj) {
+        const int li = kTH + 2 * ci + di, lj = kTH + 2 * cj + dj;
+        const int i = i0 + li, j = j0 + lj, id = li * R + lj;
+        float ns = 0.f;
+        if (i > 0) ns += zt[id - R];
+        if (i < a - 1) ns += zt[id + R];
+        if (j > 0) ns += zt[id - 1];
+        if (j < b - 1) ns += zt[id + 1];
+        sres += rt[id] - ((float)nnbr(i, j, a, b) * zt[id] - ns);
+      }
+    rc[ic * cb + jc] = sres;
+  }
   float* z = M.z + I.om[l];
-  float s;
-  if (!prolong) {
-    s = nsum(z, i, j, a, b);
-  } else {
-    const float* zc = M.z + I.om[l + 1];
-    const int bc = b >> 1;
-    s = 0.f;
-    if (i > 0) s += z[id - b] + zc[(int64_t)((i - 1) >> 1) * bc + (j >> 1)];
-    if (i < a - 1) s += z[id + b] + zc[(int64_t)((i + 1) >> 1) * bc + (j >> 1)];
-    if (j > 0) s += z[id - 1] + zc[(int64_t)(i >> 1) * bc + ((j - 1) >> 1)];
-    if (j < b - 1) s += z[id + 1] + zc[(int64_t)(i >> 1) * bc + ((j + 1) >> 1)];
+  for (int t = threadIdx.x; t < TS * TS; t += BS) {
+    const int li = t / TS, lj = t - li * TS, i = ti * TS + li, j = tj * TS + lj;
+    if (i < a && j < b) z[i * b + j] = zt[(li + kTH) * R + lj + kTH];
   }
-  z[id] = (rhs(M, I, C, w.i, l, id) + s) * inv_nn(nnbr(i, j, a, b));
 }
 
-// r_{l+1}[I] = sum over the 4 children of (r - A z) (the 2D restriction
-// scale 1/(dim - 1) is 1); one thread per coarse cell.
-__global__ void __launch_bounds__(BS) k_b_restrict(Ctx C, Dom D, MG M, int l) {
-  const Where w = where(D);
-  if (inst_done(C, w.i)) return;
-  const Inst I = C.I[w.i];
-  const int a = n_of(I, l, 0), b = n_of(I, l, 1), bc = b >> 1;
-  if (w.t >= (int64_t)(a >> 1) * bc) return;
-  const int ic = (int)(w.t / bc), jc = (int)(w.t % bc);
+// Up leg on level l: z += z_{l+1}[I/2] (piecewise-constant prolongation),
+// then two black-red sweeps.  The leg reads the down leg's z (halo included)
+// and writes zo: writing z in place would race with the neighbouring tiles'
+// reads of their halos.  The coarse correction is the tail's z at l + 1 = T,
+// else the up leg's zo.
+template <int TS>
+__global__ void __launch_bounds__(BS) k_b_up(Ctx C, Dom D, MG M, int l) {
+  constexpr int R = TS + 2 * kTH;
+  __shared__ float zt[R * R], rt[R * R];
+  const int inst = D.bt[blockIdx.x].x;
+  if (inst_done(C, inst)) return;
+  const Inst I = C.I[inst];
+  const int a = n_of(I, l, 0), b = n_of(I, l, 1), cb = b >> 1;
+  int ti, tj;
+  tile_of(D, TS, I, l, ti, tj);
+  const int i0 = ti * TS - kTH, j0 = tj * TS - kTH;
+  const float m = l == 0 ? mean_of(C, inst) : 0.f;
+  const float* r = M.r + I.om[l];
   const float* z = M.z + I.om[l];
-  float s = 0.f;
-#pragma unroll
-  for (int di = 0; di < 2; ++di)
-#pragma unroll
-    for (int dj = 0; dj < 2; ++dj) {
-      const int i = 2 * ic + di, j = 2 * jc + dj;
-      const int64_t id = (int64_t)i * b + j;
-      s += rhs(M, I, C, w.i, l, id) - ((float)nnbr(i, j, a, b) * z[id] - nsum(z, i, j, a, b));
-    }
-  M.r[I.om[l + 1] + w.t] = s;
+  const float* zc = (l + 1 == I.T ? M.z : M.zo) + I.om[l + 1];
+  for (int t = threadIdx.x; t < R * R; t += BS) {
+    const int li = t / R, lj = t - li * R, i = i0 + li, j = j0 + lj;
+    const bool in = i >= 0 && i < a && j >= 0 && j < b;
+    rt[t] = in ? r[i * b + j] - m : 0.f;
+    zt[t] = in ? z[i * b + j] + zc[(i >> 1) * cb + (j >> 1)] : 0.f;
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int h = 0; h < 4; ++h) {
+    tile_half<R>(zt, rt, i0, j0, a, b, (h & 1) ^ 1);
+    __syncthreads();
+  }
+  float* zo = M.zo + I.om[l];
+  for (int t = threadIdx.x; t < TS * TS; t += BS) {
+    const int li = t / TS, lj = t - li * TS, i = ti * TS + li, j = tj * TS + lj;
+    if (i < a && j < b) zo[i * b + j] = zt[(li + kTH) * R + lj + kTH];
+  }
 }
+
+// The V-cycle's level-0 result: the up leg's output, or the tail's when the
+// tail holds level 0.
+DFM_HD const float* z_final(const MG& M, const Inst& I) { return (I.T > 0 ? M.zo : M.z) + I.om[0]; }
 
 // One workgroup per instance runs levels T..L-1 in LDS: from r_T (global)
 // the V-cycle's remaining legs (2 pre sweeps, restriction, `bottom` sweeps
 // on the coarsest level, prolongation, 2 post sweeps), z_T written back.
-__device__ void lds_half(float* z, const float* r, int a, int b, int phase, float m) {
-  const int hb = b >> 1;
-  for (int t = threadIdx.x; t < a * hb; t += kTailThreads) {
-    const int i = t / hb, j = 2 * (t % hb) + ((i + phase) & 1);
-    const int id = i * b + j;
-    z[id] = (r[id] - m + nsum(z, i, j, a, b)) * inv_nn(nnbr(i, j, a, b));
+// Level T holds at most kTailMax cells (16 per thread): each thread keeps
+// the r values of its level-T cells in registers for the whole V-cycle (r_T
+// does not change during it), so the level-T sweeps read only LDS.  Every
+// half-sweep loop is unrolled (a colour's cells are independent), so a
+// thread's LDS loads for its cells are in flight together.
+constexpr int kTailMax = 16384;
+constexpr int kTailQ = kTailMax / 2 / kTailThreads;  // cells of one colour per thread at level T
+
+// Half-sweep of colour `phase` over an a x b LDS level, r from LDS.
+template <int Q>
+__device__ void lds_half(float* z, const float* r, int a, int b, int phase) {
+  const int hb = b >> 1, N = a * hb;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int t = threadIdx.x + q * kTailThreads;
+    if (t < N) {
+      int i, jh;
+      rc_of(t, hb, i, jh);
+      const int j = 2 * jh + ((i + phase) & 1);
+      const int id = i * b + j;
+      z[id] = (r[id] + nsum(z, i, j, a, b)) * inv_nn(nnbr(i, j, a, b));
+    }
   }
 }
+// Level-T half-sweep with the thread's r values from registers.
+__device__ void reg_half(float* z, const float (&rr)[kTailQ], int a, int b, int phase) {
+  const int hb = b >> 1, N = a * hb;
+#pragma unroll
+  for (int q = 0; q < kTailQ; ++q) {
+    const int t = threadIdx.x + q * kTailThreads;
+    if (t < N) {
+      int i, jh;
+      rc_of(t, hb, i, jh);
+      const int j = 2 * jh + ((i + phase) & 1);
+      const int id = i * b + j;
+      z[id] = (rr[q] + nsum(z, i, j, a, b)) * inv_nn(nnbr(i, j, a, b));
+    }
+  }
+}
+// The same half-sweep by the 64 lanes of wave 0 alone: a wavefront's LDS
+// operations are performed in order, so consecutive half-sweeps of one wave
+// need only a wavefront-scope fence between them (no workgroup barrier);
+// used for the `bottom` sweeps of the coarsest level (at most 1024 cells).
+__device__ void wave_half(float* z, const float* r, int a, int b, int phase) {
+  const int hb = b >> 1;
+  for (int t = threadIdx.x; t < a * hb; t += 64) {
+    int i, jh;
+    rc_of(t, hb, i, jh);
+    const int j = 2 * jh + ((i + phase) & 1);
+    const int id = i * b + j;
+    z[id] = (r[id] + nsum(z, i, j, a, b)) * inv_nn(nnbr(i, j, a, b));
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// z_l of level l (>= T) restricted: r_{l+1} = sum over 4 children of (r - A z).
+template <int Q, class RF>
+__device__ void lds_restrict(const float* z, RF r_at, float* rc, int a, int b) {
+  const int bc = b >> 1, N = (a >> 1) * bc;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int t = threadIdx.x + q * kTailThreads;
+    if (t < N) {
+      int ic, jc;
+      rc_of(t, bc, ic, jc);
+      float s = 0.f;
+#pragma unroll
+      for (int di = 0; di < 2; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+          const int i = 2 * ic + di, j = 2 * jc + dj, id = i * b + j;
+          s += r_at(id) - ((float)nnbr(i, j, a, b) * z[id] - nsum(z, i, j, a, b));
+        }
+      rc[t] = s;
+    }
+  }
+}
+// z_l += z_{l+1}[I/2] (piecewise-constant prolongation).
+template <int Q>
+__device__ void lds_prolong(float* z, const float* zc, int a, int b) {
+  const int bc = b >> 1, N = a * b;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int t = threadIdx.x + q * kTailThreads;
+    if (t < N) {
+      int i, j;
+      rc_of(t, b, i, j);
+      z[t] += zc[(i >> 1) * bc + (j >> 1)];
+    }
+  }
+}
+
+// Exact solve of the coarsest level (a, b <= 16): the all-Neumann operator
+// (#neighbours) I - adjacency is diagonalised by the orthonormal DCT-II
+// (eigenvalues (2 - 2 cos(pi k / a)) + (2 - 2 cos(pi m / b))), so z =
+// C_a^T [(C_a r C_b^T) / lambda] C_b with the constant mode set to 0.
+// w holds 4 x 256 floats of LDS scratch.  Called by every thread.
+constexpr int kBottomMax = 16;
+__device__ void dct_bottom(float* z, const float* r, int a, int b, float* w) {
+  float* Ca = w;            // Ca[k * a + i]
+  float* Cb = w + 256;      // Cb[m * b + j]
+  float* t1 = w + 512;      // a x b
+  float* t2 = w + 768;      // a x b
+  const float pi = 3.14159265358979f;
+  for (int t = threadIdx.x; t < a * a; t += kTailThreads) {
+    const int k = t / a, i = t % a;
+    Ca[t] = sqrtf((k ? 2.0f : 1.0f) / a) * cosf(pi * k * (i + 0.5f) / a);
+  }
+  for (int t = threadIdx.x; t < b * b; t += kTailThreads) {
+    const int m = t / b, j = t % b;
+    Cb[t] = sqrtf((m ? 2.0f : 1.0f) / b) * cosf(pi * m * (j + 0.5f) / b);
+  }
+  __syncthreads();
+  const int n = a * b;
+  if (threadIdx.x < n) {  // t1 = Ca r
+    const int k = threadIdx.x / b, j = threadIdx.x % b;
+    float v = 0.f;
+    for (int i = 0; i < a; ++i) v = fmaf(Ca[k * a + i], r[i * b + j], v);
+    t1[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < n) {  // t2 = (t1 Cb^T) / lambda
+    const int k = threadIdx.x / b, m = threadIdx.x % b;
+    float v = 0.f;
+    for (int j = 0; j < b; ++j) v = fmaf(t1[k * b + j], Cb[m * b + j], v);
+    const float lam = (2.0f - 2.0f * cosf(pi * k / a)) + (2.0f - 2.0f * cosf(pi * m / b));
+    t2[threadIdx.x] = (k == 0 && m == 0) ? 0.f : v / lam;
+  }
+  __syncthreads();
+  if (threadIdx.x < n) {  // t1 = Ca^T t2
+    const int i = threadIdx.x / b, m = threadIdx.x % b;
+    float v = 0.f;
+    for (int k = 0; k < a; ++k) v = fmaf(Ca[k * a + i], t2[k * b + m], v);
+    t1[threadIdx.x] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < n) {  // z = t1 Cb
+    const int i = threadIdx.x / b, j = threadIdx.x % b;
+    float v = 0.f;
+    for (int m = 0; m < b; ++m) v = fmaf(t1[i * b + m], Cb[m * b + j], v);
+    z[threadIdx.x] = v;
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(kTailThreads) k_b_tail(Ctx C, MG M, int bottom) {
   extern __shared__ float lds[];
   const int inst = blockIdx.x;
@@ -698,64 +948,101 @@ __global__ void __launch_bounds__(kTailThreads) k_b_tail(Ctx C, MG M, int bottom
   const Inst I = C.I[inst];
   const int T = I.T, L = I.L;
   // LDS: z_T..z_{L-1}, then r_{T+1}..r_{L-1}; r_T stays in global memory
-  float* zl[kMaxLv];
-  float* rl[kMaxLv];
-  int off = 0;
-  for (int l = T; l < L; ++l) {
-    zl[l] = lds + off;
-    off += n_of(I, l, 0) * n_of(I, l, 1);
-  }
-  for (int l = T + 1; l < L; ++l) {
-    rl[l] = lds + off;
-    off += n_of(I, l, 0) * n_of(I, l, 1);
-  }
+  int ztot = 0;
+  for (int l = T; l < L; ++l) ztot += n_of(I, l, 0) * n_of(I, l, 1);
+  auto zl = [&](int l) {
+    int o = 0;
+    for (int k = T; k < l; ++k) o += n_of(I, k, 0) * n_of(I, k, 1);
+    return lds + o;
+  };
+  auto rl = [&](int l) {
+    int o = ztot;
+    for (int k = T + 1; k < l; ++k) o += n_of(I, k, 0) * n_of(I, k, 1);
+    return lds + o;
+  };
+  float* dct_w = rl(L);  // 4 x 256 floats after the r levels
   const float* rT = M.r + I.om[T];
-  const float mT = T == 0 ? (float)C.sc[(int64_t)inst * S_COUNT + S_MEAN] : 0.f;
+  const float mT = T == 0 ? mean_of(C, inst) : 0.f;
+  const int aT = n_of(I, T, 0), bT = n_of(I, T, 1);
+  float rr[2][kTailQ];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int q = 0; q < kTailQ; ++q) {
+      const int t = threadIdx.x + q * kTailThreads, hb = bT >> 1;
+      rr[c][q] = 0.f;
+      if (t < aT * hb) {
+        int i, jh;
+        rc_of(t, hb, i, jh);
+        rr[c][q] = rT[i * bT + 2 * jh + ((i + c) & 1)] - mT;
+      }
+    }
+  for (int t = threadIdx.x; t < ztot; t += kTailThreads) lds[t] = 0.f;
+  __syncthreads();
+  // down: level T from registers, the others from LDS
   for (int l = T; l < L; ++l) {
     const int a = n_of(I, l, 0), b = n_of(I, l, 1);
-    const float* r = l == T ? rT : rl[l];
-    const float m = l == T ? mT : 0.f;
-    for (int t = threadIdx.x; t < a * b; t += kTailThreads) zl[l][t] = 0.f;
-    __syncthreads();
-    const int sweeps = l == L - 1 ? bottom : 2;
+    float* z = zl(l);
+    if (l == L - 1 && bottom < 0 && l > T && a <= kBottomMax && b <= kBottomMax) {
+      dct_bottom(z, rl(l), a, b, dct_w);
+      break;
+    }
+    if (l == L - 1 && a * b <= 1024) {
+      const float* r = l == T ? nullptr : rl(l);
+      if (l == T) {
+        // a single-level tail: the bottom sweeps over the register values
+        for (int s = 0; s < (bottom < 0 ? 50 : bottom); ++s) {
+          reg_half(z, rr[0], a, b, 0);
+          __syncthreads();
+          reg_half(z, rr[1], a, b, 1);
+          __syncthreads();
+        }
+      } else {
+        if (threadIdx.x < 64)
+          for (int s = 0; s < (bottom < 0 ? 50 : bottom); ++s) {
+            wave_half(z, r, a, b, 0);
+            wave_half(z, r, a, b, 1);
+          }
+        __syncthreads();
+      }
+      break;
+    }
+    const int sweeps = l == L - 1 ? (bottom < 0 ? 50 : bottom) : 2;
     for (int s = 0; s < sweeps; ++s) {
-      lds_half(zl[l], r, a, b, 0, m);
+      if (l == T) reg_half(z, rr[0], a, b, 0);
+      else lds_half<2>(z, rl(l), a, b, 0);
       __syncthreads();
-      lds_half(zl[l], r, a, b, 1, m);
+      if (l == T) reg_half(z, rr[1], a, b, 1);
+      else lds_half<2>(z, rl(l), a, b, 1);
       __syncthreads();
     }
     if (l == L - 1) break;
-    const int bc = b >> 1;
-    for (int t = threadIdx.x; t < (a >> 1) * bc; t += kTailThreads) {
-      const int ic = t / bc, jc = t % bc;
-      float s = 0.f;
-      for (int di = 0; di < 2; ++di)
-        for (int dj = 0; dj < 2; ++dj) {
-          const int i = 2 * ic + di, j = 2 * jc + dj, id = i * b + j;
-          s += (r[id] - m) - ((float)nnbr(i, j, a, b) * zl[l][id] - nsum(zl[l], i, j, a, b));
-        }
-      rl[l + 1][t] = s;
+    if (l == T)
+      lds_restrict<kTailQ / 2>(z, [&](int id) { return rT[id] - mT; }, rl(l + 1), a, b);
+    else {
+      const float* r = rl(l);
+      lds_restrict<1>(z, [&](int id) { return r[id]; }, rl(l + 1), a, b);
     }
     __syncthreads();
   }
+  // up
   for (int l = L - 2; l >= T; --l) {
-    const int a = n_of(I, l, 0), b = n_of(I, l, 1), bc = b >> 1;
-    const float* r = l == T ? rT : rl[l];
-    const float m = l == T ? mT : 0.f;
-    for (int t = threadIdx.x; t < a * b; t += kTailThreads) {
-      const int i = t / b, j = t % b;
-      zl[l][t] += zl[l + 1][(i >> 1) * bc + (j >> 1)];
-    }
+    const int a = n_of(I, l, 0), b = n_of(I, l, 1);
+    float* z = zl(l);
+    if (l == T) lds_prolong<2 * kTailQ>(z, zl(l + 1), a, b);
+    else lds_prolong<4>(z, zl(l + 1), a, b);
     __syncthreads();
     for (int s = 0; s < 2; ++s) {
-      lds_half(zl[l], r, a, b, 1, m);
+      if (l == T) reg_half(z, rr[1], a, b, 1);
+      else lds_half<2>(z, rl(l), a, b, 1);
       __syncthreads();
-      lds_half(zl[l], r, a, b, 0, m);
+      if (l == T) reg_half(z, rr[0], a, b, 0);
+      else lds_half<2>(z, rl(l), a, b, 0);
       __syncthreads();
     }
   }
-  const int nT = n_of(I, T, 0) * n_of(I, T, 1);
-  for (int t = threadIdx.x; t < nT; t += kTailThreads) M.z[I.om[T] + t] = zl[T][t];
+  const float* zT = zl(T);
+  for (int t = threadIdx.x; t < aT * bT; t += kTailThreads) M.z[I.om[T] + t] = zT[t];
 }
 
 // z.r (r with its deferred mean) -> FIN_ZR_INIT (p = z) or FIN_BETA.
@@ -763,11 +1050,17 @@ __global__ void __launch_bounds__(BS) k_b_zr(Ctx C, Dom D, MG M, int init) {
   const Where w = where(D);
   if (inst_done(C, w.i)) return;
   const Inst I = C.I[w.i];
+  const int N = (int)cells(I);
+  const float m = mean_of(C, w.i);
   double zr = 0;
-  if (w.t < cells(I)) {
-    const float z = M.z[I.om[0] + w.t];
-    zr = (double)z * (double)rhs(M, I, C, w.i, 0, w.t);
-    if (init) M.p[I.oc + w.t] = z;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int t = w.t0 + u * BS;
+    if (t < N) {
+      const float z = z_final(M, I)[t];
+      zr += (double)z * (double)(M.r[I.om[0] + t] - m);
+      if (init) M.p[I.oc + t] = z;
+    }
   }
   inst_reduce<BS>(C, D, w.i, init ? FIN_ZR_INIT : FIN_BETA, zr, 0.0);
 }
@@ -777,10 +1070,16 @@ __global__ void __launch_bounds__(BS) k_b_p(Ctx C, Dom D, MG M) {
   const Where w = where(D);
   if (inst_done(C, w.i)) return;
   const Inst I = C.I[w.i];
-  if (w.t >= cells(I)) return;
+  const int N = (int)cells(I);
   const float beta = (float)C.sc[(int64_t)w.i * S_COUNT + S_BETA];
-  float* p = M.p + I.oc + w.t;
-  *p = fmaf(beta, *p, M.z[I.om[0] + w.t]);
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int t = w.t0 + u * BS;
+    if (t < N) {
+      float* p = M.p + I.oc + t;
+      *p = fmaf(beta, *p, z_final(M, I)[t]);
+    }
+  }
 }
 
 // Ap = A p, partial p.Ap -> alpha.
@@ -788,14 +1087,19 @@ __global__ void __launch_bounds__(BS) k_b_Ap(Ctx C, Dom D, MG M) {
   const Where w = where(D);
   if (inst_done(C, w.i)) return;
   const Inst I = C.I[w.i];
+  const int a = I.n0, b = I.n1, N = a * b;
+  const float* p = M.p + I.oc;
   double pap = 0;
-  if (w.t < cells(I)) {
-    const int a = I.n0, b = I.n1;
-    const int i = (int)(w.t / b), j = (int)(w.t % b);
-    const float* p = M.p + I.oc;
-    const float ap = (float)nnbr(i, j, a, b) * p[w.t] - nsum(p, i, j, a, b);
-    M.Ap[I.oc + w.t] = ap;
-    pap = (double)p[w.t] * ap;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int t = w.t0 + u * BS;
+    if (t < N) {
+      int i, j;
+      rc_of(t, b, i, j);
+      const float ap = (float)nnbr(i, j, a, b) * p[t] - nsum(p, i, j, a, b);
+      M.Ap[I.oc + t] = ap;
+      pap += (double)p[t] * ap;
+    }
   }
   inst_reduce<BS>(C, D, w.i, FIN_ALPHA, pap, 0.0);
 }
@@ -805,16 +1109,22 @@ __global__ void __launch_bounds__(BS) k_b_update(Ctx C, Dom D, MG M) {
   const Where w = where(D);
   if (inst_done(C, w.i)) return;
   const Inst I = C.I[w.i];
+  const int N = (int)cells(I);
+  const double alpha = C.sc[(int64_t)w.i * S_COUNT + S_ALPHA];
+  const float m = mean_of(C, w.i);
   double rr = 0, rs = 0;
-  if (w.t < cells(I)) {
-    const double alpha = C.sc[(int64_t)w.i * S_COUNT + S_ALPHA];
-    const int64_t o = I.oc + w.t;
-    M.x[o] += alpha * (double)M.p[o];
-    float* r = M.r + I.om[0] + w.t;
-    const float v = fmaf(-(float)alpha, M.Ap[o], rhs(M, I, C, w.i, 0, w.t));
-    *r = v;
-    rr = (double)v * v;
-    rs = v;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int t = w.t0 + u * BS;
+    if (t < N) {
+      const int64_t o = I.oc + t;
+      M.x[o] += alpha * (double)M.p[o];
+      float* r = M.r + I.om[0] + t;
+      const float v = fmaf(-(float)alpha, M.Ap[o], *r - m);
+      *r = v;
+      rr += (double)v * v;
+      rs += v;
+    }
   }
   inst_reduce<BS>(C, D, w.i, FIN_RR, rr, rs);
 }
@@ -823,32 +1133,41 @@ __global__ void __launch_bounds__(BS) k_b_update(Ctx C, Dom D, MG M) {
 __global__ void __launch_bounds__(BS) k_b_subgrad(Ctx C, Dom D, float* u, MG M) {
   const Where w = where(D);
   const Inst I = C.I[w.i];
-  if (w.t >= npoints(I, false)) return;
-  int64_t k;
-  const int grid = point_grid(I, w.t, k);
+  const int N = npoints(I, false);
   const double* x = M.x + I.oc;
-  float* c = mac_c(u, I, grid) + k;
-  if (grid == 0) {
-    const int i = (int)(k / I.n1), j = (int)(k % I.n1);
-    *c = (i > 0 && i < I.n0) ? (float)((double)*c - (x[(int64_t)i * I.n1 + j] - x[(int64_t)(i - 1) * I.n1 + j])) : 0.f;
-  } else {
-    const int i = (int)(k / (I.n1 + 1)), j = (int)(k % (I.n1 + 1));
-    *c = (j > 0 && j < I.n1) ? (float)((double)*c - (x[(int64_t)i * I.n1 + j] - x[(int64_t)i * I.n1 + j - 1])) : 0.f;
+#pragma unroll
+  for (int uu = 0; uu < PPT; ++uu) {
+    const int t = w.t0 + uu * BS;
+    if (t >= N) break;
+    int k;
+    const int grid = point_grid(I, t, k);
+    float* c = mac_c(u, I, grid) + k;
+    if (grid == 0) {
+      int i, j;
+      rc_of(k, I.n1, i, j);
+      *c = (i > 0 && i < I.n0) ? (float)((double)*c - (x[i * I.n1 + j] - x[(i - 1) * I.n1 + j])) : 0.f;
+    } else {
+      int i, j;
+      rc_of(k, I.n1 + 1, i, j);
+      *c = (j > 0 && j < I.n1) ? (float)((double)*c - (x[i * I.n1 + j] - x[i * I.n1 + j - 1])) : 0.f;
+    }
   }
 }
 
 // ----------------------------------------------------------------- engine ----
-enum DomKind { D_FACES = 0, D_CELLS, D_ALL, D_TAILGRID, D_LV_CELLS, D_LV_HALF = D_LV_CELLS + kMaxLv,
-               D_LV_COARSE = D_LV_HALF + kMaxLv, D_COUNT = D_LV_COARSE + kMaxLv };
+enum DomKind { D_FACES = 0, D_CELLS, D_ALL, D_TAILGRID, D_LV_TILES, D_COUNT = D_LV_TILES + kMaxLv };
 
 struct Batch {
-  int count = 0, R = 0, steps = 0, bottom = 50, chunk = 4, first_iters = 12, graphs = 1, tail_cells = 16384;
+  int count = 0, R = 0, steps = 0, bottom = 50, bottom_exact = 1, chunk = 4, first_iters = 8, graphs = 1,
+      tail_cells = 16384, tile = 32;
   double tol = 1e-6;
   int max_iters = 400;
   std::vector<Inst> hI;
   Inst* dI = nullptr;
   std::vector<std::vector<int>> dom_h;  // per kind: count+1 block prefix
-  int* dom_d = nullptr;                 // all tables, D_COUNT x (count + 1)
+  std::vector<int> bt_off;              // per kind: offset of its block table in bt_d
+  int* dom_d = nullptr;                 // all prefix tables, D_COUNT x (count + 1)
+  int2* bt_d = nullptr;                 // all block tables (instance, first block)
   int64_t C = 0, F = 0, MGcells = 0;    // pool sizes
   int max_blocks = 0;
   size_t tail_lds = 0;
@@ -859,7 +1178,7 @@ struct Batch {
   float *s = nullptr, *s_new = nullptr, *init_s = nullptr, *err_s = nullptr, *tgt = nullptr, *s_start = nullptr;
   float *ls = nullptr, *ls_new = nullptr, *init_ls = nullptr;
   float *phi = nullptr, *psi = nullptr, *phis = nullptr;  // maps; phis: one per interval
-  float *mz = nullptr, *mr = nullptr, *mp = nullptr, *mAp = nullptr;
+  float *mz = nullptr, *mzo = nullptr, *mr = nullptr, *mp = nullptr, *mAp = nullptr;
   double* mx = nullptr;
   double* sc = nullptr;
   int* fl = nullptr;
@@ -874,9 +1193,9 @@ struct Batch {
   std::vector<void*> allocs;
 
   Ctx ctx() const { return Ctx{dI, sc, fl, part, ctr, tol}; }
-  Dom dom(int kind) const { return Dom{dom_d + (size_t)kind * (count + 1), count}; }
+  Dom dom(int kind) const { return Dom{dom_d + (size_t)kind * (count + 1), bt_d + bt_off[kind], count}; }
   int blocks(int kind) const { return dom_h[kind][count]; }
-  MG mg() const { return MG{mz, mr, mp, mAp, mx}; }
+  MG mg() const { return MG{mz, mzo, mr, mp, mAp, mx}; }
   int64_t map_size() const { return 2 * C + 4 * F; }
 };
 
@@ -895,7 +1214,7 @@ int levels_of(int n0, int n1) {  // int(log2(min(N))) - base_level + 1, base_lev
 int build_domains(Batch& B) {
   B.dom_h.assign(D_COUNT, std::vector<int>(B.count + 1, 0));
   auto add = [&](int kind, int i, int64_t pts) {
-    B.dom_h[kind][i + 1] = B.dom_h[kind][i] + (int)((pts + BS - 1) / BS);
+    B.dom_h[kind][i + 1] = B.dom_h[kind][i] + (int)((pts + BS * PPT - 1) / (BS * PPT));
   };
   for (int i = 0; i < B.count; ++i) {
     const Inst& I = B.hI[i];
@@ -904,15 +1223,20 @@ int build_domains(Batch& B) {
     add(D_ALL, i, faces0(I) + faces1(I) + cells(I));
     B.dom_h[D_TAILGRID][i + 1] = i + 1;
     for (int l = 0; l < kMaxLv; ++l) {
-      const bool global = l < I.T;  // levels 0..T-1 run as batched kernels
-      const int64_t nl = (int64_t)n_of(I, l, 0) * n_of(I, l, 1);
-      add(D_LV_CELLS + l, i, global ? nl : 0);
-      add(D_LV_HALF + l, i, global ? nl / 2 : 0);
-      add(D_LV_COARSE + l, i, global ? nl / 4 : 0);
+      // levels 0..T-1 run as batched tiled kernels: one block per tile
+      const int TS = B.tile;
+      const int tiles = l < I.T ? ((n_of(I, l, 0) + TS - 1) / TS) * ((n_of(I, l, 1) + TS - 1) / TS) : 0;
+      B.dom_h[D_LV_TILES + l][i + 1] = B.dom_h[D_LV_TILES + l][i] + tiles;
     }
   }
   B.max_blocks = 0;
   for (auto& v : B.dom_h) B.max_blocks = std::max(B.max_blocks, v[B.count]);
+  B.bt_off.assign(D_COUNT, 0);
+  int tot = 0;
+  for (int k = 0; k < D_COUNT; ++k) {
+    B.bt_off[k] = tot;
+    tot += B.dom_h[k][B.count];
+  }
   return DFM_OK;
 }
 
@@ -927,50 +1251,36 @@ int dalloc(Batch& B, T** p, size_t n) {
 }
 
 // ----------------------------------------------------------------- solves ----
+// One V-cycle on the recentred level-0 residual: tiled down legs on the
+// global levels, the per-instance LDS tail, tiled up legs.
+void enq_vcycle(Batch& B, hipStream_t st) {
+  const Ctx c = B.ctx();
+  const MG M = B.mg();
+  int G = 0;
+  for (auto& I : B.hI) G = std::max(G, I.T);
+  for (int l = 0; l < G; ++l) {
+    const int nb = B.blocks(D_LV_TILES + l);
+    if (B.tile == 16) launch(st, k_b_down<16>, nb, c, B.dom(D_LV_TILES + l), M, l);
+    else launch(st, k_b_down<32>, nb, c, B.dom(D_LV_TILES + l), M, l);
+  }
+  hipLaunchKernelGGL(k_b_tail, dim3(B.count), dim3(kTailThreads), B.tail_lds, st, c, M,
+                     B.bottom_exact ? -1 : B.bottom);
+  for (int l = G - 1; l >= 0; --l) {
+    const int nb = B.blocks(D_LV_TILES + l);
+    if (B.tile == 16) launch(st, k_b_up<16>, nb, c, B.dom(D_LV_TILES + l), M, l);
+    else launch(st, k_b_up<32>, nb, c, B.dom(D_LV_TILES + l), M, l);
+  }
+}
+
 void enq_pcg_iters(Batch& B, hipStream_t st, int iters) {
   const Ctx c = B.ctx();
   const MG M = B.mg();
   for (int it = 0; it < iters; ++it) {
     launch(st, k_b_Ap, B.blocks(D_CELLS), c, B.dom(D_CELLS), M);
     launch(st, k_b_update, B.blocks(D_CELLS), c, B.dom(D_CELLS), M);
-    // V-cycle on the recentred residual
-    int G = 0;
-    for (auto& I : B.hI) G = std::max(G, I.T);
-    for (int l = 0; l < G; ++l) {
-      launch(st, k_b_pre1, B.blocks(D_LV_CELLS + l), c, B.dom(D_LV_CELLS + l), M, l);
-      launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 0, 0);
-      launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 1, 0);
-      launch(st, k_b_restrict, B.blocks(D_LV_COARSE + l), c, B.dom(D_LV_COARSE + l), M, l);
-    }
-    hipLaunchKernelGGL(k_b_tail, dim3(B.count), dim3(kTailThreads), B.tail_lds, st, c, M, B.bottom);
-    for (int l = G - 1; l >= 0; --l) {
-      launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 1, 1);
-      launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 0, 0);
-      launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 1, 0);
-      launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 0, 0);
-    }
+    enq_vcycle(B, st);
     launch(st, k_b_zr, B.blocks(D_CELLS), c, B.dom(D_CELLS), M, 0);
     launch(st, k_b_p, B.blocks(D_CELLS), c, B.dom(D_CELLS), M);
-  }
-}
-
-void enq_vcycle_init(Batch& B, hipStream_t st) {
-  const Ctx c = B.ctx();
-  const MG M = B.mg();
-  int G = 0;
-  for (auto& I : B.hI) G = std::max(G, I.T);
-  for (int l = 0; l < G; ++l) {
-    launch(st, k_b_pre1, B.blocks(D_LV_CELLS + l), c, B.dom(D_LV_CELLS + l), M, l);
-    launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 0, 0);
-    launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 1, 0);
-    launch(st, k_b_restrict, B.blocks(D_LV_COARSE + l), c, B.dom(D_LV_COARSE + l), M, l);
-  }
-  hipLaunchKernelGGL(k_b_tail, dim3(B.count), dim3(kTailThreads), B.tail_lds, st, c, M, B.bottom);
-  for (int l = G - 1; l >= 0; --l) {
-    launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 1, 1);
-    launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 0, 0);
-    launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 1, 0);
-    launch(st, k_b_half, B.blocks(D_LV_HALF + l), c, B.dom(D_LV_HALF + l), M, l, 0, 0);
   }
 }
 
@@ -981,7 +1291,7 @@ void enq_pcg_begin(Batch& B, hipStream_t st, const float* u, int iters) {
   const MG M = B.mg();
   launch(st, k_b_div, B.blocks(D_CELLS), c, B.dom(D_CELLS), u, M);
   launch(st, k_b_recenter, B.blocks(D_CELLS), c, B.dom(D_CELLS), M);
-  enq_vcycle_init(B, st);
+  enq_vcycle(B, st);
   launch(st, k_b_zr, B.blocks(D_CELLS), c, B.dom(D_CELLS), M, 1);
   enq_pcg_iters(B, st, iters);
 }
@@ -1215,19 +1525,20 @@ int backward_step(Batch& B, hipStream_t st) {
 // global z / r arrays.  (Re)allocates the level pools and the partials,
 // rebuilds and uploads the instance table and the launch domains.
 int plan_mg(Batch& B, int tail_cells) {
+  tail_cells = std::min(tail_cells, kTailMax);
   int64_t om = 0;
   size_t lds_max = 0;
   for (auto& I : B.hI) {
     const int n0 = I.n0, n1 = I.n1;
     int T = I.L - 1;
     for (int l = I.L - 1; l >= 0; --l) {
-      size_t bytes = 0;
+      size_t bytes = 4 * 256 * sizeof(float);
       for (int k = l; k < I.L; ++k) bytes += sizeof(float) * (size_t)(n0 >> k) * (n1 >> k) * (k == l ? 1 : 2);
       if ((int64_t)(n0 >> l) * (n1 >> l) > tail_cells || bytes > (size_t)kLdsBytes) break;
       T = l;
     }
     I.T = T;
-    size_t bytes = 0;
+    size_t bytes = 4 * 256 * sizeof(float);  // exact-bottom scratch
     for (int k = T; k < I.L; ++k) bytes += sizeof(float) * (size_t)(n0 >> k) * (n1 >> k) * (k == T ? 1 : 2);
     lds_max = std::max(lds_max, bytes);
     for (int l = 0; l < kMaxLv; ++l) I.om[l] = 0;
@@ -1240,7 +1551,7 @@ int plan_mg(Batch& B, int tail_cells) {
   B.MGcells = om;
   B.tail_lds = lds_max;
   build_domains(B);
-  for (float** p : {&B.mz, &B.mr})
+  for (float** p : {&B.mz, &B.mzo, &B.mr})
     if (*p) {
       (void)hipFree(*p);
       *p = nullptr;
@@ -1250,11 +1561,22 @@ int plan_mg(Batch& B, int tail_cells) {
     B.part = nullptr;
   }
   hipError_t e = hipMalloc((void**)&B.mz, sizeof(float) * std::max<int64_t>(om, 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&B.mzo, sizeof(float) * std::max<int64_t>(om, 1));
   if (e == hipSuccess) e = hipMalloc((void**)&B.mr, sizeof(float) * std::max<int64_t>(om, 1));
   if (e == hipSuccess) e = hipMalloc((void**)&B.part, sizeof(double) * 2 * std::max(B.max_blocks, 1));
   std::vector<int> flat;
   for (auto& v : B.dom_h) flat.insert(flat.end(), v.begin(), v.end());
   if (e == hipSuccess) e = hipMemcpy(B.dom_d, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
+  std::vector<int2> bt;
+  for (int k = 0; k < D_COUNT; ++k)
+    for (int i = 0; i < B.count; ++i)
+      for (int b = B.dom_h[k][i]; b < B.dom_h[k][i + 1]; ++b) bt.push_back(make_int2(i, B.dom_h[k][i]));
+  if (B.bt_d) {
+    (void)hipFree(B.bt_d);
+    B.bt_d = nullptr;
+  }
+  if (e == hipSuccess) e = hipMalloc((void**)&B.bt_d, sizeof(int2) * std::max<size_t>(bt.size(), 1));
+  if (e == hipSuccess) e = hipMemcpy(B.bt_d, bt.data(), sizeof(int2) * bt.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(B.dI, B.hI.data(), sizeof(Inst) * B.count, hipMemcpyHostToDevice);
   if (e == hipSuccess && B.tail_lds > 64 * 1024)
     e = hipFuncSetAttribute((const void*)k_b_tail, hipFuncAttributeMaxDynamicSharedMemorySize, (int)B.tail_lds);
@@ -1288,9 +1610,9 @@ extern "C" int dfm_batch_create(const dfm_batch_instance* inst, int count, int r
   int64_t oc = 0, of = 0;
   for (int i = 0; i < count; ++i) {
     const int n0 = inst[i].n[0], n1 = inst[i].n[1];
-    if (n0 < 8 || n1 < 8 || n0 > 8192 || n1 > 8192) {
+    if (n0 < 8 || n1 < 8 || n0 > 2048 || n1 > 2048) {
       delete h;
-      return set_error(DFM_E_ARG, "dfm_batch_create: each extent must be 8..8192");
+      return set_error(DFM_E_ARG, "dfm_batch_create: each extent must be 8..2048");
     }
     if (!(inst[i].dx > 0) || !(inst[i].dt > 0)) {
       delete h;
@@ -1358,7 +1680,7 @@ extern "C" int dfm_batch_destroy(dfm_batch* h) {
   (void)hipDeviceSynchronize();
   for (auto& kv : B.graphs_c) (void)hipGraphExecDestroy(kv.second);
   for (void* p : B.allocs) (void)hipFree(p);
-  for (void* p : {(void*)B.mz, (void*)B.mr, (void*)B.part})
+  for (void* p : {(void*)B.mz, (void*)B.mzo, (void*)B.mr, (void*)B.part, (void*)B.bt_d})
     if (p) (void)hipFree(p);
   if (B.h_fl) (void)hipHostFree(B.h_fl);
   if (B.h_sc) (void)hipHostFree(B.h_sc);
@@ -1424,6 +1746,11 @@ extern "C" int dfm_batch_set_option(dfm_batch* h, int option, int value) {
     case DFM_BATCH_TAIL_CELLS:
       if (value < 1) return set_error(DFM_E_ARG, "dfm_batch_set_option: tail cells >= 1");
       break;
+    case DFM_BATCH_BOTTOM_EXACT: B.bottom_exact = value ? 1 : 0; break;
+    case DFM_BATCH_TILE:
+      if (value != 16 && value != 32) return set_error(DFM_E_ARG, "dfm_batch_set_option: tile must be 16 or 32");
+      B.tile = value;
+      break;
     default: return set_error(DFM_E_ARG, "dfm_batch_set_option: unknown option");
   }
   // captured graphs hold the old schedule
@@ -1431,6 +1758,7 @@ extern "C" int dfm_batch_set_option(dfm_batch* h, int option, int value) {
   for (auto& kv : B.graphs_c) (void)hipGraphExecDestroy(kv.second);
   B.graphs_c.clear();
   if (option == DFM_BATCH_TAIL_CELLS) return plan_mg(B, value);
+  if (option == DFM_BATCH_TILE) return plan_mg(B, B.tail_cells);
   return DFM_OK;
 }
 

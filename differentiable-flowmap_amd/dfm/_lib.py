@@ -119,7 +119,7 @@ MG_LEVELS = 11
 
 # dfm_batch fields and options
 BATCH_U, BATCH_S, BATCH_TARGET, BATCH_ADJ_U, BATCH_ADJ_S = 0, 1, 2, 3, 4
-BATCH_GRAPH, BATCH_FIRST_ITERS, BATCH_CHUNK, BATCH_TAIL_CELLS = 0, 1, 2, 3
+BATCH_GRAPH, BATCH_FIRST_ITERS, BATCH_CHUNK, BATCH_TAIL_CELLS, BATCH_BOTTOM_EXACT, BATCH_TILE = 0, 1, 2, 3, 4, 5
 
 
 class BatchInstance(ctypes.Structure):

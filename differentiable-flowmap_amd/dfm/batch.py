@@ -214,7 +214,7 @@ class BatchEngine:
     them."""
 
     def __init__(self, problems, tol=1e-6, max_iters=200, graphs=True, first_iters=None, chunk=None,
-                 tail_cells=None, device="cuda"):
+                 tail_cells=None, bottom_exact=True, tile=None, device="cuda"):
         import torch
 
         from . import _lib as L
@@ -244,7 +244,12 @@ class BatchEngine:
             self.set_option(L.BATCH_CHUNK, chunk)
         if tail_cells is not None:
             self.set_option(L.BATCH_TAIL_CELLS, tail_cells)
-        self.options = {"first_iters": first_iters or 12, "chunk": chunk or 4, "tail_cells": tail_cells or 16384}
+        if not bottom_exact:
+            self.set_option(L.BATCH_BOTTOM_EXACT, 0)
+        if tile is not None:
+            self.set_option(L.BATCH_TILE, tile)
+        self.options = {"first_iters": first_iters or 8, "chunk": chunk or 4, "tail_cells": tail_cells or 16384,
+                        "bottom_exact": bool(bottom_exact), "tile": tile or 32}
         # per-instance device inputs, uploaded once
         self.theta = [torch.tensor(p["theta"], dtype=torch.float32, device=self.device) for p in self.problems]
         self.density = [torch.tensor(p["density"], dtype=torch.float32, device=self.device) for p in self.problems]
@@ -375,15 +380,22 @@ class BatchEngine:
     def instance_steps(self, steps=None):
         return len(self.problems) * self.steps
 
-    def autotune(self, tails=(1024, 4096, 16384), first_iters=(8, 12, 16)):
-        """Pick the LDS-tail size (the largest level, in cells, that an
-        instance's one-workgroup tail holds; finer levels are batched global
-        launches) and then the PCG iterations run before the first
-        convergence check, each by timing a whole run (after one untimed run
-        that captures the graphs); returns {knob: {value: seconds}}."""
+    def autotune(self, tiles=(16, 32), tails=(1024, 4096, 16384), first_iters=(4, 6, 8)):
+        """Pick the tile edge of the LDS-tiled smoothing legs, the LDS-tail
+        size (the largest level, in cells, that an instance's one-workgroup
+        tail holds; finer levels are batched tiled launches) and the PCG
+        iterations run before the first convergence check, one knob after
+        the other, each by timing a whole run (after one untimed run that
+        captures the graphs); returns {knob: {value: seconds}}."""
         from . import _lib as L
 
-        out = {"tail_cells": {}, "first_iters": {}}
+        out = {"tile": {}, "tail_cells": {}, "first_iters": {}}
+        for t in tiles:
+            self.set_option(L.BATCH_TILE, t)
+            self.run()
+            out["tile"][t] = self.run()[1]
+        best_tile = min(out["tile"], key=out["tile"].get)
+        self.set_option(L.BATCH_TILE, best_tile)
         for t in tails:
             self.set_option(L.BATCH_TAIL_CELLS, t)
             self.run()
@@ -396,7 +408,7 @@ class BatchEngine:
             out["first_iters"][f] = self.run()[1]
         best_f = min(out["first_iters"], key=out["first_iters"].get)
         self.set_option(L.BATCH_FIRST_ITERS, best_f)
-        self.options.update(tail_cells=best_t, first_iters=best_f)
+        self.options.update(tile=best_tile, tail_cells=best_t, first_iters=best_f)
         self.run()  # recapture with the chosen schedule
         return out
 

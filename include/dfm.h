@@ -232,7 +232,7 @@ int dfm_axpy_mac(const dfm_grid* g, const dfm_mac* x, float alpha, const dfm_mac
  * int(log2(min n)) - 2 as 2D/vortex/mgpcg.py:28) and the launches between
  * two convergence checks replayed from captured hipGraphs.  No counterpart
  * in the reference, which runs one problem per process.  Extents must be
- * 8..8192 and divisible by 2^levels.  The engine owns its device memory
+ * 8..2048 and divisible by 2^levels.  The engine owns its device memory
  * (allocated at creation). */
 typedef struct {
   int n[2];  /* cells per axis */
@@ -264,17 +264,24 @@ int dfm_batch_copy(dfm_batch* b, int field, int inst, float* c0, float* c1, int 
 
 /* DFM_BATCH_GRAPH: 1 = replay captured graphs (default), 0 = launch eagerly.
  * DFM_BATCH_FIRST_ITERS: PCG iterations inside the segment that starts a
- * projection (default 12); DFM_BATCH_CHUNK: iterations per further replay
+ * projection (default 8); DFM_BATCH_CHUNK: iterations per further replay
  * until every instance has converged (default 4).  DFM_BATCH_TAIL_CELLS:
  * the largest level (in cells) an instance's one-workgroup LDS tail holds
  * (default 16384; each instance's finer levels run as batched global
  * launches), the batched form's tile-shape knob; setting it re-plans the
- * levels and reallocates the level arrays (between runs only).  Setting an
- * option drops the captured graphs. */
+ * levels and reallocates the level arrays (between runs only).
+ * DFM_BATCH_BOTTOM_EXACT: 1 = solve the coarsest level exactly by DCT-II
+ * (default, as the single-grid simulator's mg_bottom_exact), 0 = the
+ * reference's 50 red-black sweeps (2D/vortex/mgpcg.py:31,180-182).  Setting
+ * an option drops the captured graphs. */
 #define DFM_BATCH_GRAPH 0
 #define DFM_BATCH_FIRST_ITERS 1
 #define DFM_BATCH_CHUNK 2
 #define DFM_BATCH_TAIL_CELLS 3
+#define DFM_BATCH_BOTTOM_EXACT 4
+/* DFM_BATCH_TILE: edge (16 or 32, default 32) of the square tiles of the
+ * LDS-tiled smoothing legs on the global multigrid levels. */
+#define DFM_BATCH_TILE 5
 int dfm_batch_set_option(dfm_batch* b, int option, int value);
 
 /* LFM_Simulator.init after the fields are written: project u, init_u = u,

@@ -31,13 +31,17 @@ def small_problems(sizes, steps=8, R=4, seed=40):
     return probs
 
 
-def test_instances_match_the_oracle(device):
+@pytest.mark.parametrize("bottom_exact", [True, False])
+def test_instances_match_the_oracle(bottom_exact, device):
+    """bottom_exact: the coarsest level solved by DCT (default) or by the
+    reference's 50 red-black sweeps; either preconditioner gives the
+    projection to the solve tolerance."""
     from dfm import _lib as L
     from dfm import scenes
     from dfm.batch import BatchEngine
 
     probs = small_problems([32, 48, 64, 32])
-    eng = BatchEngine(probs, tol=1e-7, max_iters=200)
+    eng = BatchEngine(probs, tol=1e-7, max_iters=200, bottom_exact=bottom_exact)
     eng.start()
     eng.forward()
     loss = eng.init_gradient()
@@ -99,3 +103,23 @@ def test_batched_configuration_converges_and_matches_single_instance(device):
     for k, i in enumerate(sample):
         assert abs(res[i][0] - rres[k][0]) <= 1e-4 * abs(rres[k][0]), (i, res[i][0], rres[k][0])
         within(f"dtheta inst {i} (n={probs[i]['n'][0]})", rel_l2(res[i][1], rres[k][1]), 1e-3)
+
+
+def test_tile_sizes_are_bitwise_identical(device):
+    """The LDS-tiled smoothing legs reproduce the global red-black sweeps
+    exactly (the tile halo covers the legs' dependency cone), so every tile
+    edge gives bitwise the same projections; a small LDS tail puts several
+    levels on the tiled path.  (An in-place up leg, whose halo reads race
+    with the neighbouring tiles' writes, fails this.)"""
+    from dfm.batch import BatchEngine
+
+    probs = small_problems([256, 192, 128, 64], steps=4, R=4, seed=90)
+    out = {}
+    for tile in (16, 32):
+        eng = BatchEngine(probs, tol=1e-6, tile=tile, tail_cells=256)
+        res, _ = eng.run()
+        st = eng.stats()
+        assert st["noconv"] == 0 and st["max_iters"] <= 12, (tile, st)
+        out[tile] = res
+    for (la, ga), (lb, gb) in zip(out[16], out[32]):
+        assert la == lb and np.array_equal(ga, gb)
