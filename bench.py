@@ -241,6 +241,90 @@ def mgpcg_256_alone(args):
     return out
 
 
+def run_north_star(args):
+    """The mixed-precision batched form against the plain FP32 form at
+    args.size^3, forward+adjoint over one reinitialisation interval per
+    problem (seeded vortex tubes).  Plain: one problem after another, FP32
+    multigrid levels, eager launches, no side streams.  Mixed batched: k
+    problems on k worker threads and streams, each simulator with BF16
+    storage of the multigrid levels >= 2, captured PCG-iteration graphs and
+    its marches and independent projections on side streams; k from
+    --ns-instances (1..k all timed, the best is the value).  Both solve to
+    the same tolerance, so the BF16 levels only change the preconditioner;
+    the gradients of problem 0 under the two forms are compared (rel-L2)."""
+    import numpy as np
+    import torch
+
+    from concurrent.futures import ThreadPoolExecutor
+
+    from dfm import LFM_Diff_Simulator, scenes
+
+    n = (args.size,) * 3
+    R = REINIT_EVERY
+    kmax = max(1, args.ns_instances)
+    scs = [scenes.scene_3d(n, seed=s, tubes=16) for s in range(kmax)]
+    dx = scs[0]["dx"]
+
+    def make(mixed):
+        kw = dict(total_steps=R, mg_levels=args.mg_levels, tol=1e-6, max_iters=200)
+        if mixed:
+            kw.update(mg_bf16_coarse=True, overlap_march="both", overlap_projections=True)
+        else:
+            kw.update(mg_bf16_coarse=False, overlap_march=False, overlap_projections=False, mg_graph=False)
+        return LFM_Diff_Simulator(*n, dx, 0.5 * dx, R, None, **kw)
+
+    def fwd_adj(sim, sc):
+        sim.init(u0=sc["u0"], init_passive_np=sc["density"])
+        for _ in range(R):
+            sim.step_midpoint()
+        sim.init_gradient(target=sc["target"])
+        for _ in range(R):
+            sim.backtrack_step_midpoint()
+
+    plain = make(False)
+    for _ in range(2):  # warm-up: allocator, handles
+        fwd_adj(plain, scs[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fwd_adj(plain, scs[0])
+    torch.cuda.synchronize()
+    plain_sps = R / (time.perf_counter() - t0)
+    g_plain = torch.cat([c.flatten() for c in plain.adj_u.c]).double().cpu().numpy()
+    del plain
+    sims = [make(True) for _ in scs]
+    streams = [torch.cuda.Stream() for _ in scs]
+
+    def work(k):
+        with torch.cuda.stream(streams[k]):
+            fwd_adj(sims[k], scs[k])
+            streams[k].synchronize()
+
+    mixed = {}
+    for k in range(1, kmax + 1):
+        with ThreadPoolExecutor(max_workers=k) as ex:
+            for _ in range(2):  # warm-up: graphs, handles
+                list(ex.map(work, range(k)))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            list(ex.map(work, range(k)))
+            torch.cuda.synchronize()
+        mixed[k] = k * R / (time.perf_counter() - t0)
+    best = max(mixed, key=mixed.get)
+    g_mixed = torch.cat([c.flatten() for c in sims[0].adj_u.c]).double().cpu().numpy()
+    return {
+        "metric": f"north-star ratio: mixed-precision batched fwd+adjoint steps/s / plain FP32 steps/s at {args.size}^3",
+        "value": round(mixed[best] / plain_sps, 3), "unit": "ratio", "n_gpus": 1,
+        "mixed_batched_steps_per_s": {str(k): round(v, 3) for k, v in mixed.items()}, "best_instances": best,
+        "plain_fp32_steps_per_s": round(plain_sps, 3), "steps_per_instance": R,
+        "grad_rel_l2_mixed_vs_plain": float(np.linalg.norm(g_mixed - g_plain) / np.linalg.norm(g_plain)),
+        "noconv": int(sum(s.noconv_count for s in sims)),
+        "higher_is_better": True, "dtype": "f32 (BF16 storage of MG levels >= 2 in the mixed form)",
+        "data": "synthetic (seeded vortex tubes / box-union density / Gaussian targets)",
+        "config": {"workload": f"3D forward+adjoint leapfrog flow-map, {args.size}^3, RK4, one reinitialisation "
+                               f"interval ({R} steps) per instance, MGPCG tol 1e-6"},
+    }
+
+
 def run_grad_check():
     """GPU forward+adjoint vs the FP64 oracle on a 32^3 grid (2 intervals)."""
     import numpy as np
@@ -400,6 +484,9 @@ def parse_args(argv=None):
     ap.add_argument("--batch-steps", type=int, default=24)
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for --batch (1..32; 0 keeps the platform's setting: 16 measured slower than 4)")
+    ap.add_argument("--north-star", action="store_true",
+                    help="mixed-precision batched fwd+adjoint vs the plain FP32 form at --size^3 (steps/s ratio)")
+    ap.add_argument("--ns-instances", type=int, default=4, help="instances of the --north-star batched form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy used to calibrate the HBM traffic counters")
     return ap.parse_args(argv)
@@ -413,6 +500,9 @@ def main(argv=None):
         sys.exit(2)
     if args.copy_calibration:
         run_copy_calibration()
+        return
+    if args.north_star:
+        print(json.dumps(run_north_star(args)), flush=True)
         return
     if args.batch:
         from dfm.batch import configure_hw_queues

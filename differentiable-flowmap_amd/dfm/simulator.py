@@ -122,7 +122,7 @@ class LFM_Simulator:
                  mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
                  overlap_projections=True, control_num=0, BFECC_clamp=False, use_short_BFECC=False,
                  overlap_march=True, mg_fused_sweeps=2, mg_split_layout=True, mg_on_noconv="warn", mg_tail_level=0,
-                 interp_kern=2):
+                 interp_kern=2, mg_graph=True):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -147,7 +147,8 @@ class LFM_Simulator:
         self._mg_kw = dict(levels=mg_levels, pre_post=mg_pre_post, bottom=mg_bottom, tol=tol, max_iters=max_iters,
                            bf16_coarse=mg_bf16_coarse, tail=mg_tail, bottom_exact=mg_bottom_exact,
                            check_every=mg_check_every, fused_sweeps=mg_fused_sweeps,
-                           split_layout=mg_split_layout, on_noconv=mg_on_noconv, tail_level=mg_tail_level)
+                           split_layout=mg_split_layout, on_noconv=mg_on_noconv, tail_level=mg_tail_level,
+                           graph=mg_graph)
         self.solver = MGPCG(n, dx, **self._mg_kw)
         # independent projections of one adjoint step run overlapped on a
         # second handle and stream
