@@ -371,7 +371,7 @@ def run_cpu_baseline(size):
     from dfm import scenes
     from oracle import lfm_oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16))
     n = (size,) * 3
     sc = scenes.scene_3d(n, seed=0, tubes=16)
     dx = sc["dx"]
@@ -391,6 +391,66 @@ def run_cpu_baseline(size):
             "sample": f"1 forward + 1 adjoint step at {size}^3 (a step without reinitialisation; the "
                       f"1-in-10 reinitialisation cost is not in the sample), FP32, C++/OpenMP oracle with the "
                       f"restated MGPCG; {el:.1f} s"}
+
+
+def run_cpu_interval(size, steps):
+    """The CPU baseline over whole steps: the oracle (FP32, restated MGPCG,
+    OpenMP over OMP_NUM_THREADS threads) runs `steps` forward steps from t =
+    0 and then the adjoint sweep back (with `steps` = 10, one full
+    reinitialisation interval: the forward reinitialisation at step 9 and
+    the adjoint one at step 0 included); every step is timed."""
+    import numpy as np
+
+    from dfm import scenes
+    from oracle import lfm_oracle as O
+
+    threads = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1))
+    n = (size,) * 3
+    sc = scenes.scene_3d(n, seed=0, tubes=16)
+    dx = sc["dx"]
+    R = REINIT_EVERY
+    if (steps % R and steps >= R) or steps < 2:
+        raise SystemExit("bench.py: --cpu-interval steps must be 2..interval-1 or a multiple of the interval")
+    orc = O.OracleLFM(n, dx, 0.5 * dx, R, 4, adj_kernel=3, project="mgpcg", dtype=np.float32, mg_levels=5,
+                      mg_tol=1e-6)
+    t_start = time.perf_counter()
+    orc.on_stage = lambda name: print(f"cpu-interval: {name} done at {time.perf_counter() - t_start:.1f} s",
+                                      file=sys.stderr, flush=True)
+    orc.init(sc["u0"], [sc["density"]])
+    fwd, adj = [], []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        orc.step()
+        fwd.append(time.perf_counter() - t0)
+        print(f"cpu-interval: forward step {k} {fwd[-1]:.1f} s", file=sys.stderr, flush=True)
+    if steps % R == 0:
+        t0 = time.perf_counter()
+        orc.init_gradient([sc["target"].astype(np.float32)])
+        t_init = time.perf_counter() - t0
+        for k in range(steps):
+            t0 = time.perf_counter()
+            orc.backtrack_step()
+            adj.append(time.perf_counter() - t0)
+            print(f"cpu-interval: adjoint step {steps - 1 - k} {adj[-1]:.1f} s", file=sys.stderr, flush=True)
+    else:
+        # fewer steps than an interval: the adjoint of the last forward step
+        # only (not step 0, whose adjoint reinitialises from the interval's
+        # stored forward map)
+        t0 = time.perf_counter()
+        orc.init_gradient([sc["target"].astype(np.float32)])
+        t_init = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.backtrack_step()
+        adj.append(time.perf_counter() - t0)
+        print(f"cpu-interval: adjoint step {steps - 1} {adj[-1]:.1f} s", file=sys.stderr, flush=True)
+    total = sum(fwd) + sum(adj) + t_init
+    # whole intervals: step pairs over the whole time; otherwise one pair of
+    # mean forward and adjoint steps (no reinitialisation in either)
+    rate = len(fwd) / total if len(adj) == len(fwd) else 1.0 / (sum(fwd) / len(fwd) + sum(adj) / len(adj))
+    return {"cpu_interval": {"value": round(rate, 6), "unit": "fwd+adjoint steps/s", "threads": threads,
+                             "kind": "port", "size": size, "steps": steps,
+                             "forward_s": [round(x, 2) for x in fwd], "adjoint_s": [round(x, 2) for x in adj],
+                             "init_gradient_s": round(t_init, 2), "total_s": round(total, 1)}}
 
 
 def run_batch(args):
@@ -487,6 +547,9 @@ def parse_args(argv=None):
     ap.add_argument("--north-star", action="store_true",
                     help="mixed-precision batched fwd+adjoint vs the plain FP32 form at --size^3 (steps/s ratio)")
     ap.add_argument("--ns-instances", type=int, default=4, help="instances of the --north-star batched form")
+    ap.add_argument("--cpu-interval", type=int, default=0, metavar="STEPS",
+                    help="only the CPU baseline: the oracle over STEPS forward steps and (for a whole number of "
+                         "intervals) the adjoint sweep back, each step timed (OMP_NUM_THREADS threads)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy used to calibrate the HBM traffic counters")
     return ap.parse_args(argv)
@@ -500,6 +563,9 @@ def main(argv=None):
         sys.exit(2)
     if args.copy_calibration:
         run_copy_calibration()
+        return
+    if args.cpu_interval:
+        print(json.dumps(run_cpu_interval(args.size, args.cpu_interval)), flush=True)
         return
     if args.north_star:
         print(json.dumps(run_north_star(args)), flush=True)

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <omp.h>
 
 namespace {
 
@@ -480,6 +481,13 @@ void sl_points(int kern, int dim, int order, const int* n, R dx, R dt, const R* 
 
 DFM_ORACLE_EXPORTS(double, d)
 DFM_ORACLE_EXPORTS(float, f)
+
+// OpenMP threads of the oracle's parallel loops (the CPU baseline sets them
+// explicitly rather than through the environment); returns the count in use.
+extern "C" int oracle_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
 
 // B-spline kernel (k = 1,2,3,4) or its derivative at x, for the kernel tests.
 extern "C" double oracle_bspline_d(int k, int deriv, double x) { return deriv ? dNk<double>(k, x) : Nk<double>(k, x); }

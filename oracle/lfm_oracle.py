@@ -81,6 +81,14 @@ def lib():
     return _LIB
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (n <= 0: unchanged);
+    returns the count in use."""
+    f = lib().oracle_set_threads
+    f.restype = ctypes.c_int
+    return int(f(ctypes.c_int(int(n))))
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
@@ -705,6 +713,11 @@ class OracleLFM:
     mask = None
     bvel = None
     interval_targets = None  # {interval-start step: [target per passive]}
+    on_stage = None          # optional callable(name): told as each projection of a step completes
+
+    def _stage(self, name):
+        if self.on_stage is not None:
+            self.on_stage(name)
 
     def set_mask(self, mask, bvel=None):
         """Obstacle mask on cells (solid where > 0 for the projection, >= 1 for
@@ -721,6 +734,7 @@ class OracleLFM:
             project_exact(u)
         else:
             project_mgpcg(u, levels=self.mg_levels, tol=self.mg_tol)
+        self._stage("projection")
 
     def _mask_fwd(self):
         """mask_velocity_by_boundary + mask_passive_by_boundary after a forward
