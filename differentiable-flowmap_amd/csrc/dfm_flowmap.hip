@@ -2,9 +2,11 @@
 // accumulation and the adjoint sources, plus their C-ABI entry points.
 #include "dfm_common.hpp"
 #include "dfm_points.hpp"
+#include "dfm_tiled_sources.hpp"
 #include "dfm_internal.hpp"
 #include "../../include/dfm.h"
 
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 
@@ -1019,6 +1021,50 @@ extern "C" int dfm_sl_advect(const dfm_grid* g, const dfm_mac* vel, float dt, in
   return dfm_sl_advect_k(g, 2, vel, dt, order, nvec, src_vec, dst_vec, nscal, src_s, dst_s, stream);
 }
 
+namespace {
+
+// Library-wide kernel choices (dfm_flowmap_set_option).
+std::atomic<int> g_tiled_sources{1};
+
+// The 3D adjoint sources by the LDS-tiled kernel: one workgroup per output
+// tile of each face grid, grids in axis order.
+int launch_src_tiled(const Geo& G, int K, const tiled::Pairs& P, const dfm_mac* r, hipStream_t st) {
+  tiled::Tiles T{};
+  const int te[3] = {tiled::TX, tiled::TY, tiled::TZ};
+  T.off[0] = 0;
+  for (int a = 0; a < 3; ++a) {
+    int So[3];
+    shape_of_host(G, a, So);
+    int64_t nb = 1;
+    for (int b = 0; b < 3; ++b) {
+      T.nt[a][b] = (So[b] + te[b] - 1) / te[b];
+      nb *= T.nt[a][b];
+    }
+    T.off[a + 1] = (int)(T.off[a] + nb);
+    T.out[a] = r->c[a];
+  }
+  dim3 grid(T.off[3]), blk(tiled::THREADS);
+  if (K == 3) hipLaunchKernelGGL((tiled::k_src_tiled<3>), grid, blk, 0, st, G, P, T);
+  else hipLaunchKernelGGL((tiled::k_src_tiled<4>), grid, blk, 0, st, G, P, T);
+  return DFM_OK;
+}
+
+}  // namespace
+
+extern "C" int dfm_flowmap_set_option(int option, int value) {
+  if (option != DFM_FM_TILED_SOURCES) return set_error(DFM_E_ARG, "unknown flow-map option");
+  if (value != 0 && value != 1) return set_error(DFM_E_ARG, "DFM_FM_TILED_SOURCES takes 0 or 1");
+  g_tiled_sources.store(value);
+  return DFM_OK;
+}
+
+extern "C" int dfm_flowmap_get_option(int option, int* value) {
+  if (!value) return set_error(DFM_E_NULL, "value is NULL");
+  if (option != DFM_FM_TILED_SOURCES) return set_error(DFM_E_ARG, "unknown flow-map option");
+  *value = g_tiled_sources.load();
+  return DFM_OK;
+}
+
 extern "C" int dfm_adj_conv_source(const dfm_grid* g, const dfm_mac* u, const dfm_mac* w, int kern,
                                    const dfm_mac* r, void* stream) {
   Geo G;
@@ -1028,13 +1074,26 @@ extern "C" int dfm_adj_conv_source(const dfm_grid* g, const dfm_mac* u, const df
       (rc = check_mac(G, r, "r is NULL")))
     return rc;
   if (kern != 3 && kern != 4) return set_error(DFM_E_ARG, "kern must be 3 (cubic) or 4 (quartic)");
+  hipStream_t st = S_(stream);
+  if (G.dim == 3 && g_tiled_sources.load()) {
+    tiled::Pairs P{};
+    P.n = 3;
+    P.nslot = 3;
+    for (int c = 0; c < 3; ++c) {
+      P.slot[c] = c;
+      P.slot_lattice[c] = c;
+      P.fd[c] = u->c[c];
+      P.fv[c] = w->c[c];
+    }
+    launch_src_tiled(G, kern, P, r, st);
+    return launch_check("dfm_adj_conv_source");
+  }
   Sets S = face_sets(G);
   MapPtrs R{};
   for (int a = 0; a < G.dim; ++a) R.psi[a] = r->c[a];
   int64_t total = S.off[S.nset];
   dim3 grid(blocks_for(total)), blk(kBlock);
   Mac U = to_mac(u), W = to_mac(w);
-  hipStream_t st = S_(stream);
   if (G.dim == 2) {
     if (kern == 3) hipLaunchKernelGGL((k_adj_conv<2, 3>), grid, blk, 0, st, G, U, W, S, R);
     else hipLaunchKernelGGL((k_adj_conv<2, 4>), grid, blk, 0, st, G, U, W, S, R);
@@ -1058,6 +1117,19 @@ extern "C" int dfm_adj_scalar_source(const dfm_grid* g, int nscal, const float* 
     if (!s || !lam || !s[m] || !lam[m]) return set_error(DFM_E_NULL, "scalar pair is NULL");
     P.s[m] = s[m];
     P.lam[m] = lam[m];
+  }
+  if (G.dim == 3 && g_tiled_sources.load()) {
+    tiled::Pairs T{};
+    T.n = nscal;
+    T.nslot = 1;
+    T.slot_lattice[0] = 3;
+    for (int m = 0; m < nscal; ++m) {
+      T.slot[m] = 0;
+      T.fd[m] = P.s[m];
+      T.fv[m] = P.lam[m];
+    }
+    launch_src_tiled(G, 3, T, r, S_(stream));
+    return launch_check("dfm_adj_scalar_source");
   }
   Sets S = face_sets(G);
   MapPtrs R{};

@@ -76,6 +76,8 @@ SIGNATURES = {
                                    c_void_p]),
     "dfm_sl_advect_vjp": (c_int, [P(Grid), c_int, P(Mac), c_float, c_int, c_int, P(Mac), P(Mac), P(Mac), c_int, P(c_void_p),
                                   P(c_void_p), P(c_void_p), P(Mac), c_void_p]),
+    "dfm_flowmap_set_option": (c_int, [c_int, c_int]),
+    "dfm_flowmap_get_option": (c_int, [c_int, P(c_int)]),
     "dfm_adj_conv_source": (c_int, [P(Grid), P(Mac), P(Mac), c_int, P(Mac), c_void_p]),
     "dfm_adj_scalar_source": (c_int, [P(Grid), c_int, P(c_void_p), P(c_void_p), P(Mac), c_void_p]),
     "dfm_axpy_mac": (c_int, [P(Grid), P(Mac), c_float, P(Mac), c_float, P(Mac), P(Mac), c_void_p]),

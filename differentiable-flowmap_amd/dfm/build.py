@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
@@ -41,16 +42,17 @@ def build(verbose: bool = False) -> str:
     """Compile every source to an object, then link the shared library."""
     objdir = os.path.join(os.path.dirname(HERE), "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
     cc = hipcc()
-    for src in SOURCES:
-        path = os.path.join(CSRC, src)
-        obj = os.path.join(objdir, src + ".o")
-        cmd = [cc, *HIP_FLAGS, "-I", INCLUDE, "-c", path, "-o", obj]
-        if verbose:
+    objs = [os.path.join(objdir, src + ".o") for src in SOURCES]
+    cmds = [[cc, *HIP_FLAGS, "-I", INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj] for src, obj in zip(SOURCES, objs)]
+    if verbose:
+        for cmd in cmds:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    # the translation units are independent: compile them side by side
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(subprocess.run, cmd, check=True) for cmd in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

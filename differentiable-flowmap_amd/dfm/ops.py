@@ -14,6 +14,20 @@ from . import _lib as L
 from . import timing
 
 
+FM_TILED_SOURCES = 0  # DFM_FM_TILED_SOURCES
+
+
+def set_flowmap_option(option, value):
+    """Library-wide kernel choice (dfm_flowmap_set_option); returns the previous value."""
+    import ctypes
+
+    lib = L.lib()
+    old = ctypes.c_int(0)
+    L.check(lib.dfm_flowmap_get_option(option, ctypes.byref(old)), "dfm_flowmap_get_option")
+    L.check(lib.dfm_flowmap_set_option(option, int(value)), "dfm_flowmap_set_option")
+    return old.value
+
+
 def face_shape(n, a):
     return tuple(n[b] + (1 if b == a else 0) for b in range(len(n)))
 

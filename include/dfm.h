@@ -163,6 +163,15 @@ int dfm_sl_advect_k(const dfm_grid* g, int kern, const dfm_mac* vel, float dt, i
                     const dfm_mac* src_vec, const dfm_mac* dst_vec, int nscal, const float* const* src_s,
                     float* const* dst_s, void* stream);
 
+/* Library-wide kernel choices (no counterpart in the reference: A/B switches
+ * of this library's kernels; the results do not depend on them).
+ * DFM_FM_TILED_SOURCES: 3D adjoint sources (dfm_adj_conv_source,
+ * dfm_adj_scalar_source) by the LDS-tiled fixed-stencil kernel (1, default)
+ * or by per-point gathers (0); both give bitwise the same fields. */
+#define DFM_FM_TILED_SOURCES 0
+int dfm_flowmap_set_option(int option, int value);
+int dfm_flowmap_get_option(int option, int* value);
+
 /* Adjoint convective source r_a(X_a) = ((grad u)^T w)_a with the B-spline of
  * order kern (4 = quartic as in 2D, 3 = cubic as in 3D).
  * Replaces calculate_nabla_u_w, 2D/vortex/flowmap.py:361-377, 3D/flowmap.py:359-380. */

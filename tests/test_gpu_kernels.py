@@ -606,3 +606,63 @@ def test_central_vector_split_and_max_speed(n, device):
     assert abs(float(m.item()) - want) <= 1e-6 * want
     ops.max_speed(MacField([torch.zeros_like(t) for t in du.c]), m)
     assert float(m.item()) == np.float32(1e-3)
+
+
+# The 3D adjoint sources run on an LDS-tiled fixed-stencil kernel (8x8x32
+# output tiles); the per-point gather kernel stays behind an option.  Both
+# form the same weights and contract z, y, x in the same order, so they agree
+# bit for bit; grids with partial tiles on every axis, both splines.
+TILE_GRIDS = [((12, 16, 20), 1 / 20), ((9, 17, 33), 1 / 33), ((40, 8, 70), 1 / 64)]
+
+
+@pytest.mark.parametrize("gi", range(len(TILE_GRIDS)))
+@pytest.mark.parametrize("kern", [3, 4])
+def test_tiled_adj_conv_bitwise_gather_and_oracle(gi, kern, device):
+    from dfm.ops import FM_TILED_SOURCES, MacField, set_flowmap_option
+
+    n, dx = TILE_GRIDS[gi]
+    rng = np.random.default_rng(500 + 7 * gi + kern)
+    u, w = rand_mac(rng, n), rand_mac(rng, n)
+    ud, wd = mac_to_dev(u, device), mac_to_dev(w, device)
+    ops = ops_for(n, dx)
+    r_t, r_g = MacField.empty(n, device), MacField.empty(n, device)
+    prev = set_flowmap_option(FM_TILED_SOURCES, 1)
+    try:
+        ops.adj_conv_source(ud, wd, kern, r_t)
+        set_flowmap_option(FM_TILED_SOURCES, 0)
+        ops.adj_conv_source(ud, wd, kern, r_g)
+    finally:
+        set_flowmap_option(FM_TILED_SOURCES, prev)
+    for a in range(3):
+        assert torch.equal(r_t.c[a], r_g.c[a]), f"component {a}"
+    within("tiled_adj_conv", rel_l2_fields(mac_to_np(r_t), O.adj_conv_source(u, w, dx, kern)), TOL)
+
+
+@pytest.mark.parametrize("gi", range(len(TILE_GRIDS)))
+@pytest.mark.parametrize("nscal", [0, 1, 3])
+def test_tiled_adj_scalar_bitwise_gather_and_oracle(gi, nscal, device):
+    from dfm.ops import FM_TILED_SOURCES, MacField, set_flowmap_option
+
+    n, dx = TILE_GRIDS[gi]
+    rng = np.random.default_rng(600 + 7 * gi + nscal)
+    ss = [round32(rng.standard_normal(n)) for _ in range(nscal)]
+    ls = [round32(rng.standard_normal(n)) for _ in range(nscal)]
+    sd = [torch.tensor(s, device=device, dtype=torch.float32) for s in ss]
+    ld = [torch.tensor(v, device=device, dtype=torch.float32) for v in ls]
+    ops = ops_for(n, dx)
+    r_t, r_g = MacField.empty(n, device), MacField.empty(n, device)
+    prev = set_flowmap_option(FM_TILED_SOURCES, 1)
+    try:
+        ops.adj_scalar_source(sd, ld, r_t)
+        set_flowmap_option(FM_TILED_SOURCES, 0)
+        ops.adj_scalar_source(sd, ld, r_g)
+    finally:
+        set_flowmap_option(FM_TILED_SOURCES, prev)
+    for a in range(3):
+        assert torch.equal(r_t.c[a], r_g.c[a]), f"component {a}"
+    if nscal == 0:
+        assert all(float(c.abs().max()) == 0.0 for c in r_t.c)
+        return
+    parts = [O.adj_scalar_source(s, v, dx) for s, v in zip(ss, ls)]
+    ref = [sum(p[a] for p in parts) for a in range(3)]
+    within("tiled_adj_scalar", rel_l2_fields(mac_to_np(r_t), ref), TOL)
