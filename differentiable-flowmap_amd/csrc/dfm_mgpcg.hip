@@ -638,13 +638,10 @@ __global__ void __launch_bounds__(kB) k_true_resid(Dims d, const double* __restr
       if (id < w.end) {
         double v = 0.0;
         if (has[u]) {
-          double s = 0.0, nn = 0.0;
+          double s = 0.0;  // absent neighbours add +0.0 (see ap_value)
 #pragma unroll
-          for (int t = 0; t < 2 * DIM; ++t)
-            if (has[u] >> t & 1u) {
-              s += xn[u][t];
-              nn += 1.0;
-            }
+          for (int t = 0; t < 2 * DIM; ++t) s += (has[u] >> t & 1u) ? xn[u][t] : 0.0;
+          const double nn = (double)__popc(has[u] & ((1u << (2 * DIM)) - 1u));
           v = (double)bv[u] - (nn * xv[u] - s);
         }
         r[id] = (float)v;
@@ -679,15 +676,16 @@ __global__ void __launch_bounds__(kB) k_unsplit(Dims d, int64_t N, const double*
 // ~1e-2 |b| at 512^2.  k_Ap_dot and k_update_xr both call it on the same
 // loads, so the value the update uses is bitwise the one alpha was formed
 // with, and A p is never stored.
+// The neighbour count is the popcount of the presence bits, and an absent
+// neighbour adds +0.0 to the running sum, which (starting from +0.0, the sum
+// is never -0.0) leaves it bitwise unchanged: the same value as summing the
+// present neighbours only, without a conditional FP64 add per neighbour.
 template <int DIM>
 DFM_HD float ap_value(unsigned has, const float* pn, float pc) {
-  double s = 0.0, nn = 0.0;
+  double s = 0.0;
 #pragma unroll
-  for (int t = 0; t < 2 * DIM; ++t)
-    if (has >> t & 1u) {
-      s += (double)pn[t];
-      nn += 1.0;
-    }
+  for (int t = 0; t < 2 * DIM; ++t) s += (double)((has >> t & 1u) ? pn[t] : 0.0f);
+  const double nn = (double)__popc(has & ((1u << (2 * DIM)) - 1u));
   return (float)(nn * (double)pc - s);
 }
 
@@ -2123,21 +2121,32 @@ void launch_true_resid(Solver& s, hipStream_t st) {
   launch_true_resid_t<DIM, MASK, false>(s, st);
 }
 
-// One PCG iteration once z, p and z.r are set (solve(), 2D/vortex/mgpcg.py:235-278):
-// Ap and p.Ap -> alpha; x/r update and r.r -> convergence flag and mean;
-// recentring V-cycle with z.r -> beta; p update.  Every kernel is a no-op once
-// `done` is set, so iterations may be queued past convergence.
+// The PCG iteration (solve(), 2D/vortex/mgpcg.py:235-278) is queued rotated
+// so that it ends at the convergence test: the head -- Ap and p.Ap -> alpha;
+// x/r update and r.r -> convergence flag and mean -- follows the set-up (or a
+// restart) once z, p and z.r are set; every later iteration is the
+// recentring V-cycle with z.r -> beta, the p update, then the head.  A solve
+// that converges in the k-th head therefore queues nothing after it: with
+// the V-cycle last, the converging iteration's V-cycle and p update (~25
+// launches) ran as no-ops.  Every kernel is a no-op once `done` is set, so
+// iterations may still be queued past convergence.
+template <int DIM, bool MASK>
+void pcg_head(Solver& s, hipStream_t st) {
+  const unsigned gr = gridr(s.lv[0].size);
+  const Red R = red_of(s);
+  launch_Ap_dot<DIM, MASK>(s, st, gr, R);
+  launch_update_xr<DIM, MASK>(s, st, gr, R);
+}
+
 template <int DIM, bool MASK>
 void pcg_iteration(Solver& s, hipStream_t st) {
   Lvl& L0 = s.lv[0];
   int64_t N = L0.size;
   unsigned gr = gridr(N);
-  const Red R = red_of(s);
-  launch_Ap_dot<DIM, MASK>(s, st, gr, R);
-  launch_update_xr<DIM, MASK>(s, st, gr, R);
   vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_BETA, nullptr});
   hipLaunchKernelGGL(k_update_p, dim3(gr), dim3(kB), 0, st, N, (const float*)s.z0_final, s.p, (const double*)s.sc,
                      (const int*)s.flags);
+  pcg_head<DIM, MASK>(s, st);
 }
 
 void drop_graph(Solver& s) {
@@ -2210,13 +2219,18 @@ bool build_graph(Solver& s) {
   return true;
 }
 
-// Queue k PCG iterations (graph replays, then single iterations), the FP64
-// true-residual check that runs only if they converged, and an asynchronous
-// copy of the flag block to pinned host memory.
+// Queue k PCG iterations (with `head`, the first is the head alone: the
+// set-up or a restart has just formed z, p and z.r; then graph replays and
+// single iterations), the FP64 true-residual check that runs only if they
+// converged, and an asynchronous copy of the flag block to pinned host memory.
 template <int DIM, bool MASK>
-int pcg_queue(Solver& s, int k) {
+int pcg_queue(Solver& s, int k, bool head) {
   Pending& P = s.pend;
   int left = k;
+  if (head && left > 0) {
+    pcg_head<DIM, MASK>(s, P.st);
+    --left;
+  }
   while (P.use_graph && left >= s.gexec_iters) {
     if (hipGraphLaunch(s.gexec, P.st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: graph launch failed");
     left -= s.gexec_iters;
@@ -2256,7 +2270,7 @@ int pcg_begin(Solver& s, hipStream_t st, double tol, int relative, int max_iters
   int k = s.predict > 0 ? s.predict : s.check_every;
   if (k > max_iters) k = max_iters;
   P.active = 1;
-  return pcg_queue<DIM, MASK>(s, k);
+  return pcg_queue<DIM, MASK>(s, k, true);
 }
 
 template <int DIM, bool MASK>
@@ -2277,14 +2291,16 @@ int pcg_end(Solver& s) {
     // check then rejects were no-ops, and counting them would make a solve
     // that ends at max_iters depend on how many the host queued at a time
     P.it = s.h_flags[F_ITERS];
+    bool restart = false;
     if (s.h_flags[F_NTRUE] > P.refinements) {
       ++P.refinements;
       vcycle<DIM, MASK>(s, st, s.sc + S_MEAN, ZrFuse{FIN_ZR_INIT, s.p});
+      restart = true;
     }
     if (P.it >= P.max_iters) break;
     int k = s.check_every;
     if (P.it + k > P.max_iters) k = P.max_iters - P.it;
-    int rc = pcg_queue<DIM, MASK>(s, k);
+    int rc = pcg_queue<DIM, MASK>(s, k, restart);
     if (rc != DFM_OK) return rc;
   }
   // the scalars came back with the flags of the last synchronised batch; only

@@ -78,6 +78,14 @@ struct Smem {
 // T + NT lattice nodes.  The window extent is clamped to that bound anyway,
 // and every relative index to the window, so no LDS access can leave the
 // arrays.
+//
+// Cubic and linear taps are read as base + q.  The base is never below 0,
+// but it can be S - 3 (in FP32 the clamp bound S - 2 - 1e-6 rounds to S - 2
+// once S is large), and the fourth tap then lies past the lattice: the
+// gather path clamps its index to S - 1, where its weight is exactly 0.  So
+// for these splines the window is sized from the unclamped taps and loaded
+// with every index clamped to S - 1: window slot r holds f[min(lo + r, S - 1)],
+// exactly the value the gather reads for that tap.
 template <int K>
 __device__ void table_entry(Smem<K>& sm, const Geo& g, int a, int slot, int lat, int b, int pos, const int o0[3],
                             const int cnt[3]) {
@@ -99,14 +107,15 @@ __device__ void table_entry(Smem<K>& sm, const Geo& g, int a, int slot, int lat,
   setup(0, first);
   setup(cnt[b] - 1, last);
   setup(pos, mine);
+  constexpr bool kConsec = K != 4;  // taps base + q (see above)
   const int lo = first.idx[0];
-  int ext = last.idx[NT - 1] - lo + 1;
+  int ext = kConsec ? last.idx[0] + NT - lo : last.idx[NT - 1] - lo + 1;
   ext = ext < cap ? ext : cap;
 #pragma unroll
   for (int q = 0; q < NT; ++q) {
     sm.wv[slot][b][pos][q] = mine.w[q];
     if (b == a) sm.wd[slot][pos][q] = mine.dw[q];
-    const int r = mine.idx[q] - lo;
+    const int r = (kConsec ? mine.idx[0] + q : mine.idx[q]) - lo;
     sm.ri[slot][b][pos][q] = (uint8_t)(r < 0 ? 0 : (r >= ext ? ext - 1 : r));
   }
   if (pos == 0) {
@@ -166,14 +175,18 @@ __global__ void __launch_bounds__(THREADS) k_src_tiled(Geo g, Pairs P, Tiles T) 
     const int lx = __builtin_amdgcn_readfirstlane(sm.lo[slot][0]);
     const int ly = __builtin_amdgcn_readfirstlane(sm.lo[slot][1]);
     const int lz = __builtin_amdgcn_readfirstlane(sm.lo[slot][2]);
+    // window indices clamped to the lattice (slot r holds f[min(lo + r, S - 1)])
     const uint32_t S2 = (uint32_t)S[2];
+    const uint32_t zl = (uint32_t)min(lz + lane, S[2] - 1);
 #pragma unroll
     for (int kx = 0; kx < XR; ++kx) {
       const int x = wave + (THREADS / 64) * kx;
-      const float* xp = src + ((uint32_t)(lx + x) * (uint32_t)S[1] + (uint32_t)ly) * S2 + (uint32_t)lz;
+      const uint32_t xg = (uint32_t)min(lx + x, S[0] - 1);
 #pragma unroll
-      for (int y = 0; y < SM::EY; ++y)
-        v[kx][y] = (x < ex && y < ey && lane < ez) ? *at32(xp, (uint32_t)y * S2 + (uint32_t)lane) : 0.f;
+      for (int y = 0; y < SM::EY; ++y) {
+        const uint32_t yg = (uint32_t)min(ly + y, S[1] - 1);
+        v[kx][y] = (x < ex && y < ey && lane < ez) ? *at32(src, (xg * (uint32_t)S[1] + yg) * S2 + zl) : 0.f;
+      }
     }
   };
   auto store = [&](int i) {

@@ -666,3 +666,40 @@ def test_tiled_adj_scalar_bitwise_gather_and_oracle(gi, nscal, device):
     parts = [O.adj_scalar_source(s, v, dx) for s, v in zip(ss, ls)]
     ref = [sum(p[a] for p in parts) for a in range(3)]
     within("tiled_adj_scalar", rel_l2_fields(mac_to_np(r_t), ref), TOL)
+
+
+# At S >= 64 the FP32 clamp bound S - 2 - 1e-6 rounds to S - 2, so a cubic
+# sample at the upper wall has its (zero-weight) fourth tap clamped back to
+# S - 1; the tiled kernel must read that tap's value, not an unloaded LDS
+# slot.  A first call on NaN fields leaves NaN in LDS, so a stale read would
+# show as a non-finite result.
+@pytest.mark.parametrize("n", [(70, 66, 64), (64, 96, 130)])
+def test_tiled_sources_upper_wall_clamp(n, device):
+    from dfm.ops import FM_TILED_SOURCES, MacField, set_flowmap_option
+
+    dx = 1.0 / n[2]
+    rng = np.random.default_rng(700 + n[0])
+    u, w = rand_mac(rng, n), rand_mac(rng, n)
+    ops = ops_for(n, dx)
+    nan = MacField([torch.full(c.shape, float("nan"), device=device) for c in mac_to_dev(u, device).c])
+    r_t, r_g, junk = MacField.empty(n, device), MacField.empty(n, device), MacField.empty(n, device)
+    ud, wd = mac_to_dev(u, device), mac_to_dev(w, device)
+    s = [torch.tensor(round32(rng.standard_normal(n)), device=device, dtype=torch.float32)]
+    s_nan = [torch.full(n, float("nan"), device=device)]
+    prev = set_flowmap_option(FM_TILED_SOURCES, 1)
+    try:
+        ops.adj_conv_source(nan, nan, 3, junk)
+        ops.adj_conv_source(ud, wd, 3, r_t)
+        ops.adj_scalar_source(s_nan, s_nan, junk)
+        st = MacField.empty(n, device)
+        ops.adj_scalar_source(s, s, st)
+        set_flowmap_option(FM_TILED_SOURCES, 0)
+        ops.adj_conv_source(ud, wd, 3, r_g)
+        sg = MacField.empty(n, device)
+        ops.adj_scalar_source(s, s, sg)
+    finally:
+        set_flowmap_option(FM_TILED_SOURCES, prev)
+    for a in range(3):
+        assert bool(torch.isfinite(r_t.c[a]).all()) and bool(torch.isfinite(st.c[a]).all())
+        assert torch.equal(r_t.c[a], r_g.c[a]), f"adj_conv component {a}"
+        assert torch.equal(st.c[a], sg.c[a]), f"adj_scalar component {a}"
