@@ -539,12 +539,13 @@ __global__ void __launch_bounds__(kB) k_load_b(Dims d, int64_t N, const float* _
   grid_reduce(R, FIN_MEAN, acc, 0.0);
 }
 
-// r -= mean (kept as b for the refinement residual); partial r.r; x = 0.
-// Solid cells keep r = 0 (the mean is over fluid cells).
+// r -= mean (kept as b for the refinement residual); partial r.r.  Solid
+// cells keep r = 0 (the mean is over fluid cells).  x starts at 0 without
+// being written: the first x/r update of a solve (F_ITERS == 0) does not read
+// it, and a solve that ends before any iteration (b = 0) zeroes it.
 template <bool MASK>
 __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restrict__ r, float* __restrict__ bkeep,
-                                                      double* __restrict__ x, const uint8_t* __restrict__ bm,
-                                                      Red R) {
+                                                      const uint8_t* __restrict__ bm, Red R) {
   float m = (float)R.sc[S_MEAN];
   double acc = 0;
   const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
@@ -552,7 +553,6 @@ __global__ void __launch_bounds__(kB) k_recenter_norm(int64_t N, float* __restri
     float v = (!MASK || bm[id] == 0) ? r[id] - m : 0.f;
     r[id] = v;
     bkeep[id] = v;
-    x[id] = 0.0;
     acc += (double)v * (double)v;
   }
   grid_reduce(R, FIN_BNORM, acc, 0.0);
@@ -743,6 +743,8 @@ __global__ void __launch_bounds__(kB) k_update_xr(Dims d, double* __restrict__ x
   const double alpha_d = R.sc[S_ALPHA];
   const float alpha = (float)alpha_d;
   const float m = MASK ? 0.f : (float)R.sc[S_MEAN];
+  // the solve's first update: x = 0 (never written by the set-up), not read
+  const bool x_zero = R.flags[F_ITERS] == 0;
   double rr = 0, rs = 0;
   const int64_t N = (int64_t)d.n[0] * d.n[1] * d.n[2];
   const SlabWalk<uint32_t> w = slab_walk<kB>((uint32_t)N);
@@ -759,7 +761,7 @@ __global__ void __launch_bounds__(kB) k_update_xr(Dims d, double* __restrict__ x
       pc[u] = rv[u] = 0.f;
       xv[u] = 0.0;
       if (id < w.end) {
-        xv[u] = x[id];
+        if (!x_zero) xv[u] = x[id];
         rv[u] = r[id];
         if (is_fluid<MASK>(d, id)) {
           int i, j, k;
@@ -2121,6 +2123,12 @@ void launch_true_resid(Solver& s, hipStream_t st) {
   launch_true_resid_t<DIM, MASK, false>(s, st);
 }
 
+// Zero-fills n 32-bit words; a kernel rather than a memset, so that a caller
+// capturing the first solve into a graph records kernel nodes only.
+__global__ void __launch_bounds__(kB) k_zero_words(uint32_t* __restrict__ p, uint64_t n) {
+  for (uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x; id < n; id += (uint64_t)gridDim.x * kB) p[id] = 0u;
+}
+
 // The PCG iteration (solve(), 2D/vortex/mgpcg.py:235-278) is queued rotated
 // so that it ends at the convergence test: the head -- Ap and p.Ap -> alpha;
 // x/r update and r.r -> convergence flag and mean -- follows the set-up (or a
@@ -2257,7 +2265,7 @@ int pcg_begin(Solver& s, hipStream_t st, double tol, int relative, int max_iters
   Lvl& L0 = s.lv[0];
   int64_t N = L0.size;
   unsigned gr = gridr(N);
-  hipLaunchKernelGGL((k_recenter_norm<MASK>), dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, s.x, (const uint8_t*)s.bm[0],
+  hipLaunchKernelGGL((k_recenter_norm<MASK>), dim3(gr), dim3(kB), 0, st, N, L0.r, s.b, (const uint8_t*)s.bm[0],
                      red_of(s, tol, relative));
   vcycle<DIM, MASK>(s, st, nullptr, ZrFuse{FIN_ZR_INIT, s.p});
   Pending& P = s.pend;
@@ -2308,6 +2316,13 @@ int pcg_end(Solver& s) {
   if (!done && (hipMemcpyAsync(s.h_sc, s.sc, kCtlBytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess))
     return set_error(DFM_E_HIP, "mgpcg: status readback failed");
+  // a solve that ended before its first x/r update (b = 0 after recentring)
+  // never wrote x: its solution is x = 0
+  if (s.h_flags[F_ITERS] == 0) {
+    const uint64_t words = 2ull * (uint64_t)s.lv[0].size, blocks = (words + kB - 1) / kB;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(kB), 0, st,
+                       reinterpret_cast<uint32_t*>(s.x), words);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(DFM_E_HIP, "mgpcg launch", hipGetErrorString(e));
   s.last_iters = s.h_flags[F_ITERS];
@@ -2328,11 +2343,6 @@ int pcg(Solver& s, hipStream_t st, double tol, int relative, int max_iters) {
   return pcg_end<DIM, MASK>(s);
 }
 
-// Zero-fills n 32-bit words; a kernel rather than a memset, so that a caller
-// capturing the first solve into a graph records kernel nodes only.
-__global__ void __launch_bounds__(kB) k_zero_words(uint32_t* __restrict__ p, uint64_t n) {
-  for (uint64_t id = (uint64_t)blockIdx.x * kB + threadIdx.x; id < n; id += (uint64_t)gridDim.x * kB) p[id] = 0u;
-}
 
 // The first call on a handle that launches work zeroes the reduction
 // partials and counters, the control block and x, ordered on the caller's

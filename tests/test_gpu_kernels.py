@@ -703,3 +703,34 @@ def test_tiled_sources_upper_wall_clamp(n, device):
         assert bool(torch.isfinite(r_t.c[a]).all()) and bool(torch.isfinite(st.c[a]).all())
         assert torch.equal(r_t.c[a], r_g.c[a]), f"adj_conv component {a}"
         assert torch.equal(st.c[a], sg.c[a]), f"adj_scalar component {a}"
+
+
+@pytest.mark.parametrize("n,split", [((48, 40, 32), True), ((48, 40, 32), False), ((96, 64), True)])
+def test_mgpcg_zero_rhs_after_solve(n, split, device):
+    """x starts at zero without being written (the first x/r update does not
+    read it).  A projection whose right-hand side is 0 runs no iteration:
+    after a solve that left x non-zero, it must still return the field
+    unchanged, a zero pressure and a zero b -> x solution."""
+    from dfm.ops import MGPCG
+
+    dim = len(n)
+    rng = np.random.default_rng(11 + n[0])
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(dim)])
+    s = MGPCG(n, 1.0 / n[1], tol=1e-7, max_iters=200, split_layout=split)
+    s.Poisson(mac_to_dev(u, device))
+    assert s.status()[0] > 0 and float(np.abs(_pressure_of(s, n, device)).max()) > 0
+    z = [np.zeros(O.face_shape(n, a)) for a in range(dim)]
+    d = mac_to_dev(z, device)
+    s.Poisson(d)
+    assert s.status()[0] == 0
+    assert all(float(c.abs().max()) == 0.0 for c in d.c)
+    assert float(np.abs(_pressure_of(s, n, device)).max()) == 0.0
+    x = torch.full(n, 7.0, dtype=torch.float32, device=device)
+    s.solve(torch.zeros(n, dtype=torch.float32, device=device), x)
+    assert float(x.abs().max()) == 0.0
+    # and a real solve after it starts from zero again
+    d = mac_to_dev(u, device)
+    s.Poisson(d)
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    within("after_zero", rel_l2_fields(mac_to_np(d), ref), 2e-6)
