@@ -1187,6 +1187,7 @@ struct Batch {
   int* h_fl = nullptr;      // pinned mirror of the flags
   double* h_sc = nullptr;   // pinned mirror of the scalars
   hipStream_t cap = nullptr;
+  hipEvent_t rdy = nullptr;  // polled for the flag readbacks (spin_sync)
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs_c;
   int step = -1;            // forward: last completed step; adjoint: next step to undo + 1
   int projections = 0, max_seen_iters = 0, noconv = 0;
@@ -1330,7 +1331,7 @@ int run_seg(Batch& B, hipStream_t st, std::tuple<int, int, int> key, F&& enqueue
 int finish_pcg(Batch& B, hipStream_t st) {
   for (;;) {
     hipError_t e = hipMemcpyAsync(B.h_fl, B.fl, sizeof(int) * F_COUNT * B.count, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = spin_sync(st, B.rdy);
     if (e != hipSuccess) return set_error(DFM_E_HIP, "dfm_batch: convergence check", hipGetErrorString(e));
     bool all = true;
     for (int i = 0; i < B.count; ++i) all = all && B.h_fl[i * F_COUNT + F_DONE];
@@ -1664,6 +1665,7 @@ extern "C" int dfm_batch_create(const dfm_batch_instance* inst, int count, int r
     if (e == hipSuccess) e = hipHostMalloc((void**)&B.h_fl, sizeof(int) * F_COUNT * count, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&B.h_sc, sizeof(double) * S_COUNT * count, hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&B.cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&B.rdy, hipEventDisableTiming);
     if (e != hipSuccess) rc = set_error(DFM_E_HIP, "dfm_batch_create", hipGetErrorString(e));
   }
   if (rc) {
@@ -1685,6 +1687,7 @@ extern "C" int dfm_batch_destroy(dfm_batch* h) {
   if (B.h_fl) (void)hipHostFree(B.h_fl);
   if (B.h_sc) (void)hipHostFree(B.h_sc);
   if (B.cap) (void)hipStreamDestroy(B.cap);
+  if (B.rdy) (void)hipEventDestroy(B.rdy);
   delete h;
   return DFM_OK;
 }
@@ -1802,7 +1805,7 @@ extern "C" int dfm_batch_init_gradient(dfm_batch* h, double* loss, void* stream)
   launch(st, k_b_reset, B.blocks(D_ALL), c, B.dom(D_ALL), B.psi);
   B.step = B.steps;
   hipError_t e = hipMemcpyAsync(B.h_sc, B.sc, sizeof(double) * S_COUNT * B.count, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = spin_sync(st, B.rdy);
   if (e != hipSuccess) return set_error(DFM_E_HIP, "dfm_batch_init_gradient", hipGetErrorString(e));
   if (loss)
     for (int i = 0; i < B.count; ++i) loss[i] = B.h_sc[i * S_COUNT + S_LOSS];

@@ -1742,6 +1742,7 @@ struct Solver {
   int use_graph;
   int predict;       // iterations the previous solve took to first convergence
   hipStream_t cap;
+  hipEvent_t rdy;  // polled for the flag readback (spin_sync)
   hipGraphExec_t gexec;
   int gexec_iters;
   Pending pend;      // the solve between project_begin and project_end
@@ -2288,7 +2289,7 @@ int pcg_end(Solver& s) {
   hipStream_t st = P.st;
   bool done = false;
   for (;;) {
-    if (hipStreamSynchronize(st) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
+    if (spin_sync(st, s.rdy) != hipSuccess) return set_error(DFM_E_HIP, "mgpcg: flag readback failed");
     if (s.h_flags[F_DONE]) {
       if (s.h_flags[F_STATUS] == 1 && P.refinements == 0) s.predict = s.h_flags[F_ITERS];
       done = true;
@@ -2314,7 +2315,7 @@ int pcg_end(Solver& s) {
   // the scalars came back with the flags of the last synchronised batch; only
   // a solve that stopped at max_iters after queueing a restart reads again
   if (!done && (hipMemcpyAsync(s.h_sc, s.sc, kCtlBytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess))
+                spin_sync(st, s.rdy) != hipSuccess))
     return set_error(DFM_E_HIP, "mgpcg: status readback failed");
   // a solve that ended before its first x/r update (b = 0 after recentring)
   // never wrote x: its solution is x = 0
@@ -2385,6 +2386,7 @@ void destroy(Solver* s) {
     if (s->bm[l]) (void)hipFree(s->bm[l]);
   if (s->sc) (void)hipFree(s->sc);  // flags live in the same block
   if (s->h_sc) (void)hipHostFree(s->h_sc);
+  if (s->rdy) (void)hipEventDestroy(s->rdy);
   drop_graph(*s);
   if (s->cap) (void)hipStreamDestroy(s->cap);
   delete s;
@@ -2458,7 +2460,8 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
       need(hipMalloc(&s->part, sizeof(double) * 2 * kRedBlocks), "hipMalloc (partials)") &&
       need(hipMalloc(&s->ctr, sizeof(unsigned) * kCtrWords), "hipMalloc (counters)") &&
       need(hipMalloc(&s->sc, kCtlBytes), "hipMalloc (control block)") &&
-      need(hipHostMalloc(&s->h_sc, kCtlBytes, hipHostMallocDefault), "hipHostMalloc (control block)");
+      need(hipHostMalloc(&s->h_sc, kCtlBytes, hipHostMallocDefault), "hipHostMalloc (control block)") &&
+      need(hipEventCreateWithFlags(&s->rdy, hipEventDisableTiming), "hipEventCreate (readback)");
   if (failed) {
     char detail[160];
     std::snprintf(detail, sizeof(detail), "%s: %s", failed, hipGetErrorString(err));

@@ -180,6 +180,9 @@ class LFM_Simulator:
         self.passive_steps = {}  # step -> [passive], kept when step_midpoint(write_passive=True)
         self.flow_maps = {}    # interval index -> FlowMap (phi, F)
         self.u_start = {}      # interval index -> MacField (checkpoint mode)
+        # released per-step storage, reused before anything new is allocated
+        # (reserve() fills them up front, so a run allocates no device memory)
+        self._pool_mid, self._pool_fm, self._pool_pass, self._pool_u = [], [], [], []
         self.boundary_mask_t = None  # obstacle mask on cells (None: no obstacles)
         self.boundary_vel_t = None   # boundary velocity, (dim, *n), or None for 0
         # Gaussian control forces (add_control_force, 2D/vortex/lfm_midpoint_simulator.py:271-278)
@@ -384,6 +387,10 @@ class LFM_Simulator:
         self.frame_idx[None] = 0
         self.ops.reset_identity(self.phi_map)
         self.init_u.copy_(self.u)
+        self._pool_mid.extend(self.mid_u.values())
+        self._pool_fm.extend(self.flow_maps.values())
+        self._pool_pass.extend(self.passive_start.values())
+        self._pool_u.extend(self.u_start.values())
         self.mid_u.clear()
         self.passive_start.clear()
         self.flow_maps.clear()
@@ -407,9 +414,20 @@ class LFM_Simulator:
         step = self.step_num[None]
         j = step % self.reinit_every
         if j == 0 and self.record and not self._replaying:
-            self.passive_start[step] = [p.clone() for p in self.passive]
+            if self._pool_pass:
+                keep = self._pool_pass.pop()
+                for dst, src in zip(keep, self.passive):
+                    dst.copy_(src)
+            else:
+                keep = [p.clone() for p in self.passive]
+            self.passive_start[step] = keep
             if self.checkpoint:
-                self.u_start[step // self.reinit_every] = self.u.clone()
+                if self._pool_u:
+                    ku = self._pool_u.pop()
+                    ku.copy_(self.u)
+                else:
+                    ku = self.u.clone()
+                self.u_start[step // self.reinit_every] = ku
         if write_passive and self.record and not self._replaying:
             # the reference writes the passive of every step to disk (:487-491)
             self.passive_steps[step] = self.passive_start.get(step) or [p.clone() for p in self.passive]
@@ -429,9 +447,35 @@ class LFM_Simulator:
     def _mid_buffer(self, step):
         if step in self.mid_u:
             return self.mid_u[step]
-        m = MacField.empty(self.n, self.device)
+        m = self._pool_mid.pop() if self._pool_mid else MacField.empty(self.n, self.device)
         self.mid_u[step] = m
         return m
+
+    def _drop_mid(self, step):
+        m = self.mid_u.pop(step, None)
+        if m is not None:
+            self._pool_mid.append(m)
+
+    def reserve(self, steps):
+        """Allocate up front the per-step storage a forward run of `steps`
+        steps keeps (midpoint velocities, each interval's (phi, F) and
+        passives), so that neither the run nor a later one of at most that
+        length allocates device memory; init() returns the storage to the pool."""
+        steps = int(steps)
+        intervals = -(-steps // self.reinit_every)
+        while len(self._pool_mid) + len(self.mid_u) < steps + 1:
+            self._pool_mid.append(MacField.empty(self.n, self.device))
+        # checkpoint mode keeps one interval's maps at a time (while the
+        # adjoint sweep is inside it) and each interval's starting velocity
+        want_fm = 1 if self.checkpoint else (intervals if self.record else 0)
+        while len(self._pool_fm) + len(self.flow_maps) < want_fm:
+            self._pool_fm.append(FlowMap(self.n, self.device))
+        if self.record:
+            while len(self._pool_pass) + len(self.passive_start) < intervals + 1:
+                self._pool_pass.append([torch.empty_like(p) for p in self.passive])
+        if self.checkpoint:
+            while len(self._pool_u) + len(self.u_start) < intervals + 1:
+                self._pool_u.append(MacField.empty(self.n, self.device))
 
     def step_flow_map(self, j, dt, forces=None):
         """step_flow_map (2D/vortex/lfm_midpoint_simulator.py:517-661); `forces`:
@@ -482,7 +526,7 @@ class LFM_Simulator:
                 self._reinit(step, j, dt)
 
     def _store_phi(self, step):
-        fm = FlowMap(self.n, self.device)
+        fm = self._pool_fm.pop() if self._pool_fm else FlowMap(self.n, self.device)
         fm.copy_(self.phi_map)
         self.flow_maps[step // self.reinit_every] = fm
 
@@ -507,7 +551,7 @@ class LFM_Simulator:
             self._store_phi(step)
         else:
             for st in range(step - j, step + 1):
-                self.mid_u.pop(st, None)
+                self._drop_mid(st)
         ops.reset_identity(self.phi_map)
         self.init_u.copy_(self.u)
 
@@ -864,9 +908,11 @@ class LFM_Diff_Simulator(LFM_Simulator):
             self.ops.reset_identity(self.psi_map)
             if self.checkpoint:
                 m = step // self.reinit_every
-                self.flow_maps.pop(m, None)
+                fm = self.flow_maps.pop(m, None)
+                if fm is not None:
+                    self._pool_fm.append(fm)
                 for st in range(step, step + self.reinit_every):
-                    self.mid_u.pop(st, None)
+                    self._drop_mid(st)
 
     # checkpoint recomputation ------------------------------------------------
     _REPLAY_FIELDS = ("u", "_u_new", "passive", "_passive_new", "phi_map")

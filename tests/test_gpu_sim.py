@@ -259,3 +259,43 @@ def test_interval_targets_match_oracle(tmp_path, device):
         pass
     assert b.loss == sim.loss
     assert torch.equal(b.adj_passive[0], sim.adj_passive[0])
+
+
+@pytest.mark.parametrize("checkpoint", [False, True])
+def test_reserved_storage_reused_bitwise(checkpoint, device):
+    """reserve() allocates the per-step storage up front and init() returns
+    it to the simulator's pool: a second run of the same problem, and a
+    longer one within the reservation, allocate no device memory and give
+    bitwise the first run's fields and gradient."""
+    from dfm import LFM_Diff_Simulator, scenes
+
+    n, R = (24, 20, 16), 3
+    sc = scenes.scene_3d(n, seed=4, tubes=4)
+    dx = sc["dx"]
+    sim = LFM_Diff_Simulator(*n, dx, 0.5 * dx, R, None, total_steps=2 * R, tol=1e-7, max_iters=200,
+                             checkpoint=checkpoint)
+    sim.reserve(4 * R)
+
+    def run(steps):
+        sim.total_steps = steps
+        sim.init(u0=sc["u0"], init_passive_np=sc["density"])
+        for _ in range(steps):
+            sim.step_midpoint()
+        sim.init_gradient(target=sc["target"])
+        for _ in range(steps):
+            sim.backtrack_step_midpoint()
+        torch.cuda.synchronize()
+        # results kept on the host, so device memory holds only the simulator's
+        return [c.cpu() for c in sim.adj_u.c] + [sim.adj_passive[0].cpu()], sim.loss
+
+    first, l1 = run(2 * R)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    second, l2 = run(2 * R)
+    longer, _ = run(4 * R)
+    torch.cuda.synchronize()
+    assert torch.cuda.memory_allocated() == before
+    assert l1 == l2
+    for a, b in zip(first, second):
+        assert torch.equal(a, b)
+    assert all(bool(torch.isfinite(t).all()) for t in longer)
