@@ -1517,6 +1517,7 @@ __global__ void __launch_bounds__(kB) k_mask_coarsen(Dims d, const uint8_t* __re
 // sequence the reference issues per level collapsed into one kernel.
 
 constexpr int kTailMax = 8;
+constexpr int64_t kTailAutoCells = 16384;  // largest first tail level chosen automatically
 struct TailArgs {
   int nl;             // levels in the tail
   int n[kTailMax][3]; // their dims
@@ -1773,6 +1774,11 @@ void configure_tail(Solver& s, size_t cap) {
   int best = s.L;
   size_t best_bytes = 0;
   for (int l = s.L - 1; l >= s.tail_min && s.L - l <= kTailMax; --l) {
+    // one workgroup smooths the tail: a first level beyond kTailAutoCells is
+    // faster on the global (tiled) kernels even where it fits in LDS, as
+    // 32^3 does in BF16 (one tail 137 us against 2 tiled launches plus a
+    // 16^3 tail, ~45 us); an explicit DFM_MG_TAIL_LEVEL is not capped
+    if (s.tail_min == 0 && s.lv[l].size > kTailAutoCells) break;
     const size_t es = (s.bf16_coarse && l >= 2) ? 2 : 4;
     size_t tot = 0;
     for (int k = l; k < s.L; ++k) tot += 2 * (size_t)s.lv[k].size * es;

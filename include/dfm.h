@@ -378,8 +378,10 @@ int dfm_mgpcg_pressure(void* handle, double** p);
 #define DFM_MG_GRAPH 5
 #define DFM_MG_BOTTOM_EXACT 6
 /*  DFM_MG_TAIL_LEVEL: lowest level the LDS tail may start at (default 0: the
- *      largest suffix of levels that fits); levels below it use global
- *      kernels.  get: the first tail level in use (L = no tail).  The batch
+ *      largest suffix of levels that fits, with a first level of at most
+ *      16384 cells -- one workgroup runs the tail, and a larger level is
+ *      faster on the global kernels even where it fits, as 32^3 does in
+ *      BF16); levels below it use global kernels.  get: the first tail level in use (L = no tail).  The batch
  *      executor's autotuner sets it per grid size. */
 #define DFM_MG_TAIL_LEVEL 7
 /*  DFM_MG_FUSED_SWEEPS: smoothing launches of the coarse global levels (at
