@@ -233,15 +233,15 @@ def solve_alone(solver, src, n, reps=10):
             "iters_per_solve": round(its / reps, 2), "solves": reps, "not_converged": solver.noconv_count - noconv0}
 
 
-def mgpcg_256_alone(args):
-    """The solver at 256^3 (6 levels, the bandwidth-bound size: one FP32
-    field is 67 MB, past what the caches hold) projecting a seeded random
-    MAC field alone, with the run's solver options."""
+def mgpcg_256_alone(args, size=256):
+    """The solver at size^3 (at 256^3, 6 levels and the bandwidth-bound
+    case: one FP32 field is 67 MB, past what the caches hold) projecting a
+    seeded random MAC field alone, with the run's solver options."""
     import torch
 
     from dfm.ops import MGPCG, MacField
 
-    n = (256, 256, 256)
+    n = (size, size, size)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(1)
     src = MacField.empty(n, "cuda")
@@ -252,7 +252,7 @@ def mgpcg_256_alone(args):
                    check_every=args.mg_check_every, fused_sweeps=args.mg_fused_sweeps,
                    split_layout=not args.mg_natural_layout)
     out = solve_alone(solver, src, n)
-    out["field"] = "seeded uniform random MAC field in [-0.5, 0.5), 256^3, projected alone"
+    out["field"] = f"seeded uniform random MAC field in [-0.5, 0.5), {size}^3, projected alone"
     del solver, src
     torch.cuda.empty_cache()
     return out
@@ -300,15 +300,20 @@ def run_north_star(args):
         for _ in range(R):
             sim.backtrack_step_midpoint()
 
+    import statistics
+
     plain = make(False)
     quiet_gc()
     for _ in range(2):  # warm-up: allocator, handles
         fwd_adj(plain, scs[0])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    fwd_adj(plain, scs[0])
-    torch.cuda.synchronize()
-    plain_sps = R / (time.perf_counter() - t0)
+    plain_runs = []
+    for _ in range(args.ns_reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fwd_adj(plain, scs[0])
+        torch.cuda.synchronize()
+        plain_runs.append(R / (time.perf_counter() - t0))
+    plain_sps = statistics.median(plain_runs)
     g_plain = torch.cat([c.flatten() for c in plain.adj_u.c]).double().cpu().numpy()
     del plain
     sims = [make(True) for _ in scs]
@@ -320,16 +325,20 @@ def run_north_star(args):
             fwd_adj(sims[k], scs[k])
             streams[k].synchronize()
 
-    mixed = {}
+    mixed, mixed_runs = {}, {}
     for k in range(1, kmax + 1):
         with ThreadPoolExecutor(max_workers=k) as ex:
             for _ in range(2):  # warm-up: graphs, handles
                 list(ex.map(work, range(k)))
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            list(ex.map(work, range(k)))
-            torch.cuda.synchronize()
-        mixed[k] = k * R / (time.perf_counter() - t0)
+            runs = []
+            for _ in range(args.ns_reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                list(ex.map(work, range(k)))
+                torch.cuda.synchronize()
+                runs.append(k * R / (time.perf_counter() - t0))
+        mixed[k] = statistics.median(runs)
+        mixed_runs[k] = runs
     best = max(mixed, key=mixed.get)
     g_mixed = torch.cat([c.flatten() for c in sims[0].adj_u.c]).double().cpu().numpy()
     return {
@@ -337,6 +346,9 @@ def run_north_star(args):
         "value": round(mixed[best] / plain_sps, 3), "unit": "ratio", "n_gpus": 1,
         "mixed_batched_steps_per_s": {str(k): round(v, 3) for k, v in mixed.items()}, "best_instances": best,
         "plain_fp32_steps_per_s": round(plain_sps, 3), "steps_per_instance": R,
+        "runs": {"plain": [round(v, 2) for v in plain_runs],
+                 "mixed": {str(k): [round(v, 2) for v in r] for k, r in mixed_runs.items()},
+                 "statistic": f"median of {args.ns_reps} timed runs after 2 warm-up runs"},
         "grad_rel_l2_mixed_vs_plain": float(np.linalg.norm(g_mixed - g_plain) / np.linalg.norm(g_plain)),
         "noconv": int(sum(s.noconv_count for s in sims)),
         "higher_is_better": True, "dtype": "f32 (BF16 storage of MG levels >= 2 in the mixed form)",
@@ -569,9 +581,12 @@ def parse_args(argv=None):
     ap.add_argument("--north-star", action="store_true",
                     help="mixed-precision batched fwd+adjoint vs the plain FP32 form at --size^3 (steps/s ratio)")
     ap.add_argument("--ns-instances", type=int, default=4, help="instances of the --north-star batched form")
+    ap.add_argument("--ns-reps", type=int, default=3, help="timed runs per form in --north-star (median reported)")
     ap.add_argument("--cpu-interval", type=int, default=0, metavar="STEPS",
                     help="only the CPU baseline: the oracle over STEPS forward steps and (for a whole number of "
                          "intervals) the adjoint sweep back, each step timed (OMP_NUM_THREADS threads)")
+    ap.add_argument("--mg-alone", action="store_true",
+                    help="only the MGPCG solver projecting a seeded random field at --size^3 (10 timed solves)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy used to calibrate the HBM traffic counters")
     return ap.parse_args(argv)
@@ -588,6 +603,11 @@ def main(argv=None):
         return
     if args.cpu_interval:
         print(json.dumps(run_cpu_interval(args.size, args.cpu_interval)), flush=True)
+        return
+    if args.mg_alone:
+        out = mgpcg_256_alone(args, args.size)
+        out.update({"metric": f"MGPCG projection alone at {args.size}^3", "bytes_model": "156 B/cell/PCG-iter + 64 B/cell/solve"})
+        print(json.dumps(out), flush=True)
         return
     if args.north_star:
         print(json.dumps(run_north_star(args)), flush=True)
