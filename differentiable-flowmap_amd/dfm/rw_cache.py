@@ -73,66 +73,75 @@ def as_channels(arr, spatial_dim):
     return arr
 
 
+class _Series:
+    """One named series: its slot buffer, the entry index each filled slot
+    holds (in slot order), and the chunk files of the series found on disk as
+    (first entry, last entry, file name), listed on the first read."""
+
+    __slots__ = ("slots", "held", "chunks")
+
+    def __init__(self, slots=None):
+        self.slots = slots
+        self.held = []
+        self.chunks = None
+
+
 class RWCache:
     """Chunked series writer/reader with the reference's file names and
-    buffer semantics; buffers are allocated per name on first use."""
+    chunk semantics (a whole CHUNK-slot buffer per file, slot k holding
+    entry first + k); one slot buffer per series name, allocated on its
+    first write."""
 
     def __init__(self, n_steps=CHUNK):
         self.n_steps = n_steps
-        self.buffer = {}
+        self._series = {}
 
-    def _entry(self, name, shape):
-        e = self.buffer.get(name)
-        if e is None or e["buffer"].shape[1:] != tuple(shape):
-            e = {"buffer": np.zeros((self.n_steps,) + tuple(shape), np.float32), "info": [], "file_info": []}
-            self.buffer[name] = e
-        return e
+    def _for_write(self, name, shape):
+        s = self._series.get(name)
+        if s is None or s.slots is None or s.slots.shape[1:] != tuple(shape):
+            s = _Series(np.zeros((self.n_steps,) + tuple(shape), np.float32))
+            self._series[name] = s
+        return s
 
     def init(self):
-        for e in self.buffer.values():
-            e["info"].clear()
-            e["file_info"].clear()
+        for s in self._series.values():
+            s.held.clear()
+            s.chunks = None
 
     def write(self, arr, name, dir, j, shape=None):
         """write_disk_with_cache: append entry j (arr in channel layout,
-        placed at the origin of a buffer of `shape`, default arr's)."""
+        placed at the origin of a slot of `shape`, default arr's); a full
+        buffer is saved first."""
         arr = np.asarray(arr, np.float32)
-        e = self._entry(name, shape or arr.shape)
-        if len(e["info"]) >= self.n_steps:
+        s = self._for_write(name, shape or arr.shape)
+        if len(s.held) >= self.n_steps:
             self.force_write(name, dir)
-        e["buffer"][(len(e["info"]),) + tuple(slice(0, s) for s in arr.shape)] = arr
-        e["info"].append(int(j))
+        s.slots[(len(s.held),) + tuple(slice(0, k) for k in arr.shape)] = arr
+        s.held.append(int(j))
 
     def force_write(self, name, dir):
-        e = self.buffer.get(name)
-        if e is not None and e["info"]:
-            info = e["info"]
-            np.save(os.path.join(dir, f"{name}_{info[0]}_{info[-1]}.npy"), e["buffer"])
-            info.clear()
+        """Save the series' buffer as <name>_<first>_<last>.npy (all slots, as
+        the reference saves them) and start a new chunk."""
+        s = self._series.get(name)
+        if s is not None and s.held:
+            np.save(os.path.join(dir, f"{name}_{s.held[0]}_{s.held[-1]}.npy"), s.slots)
+            s.held.clear()
 
     def read(self, name, dir, j):
-        """read_disk_with_cache: the buffer slot of entry j (channel layout)."""
-        e = self.buffer.get(name)
-        if e is None:
-            e = {"buffer": None, "info": [], "file_info": []}
-            self.buffer[name] = e
-        if not e["file_info"]:
-            for fname in os.listdir(dir):
-                m = _SERIES.match(fname)
-                if m and m.group(1) == name:
-                    e["file_info"].append((int(m.group(2)), int(m.group(3)), fname))
-            e["file_info"].sort()
-        info = e["info"]
-        if info and info[0] <= j <= info[-1] and j in info:
-            return e["buffer"][info.index(j)]
-        for lo, hi, fname in e["file_info"]:
-            if lo <= j <= hi:
-                data = np.load(os.path.join(dir, fname))
-                e["buffer"] = data
-                info.clear()
-                info.extend(range(lo, hi + 1))
-                return data[info.index(j)]
-        raise ValueError(f"Frame {j} not found in any available file.")
+        """read_disk_with_cache: entry j of the series (channel layout), from
+        the chunk in memory or else from the chunk file that covers j."""
+        s = self._series.setdefault(name, _Series())
+        if s.chunks is None:
+            s.chunks = sorted((int(m.group(2)), int(m.group(3)), f) for f in os.listdir(dir)
+                              for m in [_SERIES.match(f)] if m and m.group(1) == name)
+        if j in s.held:
+            return s.slots[s.held.index(j)]
+        for first, last, fname in s.chunks:
+            if first <= j <= last:
+                s.slots = np.load(os.path.join(dir, fname))
+                s.held = list(range(first, last + 1))
+                return s.slots[j - first]
+        raise ValueError(f"series {name!r}: entry {j} lies in no chunk file under {dir}")
 
 
 def write_disk(data, name, dir, j):
@@ -144,7 +153,7 @@ def write_disk(data, name, dir, j):
 def read_disk(name, dir, j):
     path = os.path.join(dir, f"{name}_{j}.npy")
     if not os.path.exists(path):
-        raise FileNotFoundError(f"File not found: {path}")
+        raise FileNotFoundError(f"read_disk: no file {path}")
     return np.load(path)
 
 
