@@ -96,6 +96,28 @@ __global__ void __launch_bounds__(kEB) k_max_speed(Geo g, dfm_mac u, unsigned* _
   if ((threadIdx.x & 63) == 0) atomicMax(bits, best);
 }
 
+// Non-finite entries of up to kMaxCount fields, summed into *count (which
+// the first launch zeroes): a grid-stride walk per field, one atomic per wave.
+constexpr int kMaxCount = 8;
+struct CountArgs {
+  const float* f[kMaxCount];
+  int64_t n[kMaxCount];
+  int nf;
+};
+__global__ void k_count_zero(unsigned* __restrict__ count) {
+  if (threadIdx.x == 0) *count = 0u;
+}
+__global__ void __launch_bounds__(kEB) k_count_nonfinite(CountArgs A, unsigned* __restrict__ count) {
+  unsigned c = 0;
+  for (int m = 0; m < A.nf; ++m) {
+    const float* __restrict__ f = A.f[m];
+    for (int64_t i = (int64_t)blockIdx.x * kEB + threadIdx.x; i < A.n[m]; i += (int64_t)gridDim.x * kEB)
+      c += isfinite(f[i]) ? 0u : 1u;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += (unsigned)__shfl_down((int)c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
 int check_grid_fields(const dfm_grid* g, Geo& G) {
   if (!g) return set_error(DFM_E_NULL, "NULL grid");
   if (g->dim != 2 && g->dim != 3) return set_error(DFM_E_ARG, "dim must be 2 or 3");
@@ -175,4 +197,25 @@ extern "C" int dfm_max_speed(const dfm_grid* g, const dfm_mac* u, float* max_spe
   else
     hipLaunchKernelGGL(k_max_speed<3>, dim3(blocks), dim3(kEB), 0, st, G, *u, bits);
   return launched("dfm_max_speed: launch failed");
+}
+
+extern "C" int dfm_count_nonfinite(int nfields, const float* const* fields, const int64_t* sizes, unsigned* count,
+                                   void* stream) {
+  if (nfields < 0 || nfields > kMaxCount) return set_error(DFM_E_ARG, "dfm_count_nonfinite: 0..8 fields");
+  if (!count || (nfields && (!fields || !sizes))) return set_error(DFM_E_NULL, "dfm_count_nonfinite: NULL argument");
+  CountArgs A{};
+  A.nf = nfields;
+  int64_t most = 0;
+  for (int m = 0; m < nfields; ++m) {
+    if (!fields[m] || sizes[m] < 0) return set_error(DFM_E_ARG, "dfm_count_nonfinite: NULL field or negative size");
+    A.f[m] = fields[m];
+    A.n[m] = sizes[m];
+    most = sizes[m] > most ? sizes[m] : most;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_count_zero, dim3(1), dim3(64), 0, st, count);
+  int64_t b = (most + kEB - 1) / kEB;
+  const unsigned blocks = (unsigned)(b < 1 ? 1 : (b < 2048 ? b : 2048));
+  hipLaunchKernelGGL(k_count_nonfinite, dim3(blocks), dim3(kEB), 0, st, A, count);
+  return launched("dfm_count_nonfinite: launch failed");
 }

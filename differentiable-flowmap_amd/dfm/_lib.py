@@ -99,6 +99,7 @@ SIGNATURES = {
     "dfm_central_vector": (c_int, [P(Grid), P(Mac), c_void_p, c_void_p]),
     "dfm_split_central_vector": (c_int, [P(Grid), c_void_p, P(Mac), c_void_p]),
     "dfm_max_speed": (c_int, [P(Grid), P(Mac), c_void_p, c_void_p]),
+    "dfm_count_nonfinite": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dfm_tile8_convert": (c_int, [P(Grid), c_void_p, c_void_p, c_int, c_void_p]),
     "dfm_poisson_operator_check": (c_int, [P(Grid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, P(c_int),
                                            c_void_p]),
@@ -128,6 +129,7 @@ class BatchInstance(ctypes.Structure):
     _fields_ = [("n", c_int * 2), ("dx", c_float), ("dt", c_float)]
 
 DFM_E_NOCONV = -4
+DFM_E_NONFINITE = -5
 NOALLOC_MAX_VALUES = 1 << 23
 
 _lib = None

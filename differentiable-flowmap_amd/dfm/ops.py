@@ -226,6 +226,17 @@ class Ops:
         """split_central_vector: MAC u from the cell-centred (dim, *n) field."""
         L.check(self.lib.dfm_split_central_vector(self.g, central.data_ptr(), u.s, self._st()), "split_central_vector")
 
+    def count_nonfinite(self, tensors, out):
+        """Queue the count of NaN / Inf entries of the FP32 tensors into
+        `out` (a uint32-sized device tensor, e.g. int32 of shape (1,));
+        asynchronous."""
+        import ctypes
+
+        ts = list(tensors)
+        ptrs = (ctypes.c_void_p * max(1, len(ts)))(*[t.data_ptr() for t in ts])
+        sizes = (ctypes.c_int64 * max(1, len(ts)))(*[t.numel() for t in ts])
+        L.check(self.lib.dfm_count_nonfinite(len(ts), ptrs, sizes, out.data_ptr(), self._st()), "count_nonfinite")
+
     def max_speed(self, u: MacField, out):
         """calc_max_speed into the device scalar `out` (FP32)."""
         L.check(self.lib.dfm_max_speed(self.g, u.s, out.data_ptr(), self._st()), "max_speed")

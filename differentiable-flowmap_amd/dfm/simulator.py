@@ -45,6 +45,7 @@ import contextlib
 import numpy as np
 import torch
 
+from . import _lib as L
 from . import scenes, timing
 from .ops import FlowMap, MacField, MGPCG, Ops
 
@@ -122,7 +123,7 @@ class LFM_Simulator:
                  mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
                  overlap_projections=True, control_num=0, BFECC_clamp=False, use_short_BFECC=False,
                  overlap_march=True, mg_fused_sweeps=2, mg_split_layout=True, mg_on_noconv="warn", mg_tail_level=0,
-                 interp_kern=2, mg_graph=True):
+                 interp_kern=2, mg_graph=True, check_finite=True):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -153,6 +154,10 @@ class LFM_Simulator:
         # independent projections of one adjoint step run overlapped on a
         # second handle and stream
         self.overlap_projections = overlap_projections
+        # NaN / Inf in the final forward state or in the adjoint raise
+        # DfmError(DFM_E_NONFINITE) (the projections report theirs already)
+        self.check_finite = check_finite
+        self._nf_count = None
         self._solver2 = None
         self._stream2 = None
         # flow-map marches that only need the midpoint velocity run on a side
@@ -772,6 +777,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
         projection, and ``loss`` is not computed."""
         if target is not None:
             self.set_target(target)
+        self._check_finite(list(self.u.c) + list(self.passive), "init_gradient: the final forward state")
         self.final_ind = self.step_num[None] + 1
         if self.final_ind % self.reinit_every:
             raise ValueError("the adjoint sweep needs a whole number of reinitialisation intervals "
@@ -813,7 +819,22 @@ class LFM_Diff_Simulator(LFM_Simulator):
         self.step_num[None] -= 1
         j = self.step_num[None] % self.reinit_every
         self.backtrack_flow_map(j, self.dt, control_force_para)
+        if self.step_num[None] == 0:
+            self._check_finite(list(self.adj_u.c) + list(self.adj_passive), "backtrack_step_midpoint: the adjoint")
         return False, self.step_num[None] == 0
+
+    def _check_finite(self, tensors, what):
+        """Raise DfmError(DFM_E_NONFINITE) when any of the tensors holds a NaN
+        or an Inf (one device count and one small readback, at points where
+        the sweep is read back anyway: the start and the end of the adjoint)."""
+        if not self.check_finite:
+            return
+        if self._nf_count is None:
+            self._nf_count = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.ops.count_nonfinite(tensors, self._nf_count)
+        bad = int(self._nf_count.item())
+        if bad:
+            raise L.DfmError(L.DFM_E_NONFINITE, f"{what} holds {bad} non-finite values")
 
     def backtrack_flow_map(self, j, dt, control_force_para=None):
         """backtrack_flow_map (2D/vortex/lfm_midpoint_diff_simulator.py:260-503)

@@ -470,6 +470,15 @@ int dfm_split_central_vector(const dfm_grid* g, const float* central, const dfm_
  * 3D/lfm_midpoint_simulator.py:347-355). */
 int dfm_max_speed(const dfm_grid* g, const dfm_mac* u, float* max_speed, void* stream);
 
+/* *count (device, uint32) = number of NaN / Inf entries over `nfields`
+ * (<= 8) FP32 device fields of sizes[m] values; asynchronous on `stream` (no
+ * host synchronisation: the caller reads *count where it already waits).
+ * No counterpart in the reference, which has no NaN checks on this path; the
+ * simulators use it to return DFM_E_NONFINITE for a forward state or an
+ * adjoint result that is not finite. */
+int dfm_count_nonfinite(int nfields, const float* const* fields, const int64_t* sizes, unsigned* count,
+                        void* stream);
+
 /* ----------------------------------------- PoissonSolver-compatible entry */
 /* Convert a 3D cell field between the dense (n0, n1, n2) layout and the
  * 8^3-tiled flat layout MGPCG_3 passes to its external PoissonSolver

@@ -299,3 +299,55 @@ def test_reserved_storage_reused_bitwise(checkpoint, device):
     for a, b in zip(first, second):
         assert torch.equal(a, b)
     assert all(bool(torch.isfinite(t).all()) for t in longer)
+
+
+def test_nonfinite_fields_raise(device):
+    """NaN / Inf reach the caller as DfmError(DFM_E_NONFINITE): counted
+    exactly by dfm_count_nonfinite; in the final forward state at
+    init_gradient; in the velocity at the next projection; through a NaN
+    target in the adjoint."""
+    from dfm import LFM_Diff_Simulator, _lib as L, scenes
+    from dfm.ops import Ops
+
+    n, R = (20, 16, 12), 2
+    ops = Ops(n, 1.0 / 16)
+    a = torch.zeros(100, device=device)
+    b = torch.zeros(7, device=device)
+    a[[3, 50, 99]] = float("nan")
+    b[0] = float("inf")
+    b[6] = float("-inf")
+    cnt = torch.zeros(1, dtype=torch.int32, device=device)
+    ops.count_nonfinite([a, b], cnt)
+    assert int(cnt.item()) == 5
+    ops.count_nonfinite([], cnt)
+    assert int(cnt.item()) == 0
+
+    sc = scenes.scene_3d(n, seed=1, tubes=3)
+
+    def sim_after(steps):
+        sim = LFM_Diff_Simulator(*n, sc["dx"], 0.5 * sc["dx"], R, None, total_steps=steps, tol=1e-7, max_iters=200)
+        sim.init(u0=sc["u0"], init_passive_np=sc["density"])
+        for _ in range(steps):
+            sim.step_midpoint()
+        return sim
+
+    sim = sim_after(R)
+    sim.passive[0][1, 2, 3] = float("nan")
+    with pytest.raises(L.DfmError) as e:
+        sim.init_gradient(target=sc["target"])
+    assert e.value.code == L.DFM_E_NONFINITE
+
+    sim = sim_after(1)
+    sim.u.c[0][4, 4, 4] = float("inf")
+    with pytest.raises(L.DfmError) as e:
+        sim.step_midpoint()
+    assert e.value.code == L.DFM_E_NONFINITE
+
+    sim = sim_after(R)
+    tgt = np.array(sc["target"], dtype=np.float64)
+    tgt[2, 2, 2] = np.nan
+    with pytest.raises(L.DfmError) as e:
+        sim.init_gradient(target=tgt)
+        for _ in range(R):
+            sim.backtrack_step_midpoint()
+    assert e.value.code == L.DFM_E_NONFINITE
