@@ -319,6 +319,73 @@ template <int K> DFM_HD float tap_dw(float t, int a) {
   return Spline<K>::dw(t + (float)(Spline<K>::REF_OFF - a));
 }
 
+// x set-up of a quadratic sample: support base bx and offset tx (the x
+// weights are formed per slice from tx, tap_w, rather than held in arrays).
+DFM_HD void xsetup2(float p_over_dx, float bl, int S, int& bx, float& tx) {
+  const float sx = clamp_coord<2>(p_over_dx, bl, S);
+  bx = support_base<2>(sx, S);
+  tx = sx - (float)(bx + Spline<2>::REF_OFF);
+}
+
+// Quadratic 3D sample (and optionally gradient) of field f with planes of
+// S1 x S2 values, from its axis set-ups: x as (bx, tx), y and z as Axis.
+// All 27 taps (9 rows of 3, each one vector load) are requested before the
+// first is used, one memory round trip per sample; then the taps are
+// contracted z, then y, then x.  Quadratic taps are unclamped (the support
+// base is), so tap (a, b, c) sits at o00 + (a S1 + b) S2 + c.  o00 is one
+// 32-bit offset formed with 24-bit multiplies (a plane holds fewer than 2^24
+// values, a field fewer than 2^30); each row load adds a wave-uniform row
+// stride to it in 32 bits.
+template <bool GRAD>
+DFM_HD float interp2_3d(const float* __restrict__ f, int S1i, int S2i, int bx, float tx, const Axis<2, GRAD>& ay,
+                        const Axis<2, GRAD>& az, float inv_dx, float g[3]) {
+  const uint32_t S1 = (uint32_t)S1i, S2 = (uint32_t)S2i;
+  const uint32_t o00 = __umul24(__umul24((uint32_t)bx, S1) + (uint32_t)ay.idx[0], S2) + (uint32_t)az.idx[0];
+  float tv[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const float* row = at32b(f, (o00 << 2) + (((uint32_t)a * S1 + (uint32_t)b) * S2 << 2));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tv[a][b][c] = row[c];
+    }
+  }
+  float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float wxa = tap_w<2>(tx, a);
+    float yv = 0.f, yd = 0.f, yz = 0.f;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      float zv = 0.f, zd = 0.f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float val = tv[a][b][c];
+        zv = fmaf(val, az.w[c], zv);
+        if (GRAD) zd = fmaf(val, az.dw[c], zd);
+      }
+      yv = fmaf(zv, ay.w[b], yv);
+      if (GRAD) {
+        yd = fmaf(zv, ay.dw[b], yd);
+        yz = fmaf(zd, ay.w[b], yz);
+      }
+    }
+    v = fmaf(yv, wxa, v);
+    if (GRAD) {
+      gx = fmaf(yv, tap_dw<2>(tx, a), gx);
+      gy = fmaf(yd, wxa, gy);
+      gz = fmaf(yz, wxa, gz);
+    }
+  }
+  if (GRAD) {
+    g[0] = gx * inv_dx;
+    g[1] = gy * inv_dx;
+    g[2] = gz * inv_dx;
+  }
+  return v;
+}
+
 // Value (and optionally the full gradient) of the B-spline interpolant of
 // field f (shape S, node offset bl) at physical point p.  Separable: the z
 // taps are contracted first, then y, then x.  Row addresses are signed, so
@@ -358,72 +425,55 @@ DFM_HD float interp(const float* __restrict__ f, const int S[3], const float bl[
     }
     return v;
   } else {
-    // the x weights are formed per slice (tap_w) rather than held in arrays
     Axis<K, GRAD> az;
     az.setup(p[2] * inv_dx, bl[2], S[2]);
-    const float sx = clamp_coord<K>(p[0] * inv_dx, bl[0], S[0]);
-    const int bx = support_base<K>(sx, S[0]);
-    const float tx = sx - (float)(bx + Spline<K>::REF_OFF);
-    float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-    // Quadratic: all 27 taps (9 rows of 3, each one vector load) are
-    // requested before the first is used, one memory round trip per sample
-    // instead of one per x slice; the multiply-adds keep their order.  The
-    // wider splines keep one slice in flight at a time (register pressure).
-    // Quadratic taps are unclamped (the support base is), so tap (a, b, c)
-    // sits at o00 + (a S1 + b) S2 + c.  o00 is one 32-bit offset formed with
-    // 24-bit multiplies (a plane holds fewer than 2^24 values, a field fewer
-    // than 2^30); each of the 9 row loads (3 consecutive z taps) adds a
-    // wave-uniform row stride to it in 32 bits.
-    float tv[K == 2 ? NT : 1][K == 2 ? NT : 1][K == 2 ? NT : 1];
     if constexpr (K == 2) {
-      const uint32_t S1 = (uint32_t)S[1], S2 = (uint32_t)S[2];
-      const uint32_t o00 = __umul24(__umul24((uint32_t)bx, S1) + (uint32_t)ay.idx[0], S2) + (uint32_t)az.idx[0];
+      int bx;
+      float tx;
+      xsetup2(p[0] * inv_dx, bl[0], S[0], bx, tx);
+      return interp2_3d<GRAD>(f, S[1], S[2], bx, tx, ay, az, inv_dx, g);
+    } else {
+      // the x weights are formed per slice (tap_w) rather than held in
+      // arrays; one slice of taps in flight at a time (register pressure)
+      const float sx = clamp_coord<K>(p[0] * inv_dx, bl[0], S[0]);
+      const int bx = support_base<K>(sx, S[0]);
+      const float tx = sx - (float)(bx + Spline<K>::REF_OFF);
+      float v = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
       for (int a = 0; a < NT; ++a) {
+        const float wxa = tap_w<K>(tx, a);
+        const int ia = tap_index<K>(bx, a, S[0]);
+        float yv = 0.f, yd = 0.f, yz = 0.f;
 #pragma unroll
         for (int b = 0; b < NT; ++b) {
-          const float* row = at32b(f, (o00 << 2) + (((uint32_t)a * S1 + (uint32_t)b) * S2 << 2));
+          const float* row = f + ((int64_t)ia * S[1] + ay.idx[b]) * S[2];
+          float zv = 0.f, zd = 0.f;
 #pragma unroll
-          for (int c = 0; c < NT; ++c) tv[a][b][c] = row[c];
+          for (int c = 0; c < NT; ++c) {
+            const float val = row[az.idx[c]];
+            zv = fmaf(val, az.w[c], zv);
+            if (GRAD) zd = fmaf(val, az.dw[c], zd);
+          }
+          yv = fmaf(zv, ay.w[b], yv);
+          if (GRAD) {
+            yd = fmaf(zv, ay.dw[b], yd);
+            yz = fmaf(zd, ay.w[b], yz);
+          }
         }
-      }
-    }
-#pragma unroll
-    for (int a = 0; a < NT; ++a) {
-      const float wxa = tap_w<K>(tx, a);
-      const int ia = tap_index<K>(bx, a, S[0]);
-      float yv = 0.f, yd = 0.f, yz = 0.f;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) {
-        const float* row = f + ((int64_t)ia * S[1] + ay.idx[b]) * S[2];
-        float zv = 0.f, zd = 0.f;
-#pragma unroll
-        for (int c = 0; c < NT; ++c) {
-          float val;
-          if constexpr (K == 2) val = tv[a][b][c];
-          else val = row[az.idx[c]];
-          zv = fmaf(val, az.w[c], zv);
-          if (GRAD) zd = fmaf(val, az.dw[c], zd);
-        }
-        yv = fmaf(zv, ay.w[b], yv);
+        v = fmaf(yv, wxa, v);
         if (GRAD) {
-          yd = fmaf(zv, ay.dw[b], yd);
-          yz = fmaf(zd, ay.w[b], yz);
+          gx = fmaf(yv, tap_dw<K>(tx, a), gx);
+          gy = fmaf(yd, wxa, gy);
+          gz = fmaf(yz, wxa, gz);
         }
       }
-      v = fmaf(yv, wxa, v);
       if (GRAD) {
-        gx = fmaf(yv, tap_dw<K>(tx, a), gx);
-        gy = fmaf(yd, wxa, gy);
-        gz = fmaf(yz, wxa, gz);
+        g[0] = gx * inv_dx;
+        g[1] = gy * inv_dx;
+        g[2] = gz * inv_dx;
       }
+      return v;
     }
-    if (GRAD) {
-      g[0] = gx * inv_dx;
-      g[1] = gy * inv_dx;
-      g[2] = gz * inv_dx;
-    }
-    return v;
   }
 }
 
@@ -456,6 +506,35 @@ DFM_HD void mac_eval(const Geo& g, const Mac& u, const float p[3], float val[3],
       if (GRAD) { J[2][0] = gr[0]; J[2][1] = gr[1]; J[2][2] = gr[2]; }
     }
   }
+}
+
+// mac_eval<GRAD, 3> (quadratic) with the components sharing their axis
+// set-ups: along axis b, component b samples at offset 0 on n[b] + 1 nodes
+// and the other two at offset 1/2 on n[b] nodes, so 6 set-ups serve the 9
+// (component, axis) pairs.  Each component's arithmetic is interp<2, GRAD,
+// 3>'s, so the results are bitwise mac_eval's.  The components are unrolled,
+// which takes more registers than mac_eval's rolled loop: it serves the
+// kernels whose per-point work is these samples (march, back-trace).
+template <bool GRAD>
+DFM_HD void mac_eval_shared(const Geo& g, const Mac& u, const float p[3], float val[3], float J[3][3]) {
+  const float px = p[0] * g.inv_dx, py = p[1] * g.inv_dx, pz = p[2] * g.inv_dx;
+  const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  int bxN, bxC;
+  float txN, txC;
+  xsetup2(px, 0.0f, n0 + 1, bxN, txN);
+  xsetup2(px, 0.5f, n0, bxC, txC);
+  Axis<2, GRAD> yN, yC, zN, zC;
+  yN.setup(py, 0.0f, n1 + 1);
+  yC.setup(py, 0.5f, n1);
+  zN.setup(pz, 0.0f, n2 + 1);
+  zC.setup(pz, 0.5f, n2);
+  float gr[3] = {0.f, 0.f, 0.f};
+  val[0] = interp2_3d<GRAD>(u.c[0], n1, n2, bxN, txN, yC, zC, g.inv_dx, gr);
+  if (GRAD) { J[0][0] = gr[0]; J[0][1] = gr[1]; J[0][2] = gr[2]; }
+  val[1] = interp2_3d<GRAD>(u.c[1], n1 + 1, n2, bxC, txC, yN, zC, g.inv_dx, gr);
+  if (GRAD) { J[1][0] = gr[0]; J[1][1] = gr[1]; J[1][2] = gr[2]; }
+  val[2] = interp2_3d<GRAD>(u.c[2], n1, n2 + 1, bxC, txC, yC, zN, g.inv_dx, gr);
+  if (GRAD) { J[2][0] = gr[0]; J[2][1] = gr[1]; J[2][2] = gr[2]; }
 }
 
 }  // namespace dfm

@@ -7,6 +7,14 @@
 
 namespace dfm {
 
+// Velocity sample of the march and back-trace: in 3D the components share
+// their axis set-ups (mac_eval_shared, bitwise mac_eval).
+template <bool GRAD, int DIM>
+DFM_HD void vel_sample(const Geo& g, const Mac& u, const float p[3], float val[3], float J[3][3]) {
+  if constexpr (DIM == 3) mac_eval_shared<GRAD>(g, u, p, val, J);
+  else mac_eval<GRAD, DIM>(g, u, p, val, J);
+}
+
 // RK2 (midpoint) / RK4 march of point p and, with HAS_T, its Jacobian column
 // T through velocity u over dt: p += -dt u(p), T += -dt (grad u)(p) T
 // (RK_grid / RK_grid_only_psi, 2D/vortex/flowmap.py:125-215).
@@ -24,14 +32,14 @@ DFM_HD void rk_point(const Geo& g, const Mac& u, float dt, float p[3], float T[3
       out[a] = s;
     }
   };
-  mac_eval<HAS_T, DIM>(g, u, p, u1, J);
+  vel_sample<HAS_T, DIM>(g, u, p, u1, J);
   if (HAS_T) JT(T, k1);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     q[a] = p[a] + 0.5f * neg * u1[a];
     if (HAS_T) Tq[a] = T[a] + 0.5f * neg * k1[a];
   }
-  mac_eval<HAS_T, DIM>(g, u, q, u2, J);
+  vel_sample<HAS_T, DIM>(g, u, q, u2, J);
   if (HAS_T) JT(Tq, k2);
   if (ORDER == 2) {
 #pragma unroll
@@ -46,14 +54,14 @@ DFM_HD void rk_point(const Geo& g, const Mac& u, float dt, float p[3], float T[3
     q[a] = p[a] + 0.5f * neg * u2[a];
     if (HAS_T) Tq[a] = T[a] + 0.5f * neg * k2[a];
   }
-  mac_eval<HAS_T, DIM>(g, u, q, u3, J);
+  vel_sample<HAS_T, DIM>(g, u, q, u3, J);
   if (HAS_T) JT(Tq, k3);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     q[a] = p[a] + neg * u3[a];
     if (HAS_T) Tq[a] = T[a] + neg * k3[a];
   }
-  mac_eval<HAS_T, DIM>(g, u, q, u4, J);
+  vel_sample<HAS_T, DIM>(g, u, q, u4, J);
   if (HAS_T) JT(Tq, k4);
   const float sixth = 1.0f / 6.0f;
 #pragma unroll
@@ -68,10 +76,10 @@ DFM_HD void rk_point(const Geo& g, const Mac& u, float dt, float p[3], float T[3
 template <int DIM, int ORDER>
 DFM_HD void backtrace(const Geo& g, const Mac& vel, float dt, const float x[3], float p[3]) {
   float v1[3], v2[3], v3[3], v4[3], q[3];
-  mac_eval<false, DIM>(g, vel, x, v1, nullptr);
+  vel_sample<false, DIM>(g, vel, x, v1, nullptr);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) q[a] = x[a] - v1[a] * dt * 0.5f;
-  mac_eval<false, DIM>(g, vel, q, v2, nullptr);
+  vel_sample<false, DIM>(g, vel, q, v2, nullptr);
   if (ORDER == 2) {
 #pragma unroll
     for (int a = 0; a < DIM; ++a) p[a] = x[a] - v2[a] * dt;
@@ -79,10 +87,10 @@ DFM_HD void backtrace(const Geo& g, const Mac& vel, float dt, const float x[3], 
   }
 #pragma unroll
   for (int a = 0; a < DIM; ++a) q[a] = x[a] - v2[a] * dt * 0.5f;
-  mac_eval<false, DIM>(g, vel, q, v3, nullptr);
+  vel_sample<false, DIM>(g, vel, q, v3, nullptr);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) q[a] = x[a] - v3[a] * dt;
-  mac_eval<false, DIM>(g, vel, q, v4, nullptr);
+  vel_sample<false, DIM>(g, vel, q, v4, nullptr);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) p[a] = x[a] - (v1[a] + 2.0f * v2[a] + 2.0f * v3[a] + v4[a]) / 6.0f * dt;
 }
