@@ -1135,6 +1135,216 @@ __global__ void __launch_bounds__(kB) k_smooth(Dims d, float* __restrict__ r, fl
   if (ZR) grid_reduce(R, op, acc, 0.0);
 }
 
+// ------------------------------------------------ level-0 row kernels ----
+// The colour-split finest level in 3D with an innermost extent that is a
+// multiple of 8.  The per-pair kernels above spend ~135 VALU instructions
+// per cell on index arithmetic, bounds tests and 64-bit addresses, which
+// makes them VALU-bound at 256^3.  Here a thread owns half indices h0 ..
+// h0 + 3 of one row (i, j) of colour `phase`, so the arithmetic is done once
+// per 4 cells and every row is read with 16-byte loads.  The thread's cells
+// sit at natural k = 2h + off, off = (phase - i - j) & 1.  Their x and y
+// neighbours are the other colour's cells of the same half indices in rows
+// (i +- 1, j) and (i, j +- 1); their -z and +z neighbours are the other
+// colour's half indices h - 1 + off and h + off of the own row, i.e. the
+// centre row's 4 values plus one beyond the segment (h0 - 1 when off = 0,
+// h0 + 4 when off = 1).  Coarse parents follow the same pattern: the parent
+// of natural k is k >> 1 = h for the x and y neighbours, h - 1 + off and
+// h + off for the z neighbours.  Each cell is formed with the per-pair
+// kernels' operations in their order, so the smoothed values are bitwise
+// theirs; the z.r sums add the same terms in another order.
+constexpr int kRowV = 4;
+
+struct F4 {
+  float v[kRowV];
+};
+DFM_HD F4 ld4(const float* __restrict__ p, uint32_t i) {
+  const float4 q = *reinterpret_cast<const float4*>(p + i);
+  return F4{{q.x, q.y, q.z, q.w}};
+}
+DFM_HD void st4(float* __restrict__ p, uint32_t i, const F4& a) {
+  *reinterpret_cast<float4*>(p + i) = make_float4(a.v[0], a.v[1], a.v[2], a.v[3]);
+}
+
+// A thread's row segment.
+struct RowSeg {
+  int i, j, h0, off;
+  uint32_t rb;    // half index of the row's first pair
+  unsigned nbr;   // bits 0..3: the -x, +x, -y, +y neighbour rows exist
+  bool zm0, zp3;  // cell 0 has a -z neighbour, cell 3 a +z neighbour
+};
+
+DFM_HD RowSeg row_seg(const Dims& d, uint32_t t, int phase) {
+  RowSeg f;
+  const uint32_t half = d.fh.d;
+  const uint32_t p0 = t * kRowV;
+  const uint32_t row = fdiv(p0, d.fh);
+  const uint32_t q = fdiv(row, d.f1);
+  f.i = (int)q;
+  f.j = (int)(row - q * (uint32_t)d.n[1]);
+  f.h0 = (int)(p0 - row * half);
+  f.rb = row * half;
+  f.off = (phase - f.i - f.j) & 1;
+  f.nbr = (f.i > 0 ? 1u : 0u) | (f.i < d.n[0] - 1 ? 2u : 0u) | (f.j > 0 ? 4u : 0u) | (f.j < d.n[1] - 1 ? 8u : 0u);
+  f.zm0 = f.off == 1 || f.h0 > 0;
+  f.zp3 = f.off == 0 || f.h0 + kRowV < (int)half;
+  return f;
+}
+
+// Threads of a row launch: one per kRowV pairs.
+__host__ __device__ inline uint32_t row_threads(const Dims& d) { return (uint32_t)d.n[0] * (uint32_t)d.n[1] * (d.fh.d / kRowV); }
+
+// The 6 neighbour rows of a segment in array f (x, y rows of the same half
+// indices; the own row's 4 values and the one beyond), with c the segment's
+// first index in the other colour's half and (sx, sy) the row strides.  An
+// absent row reads the centre row (its values are not used).
+template <typename T>
+struct Nbr6 {
+  F4 xm, xp, ym, yp, c;
+  float e;
+};
+template <typename T>
+DFM_HD Nbr6<T> nbr6(const T* __restrict__ f, const RowSeg& g, uint32_t c, uint32_t sx, uint32_t sy) {
+  Nbr6<T> n;
+  n.xm = ld4(f, g.nbr & 1u ? c - sx : c);
+  n.xp = ld4(f, g.nbr & 2u ? c + sx : c);
+  n.ym = ld4(f, g.nbr & 4u ? c - sy : c);
+  n.yp = ld4(f, g.nbr & 8u ? c + sy : c);
+  n.c = ld4(f, c);
+  const bool ext = g.off ? g.zp3 : g.zm0;
+  n.e = Sto<T>::ld(f, ext ? (g.off ? c + kRowV : c - 1u) : c);
+  return n;
+}
+// the -z / +z neighbour value of cell v
+template <typename T>
+DFM_HD float zm_of(const Nbr6<T>& n, int off, int v) {
+  return off ? n.c.v[v] : (v == 0 ? n.e : n.c.v[v - 1]);
+}
+template <typename T>
+DFM_HD float zp_of(const Nbr6<T>& n, int off, int v) {
+  return off ? (v == kRowV - 1 ? n.e : n.c.v[v + 1]) : n.c.v[v];
+}
+
+// Gauss-Seidel values of the segment's 4 cells (gs_value's operations: the
+// neighbours summed -x, +x, -y, +y, -z, +z, each as z + parent correction
+// with ZC, then (r - m + sum) / nn).
+template <bool ZC, typename ZCT>
+DFM_HD F4 row_gs(const Dims& d, const RowSeg& g, const F4& rv, float m, const Nbr6<float>& z, const Nbr6<ZCT>& pc) {
+  F4 out;
+#pragma unroll
+  for (int v = 0; v < kRowV; ++v) {
+    float sum = 0.f, nn = 0.f;
+    if (g.nbr & 1u) { sum += ZC ? z.xm.v[v] + pc.xm.v[v] : z.xm.v[v]; nn += 1.f; }
+    if (g.nbr & 2u) { sum += ZC ? z.xp.v[v] + pc.xp.v[v] : z.xp.v[v]; nn += 1.f; }
+    if (g.nbr & 4u) { sum += ZC ? z.ym.v[v] + pc.ym.v[v] : z.ym.v[v]; nn += 1.f; }
+    if (g.nbr & 8u) { sum += ZC ? z.yp.v[v] + pc.yp.v[v] : z.yp.v[v]; nn += 1.f; }
+    if (v > 0 || g.zm0) {
+      const float a = zm_of(z, g.off, v);
+      sum += ZC ? a + zm_of(pc, g.off, v) : a;
+      nn += 1.f;
+    }
+    if (v < kRowV - 1 || g.zp3) {
+      const float a = zp_of(z, g.off, v);
+      sum += ZC ? a + zp_of(pc, g.off, v) : a;
+      nn += 1.f;
+    }
+    out.v[v] = (rv.v[v] - m + sum) * inv_count(nn);
+  }
+  return out;
+}
+
+// The coarse parents of a segment's neighbours: the parent rows of the x and
+// y neighbour rows and of the own row, in the coarse level's natural order
+// (whose innermost extent is the fine level's pairs per row).
+template <typename ZCT>
+DFM_HD Nbr6<ZCT> parents6(const ZCT* __restrict__ zc, const Dims& dc, const RowSeg& g) {
+  auto row = [&](int ci, int cj) { return ((uint32_t)ci * (uint32_t)dc.n[1] + (uint32_t)cj) * (uint32_t)dc.n[2]; };
+  const uint32_t h = (uint32_t)g.h0, own = row(g.i >> 1, g.j >> 1) + h;
+  Nbr6<ZCT> n;
+  n.xm = ld4(zc, g.nbr & 1u ? row((g.i - 1) >> 1, g.j >> 1) + h : own);
+  n.xp = ld4(zc, g.nbr & 2u ? row((g.i + 1) >> 1, g.j >> 1) + h : own);
+  n.ym = ld4(zc, g.nbr & 4u ? row(g.i >> 1, (g.j - 1) >> 1) + h : own);
+  n.yp = ld4(zc, g.nbr & 8u ? row(g.i >> 1, (g.j + 1) >> 1) + h : own);
+  n.c = ld4(zc, own);
+  const bool ext = g.off ? g.zp3 : g.zm0;
+  n.e = Sto<ZCT>::ld(zc, ext ? (g.off ? own + kRowV : own - 1u) : own);
+  return n;
+}
+
+// One level-0 half-sweep of colour `phase` (MODE 1; MODE 2 with the coarse
+// correction zc added to the neighbour values, the fused prolongation), or
+// the first half-sweep from z = 0 (MODE 0: (r - m) / nn, k_first_half).
+template <int MODE, typename ZCT = float>
+__global__ void __launch_bounds__(kB) k_row_half(Dims d, const float* __restrict__ r, float* __restrict__ z,
+                                                 int phase, const double* shift, const ZCT* __restrict__ zc,
+                                                 Dims dc, const int* flags) {
+#pragma clang fp contract(off)
+  if (is_done(flags)) return;
+  const uint32_t t = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  if (t >= row_threads(d)) return;
+  const float m = shift ? (float)*shift : 0.f;
+  const RowSeg g = row_seg(d, t, phase);
+  const uint32_t own = (phase ? d.hh : 0u) + g.rb + (uint32_t)g.h0;
+  const F4 rv = ld4(r, own);
+  F4 out;
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) {
+      const int k = 2 * (g.h0 + v) + g.off;
+      float nn = 0.f;
+      nn += (g.i > 0) + (g.i < d.n[0] - 1) + (g.j > 0) + (g.j < d.n[1] - 1);
+      nn += (k > 0) + (k < d.n[2] - 1);
+      out.v[v] = (rv.v[v] - m) * inv_count(nn);
+    }
+  } else {
+    const uint32_t c = (phase ? 0u : d.hh) + g.rb + (uint32_t)g.h0;
+    const uint32_t sy = d.fh.d, sx = (uint32_t)d.n[1] * sy;
+    const Nbr6<float> zn = nbr6(z, g, c, sx, sy);
+    if constexpr (MODE == 2) {
+      const Nbr6<ZCT> pc = parents6(zc, dc, g);
+      out = row_gs<true>(d, g, rv, m, zn, pc);
+    } else {
+      out = row_gs<false, float>(d, g, rv, m, zn, zn);
+    }
+  }
+  st4(z, own, out);
+}
+
+// The last level-0 half-sweep of a V-cycle fused with z.r (k_smooth<3, true,
+// false, true>): the segment's cells are smoothed, and z.r is summed over
+// them and the other colour's cells of the same pairs (whose z is final),
+// own cell first in each pair; p = z for both when p is given.  Then the
+// scalar step `op` runs on the total.
+__global__ void __launch_bounds__(kB) k_row_zr(Dims d, const float* __restrict__ r, float* __restrict__ z, int phase,
+                                               const double* shift, Red R, int op, float* __restrict__ p) {
+  if (is_done(R.flags)) return;
+  const float m = shift ? (float)*shift : 0.f;
+  double acc = 0;
+  const SlabWalk<uint32_t> w = slab_walk<kB>(row_threads(d));
+  const uint32_t sy = d.fh.d, sx = (uint32_t)d.n[1] * sy;
+  for (uint32_t t = w.first; t < w.end; t += w.stride) {
+    const RowSeg g = row_seg(d, t, phase);
+    const uint32_t own = (phase ? d.hh : 0u) + g.rb + (uint32_t)g.h0;
+    const uint32_t c = (phase ? 0u : d.hh) + g.rb + (uint32_t)g.h0;
+    const F4 rv = ld4(r, own);
+    const F4 ro = ld4(r, c);
+    const Nbr6<float> zn = nbr6(z, g, c, sx, sy);
+    const F4 zo = row_gs<false, float>(d, g, rv, m, zn, zn);
+    st4(z, own, zo);
+    if (p) {
+      st4(p, own, zo);
+      st4(p, c, zn.c);
+    }
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) {
+      const float rvo = rv.v[v] - m;
+      acc += (double)zo.v[v] * (double)rvo;
+      const float rvt = ro.v[v] - m;
+      acc += (double)zn.c.v[v] * (double)rvt;
+    }
+  }
+  grid_reduce(R, op, acc, 0.0);
+}
+
 // One full RBGS sweep (colour c1, then the other) as a single launch, for the
 // levels whose half-sweeps are launch-latency bound.  The second colour's
 // update needs the first colour's new values at its neighbours; each such
@@ -1755,6 +1965,7 @@ struct Solver {
   int fused_sweeps;  // option: one launch per RBGS sweep on levels with z2
   int tiled_sweeps;  // option: LDS-tiled pre/post smoothing on eligible coarse levels
   int split_opt;     // option: level 0 colour-split where eligible (set_layout)
+  int row_opt;       // option: the colour-split level's row kernels where eligible
   double* xn;        // natural-order copy of a colour-split pressure (on request)
   hipStream_t last_st;  // stream of the last solve
   int ready;         // partials, counters, control block and x zeroed (ensure_ready)
@@ -1866,10 +2077,32 @@ struct ZrFuse {
 // On return z names the buffer holding the result.  Returns whether zr was done.
 // One global half-sweep of colour c (no z.r): several pairs per thread
 // without an obstacle mask, one per thread with it.
+// Whether the colour-split level with dims d runs the row kernels (3D, an
+// innermost extent that is a multiple of 8; with a coarse correction, a
+// coarse innermost extent of half the fine one).
+inline bool row_ok(const Solver& s, const Dims& d, const Dims* dc = nullptr) {
+  return s.row_opt && d.hh != 0 && d.n[2] % (2 * kRowV) == 0 && (!dc || 2 * dc->n[2] == d.n[2]);
+}
+
+inline unsigned row_grid(const Dims& d) { return (row_threads(d) + kB - 1) / kB; }
+
 template <int DIM, bool MASK, bool SPLIT>
 void half_sweep(Solver& s, hipStream_t st, const Dims& d, float* r, float* z, int c, const double* shift,
                 const float* zc, const Dims& dc, const Red& R, bool zc_bf16 = false) {
   const uint32_t pairs = pairs_of(d, DIM);
+  if constexpr (SPLIT && DIM == 3 && !MASK) {
+    // (the finest level's coarse correction, level 1, is never stored in BF16)
+    if (!zc_bf16 && row_ok(s, d, zc ? &dc : nullptr)) {
+      const dim3 g(row_grid(d));
+      if (!zc)
+        hipLaunchKernelGGL((k_row_half<1>), g, dim3(kB), 0, st, d, (const float*)r, z, c, shift,
+                           (const float*)nullptr, dc, (const int*)s.flags);
+      else
+        hipLaunchKernelGGL((k_row_half<2>), g, dim3(kB), 0, st, d, (const float*)r, z, c, shift, zc, dc,
+                           (const int*)s.flags);
+      return;
+    }
+  }
   if constexpr (!MASK) {
     const unsigned g = (unsigned)((pairs + (uint32_t)(kB * kSweepU) - 1) / (uint32_t)(kB * kSweepU));
     if (zc_bf16)
@@ -1904,8 +2137,12 @@ bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool 
       // colour c1 from zero, then an ordinary half-sweep of the other colour
       // (k_first_sweep's result bitwise)
       const unsigned g = (unsigned)((pairs_of(d, DIM) + (uint32_t)(kB * kSweepU) - 1) / (uint32_t)(kB * kSweepU));
-      hipLaunchKernelGGL((k_first_half<DIM, SPLIT>), dim3(g), dim3(kB), 0, st, d, (const float*)L.r, z, c1, shift,
-                         (const int*)s.flags);
+      if (SPLIT && DIM == 3 && row_ok(s, d))
+        hipLaunchKernelGGL((k_row_half<0>), dim3(row_grid(d)), dim3(kB), 0, st, d, (const float*)L.r, z, c1, shift,
+                           (const float*)nullptr, dc, (const int*)s.flags);
+      else
+        hipLaunchKernelGGL((k_first_half<DIM, SPLIT>), dim3(g), dim3(kB), 0, st, d, (const float*)L.r, z, c1, shift,
+                           (const int*)s.flags);
       half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, 1 - c1, shift, (const float*)nullptr, dc, R);
     }
     k0 = 1;
@@ -1929,8 +2166,12 @@ bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool 
     }
     half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, c1, shift, zc, dc, R, zc != nullptr && zc_bf16);
     if (zr && it == sweeps - 1) {
-      hipLaunchKernelGGL((k_smooth<DIM, true, MASK, SPLIT>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, z, 1 - c1,
-                         shift, (const float*)nullptr, dc, R, zr->op, zr->p);
+      if (SPLIT && DIM == 3 && !MASK && row_ok(s, d))
+        hipLaunchKernelGGL(k_row_zr, dim3(gridr(row_threads(d))), dim3(kB), 0, st, d, (const float*)L.r, z, 1 - c1,
+                           shift, R, zr->op, zr->p);
+      else
+        hipLaunchKernelGGL((k_smooth<DIM, true, MASK, SPLIT>), dim3(gridr(pairs)), dim3(kB), 0, st, d, L.r, z,
+                           1 - c1, shift, (const float*)nullptr, dc, R, zr->op, zr->p);
       zr_done = true;
     } else {
       half_sweep<DIM, MASK, SPLIT>(s, st, d, L.r, z, 1 - c1, shift, (const float*)nullptr, dc, R);
@@ -2438,6 +2679,7 @@ extern "C" int dfm_mgpcg_create(const dfm_grid* g, int levels, int pre_post, int
   s->max_refine = 3;
   s->use_graph = 1;
   s->fused_sweeps = 1;
+  s->row_opt = 1;
   s->tiled_sweeps = 1;
   s->split_opt = 1;
   // Allocation only (ensure_ready zeroes on the first call that launches
@@ -2537,6 +2779,10 @@ extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
       s->split_opt = value ? 1 : 0;
       drop_graph(*s);
       return DFM_OK;
+    case DFM_MG_ROW_KERNELS:
+      s->row_opt = value ? 1 : 0;
+      drop_graph(*s);
+      return DFM_OK;
     default:
       return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
@@ -2557,6 +2803,7 @@ extern "C" int dfm_mgpcg_get_option(void* handle, int option, int* value) {
     case DFM_MG_SPLIT_LAYOUT: *value = split_eligible(*s) ? 1 : 0; return DFM_OK;
     case DFM_MG_BF16_LEVEL: *value = bf16_first_level(*s); return DFM_OK;
     case DFM_MG_LEVELS: *value = s->L; return DFM_OK;
+    case DFM_MG_ROW_KERNELS: *value = s->row_opt; return DFM_OK;
     default: return set_error(DFM_E_ARG, "unknown MGPCG option");
   }
 }

@@ -119,6 +119,7 @@ MG_FUSED_SWEEPS = 8
 MG_SPLIT_LAYOUT = 9
 MG_BF16_LEVEL = 10
 MG_LEVELS = 11
+MG_ROW_KERNELS = 12
 
 # dfm_batch fields and options
 BATCH_U, BATCH_S, BATCH_TARGET, BATCH_ADJ_U, BATCH_ADJ_S = 0, 1, 2, 3, 4

@@ -495,6 +495,41 @@ def test_mgpcg_split_layout_matches_natural(n, levels, device):
     within("xs", rel_l2(xs, xn), 1e-5)
 
 
+@pytest.mark.parametrize("n,levels,bf16", [((64, 48, 32), 3, False), ((40, 36, 24), 3, False),
+                                           ((128, 128, 128), 5, True), ((40, 36, 20), 3, False)])
+def test_mgpcg_row_kernels_match_pair_kernels(n, levels, bf16, device):
+    """The finest colour-split level's row kernels (4 cells of a row and
+    colour per thread) give the per-pair kernels' projection up to rounding
+    (the smoothed values are the same; the z.r sums add in another order),
+    within one iteration, and the exact projection.  (40, 36, 20) has an
+    innermost extent that is not a multiple of 8: the option is on but the
+    pair kernels run, so the two solves are bitwise equal."""
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    rng = np.random.default_rng(31 + n[0] + n[2])
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(3)])
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    got = {}
+    for rows in (1, 0):
+        s = MGPCG(n, 1.0 / n[1], levels=levels, tol=1e-7, max_iters=200, bf16_coarse=bf16)
+        assert s.get_option(L.MG_ROW_KERNELS) == 1
+        s.set_option(L.MG_ROW_KERNELS, rows)
+        assert s.get_option(L.MG_ROW_KERNELS) == rows
+        d = mac_to_dev(u, device)
+        s.Poisson(d)
+        got[rows] = ([c.clone() for c in d.c], s.status()[0])
+        within("row-kernel projection vs exact", rel_l2_fields(mac_to_np(d), ref), 2e-5 if bf16 else 2e-6)
+    (fr, itr), (fp, itp) = got[1], got[0]
+    assert abs(itr - itp) <= 1
+    if n[2] % 8:
+        assert itr == itp and all(torch.equal(a, b) for a, b in zip(fr, fp))
+    else:
+        within("row vs pair kernels", rel_l2_fields([c.cpu().numpy() for c in fr], [c.cpu().numpy() for c in fp]),
+               1e-6)
+
+
 def test_mgpcg_first_solve_on_side_stream(device):
     """Creating a handle only allocates.  Its reduction counters, control
     block and x are zeroed on the stream of the first call that launches
