@@ -1345,6 +1345,117 @@ __global__ void __launch_bounds__(kB) k_row_zr(Dims d, const float* __restrict__
   grid_reduce(R, op, acc, 0.0);
 }
 
+// The same segment of the other colour (cells at natural k = 2h + 1 - off).
+DFM_HD RowSeg other_colour(const Dims& d, const RowSeg& g) {
+  RowSeg f = g;
+  f.off = 1 - g.off;
+  f.zm0 = f.off == 1 || f.h0 > 0;
+  f.zp3 = f.off == 0 || f.h0 + kRowV < (int)d.fh.d;
+  return f;
+}
+
+// A p of the segment's 4 cells of one colour (ap_value: the neighbour values
+// summed in FP64, -x, +x, -y, +y, -z, +z, an absent one as 0; nn pc - sum).
+DFM_HD F4 row_Ap(const RowSeg& g, const F4& pc, const Nbr6<float>& n) {
+  F4 ap;
+#pragma unroll
+  for (int v = 0; v < kRowV; ++v) {
+    const bool zm = v > 0 || g.zm0, zp = v < kRowV - 1 || g.zp3;
+    double sum = 0.0;
+    sum += (double)((g.nbr & 1u) ? n.xm.v[v] : 0.0f);
+    sum += (double)((g.nbr & 2u) ? n.xp.v[v] : 0.0f);
+    sum += (double)((g.nbr & 4u) ? n.ym.v[v] : 0.0f);
+    sum += (double)((g.nbr & 8u) ? n.yp.v[v] : 0.0f);
+    sum += (double)(zm ? zm_of(n, g.off, v) : 0.0f);
+    sum += (double)(zp ? zp_of(n, g.off, v) : 0.0f);
+    const double nn = (double)(__popc(g.nbr) + (zm ? 1 : 0) + (zp ? 1 : 0));
+    ap.v[v] = (float)(nn * (double)pc.v[v] - sum);
+  }
+  return ap;
+}
+
+// p.Ap over the colour-split finest level (k_Ap_dot): a thread walks row
+// segments, both colours of each (8 cells).
+__global__ void __launch_bounds__(kB) k_row_Ap_dot(Dims d, const float* __restrict__ p, Red R) {
+  if (is_done(R.flags)) return;
+  double acc = 0;
+  const SlabWalk<uint32_t> w = slab_walk<kB>(row_threads(d));
+  const uint32_t sy = d.fh.d, sx = (uint32_t)d.n[1] * sy;
+  for (uint32_t t = w.first; t < w.end; t += w.stride) {
+    const RowSeg g0 = row_seg(d, t, 0), g1 = other_colour(d, g0);
+    const uint32_t c0 = g0.rb + (uint32_t)g0.h0, c1 = d.hh + c0;
+    const Nbr6<float> n0 = nbr6(p, g0, c1, sx, sy);  // colour-0 cells' neighbours (colour 1)
+    const Nbr6<float> n1 = nbr6(p, g1, c0, sx, sy);
+    const F4 a0 = row_Ap(g0, n1.c, n0), a1 = row_Ap(g1, n0.c, n1);
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) acc += (double)n1.c.v[v] * (double)a0.v[v];
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) acc += (double)n0.c.v[v] * (double)a1.v[v];
+  }
+  grid_reduce(R, FIN_ALPHA, acc, 0.0);
+}
+
+// x += alpha p, r = (r - mean) - alpha A p over the colour-split finest
+// level (k_update_xr's operations per cell), both colours of each segment.
+__global__ void __launch_bounds__(kB) k_row_update_xr(Dims d, double* __restrict__ x, float* __restrict__ r,
+                                                      const float* __restrict__ p, Red R) {
+  if (is_done(R.flags)) return;
+  const double alpha_d = R.sc[S_ALPHA];
+  const float alpha = (float)alpha_d;
+  const float m = (float)R.sc[S_MEAN];
+  const bool x_zero = R.flags[F_ITERS] == 0;
+  double rr = 0, rs = 0;
+  const SlabWalk<uint32_t> w = slab_walk<kB>(row_threads(d));
+  const uint32_t sy = d.fh.d, sx = (uint32_t)d.n[1] * sy;
+  for (uint32_t t = w.first; t < w.end; t += w.stride) {
+    const RowSeg g0 = row_seg(d, t, 0), g1 = other_colour(d, g0);
+    const uint32_t c0 = g0.rb + (uint32_t)g0.h0, c1 = d.hh + c0;
+    const Nbr6<float> n0 = nbr6(p, g0, c1, sx, sy);
+    const Nbr6<float> n1 = nbr6(p, g1, c0, sx, sy);
+    const F4 r0 = ld4(r, c0), r1 = ld4(r, c1);
+    double x0[kRowV], x1[kRowV];
+#pragma unroll
+    for (int v = 0; v < kRowV; v += 2) {
+      double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+      if (!x_zero) {
+        a = *reinterpret_cast<const double2*>(x + c0 + v);
+        b = *reinterpret_cast<const double2*>(x + c1 + v);
+      }
+      x0[v] = a.x;
+      x0[v + 1] = a.y;
+      x1[v] = b.x;
+      x1[v + 1] = b.y;
+    }
+    const F4 a0 = row_Ap(g0, n1.c, n0), a1 = row_Ap(g1, n0.c, n1);
+    F4 o0, o1;
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) {
+      x0[v] = fma(alpha_d, (double)n1.c.v[v], x0[v]);
+      x1[v] = fma(alpha_d, (double)n0.c.v[v], x1[v]);
+      o0.v[v] = fmaf(-alpha, a0.v[v], r0.v[v] - m);
+      o1.v[v] = fmaf(-alpha, a1.v[v], r1.v[v] - m);
+    }
+#pragma unroll
+    for (int v = 0; v < kRowV; v += 2) {
+      *reinterpret_cast<double2*>(x + c0 + v) = make_double2(x0[v], x0[v + 1]);
+      *reinterpret_cast<double2*>(x + c1 + v) = make_double2(x1[v], x1[v + 1]);
+    }
+    st4(r, c0, o0);
+    st4(r, c1, o1);
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) {
+      rr += (double)o0.v[v] * (double)o0.v[v];
+      rs += (double)o0.v[v];
+    }
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) {
+      rr += (double)o1.v[v] * (double)o1.v[v];
+      rs += (double)o1.v[v];
+    }
+  }
+  grid_reduce(R, FIN_RR, rr, rs);
+}
+
 // One full RBGS sweep (colour c1, then the other) as a single launch, for the
 // levels whose half-sweeps are launch-latency bound.  The second colour's
 // update needs the first colour's new values at its neighbours; each such
@@ -2339,6 +2450,11 @@ void launch_update_xr_t(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
 template <int DIM, bool MASK>
 void launch_update_xr(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
   if constexpr (!MASK) {
+    if (DIM == 3 && split_level(s, 0) && row_ok(s, s.ld[0])) {
+      hipLaunchKernelGGL(k_row_update_xr, dim3(gridr(row_threads(s.ld[0]))), dim3(kB), 0, st, s.ld[0], s.x,
+                         s.lv[0].r, (const float*)s.p, R);
+      return;
+    }
     if (split_level(s, 0)) return launch_update_xr_t<DIM, false, true>(s, st, gr, R);
   }
   launch_update_xr_t<DIM, MASK, false>(s, st, gr, R);
@@ -2347,6 +2463,11 @@ void launch_update_xr(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
 template <int DIM, bool MASK>
 void launch_Ap_dot(Solver& s, hipStream_t st, unsigned gr, const Red& R) {
   if constexpr (!MASK) {
+    if (DIM == 3 && split_level(s, 0) && row_ok(s, s.ld[0])) {
+      hipLaunchKernelGGL(k_row_Ap_dot, dim3(gridr(row_threads(s.ld[0]))), dim3(kB), 0, st, s.ld[0],
+                         (const float*)s.p, R);
+      return;
+    }
     if (split_level(s, 0)) return launch_Ap_dot_t<DIM, false, true>(s, st, gr, R);
   }
   launch_Ap_dot_t<DIM, MASK, false>(s, st, gr, R);
