@@ -31,18 +31,22 @@ HBM_PEAK_GBPS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 # Algorithmic work per unit (flop / byte models).
-# 3D quadratic B-spline interpolation of one scalar in the reference's tap form
-# (a 4x4x4 loop, separable weights): value + gradient ~480 flop, value only
-# ~220 flop.  The kernels skip the 37 taps whose weight is exactly zero.
-FLOP_INTERP_GRAD_3D = 480.0
-FLOP_INTERP_VAL_3D = 220.0
+# 3D quadratic B-spline sample of one scalar over its 27 support taps
+# (separable weights, contracted z, then y, then x): value + gradient = 93
+# multiply-adds (186 flop) + 3 axes of weights and derivative weights (33) +
+# the clamped coordinates (21) + the gradient scale (3) = 243 flop; value
+# only = 39 multiply-adds (78) + weights (24) + coordinates (21) = 123 flop.
+# (The reference's tap loop visits 4x4x4 taps; the 37 beyond the support
+# have weight exactly 0 and are not counted.)
+FLOP_INTERP_GRAD_3D = 243.0
+FLOP_INTERP_VAL_3D = 123.0
+# Issue ceiling of the vector ALUs without packed FP32 (which the build
+# turns off): one wave64 instruction per CU per clock, 256 CUs at 2.4 GHz.
+VALU_ISSUE_PER_S = 256 * 2.4e9
 # MGPCG compulsory FP32 traffic: bytes per fine cell per PCG iteration (3D) and
 # per solve (divergence, init, gradient subtraction).
 MG_BYTES_CELL_ITER_3D = 156.0
 MG_BYTES_CELL_SOLVE_3D = 64.0
-# Gathered 4-byte taps per face point of one RK4 march: 4 stages x 3 MAC
-# components x 27 support taps (the gradient reuses the value's taps).
-MARCH_TAPS_FACE = 4 * 3 * 27
 
 
 REINIT_EVERY = 10
@@ -84,6 +88,19 @@ def sl_flop(n, nsrc_vec, nsrc_scal, order=4):
     f = faces * (back + nsrc_vec * FLOP_INTERP_VAL_3D) if nsrc_vec else 0.0
     c = cells * (back + nsrc_scal * FLOP_INTERP_VAL_3D) if nsrc_scal else 0.0
     return f + c
+
+
+def measured_valu(kernel, grid):
+    """Vector-ALU wave instructions per launch of a kernel at a grid size from
+    the committed rocprofv3 SQ_INSTS_VALU summary of the 128^3 bench workload,
+    with the kernel's duration when it ran alone in that pass; None when the
+    summary does not hold them."""
+    path = os.path.join(ROOT, "profiles", "pmc_valu_3d128.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get(kernel, {}).get(str(int(grid)))
 
 
 def measured_traffic(dom, n):
@@ -159,31 +176,36 @@ def run_gpu(args):
     assert bool(torch.isfinite(lam).all()), "non-finite adjoint"
 
     # dominant flow-map kernel by total device time
+    faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
+    cells = float(np.prod(n))
     models = {
-        "march": (march_flop(n), "dfm_march (k_march<3,4>)", MARCH_TAPS_FACE),
-        "sl_advect": (sl_flop(n, 2, 2), "dfm_sl_advect (k_sl_advect<3,4>)", None),
-        "adj_conv": (None, "dfm_adj_conv_source (k_adj_conv<3,3>)", None),
+        "march": (march_flop(n), "dfm_march (k_march<3,4>)", "k_march<3, 4>", faces + cells),
+        "sl_advect": (sl_flop(n, 2, 2), "dfm_sl_advect (k_sl_advect<3,4>)", None, None),
+        "adj_conv": (faces * 3 * 216 * 4, "dfm_adj_conv_source (k_src_tiled<3>)", None, None),
     }
     dom = max((k for k in models if k in summ), key=lambda k: summ[k][1])
     calls, tot_ms = summ[dom]
     avg_s = tot_ms / calls / 1e3
     flop = models[dom][0]
-    if flop is None:
-        faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
-        flop = faces * 3 * 216 * 4
     achieved = flop / avg_s / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, n),
                 "kernel": models[dom][1], "avg_launch_ms": round(avg_s * 1e3, 4), "calls": calls,
                 "flop_per_launch": flop,
-                "note": "B-spline stencil gathers with no dense contraction, so no MFMA: priced against the FP32 "
-                        "vector peak with the reference's 64-tap flop count; the kernel executes only the 27 "
-                        "support taps per sample (executed_flop_frac), and its limit is the rate of gathered "
-                        "L1 loads (gathered_TBps)"}
-    if models[dom][2] is not None:
-        faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
-        roofline["executed_flop_frac"] = round(achieved * 27.0 / 64.0 / FP32_PEAK_TFLOPS, 4)
-        roofline["gathered_TBps"] = round(faces * models[dom][2] * 4.0 / avg_s / 1e12, 2)
+                "flop_model": "27 support taps per quadratic sample: 243 flop with the gradient, 123 value only",
+                "note": "B-spline samples with no dense contraction, so no MFMA: priced against the published FP32 "
+                        "vector peak, which counts packed FP32; the build issues unpacked FP32, whose ceiling is "
+                        "half of it.  valu_issue_frac: the kernel's vector-ALU wave instructions per launch "
+                        "(rocprofv3 SQ_INSTS_VALU over this workload) over the issue ceiling of one wave64 "
+                        "instruction per CU per clock, in this run's launch time (beside the projections) and "
+                        "in the counter pass's own (the kernel alone)"}
+    if models[dom][2] is not None and tuple(n) == (128, 128, 128):
+        grid = -(-int(models[dom][3]) // 256) * 256
+        v = measured_valu(models[dom][2], grid)
+        if v:
+            roofline["valu_per_launch"] = v["valu"]
+            roofline["valu_issue_frac"] = round(v["valu"] / (VALU_ISSUE_PER_S * avg_s), 3)
+            roofline["valu_issue_frac_alone"] = round(v["valu"] / (VALU_ISSUE_PER_S * v["us"] * 1e-6), 3)
     mg_calls, mg_ms = summ.get("mgpcg", (0, 0.0))
     mg_iters = cnt.get("mgpcg_iters", 0)
     cells = float(np.prod(n))
