@@ -649,7 +649,7 @@ def main(argv=None):
     elapsed, roofline, mgpcg, breakdown = run_gpu(args)
     K = args.steps
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.size == 128 else METRIC.replace("128³", f"{args.size}³"),
         "value": round(K / elapsed, 3),
         "unit": "steps/s",
         "n_gpus": 1,
