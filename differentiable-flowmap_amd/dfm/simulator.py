@@ -857,9 +857,14 @@ class LFM_Diff_Simulator(LFM_Simulator):
             g = self._force_grads
             ops.gaussian_force_adjoint(self._fc, self._fr, self._fs, forces, self.adj_u, g["adj_center"],
                                        g["adj_strength"], g["pen_center"], g["pen_strength"])
-        # adjoint sources at the grid faces, from the current (end-of-step) state
-        ops.adj_conv_source(self.u, self.adj_u, self.adj_kernel, self.r_u)
-        ops.adj_scalar_source(self.passive, self.adj_passive, self.r_s)
+        # adjoint sources at the grid faces, from the current (end-of-step)
+        # state; they only read fields that the advection below reads too and
+        # write r_u / r_s, which the axpy after it consumes, so they run on the
+        # side stream beside the advection (the current stream waits for them
+        # before the axpy)
+        with self._on_side("adjoint"):
+            ops.adj_conv_source(self.u, self.adj_u, self.adj_kernel, self.r_u)
+            ops.adj_scalar_source(self.passive, self.adj_passive, self.r_s)
         if self.short_bfecc:
             # BFECC of lam_u, lam_s, s and u through one-step maps of mid; the
             # convective source then enters at -2 dt
@@ -882,6 +887,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
         self.passive, self._passive_new = self._passive_new, self.passive
         self.adj_passive, self._adj_passive_new = self._adj_passive_new, self.adj_passive
         # lam_u += src_dt r_u - dt r_s  (2D/vortex/lfm_midpoint_diff_simulator.py:297-298,359-374)
+        self._join_side()
         ops.axpy(self.adj_u, src_dt, self.r_u, self.adj_u, beta=-dt, z=self.r_s)
         # psi/T march and source accumulation (AB3 at -2dt for r_u, -dt for r_s);
         # they write only psi/T, init_adj_u (init_u) and the AB3 history, so
