@@ -57,17 +57,23 @@ template <int K>
 struct Smem {
   static constexpr int NT = Spline<K>::NT;
   static constexpr int EX = TX + NT, EY = TY + NT, EZ = TZ + NT;  // window bounds (see tables())
-  static constexpr int PZ = EZ | 1;                                // odd row pitches
-  static constexpr int PZ2 = TZ + 1;
+  static constexpr int PZ = EZ | 1;                                // odd row pitch
   static_assert(EZ <= 64, "a window row is one wave wide");
-  __attribute__((aligned(16))) float A[EX * EY * PZ];  // lattice window, later the y-pass output
-  float Z[EX * EY * PZ2];  // z-pass output
+  // lattice window; the z pass overwrites each row with its outputs
+  __attribute__((aligned(16))) float A[EX * EY * PZ];
+  float Y[EX * TY * TZ];  // y-pass output
   static constexpr int NW = (NT + 3) & ~3;  // table rows padded to 16 bytes (vector reads)
-  __attribute__((aligned(16))) float wv[3][3][kMaxPos][NW];  // [slot][axis][output][tap] value weights
-  __attribute__((aligned(16))) float wd[3][kMaxPos][NW];  // [slot][output][tap] derivative weights along the output grid's axis
-  __attribute__((aligned(16))) uint8_t ri[3][3][kMaxPos][NW];  // tap index relative to the window
+  static constexpr int TXY = TX > TY ? TX : TY;
+  // [slot][output][tap] value weights along z and along x, y; derivative
+  // weights along the output grid's axis; tap indices relative to the window
+  __attribute__((aligned(16))) float wvz[3][TZ][NW];
+  __attribute__((aligned(16))) float wvxy[3][2][TXY][NW];
+  __attribute__((aligned(16))) float wd[3][kMaxPos][NW];
+  __attribute__((aligned(16))) uint8_t riz[3][TZ][NW];
+  __attribute__((aligned(16))) uint8_t rixy[3][2][TXY][NW];
   int lo[3][3], ext[3][3];
-  static_assert(EX * TY * TZ <= EX * EY * PZ, "y-pass output must fit in A");
+  DFM_HD float& wv(int slot, int b, int pos, int q) { return b == 2 ? wvz[slot][pos][q] : wvxy[slot][b][pos][q]; }
+  DFM_HD uint8_t& ri(int slot, int b, int pos, int q) { return b == 2 ? riz[slot][pos][q] : rixy[slot][b][pos][q]; }
   static_assert(EX < 256 && EY < 256 && EZ < 256, "relative tap indices are bytes");
 };
 
@@ -113,10 +119,10 @@ __device__ void table_entry(Smem<K>& sm, const Geo& g, int a, int slot, int lat,
   ext = ext < cap ? ext : cap;
 #pragma unroll
   for (int q = 0; q < NT; ++q) {
-    sm.wv[slot][b][pos][q] = mine.w[q];
+    sm.wv(slot, b, pos, q) = mine.w[q];
     if (b == a) sm.wd[slot][pos][q] = mine.dw[q];
     const int r = (kConsec ? mine.idx[0] + q : mine.idx[q]) - lo;
-    sm.ri[slot][b][pos][q] = (uint8_t)(r < 0 ? 0 : (r >= ext ? ext - 1 : r));
+    sm.ri(slot, b, pos, q) = (uint8_t)(r < 0 ? 0 : (r >= ext ? ext - 1 : r));
   }
   if (pos == 0) {
     sm.lo[slot][b] = lo;
@@ -204,27 +210,33 @@ __global__ void __launch_bounds__(THREADS) k_src_tiled(Geo g, Pairs P, Tiles T) 
   };
   __syncthreads();  // tables
   const int nf = 2 * P.n;
-  if (nf > 0) issue(0);
+  // Three barriers per field: the z pass runs in place in A (each window row
+  // is read by the 32 lanes of one wave before they write its outputs), the
+  // y pass writes Y, and the next field's window goes into A during the x
+  // pass, which reads Y.
+  if (nf > 0) {
+    issue(0);
+    store(0);
+    lds_barrier();
+    if (nf > 1) issue(1);
+  }
 #pragma unroll 1
   for (int i = 0; i < nf; ++i) {
     const int slot = P.slot[i >> 1];
     const bool deriv = !(i & 1);
     const int ex = __builtin_amdgcn_readfirstlane(sm.ext[slot][0]);
     const int ey = __builtin_amdgcn_readfirstlane(sm.ext[slot][1]);
-    store(i);
-    lds_barrier();
-    if (i + 1 < nf) issue(i + 1);
-    // z pass: A (ex*ey rows) -> Z
+    // z pass: A (ex*ey rows) -> A, in place
     {
       float w[NT];
       int rz[NT];
 #pragma unroll
       for (int q = 0; q < NT; ++q) {
-        w[q] = (deriv && a == 2) ? sm.wd[slot][zo][q] : sm.wv[slot][2][zo][q];
-        rz[q] = sm.ri[slot][2][zo][q];
+        w[q] = (deriv && a == 2) ? sm.wd[slot][zo][q] : sm.wvz[slot][zo][q];
+        rz[q] = sm.riz[slot][zo][q];
       }
       const float* in = sm.A + yo * SM::PZ + (kConsec ? rz[0] : 0);
-      float* out = sm.Z + yo * SM::PZ2 + zo;
+      float* out = sm.A + yo * SM::PZ + zo;
 #pragma unroll 6
       for (int row = yo; row < ex * ey; row += THREADS / TZ) {
         float acc = 0.f;
@@ -232,45 +244,46 @@ __global__ void __launch_bounds__(THREADS) k_src_tiled(Geo g, Pairs P, Tiles T) 
         for (int q = 0; q < NT; ++q) acc = fmaf(in[kConsec ? q : rz[q]], w[q], acc);
         *out = acc;
         in += (THREADS / TZ) * SM::PZ;
-        out += (THREADS / TZ) * SM::PZ2;
+        out += (THREADS / TZ) * SM::PZ;
       }
     }
     lds_barrier();
-    // y pass: Z -> A as (x, yo, z), x over the window
+    // y pass: A -> Y as (x, yo, z), x over the window
     {
       float w[NT];
       int ry[NT];
 #pragma unroll
       for (int q = 0; q < NT; ++q) {
-        w[q] = (deriv && a == 1) ? sm.wd[slot][yo][q] : sm.wv[slot][1][yo][q];
-        ry[q] = sm.ri[slot][1][yo][q];
+        w[q] = (deriv && a == 1) ? sm.wd[slot][yo][q] : sm.wvxy[slot][1][yo][q];
+        ry[q] = sm.rixy[slot][1][yo][q];
       }
-      const float* in = sm.Z + zo + (kConsec ? ry[0] * SM::PZ2 : 0);
-      float* out = sm.A + yo * TZ + zo;
+      const float* in = sm.A + zo + (kConsec ? ry[0] * SM::PZ : 0);
+      float* out = sm.Y + yo * TZ + zo;
 #pragma unroll 4
       for (int x = 0; x < ex; ++x) {
         float acc = 0.f;
 #pragma unroll
-        for (int q = 0; q < NT; ++q) acc = fmaf(in[(kConsec ? q : ry[q]) * SM::PZ2], w[q], acc);
+        for (int q = 0; q < NT; ++q) acc = fmaf(in[(kConsec ? q : ry[q]) * SM::PZ], w[q], acc);
         *out = acc;
-        in += ey * SM::PZ2;
+        in += ey * SM::PZ;
         out += TY * TZ;
       }
     }
     lds_barrier();
-    // x pass: A -> registers, outputs (xo, yo, zo); the position's weights
+    if (i + 1 < nf) store(i + 1);  // A is free; the x pass reads Y
+    // x pass: Y -> registers, outputs (xo, yo, zo); the position's weights
     // and tap indices are block-uniform
     {
-      const float* col = sm.A + yo * TZ + zo;
+      const float* col = sm.Y + yo * TZ + zo;
 #pragma unroll
       for (int xo = 0; xo < TX; ++xo) {
         float w[NT];
         int rx[NT];
-        const float* wsrc = (deriv && a == 0) ? sm.wd[slot][xo] : sm.wv[slot][0][xo];
+        const float* wsrc = (deriv && a == 0) ? sm.wd[slot][xo] : sm.wvxy[slot][0][xo];
 #pragma unroll
         for (int q = 0; q < NT; ++q) {
           w[q] = wsrc[q];
-          rx[q] = sm.ri[slot][0][xo][q];
+          rx[q] = sm.rixy[slot][0][xo][q];
         }
         float acc = 0.f;
         const float* in = col + (kConsec ? rx[0] * (TY * TZ) : 0);
@@ -280,7 +293,8 @@ __global__ void __launch_bounds__(THREADS) k_src_tiled(Geo g, Pairs P, Tiles T) 
         else r[xo] = fmaf(d[xo], acc, r[xo]);
       }
     }
-    lds_barrier();  // A is free for the next field
+    lds_barrier();  // Y is free, A holds the next window
+    if (i + 2 < nf) issue(i + 2);
   }
   if (yo < cnt[1] && zo < cnt[2]) {
     float* out = T.out[a];
