@@ -31,15 +31,18 @@ HBM_PEAK_GBPS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 # Algorithmic work per unit (flop / byte models).
-# 3D quadratic B-spline sample of one scalar over its 27 support taps
-# (separable weights, contracted z, then y, then x): value + gradient = 93
+# 3D quadratic B-spline interpolation of one scalar in the reference's tap
+# form (a 4x4x4 loop, separable weights): value + gradient ~480 flop, value
+# only ~220 flop.  This is the per-unit figure `roofline.achieved` uses.
+FLOP_INTERP_GRAD_3D = 480.0
+FLOP_INTERP_VAL_3D = 220.0
+# The same samples over their 27 support taps, as the kernels execute them
+# (the 37 other taps have weight exactly 0): value + gradient = 93
 # multiply-adds (186 flop) + 3 axes of weights and derivative weights (33) +
 # the clamped coordinates (21) + the gradient scale (3) = 243 flop; value
 # only = 39 multiply-adds (78) + weights (24) + coordinates (21) = 123 flop.
-# (The reference's tap loop visits 4x4x4 taps; the 37 beyond the support
-# have weight exactly 0 and are not counted.)
-FLOP_INTERP_GRAD_3D = 243.0
-FLOP_INTERP_VAL_3D = 123.0
+EXEC_FLOP_INTERP_GRAD_3D = 243.0
+EXEC_FLOP_INTERP_VAL_3D = 123.0
 # Issue ceiling of the vector ALUs without packed FP32 (which the build
 # turns off): one wave64 instruction per CU per clock, 256 CUs at 2.4 GHz.
 VALU_ISSUE_PER_S = 256 * 2.4e9
@@ -66,15 +69,15 @@ def plan_steps(steps, warmup, reinit_every=REINIT_EVERY):
     return {"steps": up(steps), "warmup": up(warmup), "steps_requested": steps, "warmup_requested": warmup}
 
 
-def march_flop(n, order=4):
+def march_flop(n, order=4, grad=FLOP_INTERP_GRAD_3D, val=FLOP_INTERP_VAL_3D):
     """Flop of one dfm_march launch: face points carry (psi, T), cells psi only."""
     import numpy as np
 
     cells = float(np.prod(n))
     faces = sum(float(np.prod([n[b] + (b == a) for b in range(3)])) for a in range(3))
     stages = 4 if order == 4 else 2
-    per_face = stages * (3 * FLOP_INTERP_GRAD_3D + 2 * 9 + 6 * 2) + 12
-    per_cell = stages * (3 * FLOP_INTERP_VAL_3D + 6) + 6
+    per_face = stages * (3 * grad + 2 * 9 + 6 * 2) + 12
+    per_cell = stages * (3 * val + 6) + 6
     return faces * per_face + cells * per_cell
 
 
@@ -192,13 +195,19 @@ def run_gpu(args):
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, n),
                 "kernel": models[dom][1], "avg_launch_ms": round(avg_s * 1e3, 4), "calls": calls,
                 "flop_per_launch": flop,
-                "flop_model": "27 support taps per quadratic sample: 243 flop with the gradient, 123 value only",
+                "flop_model": "the reference's tap form: 480 flop per quadratic sample with the gradient, "
+                              "220 value only",
                 "note": "B-spline samples with no dense contraction, so no MFMA: priced against the published FP32 "
                         "vector peak, which counts packed FP32; the build issues unpacked FP32, whose ceiling is "
-                        "half of it.  valu_issue_frac: the kernel's vector-ALU wave instructions per launch "
+                        "half of it.  executed_*: the 27 support taps the kernel evaluates (243 / 123 flop per "
+                        "sample).  valu_issue_frac: the kernel's vector-ALU wave instructions per launch "
                         "(rocprofv3 SQ_INSTS_VALU over this workload) over the issue ceiling of one wave64 "
                         "instruction per CU per clock, in this run's launch time (beside the projections) and "
                         "in the counter pass's own (the kernel alone)"}
+    if dom == "march":
+        ex = march_flop(n, grad=EXEC_FLOP_INTERP_GRAD_3D, val=EXEC_FLOP_INTERP_VAL_3D)
+        roofline["executed_flop_per_launch"] = ex
+        roofline["executed_frac"] = round(ex / avg_s / 1e12 / FP32_PEAK_TFLOPS, 4)
     if models[dom][2] is not None and tuple(n) == (128, 128, 128):
         grid = -(-int(models[dom][3]) // 256) * 256
         v = measured_valu(models[dom][2], grid)
