@@ -1161,6 +1161,11 @@ DFM_HD F4 ld4(const float* __restrict__ p, uint32_t i) {
   const float4 q = *reinterpret_cast<const float4*>(p + i);
   return F4{{q.x, q.y, q.z, q.w}};
 }
+DFM_HD F4 ld4(const uint16_t* __restrict__ p, uint32_t i) {
+  const uint2 q = *reinterpret_cast<const uint2*>(p + i);
+  return F4{{__uint_as_float(q.x << 16), __uint_as_float(q.x & 0xffff0000u), __uint_as_float(q.y << 16),
+             __uint_as_float(q.y & 0xffff0000u)}};
+}
 DFM_HD void st4(float* __restrict__ p, uint32_t i, const F4& a) {
   *reinterpret_cast<float4*>(p + i) = make_float4(a.v[0], a.v[1], a.v[2], a.v[3]);
 }
@@ -1307,6 +1312,73 @@ __global__ void __launch_bounds__(kB) k_row_half(Dims d, const float* __restrict
     }
   }
   st4(z, own, out);
+}
+
+// The 4 cells of a segment's own colour (off = 0: cells 0, 2, 4, 6; off = 1:
+// cells 1, 3, 5, 7) and of the other colour, from its two 16-byte halves.
+// The cells are picked from named vector components: selects between
+// elements of a register array can be folded into a run-time index into
+// that array, which then lives in scratch memory.
+struct Seg8 {
+  float4 lo, hi;
+};
+DFM_HD Seg8 ld8(const float* __restrict__ p, uint32_t i) {
+  return Seg8{*reinterpret_cast<const float4*>(p + i), *reinterpret_cast<const float4*>(p + i + kRowV)};
+}
+DFM_HD F4 pick(const Seg8& s, bool odd) {
+  return F4{{odd ? s.lo.y : s.lo.x, odd ? s.lo.w : s.lo.z, odd ? s.hi.y : s.hi.x, odd ? s.hi.w : s.hi.z}};
+}
+
+// The same half-sweeps on a natural-order 3D level (the coarse global levels,
+// or level 0 when it is not colour-split): a thread owns the 8 cells of pairs
+// h0 .. h0 + 3 of row (i, j) and updates its 4 cells of colour `phase` (at
+// 2 (h0 + v) + off).  Each neighbour row is two 16-byte loads; the segment
+// is stored back whole, its other-colour cells with the values just read
+// (no other thread writes them in a half-sweep).  Bitwise k_smooth_u /
+// k_first_half.
+template <int MODE, typename ZCT = float>
+__global__ void __launch_bounds__(kB) k_row_half_nat(Dims d, const float* __restrict__ r, float* __restrict__ z,
+                                                     int phase, const double* shift, const ZCT* __restrict__ zc,
+                                                     Dims dc, const int* flags) {
+#pragma clang fp contract(off)
+  if (is_done(flags)) return;
+  const uint32_t t = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+  if (t >= row_threads(d)) return;
+  const float m = shift ? (float)*shift : 0.f;
+  const RowSeg g = row_seg(d, t, phase);
+  const uint32_t base = 2u * (g.rb + (uint32_t)g.h0);  // natural index of the segment's first cell
+  const bool odd = g.off != 0;
+  const F4 rv = pick(ld8(r, base), odd);
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int v = 0; v < kRowV; ++v) {
+      const int k = 2 * (g.h0 + v) + g.off;
+      float nn = 0.f;
+      nn += (g.i > 0) + (g.i < d.n[0] - 1) + (g.j > 0) + (g.j < d.n[1] - 1);
+      nn += (k > 0) + (k < d.n[2] - 1);
+      z[base + (uint32_t)(2 * v + g.off)] = (rv.v[v] - m) * inv_count(nn);
+    }
+  } else {
+    const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
+    const Seg8 c8 = ld8(z, base);
+    Nbr6<float> zn;
+    zn.xm = pick(ld8(z, g.nbr & 1u ? base - sx : base), odd);
+    zn.xp = pick(ld8(z, g.nbr & 2u ? base + sx : base), odd);
+    zn.ym = pick(ld8(z, g.nbr & 4u ? base - sy : base), odd);
+    zn.yp = pick(ld8(z, g.nbr & 8u ? base + sy : base), odd);
+    zn.c = pick(c8, !odd);  // the other colour's cell of each pair
+    const bool ext = odd ? g.zp3 : g.zm0;
+    zn.e = z[ext ? (odd ? base + 2 * kRowV : base - 1u) : base];
+    F4 out;
+    if constexpr (MODE == 2) out = row_gs<true>(d, g, rv, m, zn, parents6(zc, dc, g));
+    else out = row_gs<false, float>(d, g, rv, m, zn, zn);
+    const float4 lo = odd ? make_float4(c8.lo.x, out.v[0], c8.lo.z, out.v[1])
+                          : make_float4(out.v[0], c8.lo.y, out.v[1], c8.lo.w);
+    const float4 hi = odd ? make_float4(c8.hi.x, out.v[2], c8.hi.z, out.v[3])
+                          : make_float4(out.v[2], c8.hi.y, out.v[3], c8.hi.w);
+    *reinterpret_cast<float4*>(z + base) = lo;
+    *reinterpret_cast<float4*>(z + base + kRowV) = hi;
+  }
 }
 
 // The last level-0 half-sweep of a V-cycle fused with z.r (k_smooth<3, true,
@@ -2076,7 +2148,7 @@ struct Solver {
   int fused_sweeps;  // option: one launch per RBGS sweep on levels with z2
   int tiled_sweeps;  // option: LDS-tiled pre/post smoothing on eligible coarse levels
   int split_opt;     // option: level 0 colour-split where eligible (set_layout)
-  int row_opt;       // option: the colour-split level's row kernels where eligible
+  int row_opt;       // option: row half-sweeps (1: where eligible and wide enough, 2: where eligible)
   double* xn;        // natural-order copy of a colour-split pressure (on request)
   hipStream_t last_st;  // stream of the last solve
   int ready;         // partials, counters, control block and x zeroed (ensure_ready)
@@ -2197,10 +2269,34 @@ inline bool row_ok(const Solver& s, const Dims& d, const Dims* dc = nullptr) {
 
 inline unsigned row_grid(const Dims& d) { return (row_threads(d) + kB - 1) / kB; }
 
+// The natural-order counterpart of row_ok (k_row_half_nat).  With option
+// value 1 only on levels whose row launch has a workgroup for each of the
+// 256 CUs: below that (a 64^3 level has 128) the per-pair sweep, with twice
+// the threads, measured as fast or faster.
+inline bool row_ok_nat(const Solver& s, const Dims& d, const Dims* dc = nullptr) {
+  return s.row_opt && d.hh == 0 && d.n[2] % (2 * kRowV) == 0 && (!dc || 2 * dc->n[2] == d.n[2]) &&
+         (s.row_opt == 2 || row_grid(d) >= 256);
+}
+
 template <int DIM, bool MASK, bool SPLIT>
 void half_sweep(Solver& s, hipStream_t st, const Dims& d, float* r, float* z, int c, const double* shift,
                 const float* zc, const Dims& dc, const Red& R, bool zc_bf16 = false) {
   const uint32_t pairs = pairs_of(d, DIM);
+  if constexpr (!SPLIT && DIM == 3 && !MASK) {
+    if (row_ok_nat(s, d, zc ? &dc : nullptr)) {
+      const dim3 g(row_grid(d));
+      if (!zc)
+        hipLaunchKernelGGL((k_row_half_nat<1>), g, dim3(kB), 0, st, d, (const float*)r, z, c, shift,
+                           (const float*)nullptr, dc, (const int*)s.flags);
+      else if (zc_bf16)
+        hipLaunchKernelGGL((k_row_half_nat<2, bf16_t>), g, dim3(kB), 0, st, d, (const float*)r, z, c, shift,
+                           reinterpret_cast<const bf16_t*>(zc), dc, (const int*)s.flags);
+      else
+        hipLaunchKernelGGL((k_row_half_nat<2>), g, dim3(kB), 0, st, d, (const float*)r, z, c, shift, zc, dc,
+                           (const int*)s.flags);
+      return;
+    }
+  }
   if constexpr (SPLIT && DIM == 3 && !MASK) {
     // (the finest level's coarse correction, level 1, is never stored in BF16)
     if (!zc_bf16 && row_ok(s, d, zc ? &dc : nullptr)) {
@@ -2251,6 +2347,9 @@ bool sweeps_global_t(Solver& s, hipStream_t st, int l, int sweeps, int c1, bool 
       if (SPLIT && DIM == 3 && row_ok(s, d))
         hipLaunchKernelGGL((k_row_half<0>), dim3(row_grid(d)), dim3(kB), 0, st, d, (const float*)L.r, z, c1, shift,
                            (const float*)nullptr, dc, (const int*)s.flags);
+      else if (!SPLIT && DIM == 3 && row_ok_nat(s, d))
+        hipLaunchKernelGGL((k_row_half_nat<0>), dim3(row_grid(d)), dim3(kB), 0, st, d, (const float*)L.r, z, c1,
+                           shift, (const float*)nullptr, dc, (const int*)s.flags);
       else
         hipLaunchKernelGGL((k_first_half<DIM, SPLIT>), dim3(g), dim3(kB), 0, st, d, (const float*)L.r, z, c1, shift,
                            (const int*)s.flags);
@@ -2901,7 +3000,8 @@ extern "C" int dfm_mgpcg_set_option(void* handle, int option, int value) {
       drop_graph(*s);
       return DFM_OK;
     case DFM_MG_ROW_KERNELS:
-      s->row_opt = value ? 1 : 0;
+      if (value < 0 || value > 2) return set_error(DFM_E_ARG, "row kernels option must be 0, 1 or 2");
+      s->row_opt = value;
       drop_graph(*s);
       return DFM_OK;
     default:

@@ -411,11 +411,14 @@ int dfm_mgpcg_pressure(void* handle, double** p);
 #define DFM_MG_BF16_LEVEL 10
 /*  DFM_MG_LEVELS (get only): the handle's number of MG levels L. */
 #define DFM_MG_LEVELS 11
-/*  DFM_MG_ROW_KERNELS: 1 (default) = on a colour-split 3D finest level whose
- *  innermost extent is a multiple of 8, the half-sweeps run one thread per 4
- *  consecutive cells of a row and colour (16-byte row loads, the index
- *  arithmetic shared by the 4); 0 = one thread per cell pair.  The smoothed
- *  values are bitwise the same; the z.r sums add in another order. */
+/*  DFM_MG_ROW_KERNELS: 1 (default) = on a 3D level whose innermost extent is
+ *  a multiple of 8 (and twice the next coarser one's), the half-sweeps run one
+ *  thread per 4 consecutive cells of a row and colour (16-byte row loads, the
+ *  index arithmetic shared by the 4): always on a colour-split finest level,
+ *  on a natural-order level when that makes at least 256 workgroups of 256
+ *  threads; 2 = the same on every such level; 0 = one thread per cell pair.
+ *  The smoothed values are bitwise the same; on a colour-split level the
+ *  z.r sums add in another order.  Other values: DFM_E_ARG. */
 #define DFM_MG_ROW_KERNELS 12
 int dfm_mgpcg_set_option(void* handle, int option, int value);
 int dfm_mgpcg_get_option(void* handle, int option, int* value);

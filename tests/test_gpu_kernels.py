@@ -530,6 +530,49 @@ def test_mgpcg_row_kernels_match_pair_kernels(n, levels, bf16, device):
                1e-6)
 
 
+@pytest.mark.parametrize("n,levels,bf16", [((64, 48, 32), 3, False), ((48, 40, 40), 3, True),
+                                           ((128, 128, 128), 5, True), ((40, 36, 20), 3, False)])
+def test_mgpcg_natural_row_kernels_bitwise(n, levels, bf16, device):
+    """On natural-order levels (every level here: the finest is not
+    colour-split) the row half-sweeps, which update a row's 4 cells of one
+    colour from two 16-byte loads per neighbour row, are the per-pair
+    half-sweeps operation for operation: the solves with them on every
+    eligible level (option 2), on the wide levels only (1: here only the
+    128^3 level) and with none (0) give the same bits, iteration counts and
+    pressure.  (48, 40, 40) has a level (24, 20, 20) whose innermost extent
+    is not a multiple of 8, so row and pair sweeps alternate between levels;
+    (40, 36, 20) runs no row sweep.  The BF16 cases read a BF16 coarse
+    correction in the fused prolongation.  Option values outside 0..2 are
+    refused."""
+    from dfm import _lib as L
+    from dfm.ops import MGPCG
+
+    rng = np.random.default_rng(7 + n[0] + n[2])
+    u = round32([rng.standard_normal(O.face_shape(n, a)) for a in range(3)])
+    ref = [c.copy() for c in u]
+    O.project_exact(ref)
+    got = {}
+    for rows in (2, 1, 0):
+        s = MGPCG(n, 1.0 / n[1], levels=levels, tol=1e-7, max_iters=200, bf16_coarse=bf16, split_layout=False)
+        assert s.get_option(L.MG_SPLIT_LAYOUT) == 0
+        s.set_option(L.MG_ROW_KERNELS, rows)
+        assert s.get_option(L.MG_ROW_KERNELS) == rows
+        for bad in (-1, 3):
+            with pytest.raises(L.DfmError):
+                s.set_option(L.MG_ROW_KERNELS, bad)
+        assert s.get_option(L.MG_ROW_KERNELS) == rows
+        d = mac_to_dev(u, device)
+        s.Poisson(d)
+        got[rows] = ([c.clone() for c in d.c], s.status()[0], _pressure_of(s, n, device))
+        within("natural row-kernel projection vs exact", rel_l2_fields(mac_to_np(d), ref), 2e-5 if bf16 else 2e-6)
+    fp, itp, pp = got[0]
+    for rows in (2, 1):
+        fr, itr, pr = got[rows]
+        assert itr == itp
+        assert all(torch.equal(a, b) for a, b in zip(fr, fp))
+        assert np.array_equal(pr, pp)
+
+
 def test_mgpcg_first_solve_on_side_stream(device):
     """Creating a handle only allocates.  Its reduction counters, control
     block and x are zeroed on the stream of the first call that launches
