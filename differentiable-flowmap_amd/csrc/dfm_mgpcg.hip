@@ -1336,6 +1336,37 @@ DFM_HD F4 pick(const Seg8& s, bool odd) {
 // is stored back whole, its other-colour cells with the values just read
 // (no other thread writes them in a half-sweep).  Bitwise k_smooth_u /
 // k_first_half.
+// Gauss-Seidel update, in place in z, of the own-colour cells of segment g
+// of a natural-order 3D level (with ZC the coarse correction zc added to the
+// neighbour values, the fused prolongation).
+template <bool ZC, typename ZCT>
+DFM_HD void nat_seg_update(const Dims& d, const RowSeg& g, const float* __restrict__ r, float* __restrict__ z,
+                           float m, const ZCT* __restrict__ zc, const Dims& dc) {
+#pragma clang fp contract(off)
+  const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
+  const uint32_t base = 2u * (g.rb + (uint32_t)g.h0);  // natural index of the segment's first cell
+  const bool odd = g.off != 0;
+  const F4 rv = pick(ld8(r, base), odd);
+  const Seg8 c8 = ld8(z, base);
+  Nbr6<float> zn;
+  zn.xm = pick(ld8(z, g.nbr & 1u ? base - sx : base), odd);
+  zn.xp = pick(ld8(z, g.nbr & 2u ? base + sx : base), odd);
+  zn.ym = pick(ld8(z, g.nbr & 4u ? base - sy : base), odd);
+  zn.yp = pick(ld8(z, g.nbr & 8u ? base + sy : base), odd);
+  zn.c = pick(c8, !odd);  // the other colour's cell of each pair
+  const bool ext = odd ? g.zp3 : g.zm0;
+  zn.e = z[ext ? (odd ? base + 2 * kRowV : base - 1u) : base];
+  F4 out;
+  if constexpr (ZC) out = row_gs<true>(d, g, rv, m, zn, parents6(zc, dc, g));
+  else out = row_gs<false, float>(d, g, rv, m, zn, zn);
+  const float4 lo = odd ? make_float4(c8.lo.x, out.v[0], c8.lo.z, out.v[1])
+                        : make_float4(out.v[0], c8.lo.y, out.v[1], c8.lo.w);
+  const float4 hi = odd ? make_float4(c8.hi.x, out.v[2], c8.hi.z, out.v[3])
+                        : make_float4(out.v[2], c8.hi.y, out.v[3], c8.hi.w);
+  *reinterpret_cast<float4*>(z + base) = lo;
+  *reinterpret_cast<float4*>(z + base + kRowV) = hi;
+}
+
 template <int MODE, typename ZCT = float>
 __global__ void __launch_bounds__(kB) k_row_half_nat(Dims d, const float* __restrict__ r, float* __restrict__ z,
                                                      int phase, const double* shift, const ZCT* __restrict__ zc,
@@ -1346,10 +1377,9 @@ __global__ void __launch_bounds__(kB) k_row_half_nat(Dims d, const float* __rest
   if (t >= row_threads(d)) return;
   const float m = shift ? (float)*shift : 0.f;
   const RowSeg g = row_seg(d, t, phase);
-  const uint32_t base = 2u * (g.rb + (uint32_t)g.h0);  // natural index of the segment's first cell
-  const bool odd = g.off != 0;
-  const F4 rv = pick(ld8(r, base), odd);
   if constexpr (MODE == 0) {
+    const uint32_t base = 2u * (g.rb + (uint32_t)g.h0);
+    const F4 rv = pick(ld8(r, base), g.off != 0);
 #pragma unroll
     for (int v = 0; v < kRowV; ++v) {
       const int k = 2 * (g.h0 + v) + g.off;
@@ -1359,25 +1389,7 @@ __global__ void __launch_bounds__(kB) k_row_half_nat(Dims d, const float* __rest
       z[base + (uint32_t)(2 * v + g.off)] = (rv.v[v] - m) * inv_count(nn);
     }
   } else {
-    const uint32_t sy = (uint32_t)d.n[2], sx = (uint32_t)d.n[1] * sy;
-    const Seg8 c8 = ld8(z, base);
-    Nbr6<float> zn;
-    zn.xm = pick(ld8(z, g.nbr & 1u ? base - sx : base), odd);
-    zn.xp = pick(ld8(z, g.nbr & 2u ? base + sx : base), odd);
-    zn.ym = pick(ld8(z, g.nbr & 4u ? base - sy : base), odd);
-    zn.yp = pick(ld8(z, g.nbr & 8u ? base + sy : base), odd);
-    zn.c = pick(c8, !odd);  // the other colour's cell of each pair
-    const bool ext = odd ? g.zp3 : g.zm0;
-    zn.e = z[ext ? (odd ? base + 2 * kRowV : base - 1u) : base];
-    F4 out;
-    if constexpr (MODE == 2) out = row_gs<true>(d, g, rv, m, zn, parents6(zc, dc, g));
-    else out = row_gs<false, float>(d, g, rv, m, zn, zn);
-    const float4 lo = odd ? make_float4(c8.lo.x, out.v[0], c8.lo.z, out.v[1])
-                          : make_float4(out.v[0], c8.lo.y, out.v[1], c8.lo.w);
-    const float4 hi = odd ? make_float4(c8.hi.x, out.v[2], c8.hi.z, out.v[3])
-                          : make_float4(out.v[2], c8.hi.y, out.v[3], c8.hi.w);
-    *reinterpret_cast<float4*>(z + base) = lo;
-    *reinterpret_cast<float4*>(z + base + kRowV) = hi;
+    nat_seg_update<MODE == 2, ZCT>(d, g, r, z, m, zc, dc);
   }
 }
 
@@ -2034,8 +2046,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const T* __restrict__
     T* zc = rc + size_of(l + 1);
     const int* nc = A.n[l + 1];
     for (int cid = threadIdx.x; cid < size_of(l + 1); cid += blockDim.x) {
-      int ck = cid % nc[2], ct = cid / nc[2];
-      int cj = ct % nc[1], ci = ct / nc[1];
+      int ci, cj, ck;
+      cell_of(A.d[l + 1], (uint32_t)cid, ci, cj, ck);
       float acc = 0.f;
       for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
@@ -2080,8 +2092,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs A, const T* __restrict__
     const int* n = A.n[l];
     const int* nc = A.n[l + 1];
     for (int id = threadIdx.x; id < size_of(l); id += blockDim.x) {
-      int k = id % n[2], t = id / n[2];
-      int j = t % n[1], i = t / n[1];
+      int i, j, k;
+      cell_of(A.d[l], (uint32_t)id, i, j, k);
       int cid = ((i >> 1) * nc[1] + (j >> 1)) * nc[2] + (DIM == 3 ? (k >> 1) : 0);
       Sto<T>::st(z, id, Sto<T>::ld(z, id) + Sto<T>::ld(zc, cid));
     }

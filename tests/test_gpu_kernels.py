@@ -530,9 +530,10 @@ def test_mgpcg_row_kernels_match_pair_kernels(n, levels, bf16, device):
                1e-6)
 
 
-@pytest.mark.parametrize("n,levels,bf16", [((64, 48, 32), 3, False), ((48, 40, 40), 3, True),
-                                           ((128, 128, 128), 5, True), ((40, 36, 20), 3, False)])
-def test_mgpcg_natural_row_kernels_bitwise(n, levels, bf16, device):
+@pytest.mark.parametrize("n,levels,bf16,exact", [((64, 48, 32), 3, False, False), ((64, 48, 32), 3, False, True),
+                                                 ((48, 40, 40), 3, True, False), ((128, 128, 128), 5, True, False),
+                                                 ((128, 128, 128), 5, False, True), ((40, 36, 20), 3, False, False)])
+def test_mgpcg_natural_row_kernels_bitwise(n, levels, bf16, exact, device):
     """On natural-order levels (every level here: the finest is not
     colour-split) the row half-sweeps, which update a row's 4 cells of one
     colour from two 16-byte loads per neighbour row, are the per-pair
@@ -542,8 +543,9 @@ def test_mgpcg_natural_row_kernels_bitwise(n, levels, bf16, device):
     pressure.  (48, 40, 40) has a level (24, 20, 20) whose innermost extent
     is not a multiple of 8, so row and pair sweeps alternate between levels;
     (40, 36, 20) runs no row sweep.  The BF16 cases read a BF16 coarse
-    correction in the fused prolongation.  Option values outside 0..2 are
-    refused."""
+    correction in the fused prolongation; the exact cases end the V-cycle
+    with the DCT bottom solve in the one-workgroup LDS tail.  Option values
+    outside 0..2 are refused."""
     from dfm import _lib as L
     from dfm.ops import MGPCG
 
@@ -553,8 +555,10 @@ def test_mgpcg_natural_row_kernels_bitwise(n, levels, bf16, device):
     O.project_exact(ref)
     got = {}
     for rows in (2, 1, 0):
-        s = MGPCG(n, 1.0 / n[1], levels=levels, tol=1e-7, max_iters=200, bf16_coarse=bf16, split_layout=False)
+        s = MGPCG(n, 1.0 / n[1], levels=levels, tol=1e-7, max_iters=200, bf16_coarse=bf16, split_layout=False,
+                  bottom_exact=exact)
         assert s.get_option(L.MG_SPLIT_LAYOUT) == 0
+        assert s.get_option(L.MG_BOTTOM_EXACT) == int(exact)
         s.set_option(L.MG_ROW_KERNELS, rows)
         assert s.get_option(L.MG_ROW_KERNELS) == rows
         for bad in (-1, 3):
