@@ -289,7 +289,7 @@ def mgpcg_256_alone(args, size=256):
     return out
 
 
-def run_north_star(args):
+def run_batched_vs_plain(args):
     """The mixed-precision batched form against the plain FP32 form at
     args.size^3, forward+adjoint over one reinitialisation interval per
     problem (seeded vortex tubes).  Plain: one problem after another, FP32
@@ -297,7 +297,7 @@ def run_north_star(args):
     problems on k worker threads and streams, each simulator with BF16
     storage of the multigrid levels >= 2, captured PCG-iteration graphs and
     its marches and independent projections on side streams; k from
-    --ns-instances (1..k all timed, the best is the value).  Both solve to
+    --batched-instances (1..k all timed, the best is the value).  Both solve to
     the same tolerance, so the BF16 levels only change the preconditioner;
     the gradients of problem 0 under the two forms are compared (rel-L2)."""
     import numpy as np
@@ -309,7 +309,7 @@ def run_north_star(args):
 
     n = (args.size,) * 3
     R = REINIT_EVERY
-    kmax = max(1, args.ns_instances)
+    kmax = max(1, args.batched_instances)
     scs = [scenes.scene_3d(n, seed=s, tubes=16) for s in range(kmax)]
     dx = scs[0]["dx"]
 
@@ -338,7 +338,7 @@ def run_north_star(args):
     for _ in range(2):  # warm-up: allocator, handles
         fwd_adj(plain, scs[0])
     plain_runs = []
-    for _ in range(args.ns_reps):
+    for _ in range(args.batched_reps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         fwd_adj(plain, scs[0])
@@ -362,7 +362,7 @@ def run_north_star(args):
             for _ in range(2):  # warm-up: graphs, handles
                 list(ex.map(work, range(k)))
             runs = []
-            for _ in range(args.ns_reps):
+            for _ in range(args.batched_reps):
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 list(ex.map(work, range(k)))
@@ -373,13 +373,13 @@ def run_north_star(args):
     best = max(mixed, key=mixed.get)
     g_mixed = torch.cat([c.flatten() for c in sims[0].adj_u.c]).double().cpu().numpy()
     return {
-        "metric": f"north-star ratio: mixed-precision batched fwd+adjoint steps/s / plain FP32 steps/s at {args.size}^3",
+        "metric": f"mixed-precision batched fwd+adjoint steps/s / plain FP32 steps/s at {args.size}^3",
         "value": round(mixed[best] / plain_sps, 3), "unit": "ratio", "n_gpus": 1,
         "mixed_batched_steps_per_s": {str(k): round(v, 3) for k, v in mixed.items()}, "best_instances": best,
         "plain_fp32_steps_per_s": round(plain_sps, 3), "steps_per_instance": R,
         "runs": {"plain": [round(v, 2) for v in plain_runs],
                  "mixed": {str(k): [round(v, 2) for v in r] for k, r in mixed_runs.items()},
-                 "statistic": f"median of {args.ns_reps} timed runs after 2 warm-up runs"},
+                 "statistic": f"median of {args.batched_reps} timed runs after 2 warm-up runs"},
         "grad_rel_l2_mixed_vs_plain": float(np.linalg.norm(g_mixed - g_plain) / np.linalg.norm(g_plain)),
         "noconv": int(sum(s.noconv_count for s in sims)),
         "higher_is_better": True, "dtype": "f32 (BF16 storage of MG levels >= 2 in the mixed form)",
@@ -609,10 +609,10 @@ def parse_args(argv=None):
     ap.add_argument("--batch-steps", type=int, default=24)
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for --batch (1..32; 0 keeps the platform's setting: 16 measured slower than 4)")
-    ap.add_argument("--north-star", action="store_true",
+    ap.add_argument("--batched-vs-plain", action="store_true",
                     help="mixed-precision batched fwd+adjoint vs the plain FP32 form at --size^3 (steps/s ratio)")
-    ap.add_argument("--ns-instances", type=int, default=4, help="instances of the --north-star batched form")
-    ap.add_argument("--ns-reps", type=int, default=3, help="timed runs per form in --north-star (median reported)")
+    ap.add_argument("--batched-instances", type=int, default=4, help="instances of the --batched-vs-plain batched form")
+    ap.add_argument("--batched-reps", type=int, default=3, help="timed runs per form in --batched-vs-plain (median reported)")
     ap.add_argument("--cpu-interval", type=int, default=0, metavar="STEPS",
                     help="only the CPU baseline: the oracle over STEPS forward steps and (for a whole number of "
                          "intervals) the adjoint sweep back, each step timed (OMP_NUM_THREADS threads)")
@@ -640,8 +640,8 @@ def main(argv=None):
         out.update({"metric": f"MGPCG projection alone at {args.size}^3", "bytes_model": "156 B/cell/PCG-iter + 64 B/cell/solve"})
         print(json.dumps(out), flush=True)
         return
-    if args.north_star:
-        print(json.dumps(run_north_star(args)), flush=True)
+    if args.batched_vs_plain:
+        print(json.dumps(run_batched_vs_plain(args)), flush=True)
         return
     if args.batch:
         from dfm.batch import configure_hw_queues

@@ -11,7 +11,7 @@ import pytest
 
 CSRC = os.path.join(os.path.dirname(__file__), "..", "differentiable-flowmap_amd", "csrc")
 
-DRIVER = r"""
+MAIN_SRC = r"""
 #include "dfm_fastdiv.hpp"
 #include <cstdio>
 #include <random>
@@ -49,7 +49,7 @@ def test_fastdiv_matches_hardware_division():
         src = os.path.join(td, "fd.cpp")
         exe = os.path.join(td, "fd")
         with open(src, "w") as f:
-            f.write(DRIVER)
+            f.write(MAIN_SRC)
         subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, src, "-o", exe], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
         assert out.returncode == 0, out.stdout[-2000:]
