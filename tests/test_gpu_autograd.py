@@ -157,7 +157,7 @@ def test_rollout_gradient_equals_simulator_adjoint(device):
 
 
 def test_u0_gradient_3d_matches_oracle(device):
-    """Config-3 form (gradient w.r.t. the initial velocity field) through
+    """The 3D form (gradient w.r.t. the initial velocity field) through
     autograd at 16^3 against the FP64 oracle's adjoint."""
     from dfm import LFM_Diff_Simulator, scenes
     from dfm.autograd import density_loss
@@ -184,7 +184,7 @@ def test_u0_gradient_3d_matches_oracle(device):
 
 
 def test_theta_gradient_vs_central_differences_and_oracle(device):
-    """Config-2 form (control theta, 8 vortices x 4) at 64^2: the autograd
+    """The 2D vortex control (theta, 8 vortices x 4) at 64^2: the autograd
     gradient (adjoint sweep + VJP kernel) against (a) the FP64 oracle's
     adjoint pulled back through the host initializer (rel-L2 <= 1e-3) and
     (b) central differences of the GPU loss, h = 1e-3, along 4 seeded unit

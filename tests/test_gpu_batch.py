@@ -1,4 +1,4 @@
-"""Batched uneven instances (config-5 form at reduced sizes): running the
+"""Batched uneven 2D instances (at reduced sizes): running the
 instances concurrently on several streams gives, per instance, bitwise the
 results of running them one by one, each instance's theta gradient matches
 the FP64 oracle, and the worker-count autotuner picks its fastest probe."""

@@ -86,7 +86,7 @@ def test_graphs_match_eager_and_replays_are_bitwise(device):
 
 
 def test_batched_configuration_converges_and_matches_single_instance(device):
-    """Config-5 sizes: 32 instances drawn from 128^2 .. 512^2; every solve
+    """The full batched workload: 32 instances drawn from 128^2 .. 512^2; every solve
     converges and the theta gradients of a sample of instances match the
     single-instance GPU simulator's."""
     from dfm.batch import BatchedLFM, BatchEngine, make_problems
