@@ -147,7 +147,8 @@ def run_gpu(args):
                              mg_bottom=50, tol=1e-6, max_iters=200, mg_tail=not args.mg_no_tail,
                              mg_bf16_coarse=args.mg_bf16_coarse, mg_bottom_exact=args.mg_bottom_exact,
                              mg_check_every=args.mg_check_every, overlap_march=False if args.overlap_march == "none" else args.overlap_march,
-                             mg_fused_sweeps=args.mg_fused_sweeps, mg_split_layout=not args.mg_natural_layout)
+                             mg_fused_sweeps=args.mg_fused_sweeps, mg_split_layout=not args.mg_natural_layout,
+                             side_march_blocks=args.side_march_blocks)
     u0, s0, tgt = sc["u0"], sc["density"], sc["target"]
     # the storage of a K-step run, allocated before the warm-up: a warm-up
     # shorter than K must not leave allocations to the timed region
@@ -203,11 +204,29 @@ def run_gpu(args):
                         "sample).  valu_issue_frac: the kernel's vector-ALU wave instructions per launch "
                         "(rocprofv3 SQ_INSTS_VALU over this workload) over the issue ceiling of one wave64 "
                         "instruction per CU per clock, in this run's launch time (beside the projections) and "
-                        "in the counter pass's own (the kernel alone)"}
+                        "in the counter pass's own (the kernel alone).  The side-stream marches run with at most "
+                        "side_march_blocks workgroups per CU (dfm_march_capped), which lengthens their in-run "
+                        "launches; alone_ms / frac_alone: the uncapped launch with nothing beside it"}
     if dom == "march":
         ex = march_flop(n, grad=EXEC_FLOP_INTERP_GRAD_3D, val=EXEC_FLOP_INTERP_VAL_3D)
         roofline["executed_flop_per_launch"] = ex
         roofline["executed_frac"] = round(ex / avg_s / 1e12 / FP32_PEAK_TFLOPS, 4)
+        # the same launch with nothing beside it and no workgroup cap, after
+        # the timed region (on the run's own maps and velocity, which are no
+        # longer needed): the kernel's own rate, against the in-run rate of
+        # the side-stream marches that hold at most side_march_blocks
+        # workgroups per CU so that the projections beside them find free CUs
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        sim.ops.march(sim.u, sim.phi_map, sim.dt, 4)
+        ev0.record()
+        for _ in range(5):
+            sim.ops.march(sim.u, sim.phi_map, sim.dt, 4)
+        ev1.record()
+        torch.cuda.synchronize()
+        alone_s = ev0.elapsed_time(ev1) / 5 / 1e3
+        roofline["side_march_blocks"] = args.side_march_blocks
+        roofline["alone_ms"] = round(alone_s * 1e3, 4)
+        roofline["frac_alone"] = round(flop / alone_s / 1e12 / FP32_PEAK_TFLOPS, 4)
     if models[dom][2] is not None and tuple(n) == (128, 128, 128):
         grid = -(-int(models[dom][3]) // 256) * 256
         v = measured_valu(models[dom][2], grid)
@@ -616,6 +635,8 @@ def parse_args(argv=None):
     ap.add_argument("--cpu-interval", type=int, default=0, metavar="STEPS",
                     help="only the CPU baseline: the oracle over STEPS forward steps and (for a whole number of "
                          "intervals) the adjoint sweep back, each step timed (OMP_NUM_THREADS threads)")
+    ap.add_argument("--side-march-blocks", type=int, default=2,
+                    help="workgroups per CU a side-stream march may hold (0: no cap)")
     ap.add_argument("--mg-alone", action="store_true",
                     help="only the MGPCG solver projecting a seeded random field at --size^3 (10 timed solves)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -22,6 +22,7 @@ static __device__ __align__(256) float scratch[DFM_NOALLOC_MAX_VALUES];
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMarchCapMax = 16;  // largest workgroups-per-CU cap of dfm_march_capped
 
 // Up to four point sets (3 face grids + cells) addressed by one flat index.
 struct Sets {
@@ -760,8 +761,11 @@ extern "C" int dfm_reset_identity(const dfm_grid* g, const dfm_flowmap* m, void*
   return launch_check("dfm_reset_identity");
 }
 
-extern "C" int dfm_march(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m, float dt, int order,
-                         void* stream) {
+// The march of every point set of m, with `lds` bytes of dynamic LDS
+// requested per workgroup (the kernel uses none; a request caps how many of
+// its workgroups a CU holds at once).
+static int march_launch(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m, float dt, int order,
+                        size_t lds, void* stream) {
   Geo G;
   int rc = check_grid(g, G);
   if (rc) return rc;
@@ -787,13 +791,58 @@ extern "C" int dfm_march(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap*
   dim3 grid(blocks_for(total)), blk(kBlock);
   hipStream_t st = S_(stream);
   if (G.dim == 2) {
-    if (order == 2) hipLaunchKernelGGL((k_march<2, 2>), grid, blk, 0, st, G, U, dt, S, M);
-    else hipLaunchKernelGGL((k_march<2, 4>), grid, blk, 0, st, G, U, dt, S, M);
+    if (order == 2) hipLaunchKernelGGL((k_march<2, 2>), grid, blk, lds, st, G, U, dt, S, M);
+    else hipLaunchKernelGGL((k_march<2, 4>), grid, blk, lds, st, G, U, dt, S, M);
   } else {
-    if (order == 2) hipLaunchKernelGGL((k_march<3, 2>), grid, blk, 0, st, G, U, dt, S, M);
-    else hipLaunchKernelGGL((k_march<3, 4>), grid, blk, 0, st, G, U, dt, S, M);
+    if (order == 2) hipLaunchKernelGGL((k_march<3, 2>), grid, blk, lds, st, G, U, dt, S, M);
+    else hipLaunchKernelGGL((k_march<3, 4>), grid, blk, lds, st, G, U, dt, S, M);
   }
   return launch_check("dfm_march");
+}
+
+// Dynamic LDS that lets at most n march workgroups share a CU: the least
+// request of which n + 1 no longer fit in the CU's LDS (1 KB steps), so the
+// rest of the CU's LDS stays free for other kernels.  Instantiations that
+// need more than the default per-workgroup limit are opted in once per
+// device.
+static int march_cap_lds(int n, size_t* lds) {
+  int dev = 0, per_cu = 0, per_block = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || per_cu <= 0)
+    return set_error(DFM_E_HIP, "march: LDS size query failed");
+  const size_t kb = 1024, req = ((size_t)per_cu / (size_t)(n + 1) / kb + 1) * kb;
+  if ((size_t)n * req > (size_t)per_cu) return set_error(DFM_E_ARG, "march: blocks_per_cu too large for the CU's LDS");
+  if (req > (size_t)per_block) {
+    static std::atomic<uint64_t> opted{0};  // bit per device
+    if (dev < 64 && !(opted.load() >> dev & 1u)) {
+      const void* fns[4] = {(const void*)k_march<2, 2>, (const void*)k_march<2, 4>, (const void*)k_march<3, 2>,
+                            (const void*)k_march<3, 4>};
+      for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)req) != hipSuccess)
+          return set_error(DFM_E_HIP, "march: LDS opt-in refused");
+      opted.fetch_or(uint64_t(1) << dev);
+    }
+  }
+  *lds = req;
+  return DFM_OK;
+}
+
+extern "C" int dfm_march(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m, float dt, int order,
+                         void* stream) {
+  return march_launch(g, u, m, dt, order, 0, stream);
+}
+
+extern "C" int dfm_march_capped(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m, float dt, int order,
+                                int blocks_per_cu, void* stream) {
+  if (blocks_per_cu < 0 || blocks_per_cu > kMarchCapMax)
+    return set_error(DFM_E_ARG, "blocks_per_cu must be 0 (no cap) .. 16");
+  size_t lds = 0;
+  if (blocks_per_cu > 0) {
+    const int rc = march_cap_lds(blocks_per_cu, &lds);
+    if (rc) return rc;
+  }
+  return march_launch(g, u, m, dt, order, lds, stream);
 }
 
 extern "C" int dfm_pullback_u_k(const dfm_grid* g, int kern, const dfm_mac* src, const dfm_flowmap* m, float alpha,

@@ -41,6 +41,7 @@ SIGNATURES = {
     "dfm_version": (c_int, []),
     "dfm_reset_identity": (c_int, [P(Grid), P(FlowMapS), c_void_p]),
     "dfm_march": (c_int, [P(Grid), P(Mac), P(FlowMapS), c_float, c_int, c_void_p]),
+    "dfm_march_capped": (c_int, [P(Grid), P(Mac), P(FlowMapS), c_float, c_int, c_int, c_void_p]),
     "dfm_pullback_u": (c_int, [P(Grid), P(Mac), P(FlowMapS), c_float, c_float, P(Mac), P(Mac), c_void_p]),
     "dfm_pullback_s": (c_int, [P(Grid), c_void_p, c_void_p, c_float, c_float, c_void_p, c_void_p, c_void_p]),
     "dfm_bfecc_u": (c_int, [P(Grid), P(Mac), P(FlowMapS), P(FlowMapS), P(Mac), P(Mac), c_void_p]),

@@ -125,9 +125,15 @@ class Ops:
     def reset_identity(self, fm: FlowMap):
         L.check(self.lib.dfm_reset_identity(self.g, fm.s, self._st()), "reset_identity")
 
-    def march(self, u: MacField, fm: FlowMap, dt, order):
+    def march(self, u: MacField, fm: FlowMap, dt, order, blocks_per_cu=0):
+        """blocks_per_cu > 0: at most that many march workgroups per CU
+        (dfm_march_capped), for a march beside other work on another stream."""
         with timing.region("march"):
-            L.check(self.lib.dfm_march(self.g, u.s, fm.s, float(dt), int(order), self._st()), "march")
+            if blocks_per_cu:
+                L.check(self.lib.dfm_march_capped(self.g, u.s, fm.s, float(dt), int(order), int(blocks_per_cu),
+                                                  self._st()), "march_capped")
+            else:
+                L.check(self.lib.dfm_march(self.g, u.s, fm.s, float(dt), int(order), self._st()), "march")
 
     # kern: sampling spline of the advected quantity, 1 linear / 2 quadratic
     # (the reference's) / 3 cubic; the velocity moving points stays quadratic

@@ -123,7 +123,7 @@ class LFM_Simulator:
                  mg_bf16_coarse=False, mg_tail=True, mg_bottom_exact=True, mg_check_every=1, checkpoint=False,
                  overlap_projections=True, control_num=0, BFECC_clamp=False, use_short_BFECC=False,
                  overlap_march=True, mg_fused_sweeps=2, mg_split_layout=True, mg_on_noconv="warn", mg_tail_level=0,
-                 interp_kern=2, mg_graph=True, check_finite=True):
+                 interp_kern=2, mg_graph=True, check_finite=True, side_march_blocks=2):
         n, dx, dt, R, log_dir, log_target_dir = _parse_ctor(args)
         self.n = n
         self.dim = len(n)
@@ -164,6 +164,11 @@ class LFM_Simulator:
         # stream beside the (latency-bound) projections: True / "both",
         # "forward", "adjoint" or False
         self.overlap_march = overlap_march
+        # marches on the side stream hold at most this many workgroups per CU
+        # (dfm_march_capped; 0 = no cap), so that the projection's
+        # one-workgroup coarse levels find a CU without waiting
+        self.side_march_blocks = side_march_blocks
+        self._side_active = False
         self._side = None
         self._side_done = None
         dev = self.device
@@ -247,10 +252,18 @@ class LFM_Simulator:
         ready.record(main)
         with torch.cuda.stream(self._side):
             self._side.wait_event(ready)
-            yield
+            self._side_active = True
+            try:
+                yield
+            finally:
+                self._side_active = False
         done = torch.cuda.Event()
         done.record(self._side)
         self._side_done = done
+
+    def _side_cap(self):
+        """Workgroups-per-CU cap for a march queued now (only on the side stream)."""
+        return self.side_march_blocks if self._side_active else 0
 
     def _join_side(self):
         if self._side_done is not None:
@@ -499,7 +512,7 @@ class LFM_Simulator:
         # nothing before the reinitialisation reads it, so it runs beside the
         # advection and the projection below
         with self._on_side("forward"):
-            ops.march(mid, self.phi_map, -dt, order)
+            ops.march(mid, self.phi_map, -dt, order, blocks_per_cu=self._side_cap())
         if self.short_bfecc:
             # BFECC from the step's start through one-step maps of the midpoint
             # velocity (use_short_BFECC, 2D/vortex/lfm_midpoint_simulator.py:560-583)
@@ -893,7 +906,7 @@ class LFM_Diff_Simulator(LFM_Simulator):
         # they write only psi/T, init_adj_u (init_u) and the AB3 history, so
         # they run on the side stream beside the two projections
         with self._on_side("adjoint"):
-            ops.march(mid, self.psi_map, dt, order)
+            ops.march(mid, self.psi_map, dt, order, blocks_per_cu=self._side_cap())
             ops.accumulate(self.r_u, self.psi_map, self.init_adj_u, self.ab_h1, self.ab_h2,
                            self.reinit_every - 1 - j, -2.0 * dt, f2=self.r_s, dt2=-dt)
             if forces is not None:

@@ -71,6 +71,16 @@ int dfm_reset_identity(const dfm_grid* g, const dfm_flowmap* m, void* stream);
  * 3D/flowmap.py:6-25,85-179. */
 int dfm_march(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m, float dt, int order, void* stream);
 
+/* dfm_march with at most blocks_per_cu of its workgroups resident on a CU at
+ * once (0: no cap; 1..16), through an LDS request the kernel does not use.
+ * The results are dfm_march's, bit for bit.  No counterpart in the
+ * reference: for a march queued on a second stream beside latency-bound work
+ * (the projection's one-workgroup coarse levels need a whole CU free), which
+ * otherwise waits for march workgroups to retire.  Caps the LDS cannot
+ * enforce return DFM_E_ARG. */
+int dfm_march_capped(const dfm_grid* g, const dfm_mac* u, const dfm_flowmap* m, float dt, int order,
+                     int blocks_per_cu, void* stream);
+
 /* Impulse pullback with epilogue: out_a = alpha * (T_a . src(psi_a)) + beta * add_a
  * (add may be NULL when beta == 0; out may alias add).
  * alpha=1, beta=0: advect_u, 2D/vortex/flowmap.py:83-97, 3D/flowmap.py:51-69.

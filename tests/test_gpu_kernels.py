@@ -85,6 +85,35 @@ def test_march(dim, order, device):
         within("T_f[a]", rel_l2(T_f[a], om.T_f[a]), TOL)
 
 
+@pytest.mark.parametrize("dim,order", [(2, 2), (3, 4)])
+def test_march_capped_is_march(dim, order, device):
+    """dfm_march_capped (at most k march workgroups per CU, through an LDS
+    request the kernel does not use) gives dfm_march's maps bit for bit, for
+    caps that fit in the default per-workgroup LDS (2, 3, 16) and one that
+    needs the opt-in (1); caps outside 0..16 are refused."""
+    from dfm import _lib as L
+
+    n, dx = GRIDS[dim]
+    rng = np.random.default_rng(70 + dim)
+    u = smooth_mac(rng, n, dx)
+    om = random_map(rng, n, dx)
+    dt = 0.5 * dx
+    ops = ops_for(n, dx)
+    ref = fmap_to_dev(om, device)
+    ops.march(mac_to_dev(u, device), ref, dt, order)
+    want = fmap_to_np(ref)
+    for cap in (1, 2, 3, 16):
+        fm = fmap_to_dev(om, device)
+        ops.march(mac_to_dev(u, device), fm, dt, order, blocks_per_cu=cap)
+        got = fmap_to_np(fm)
+        assert np.array_equal(got[0], want[0])
+        for a in range(dim):
+            assert np.array_equal(got[1][a], want[1][a]) and np.array_equal(got[2][a], want[2][a])
+    for bad in (-1, 17):
+        with pytest.raises(L.DfmError):
+            ops.march(mac_to_dev(u, device), fmap_to_dev(om, device), dt, order, blocks_per_cu=bad)
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_pullback_u_and_epilogue(dim, device):
     n, dx = GRIDS[dim]
